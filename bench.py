@@ -1,0 +1,158 @@
+"""Benchmark of the NMF multiplicative-update hot path on one MI355X.
+
+A step is one multiplicative-update iteration (H update then W update) over the
+whole matrix.  Default workload: Euclidean NMF, V 16384 x 8192 = W0 H0 + U[0,0.01)
+noise (W0, H0 ~ U[0,1)), r = 64, synthetic data from the seeded generator.
+Prints ONE JSON line (rank 0).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "hbm_gbs": 8000.0}
+
+WORKLOADS = {
+    # name: (rule, n, m, r, r0, seed)
+    "euclid-16384x8192-r64": ("euclidean", 16384, 8192, 64, 64, 0x5EED + 4),
+    "euclid-8192x4096-r64": ("euclidean", 8192, 4096, 64, 64, 0x5EED + 2),
+    "kl-4096x4096-r32": ("kl", 4096, 4096, 32, 32, 0x5EED + 3),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="euclid-16384x8192-r64", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp32", choices=["fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the NMF path "
+              "has no multi-GPU mode (the row-sharded design is documented, not built). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gpumlib_amd import capi
+    from gpumlib_amd import nmf as G
+
+    lib = capi.load()
+    rule_name, n, m, r, r0, seed = WORKLOADS[args.workload]
+    rule = G.EUCLIDEAN if rule_name == "euclidean" else G.KL
+    prec = G.FP32
+    eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
+
+    t_gen = time.time()
+    V = G.gen_euclidean(n, m, r0, seed) if rule == G.EUCLIDEAN else G.gen_poisson(n, m, r0, seed)
+    W0, H0 = G.gen_init(n, m, r, seed)
+    t_gen = time.time() - t_gen
+
+    d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
+    stream = d.stream()
+
+    def run(k):
+        capi.check(lib.gpumlib_nmf_iterate(rule, prec, n, m, r, d.V.data_ptr(), d.W.data_ptr(), d.H.data_ptr(),
+                                           eps, k, 1, G.OBJ_NONE, None, d.ws.data_ptr(), d.ws_bytes, stream))
+
+    obj_kind = G.OBJ_FROBENIUS if rule == G.EUCLIDEAN else G.OBJ_KL
+    obj0 = d.objective(obj_kind)
+    run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    obj1 = d.objective(obj_kind)
+
+    # Per-kernel durations, HIP events on the stream the kernels are launched on.
+    def time_pass(which):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(3):
+            capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
+                                            d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
+        ev0.record()
+        for _ in range(args.kernel_reps):
+            capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
+                                            d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
+        ev1.record()
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1) / args.kernel_reps  # ms per launch
+
+    ms_h, ms_w = time_pass(0), time_pass(1)
+    if rule == G.EUCLIDEAN:
+        flops_h = 2.0 * n * m * r + 2.0 * n * r * r
+        flops_w = 2.0 * n * m * r + 2.0 * m * r * r
+        flops_iter = 4.0 * n * m * r + 4.0 * r * r * (n + m)
+    else:
+        flops_h = flops_w = 4.0 * n * m * r
+        flops_iter = 8.0 * n * m * r
+    vbytes = 4.0 * n * m
+    dom_ms, dom_flops, dom_name = (ms_h, flops_h, "pass_h") if ms_h >= ms_w else (ms_w, flops_w, "pass_w")
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    iters_per_s = 1e3 / ms
+
+    line = {
+        "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",
+        "value": iters_per_s,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded SplitMix64 generator: V = W0 H0 + U[0,0.01) noise)" if rule == G.EUCLIDEAN
+        else "synthetic (seeded SplitMix64 generator: V ~ Poisson(W0 H0), W0, H0 ~ Exp(1))",
+        "config": {"workload": args.workload, "rule": rule_name, "n": n, "m": m, "r": r, "precision": "fp32",
+                   "eps": eps, "parallelism": "single GPU"},
+        "mfma_peak_frac": flops_iter / (ms * 1e-3) / 1e12 / PEAK["fp32_mfma_tflops"],
+        "hbm_gbs_v_passes": 2 * vbytes / (ms * 1e-3) / 1e9,
+        "objective": {"entry": obj0, "after": obj1, "kind": "frobenius" if rule == G.EUCLIDEAN else "kl"},
+        "kernels": {"pass_h_ms": ms_h, "pass_w_ms": ms_w,
+                    "pass_h_tflops": flops_h / (ms_h * 1e-3) / 1e12, "pass_w_tflops": flops_w / (ms_w * 1e-3) / 1e12},
+        "roofline": {"kernel": dom_name, "bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": None},
+        "gen_seconds": t_gen,
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        Vh = np.ascontiguousarray(V)
+        Wh, Hh = W0.copy(), H0.copy()
+        fn = orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter
+        fn(n, m, r, Vh, Wh, Hh, eps, iters=1, omp=True)  # warm-up
+        k, t = 0, 0.0
+        while t < args.cpu_seconds or k < 2:
+            s0 = time.perf_counter()
+            fn(n, m, r, Vh, Wh, Hh, eps, iters=1, omp=True)
+            t += time.perf_counter() - s0
+            k += 1
+        line["cpu_baseline"] = {"value": k / t, "unit": "iter/s", "cores": orc.omp_threads(), "kind": "port",
+                                "sample": f"{k} iterations of the same workload ({args.workload}, FP32 update rules, "
+                                          f"OpenMP host-cores restatement) after 1 warm-up"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

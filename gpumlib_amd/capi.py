@@ -1,0 +1,87 @@
+"""ctypes binding of libgpumlib_nmf.so (the C-ABI in include/gpumlib_nmf.h).
+
+The library is built in-tree by gpumlib_amd/build.py.  Loading fails loudly when
+it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgpumlib_nmf.so")
+
+EUCLIDEAN, KL = 0, 1
+FP32, BF16, FP8 = 0, 1, 2
+OBJ_NONE, OBJ_FROBENIUS, OBJ_KL, OBJ_QUALITY = 0, 1, 2, 3
+REFERENCE_EPS = 1e-8  # float(10e-9f) == float(1e-8f)
+
+_c = ctypes
+_vp, _i, _l, _f, _d, _sz, _u64 = _c.c_void_p, _c.c_int, _c.c_long, _c.c_float, _c.c_double, _c.c_size_t, _c.c_uint64
+
+
+class Dim3(ctypes.Structure):
+    _fields_ = [("x", _c.c_uint32), ("y", _c.c_uint32), ("z", _c.c_uint32)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix": (_i, [_c.c_uint, _c.c_uint, _vp, _vp, _vp, _i]),
+    "gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+    "gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+    "gpumlib_nmf_mu_update": (_i, [_vp, _vp, _vp, _l, _f, _vp]),
+    "gpumlib_nmf_sum_w": (_i, [_vp, _i, _i, _f, _vp, _vp]),
+    "gpumlib_nmf_sum_h": (_i, [_vp, _i, _i, _f, _vp, _vp]),
+    "gpumlib_nmf_kl_update_h": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
+    "gpumlib_nmf_kl_update_w": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
+    "gpumlib_nmf_quality_workspace_bytes": (_sz, [_l]),
+    "gpumlib_nmf_quality": (_i, [_vp, _vp, _l, _f, _vp, _sz, _vp, _vp]),
+    "gpumlib_nmf_sgemm": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
+    "gpumlib_nmf_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "gpumlib_nmf_iterate": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "gpumlib_nmf_objective": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
+    "gpumlib_nmf_pass": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _vp]),
+    "gpumlib_nmf_object_create": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _c.POINTER(_vp)]),
+    "gpumlib_nmf_object_set_epsilon": (_i, [_vp, _f]),
+    "gpumlib_nmf_object_set_quality_interval": (_i, [_vp, _i]),
+    "gpumlib_nmf_object_do_iteration": (_i, [_vp, _i]),
+    "gpumlib_nmf_object_do_iterations": (_i, [_vp, _i, _i]),
+    "gpumlib_nmf_object_quality_improvement": (_f, [_vp]),
+    "gpumlib_nmf_object_get_w": (_i, [_vp, _vp]),
+    "gpumlib_nmf_object_get_h": (_i, [_vp, _vp]),
+    "gpumlib_nmf_object_get_wh": (_i, [_vp, _vp]),
+    "gpumlib_nmf_object_objective": (_i, [_vp, _i, _c.POINTER(_d)]),
+    "gpumlib_nmf_object_destroy": (None, [_vp]),
+    "gpumlib_nmf_rng_u64": (_u64, [_u64, _u64, _u64]),
+    "gpumlib_nmf_gen_uniform_f32": (None, [_u64, _u64, _sz, _vp]),
+    "gpumlib_nmf_gen_euclidean_v": (None, [_u64, _i, _i, _i, _f, _vp]),
+    "gpumlib_nmf_gen_poisson_v": (None, [_u64, _i, _i, _i, _vp]),
+    "gpumlib_nmf_gen_init": (None, [_u64, _i, _i, _i, _vp, _vp]),
+}
+
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load the native library (once) and declare every C-ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryMissing(
+            f"{LIB_PATH} is missing; build it with `python gpumlib_amd/build.py` "
+            "(or __graft_entry__.build()).  There is no non-native fallback.")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(err, what="gpumlib call"):
+    if err != 0:
+        raise RuntimeError(f"{what} failed with hipError_t {err}")

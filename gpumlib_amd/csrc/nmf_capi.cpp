@@ -1,0 +1,256 @@
+// C-ABI of the NMF library: argument checking, workspace planning and the
+// launch sequence of one multiplicative-update iteration.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/gpumlib_nmf.h"
+#include "nmf_common.hpp"
+#include "nmf_internal.hpp"
+#include "nmf_plan.hpp"
+
+using namespace gpumlib_nmf;
+
+namespace gpumlib_nmf {
+
+static int choose_slices(int tiles, int K, int* kslice) {
+    const int target = kTargetWorkgroups;
+    int S = (target + tiles - 1) / tiles;
+    const int maxS = (K + KC - 1) / KC;
+    if (S > maxS) S = maxS;
+    if (S < 1) S = 1;
+    int ks = round_up((K + S - 1) / S, KC);
+    if (ks < KC) ks = KC;
+    S = (K + ks - 1) / ks;
+    if (S < 1) S = 1;
+    *kslice = ks;
+    return S;
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl) {
+    if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return false;
+    if (precision != GPUMLIB_NMF_FP32) return false;
+    if (n < 1 || m < 1 || r < 1 || r > kMaxFusedRank) return false;
+    memset(pl, 0, sizeof(*pl));
+    pl->rule = rule;
+    pl->precision = precision;
+    pl->n = n;
+    pl->m = m;
+    pl->r = r;
+    pl->npad = round_up(n, 4);
+    pl->rpad = round_up(r, 4);
+    pl->ldr = pl->rpad;
+    pl->ldp = pl->npad + pl->rpad;
+    const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
+    pl->tiles1 = (m + TILE - 1) / TILE + euc;
+    pl->S1 = choose_slices(pl->tiles1, n, &pl->ks1);
+    pl->pstride1 = (long)(m + r) * pl->ldr;
+    pl->tiles3 = (n + TILE - 1) / TILE + euc;
+    pl->S3 = choose_slices(pl->tiles3, m, &pl->ks3);
+    pl->pstride3 = (long)r * pl->ldp;
+    const size_t p1 = (size_t)pl->S1 * pl->pstride1, p3 = (size_t)pl->S3 * pl->pstride3;
+    const size_t pbytes = align256(sizeof(float) * (p1 > p3 ? p1 : p3));
+    const int smax = pl->S1 > pl->S3 ? pl->S1 : pl->S3;
+    const size_t sbytes = align256(sizeof(float) * (size_t)smax * pl->rpad);
+    pl->nobjp = pl->tiles1 * pl->S1;  // one objective partial per H-pass workgroup
+    const size_t obytes = align256(sizeof(double) * (size_t)pl->nobjp);
+    pl->off_P = 0;
+    pl->off_Psum = pbytes;
+    pl->off_objp = pbytes + sbytes;
+    pl->bytes = pbytes + sbytes + obytes;
+    return true;
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+void pass_args(const Plan& pl, const float* V, const float* W, const float* H, void* ws, float eps, int which,
+               PassArgs* a) {
+    char* base = (char*)ws;
+    memset(a, 0, sizeof(*a));
+    a->V = V;
+    a->W = W;
+    a->H = H;
+    a->P = (float*)(base + pl.off_P);
+    a->Psum = (float*)(base + pl.off_Psum);
+    a->objp = (double*)(base + pl.off_objp);
+    a->n = pl.n;
+    a->m = pl.m;
+    a->r = pl.r;
+    a->ldr = pl.ldr;
+    a->ldp = pl.ldp;
+    a->npad = pl.npad;
+    a->eps = eps;
+    a->vec = (pl.n % 4 == 0) && (pl.r % 4 == 0) && aligned16(V) && aligned16(W) && aligned16(H);
+    if (which == 1) {
+        a->kslice = pl.ks1;
+        a->S = pl.S1;
+        a->tiles = pl.tiles1;
+        a->pstride = pl.pstride1;
+    } else {
+        a->kslice = pl.ks3;
+        a->S = pl.S3;
+        a->tiles = pl.tiles3;
+        a->pstride = pl.pstride3;
+    }
+}
+
+hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations, int update_w,
+                          int objective, double* d_objective, void* ws, hipStream_t st) {
+    const int kl = (pl.rule == GPUMLIB_NMF_KL);
+    PassArgs a1, a3;
+    pass_args(pl, V, W, H, ws, eps, 1, &a1);
+    pass_args(pl, V, W, H, ws, eps, 3, &a3);
+    hipError_t e;
+#define CK(x)                             \
+    do {                                  \
+        if ((e = (x)) != hipSuccess) return e; \
+    } while (0)
+    for (int it = 0; it < iterations; it++) {
+        const int obj = (it == 0 && d_objective) ? objective : GPUMLIB_NMF_OBJ_NONE;
+        CK(launch_pass_h_f32(a1, kl, obj, 1, st));
+        if (obj != GPUMLIB_NMF_OBJ_NONE) CK(launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st));
+        if (kl)
+            CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
+        else
+            CK(launch_update_h_euc(H, a1.P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
+        if (!update_w) continue;
+        CK(launch_pass_w_f32(a3, kl, st));
+        if (kl)
+            CK(launch_update_w_kl(W, a3.P, a3.Psum, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st));
+        else
+            CK(launch_update_w_euc(W, a3.P, pl.n, pl.r, pl.ldp, pl.npad, pl.S3, pl.pstride3, eps, st));
+    }
+    return hipSuccess;
+}
+
+hipError_t run_objective(const Plan& pl, const float* V, const float* W, const float* H, float eps, int objective,
+                         double* d_objective, void* ws, hipStream_t st) {
+    PassArgs a1;
+    pass_args(pl, V, W, H, ws, eps, 1, &a1);
+    a1.tiles = (pl.m + TILE - 1) / TILE;  // no gram tile
+    hipError_t e = launch_pass_h_f32(a1, 0, objective, 0, st);
+    if (e != hipSuccess) return e;
+    return launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st);
+#undef CK
+}
+
+}  // namespace gpumlib_nmf
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+hipError_t gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix(unsigned int, unsigned int, float* nm, float* dm,
+                                                                   float* m, int elements) {
+    return launch_mu_update(nm, dm, m, elements, kReferenceEpsilon, nullptr);
+}
+
+hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w(dim3, dim3, float* W, float* H, float* V,
+                                                                      float* WH, float* sumH, int n, int m, int r) {
+    return launch_kl_update_w_wh(W, H, V, WH, sumH, n, m, r, kReferenceEpsilon, nullptr);
+}
+
+hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(dim3, dim3, float* H, float* W, float* V,
+                                                                      float* WH, float* sumW, int n, int m, int r) {
+    return launch_kl_update_h_wh(H, W, V, WH, sumW, n, m, r, kReferenceEpsilon, nullptr);
+}
+
+hipError_t gpumlib_nmf_mu_update(const float* nm, const float* dm, float* m, long elements, float eps,
+                                 hipStream_t stream) {
+    if (elements < 0) return hipErrorInvalidValue;
+    return launch_mu_update(nm, dm, m, elements, eps, stream);
+}
+
+hipError_t gpumlib_nmf_sum_w(const float* W, int n, int r, float eps, float* sumW, hipStream_t stream) {
+    if (n < 0 || r < 0) return hipErrorInvalidValue;
+    return launch_sum_w(W, n, r, eps, sumW, stream);
+}
+
+hipError_t gpumlib_nmf_sum_h(const float* H, int r, int m, float eps, float* sumH, hipStream_t stream) {
+    if (m < 0 || r < 0 || r > 256) return hipErrorInvalidValue;
+    return launch_sum_h(H, r, m, eps, sumH, stream);
+}
+
+hipError_t gpumlib_nmf_kl_update_h(float* H, const float* W, const float* V, const float* WH, const float* sumW, int n,
+                                   int m, int r, float eps, hipStream_t stream) {
+    if (n < 0 || m < 0 || r < 0) return hipErrorInvalidValue;
+    return launch_kl_update_h_wh(H, W, V, WH, sumW, n, m, r, eps, stream);
+}
+
+hipError_t gpumlib_nmf_kl_update_w(float* W, const float* H, const float* V, const float* WH, const float* sumH, int n,
+                                   int m, int r, float eps, hipStream_t stream) {
+    if (n < 0 || m < 0 || r < 0) return hipErrorInvalidValue;
+    return launch_kl_update_w_wh(W, H, V, WH, sumH, n, m, r, eps, stream);
+}
+
+static int quality_partials(long count) {
+    long p = (count + (long)NTHREADS * 16 - 1) / ((long)NTHREADS * 16);
+    if (p < 1) p = 1;
+    if (p > 1024) p = 1024;
+    return (int)p;
+}
+
+size_t gpumlib_nmf_quality_workspace_bytes(long count) {
+    return sizeof(double) * (size_t)quality_partials(count < 0 ? 0 : count);
+}
+
+hipError_t gpumlib_nmf_quality(const float* V, const float* WH, long count, float eps, void* workspace,
+                               size_t workspace_bytes, double* d_quality, hipStream_t stream) {
+    if (count < 0 || !d_quality) return hipErrorInvalidValue;
+    const int np = quality_partials(count);
+    if (!workspace || workspace_bytes < sizeof(double) * (size_t)np) return hipErrorInvalidValue;
+    return launch_quality(V, WH, count, eps, (double*)workspace, np, d_quality, stream);
+}
+
+hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, float alpha, const float* A, int lda,
+                             const float* B, int ldb, float beta, float* C, int ldc, hipStream_t stream) {
+    return launch_sgemm(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+}
+
+size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r) {
+    Plan pl;
+    if (!make_plan(rule, precision, n, m, r, &pl)) return 0;
+    return pl.bytes;
+}
+
+hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, const void* V, float* W, float* H,
+                               float eps, int iterations, int update_w, int objective, double* d_objective,
+                               void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    Plan pl;
+    if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
+    if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || iterations < 0) return hipErrorInvalidValue;
+    if (objective < GPUMLIB_NMF_OBJ_NONE || objective > GPUMLIB_NMF_OBJ_QUALITY) return hipErrorInvalidValue;
+    if (objective != GPUMLIB_NMF_OBJ_NONE && !d_objective) return hipErrorInvalidValue;
+    if (iterations == 0) {
+        if (objective == GPUMLIB_NMF_OBJ_NONE) return hipSuccess;
+        return run_objective(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
+    }
+    return run_iterations(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective, workspace,
+                          stream);
+}
+
+hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, const void* V, const float* W,
+                                 const float* H, float eps, int objective, double* d_objective, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream) {
+    Plan pl;
+    if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
+    if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || !d_objective) return hipErrorInvalidValue;
+    if (objective < GPUMLIB_NMF_OBJ_FROBENIUS || objective > GPUMLIB_NMF_OBJ_QUALITY) return hipErrorInvalidValue;
+    return run_objective(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
+}
+
+hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void* V, const float* W,
+                            const float* H, float eps, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    Plan pl;
+    if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
+    if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
+        return hipErrorInvalidValue;
+    const int kl = (rule == GPUMLIB_NMF_KL);
+    PassArgs a;
+    pass_args(pl, (const float*)V, W, H, workspace, eps, which == 0 ? 1 : 3, &a);
+    if (which == 0) return launch_pass_h_f32(a, kl, GPUMLIB_NMF_OBJ_NONE, 1, stream);
+    return launch_pass_w_f32(a, kl, stream);
+}
+
+}  // extern "C"

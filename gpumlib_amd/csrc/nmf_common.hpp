@@ -1,0 +1,90 @@
+// Shared definitions for the gfx950 NMF kernels.
+//
+// Storage is column-major, as in GPUMLib (src/algorithms/nmf/nmf_base.h:109-111):
+//   V is n x m, V(i,u) = V[u*n + i];  W is n x r, W(i,a) = W[a*n + i];
+//   H is r x m, H(a,u) = H[u*r + a].
+// The two long contractions of one multiplicative-update iteration are
+//   "H pass": reduce over the rows i of V   -> r x m   (W^T V, W^T W, KL numerator of H)
+//   "W pass": reduce over the columns u of V -> n x r  (V H^T, H H^T, KL numerator of W)
+// Each is split over K into S slices written as FP32 partials; a small update
+// kernel ("epilogue") sums the S partials in slice order (deterministic) and
+// applies the multiplicative rule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpumlib_nmf {
+
+
+// K depth staged through LDS per pipeline step, and the LDS row pitch used for
+// K-contiguous tiles (4 floats of padding keep 16-byte alignment for 128-bit LDS
+// reads and spread 8-lane groups over distinct banks).
+constexpr int KC = 32;
+constexpr int LDK = KC + 4;
+// Output columns (H pass) or rows (W pass) of V handled by one workgroup.
+constexpr int TILE = 128;
+constexpr int NTHREADS = 256;
+
+// Rank padding: the fused kernels hold r rounded up to 32-row MFMA blocks.
+__host__ __device__ inline int round_up(int x, int q) { return (x + q - 1) / q * q; }
+inline int xcd_grid(int work) { return (work + 7) / 8 * 8; }
+
+#if defined(__HIPCC__)
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One 32x32x2 FP32 MFMA.  Operand lane layout (lane l, c = l % 32, h = l / 32):
+//   A[row c][k h], B[k h][col c];  result register v holds D[row 8*(v/4) + 4*h + v%4][col c].
+__device__ __forceinline__ f32x16 mfma32x2(float a, float b, f32x16 acc) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+}
+
+// Row of the 32x32 MFMA result held in accumulator register v by half-wave h.
+__device__ __forceinline__ int acc_row(int v, int h) { return ((v >> 2) << 3) + (h << 2) + (v & 3); }
+
+// Workgroups are handed to the 8 XCDs round-robin by hardware block id.  This
+// maps block ids so that consecutive work items (which share a K slice and so a
+// slice of W or H) run on the same XCD and hit the same L2.
+__device__ __forceinline__ int xcd_work_index(int bid, int grid) {
+    const int per = grid >> 3;  // grid is a multiple of 8
+    return (bid & 7) * per + (bid >> 3);
+}
+
+// Guarded 4-float load of p[0..3] where element j exists iff ok && k + j < kend.
+// vec: the address is 16-byte aligned whenever k is a multiple of 4.
+__device__ __forceinline__ float4 load4(const float* p, bool ok, int k, int kend, bool vec) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!ok) return v;
+    if (vec && k + 3 < kend) return *reinterpret_cast<const float4*>(p);
+    if (k < kend) v.x = p[0];
+    if (k + 1 < kend) v.y = p[1];
+    if (k + 2 < kend) v.z = p[2];
+    if (k + 3 < kend) v.w = p[3];
+    return v;
+}
+#endif  // __HIPCC__
+
+// Launch description of one pass (H or W) of one problem.
+struct PassArgs {
+    const float* V;   // n x m
+    const float* W;   // n x r
+    const float* H;   // r x m
+    float* P;         // partials, S slices of pstride floats
+    float* Psum;      // KL: S slices of rpad floats (column sums of W / row sums of H)
+    double* objp;     // per-workgroup objective partials (null: no objective)
+    int n, m, r;
+    int ldr;          // H pass partial: P[s][u*ldr + a], u in [0, m + r) (gram columns at u >= m)
+    int ldp;          // W pass partial: P[s][a*ldp + i], i in [0, npad + r) (gram rows at i >= npad)
+    int npad;         // round_up(n, 4)
+    int kslice;       // K extent of one slice, multiple of KC
+    int S;            // slices
+    int tiles;        // tiles per slice (including the gram tile for the Euclidean rule)
+    long pstride;     // floats per slice of P
+    float eps;
+    int vec;          // V, W, H rows/columns are 16-byte aligned (n % 4 == 0 and r % 4 == 0)
+};
+
+// Objective selector for the fused H pass.
+enum ObjKind { OBJ_NONE = 0, OBJ_FROB = 1, OBJ_KL = 2, OBJ_QUALITY = 3 };
+
+}  // namespace gpumlib_nmf

@@ -1,0 +1,506 @@
+// FP32 MFMA pass kernels of the multiplicative-update iteration.
+//
+// H pass (one workgroup = 128 columns u of V x one K slice of rows i):
+//   Euclidean:  P[s](a,u) = sum_i W(i,a) V(i,u)              (W^T V, reference
+//               nmf_multiplicative_euclidian.cpp:36 via DeviceMatrix.h:217-227)
+//               plus one extra tile with V replaced by W      (W^T W, :39)
+//   KL:         P[s](a,u) = sum_i W(i,a) V(i,u) / ((W H)(i,u) + eps)
+//               (the numerator of nmf_kernels.cu:295-341, with W H formed per
+//               tile on MFMA instead of being materialised by a separate SGEMM
+//               as in nmf_multiplicative_divergence.cpp:37)
+// W pass (one workgroup = 128 rows i of V x one K slice of columns u):
+//   Euclidean:  P[s](i,a) = sum_u V(i,u) H(a,u)               (V H^T, :56)
+//               plus one extra tile with V replaced by H      (H H^T, :59-61)
+//   KL:         P[s](i,a) = sum_u V(i,u) H(a,u) / ((W H)(i,u) + eps)
+//               (numerator of nmf_kernels.cu:247-293)
+//
+// Every workgroup has 4 waves; a wave owns 32 columns (H pass) or 32 rows
+// (W pass) of the tile and all r (padded to MB*32) rank rows, i.e. MB 32x32
+// MFMA accumulators.  K is staged through double-buffered LDS in steps of KC
+// with one barrier per step; the next step's global loads are in flight while
+// the current step's MFMAs run.
+#include "nmf_common.hpp"
+#include "nmf_internal.hpp"
+
+namespace gpumlib_nmf {
+
+// ---------------------------------------------------------------------------
+// Workgroup-wide deterministic sum of one double per thread; thread 0 returns it.
+__device__ double block_sum_f64(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NTHREADS / 64; i++) t += red[i];
+    }
+    return t;
+}
+
+template <int OBJ>
+__device__ __forceinline__ double obj_term(float v, float wh, float eps) {
+    if (OBJ == OBJ_FROB) {
+        const double d = (double)v - (double)wh;
+        return d * d;
+    } else if (OBJ == OBJ_KL) {
+        const double dv = v, dwh = wh;
+        const double t = (v > 0.f) ? dv * log(dv / dwh) : 0.0;
+        return t - dv + dwh;
+    } else if (OBJ == OBJ_QUALITY) {
+        // reference: V[k] * log10(WH[k] + eps) - WH[k]  (nmf_kernels_quality.cu:35)
+        const float whe = wh + eps;
+        return (double)v * log10((double)whe) - (double)wh;
+    }
+    return 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
+// entry; GEMM2 false: objective only (no partials written).
+template <int MB, bool KL, int OBJ, bool GEMM2>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
+    constexpr int RP = MB * 32;
+    constexpr bool NEED_WH = KL || (OBJ != OBJ_NONE);
+    __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];    // W chunk, [a][i]
+    __shared__ __attribute__((aligned(16))) float sB[2][TILE][LDK];  // V (or W) chunk, [u][i]
+    __shared__ double red[NTHREADS / 64];
+
+    const int total = p.tiles * p.S;
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= total) return;
+    const int s = w / p.tiles, t = w - s * p.tiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    const int vt = (p.m + TILE - 1) / TILE;
+    const bool gram = (!KL) && (t == vt);
+    const float* B = gram ? p.W : p.V;
+    const int col0 = gram ? 0 : t * TILE;
+    const int ncols = gram ? p.r : min(TILE, p.m - col0);
+    const int kbeg = s * p.kslice;
+    const int kend = min(p.n, kbeg + p.kslice);
+    const bool vec = p.vec != 0;
+
+    // Per-wave B operand of the W H product: H(b, u) for b = 2*kk + h, u = this lane's column.
+    float hreg[NEED_WH ? MB * 16 : 1];
+    if (NEED_WH) {
+        const int u = col0 + wave * 32 + c;
+#pragma unroll
+        for (int kk = 0; kk < MB * 16; kk++) {
+            const int b = 2 * kk + h;
+            hreg[kk] = (!gram && u < p.m && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
+        }
+    }
+
+    float4 ra[MB], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
+            rb[j] = load4(B + (size_t)(col0 + col) * p.n + k0 + kq, col < ncols, k0 + kq, kend, vec);
+        }
+#pragma unroll
+        for (int j = 0; j < MB; j++) {
+            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 4;
+            ra[j] = load4(p.W + (size_t)a * p.n + k0 + kq, a < p.r, k0 + kq, kend, vec);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
+            *reinterpret_cast<float4*>(&sB[buf][col][kq]) = rb[j];
+        }
+#pragma unroll
+        for (int j = 0; j < MB; j++) {
+            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 4;
+            *reinterpret_cast<float4*>(&sA[buf][a][kq]) = ra[j];
+        }
+    };
+
+    f32x16 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    double objacc = 0.0;
+    float colsum = 0.f;  // KL: sum over this slice of W(i, tid) (tile 0 only)
+    const bool do_sums = KL && GEMM2 && (t == 0) && (tid < RP);
+
+    const int nchunks = (kend - kbeg + KC - 1) / KC;
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
+
+        if (do_sums) {
+#pragma unroll
+            for (int j = 0; j < KC; j += 4) {
+                const float4 x = *reinterpret_cast<const float4*>(&sA[buf][tid][j]);
+                colsum += (x.x + x.y) + (x.z + x.w);
+            }
+        }
+
+        if (NEED_WH && !gram) {
+            // wh(i, u) for the 32 rows of this step and the wave's 32 columns.
+            f32x16 wh;
+            for (int v = 0; v < 16; v++) wh[v] = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < MB * 16; kk++) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
+            // wh[v] is row i = acc_row(v, h) of the chunk, column c of the wave.
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 vv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
+                const float vj[4] = {vv.x, vv.y, vv.z, vv.w};
+                float rj[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (OBJ != OBJ_NONE) objacc += obj_term<OBJ>(vj[j], wh[g * 4 + j], p.eps);
+                    if (KL) rj[j] = vj[j] / (wh[g * 4 + j] + p.eps);
+                }
+                if (KL && GEMM2) {
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++) {
+                        const float4 wa = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][g * 8 + h * 4]);
+                        acc[mb] = mfma32x2(wa.x, rj[0], acc[mb]);
+                        acc[mb] = mfma32x2(wa.y, rj[1], acc[mb]);
+                        acc[mb] = mfma32x2(wa.z, rj[2], acc[mb]);
+                        acc[mb] = mfma32x2(wa.w, rj[3], acc[mb]);
+                    }
+                }
+            }
+        }
+        if (!KL && GEMM2) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 bv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) {
+                    const float4 av = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][g * 8 + h * 4]);
+                    acc[mb] = mfma32x2(av.x, bv.x, acc[mb]);
+                    acc[mb] = mfma32x2(av.y, bv.y, acc[mb]);
+                    acc[mb] = mfma32x2(av.z, bv.z, acc[mb]);
+                    acc[mb] = mfma32x2(av.w, bv.w, acc[mb]);
+                }
+            }
+        }
+        if (ch + 1 < nchunks) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    if (GEMM2) {
+        // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
+        const int ul = wave * 32 + c;
+        if (ul < ncols) {
+            const int ucol = (gram ? p.m : col0) + ul;
+            float* dst = p.P + (size_t)s * p.pstride + (size_t)ucol * p.ldr;
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int a0 = mb * 32 + g * 8 + h * 4;
+                    if (a0 < p.ldr) {
+                        float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
+                        *reinterpret_cast<float4*>(dst + a0) = o;
+                    }
+                }
+            }
+        }
+        if (do_sums && tid < p.r) p.Psum[(size_t)s * round_up(p.r, 4) + tid] = colsum;
+    }
+    if (OBJ != OBJ_NONE) {
+        const double tot = block_sum_f64(objacc, red);
+        if (tid == 0) p.objp[w] = tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// W pass, Euclidean rule: P[s](i,a) = sum_u A(i,u) H(a,u), A = V (row tiles) or H (gram tile).
+template <int MB>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_euc_f32(PassArgs p) {
+    constexpr int RP = MB * 32;
+    constexpr int PA = TILE + 32;  // rows k and k+1 land 32 banks apart
+    constexpr int PB = RP + 32;
+    __shared__ __attribute__((aligned(16))) float sA[2][KC][PA];  // [u][i]
+    __shared__ __attribute__((aligned(16))) float sB[2][KC][PB];  // [u][a]
+
+    const int total = p.tiles * p.S;
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= total) return;
+    const int s = w / p.tiles, t = w - s * p.tiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    const int vt = (p.n + TILE - 1) / TILE;
+    const bool gram = (t == vt);
+    const float* A = gram ? p.H : p.V;
+    const int lda = gram ? p.r : p.n;
+    const int row0 = gram ? 0 : t * TILE;
+    const int nrows = gram ? p.r : min(TILE, p.n - row0);
+    const int kbeg = s * p.kslice;
+    const int kend = min(p.m, kbeg + p.kslice);
+    const bool vec = p.vec != 0;
+    constexpr int BQ = RP / 4;                // float4 per H column chunk
+    constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
+
+    float4 ra[4], rb[BLOADS];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+            ra[j] = load4(A + (size_t)(k0 + uu) * lda + row0 + iq, k0 + uu < kend, iq, nrows, vec);
+        }
+#pragma unroll
+        for (int j = 0; j < BLOADS; j++) {
+            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
+            rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+            *reinterpret_cast<float4*>(&sA[buf][uu][iq]) = ra[j];
+        }
+#pragma unroll
+        for (int j = 0; j < BLOADS; j++) {
+            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
+            if (uu < KC) *reinterpret_cast<float4*>(&sB[buf][uu][aq]) = rb[j];
+        }
+    };
+
+    f32x16 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+
+    const int nchunks = (kend - kbeg + KC - 1) / KC;
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
+#pragma unroll
+        for (int kk = 0; kk < KC / 2; kk++) {
+            const float av = sA[buf][2 * kk + h][wave * 32 + c];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
+        }
+        if (ch + 1 < nchunks) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    // acc[mb][v] = P(i = wave*32 + acc_row(v,h), a = mb*32 + c)
+    const int orow0 = gram ? p.npad : row0;
+    const int rlim = gram ? round_up(p.r, 4) : (p.npad - row0);
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+        const int a = mb * 32 + c;
+        if (a >= p.r) continue;
+        float* dst = p.P + (size_t)s * p.pstride + (size_t)a * p.ldp + orow0;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int il = wave * 32 + g * 8 + h * 4;
+            if (il < rlim) {
+                float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
+                *reinterpret_cast<float4*>(dst + il) = o;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// W pass, KL rule: P[s](i,a) = sum_u V(i,u) H(a,u) / ((W H)(i,u) + eps).
+// (W H)^T is formed per step as H^T W^T so that the ratio lands in the A-operand
+// layout of the second product without a shuffle.
+template <int MB>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
+    constexpr int RP = MB * 32;
+    constexpr int PV = TILE + 8;  // rows u and u+4 land 32 banks apart
+    __shared__ __attribute__((aligned(16))) float sV[2][KC][PV];   // [u][i]
+    __shared__ __attribute__((aligned(16))) float sH[2][RP][LDK];  // [a][u]
+
+    const int total = p.tiles * p.S;
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= total) return;
+    const int s = w / p.tiles, t = w - s * p.tiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    const int row0 = t * TILE;
+    const int nrows = min(TILE, p.n - row0);
+    const int kbeg = s * p.kslice;
+    const int kend = min(p.m, kbeg + p.kslice);
+    const bool vec = p.vec != 0;
+    constexpr int BQ = RP / 4;
+    constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
+
+    // B operand of H^T W^T: W(i, b) for i = this lane's row, b = 2*kk + h.
+    float wreg[MB * 16];
+    {
+        const int i = row0 + wave * 32 + c;
+#pragma unroll
+        for (int kk = 0; kk < MB * 16; kk++) {
+            const int b = 2 * kk + h;
+            wreg[kk] = (i < p.n && b < p.r) ? p.W[(size_t)b * p.n + i] : 0.f;
+        }
+    }
+
+    float4 ra[4], rb[BLOADS];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+            ra[j] = load4(p.V + (size_t)(k0 + uu) * p.n + row0 + iq, k0 + uu < kend, iq, nrows, vec);
+        }
+#pragma unroll
+        for (int j = 0; j < BLOADS; j++) {
+            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
+            rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+            *reinterpret_cast<float4*>(&sV[buf][uu][iq]) = ra[j];
+        }
+#pragma unroll
+        for (int j = 0; j < BLOADS; j++) {
+            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
+            if (uu < KC) {
+                sH[buf][aq + 0][uu] = rb[j].x;
+                sH[buf][aq + 1][uu] = rb[j].y;
+                sH[buf][aq + 2][uu] = rb[j].z;
+                sH[buf][aq + 3][uu] = rb[j].w;
+            }
+        }
+    };
+
+    f32x16 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    float rowsum = 0.f;  // sum over this slice of H(tid, u) (tile 0 only)
+    const bool do_sums = (t == 0) && (tid < RP);
+
+    const int nchunks = (kend - kbeg + KC - 1) / KC;
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
+        if (do_sums) {
+#pragma unroll
+            for (int j = 0; j < KC; j += 4) {
+                const float4 x = *reinterpret_cast<const float4*>(&sH[buf][tid][j]);
+                rowsum += (x.x + x.y) + (x.z + x.w);
+            }
+        }
+        // whT(u, i): rows u = acc_row(v,h) of this step, column i = wave*32 + c.
+        f32x16 wt;
+        for (int v = 0; v < 16; v++) wt[v] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < MB * 16; kk++) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            float rj[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float vv = sV[buf][g * 8 + h * 4 + j][wave * 32 + c];
+                rj[j] = vv / (wt[g * 4 + j] + p.eps);
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                const float4 hb = *reinterpret_cast<const float4*>(&sH[buf][mb * 32 + c][g * 8 + h * 4]);
+                acc[mb] = mfma32x2(rj[0], hb.x, acc[mb]);
+                acc[mb] = mfma32x2(rj[1], hb.y, acc[mb]);
+                acc[mb] = mfma32x2(rj[2], hb.z, acc[mb]);
+                acc[mb] = mfma32x2(rj[3], hb.w, acc[mb]);
+            }
+        }
+        if (ch + 1 < nchunks) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const int rlim = p.npad - row0;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+        const int a = mb * 32 + c;
+        if (a >= p.r) continue;
+        float* dst = p.P + (size_t)s * p.pstride + (size_t)a * p.ldp + row0;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int il = wave * 32 + g * 8 + h * 4;
+            if (il < rlim) {
+                float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
+                *reinterpret_cast<float4*>(dst + il) = o;
+            }
+        }
+    }
+    if (do_sums && tid < p.r) p.Psum[(size_t)s * round_up(p.r, 4) + tid] = rowsum;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+template <int MB>
+static hipError_t launch_h_mb(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st) {
+    const int grid = xcd_grid(a.tiles * a.S);
+#define LH(KLV, OBJV, G2)                                                                        \
+    hipLaunchKernelGGL((k_pass_h_f32<MB, KLV, OBJV, G2>), dim3(grid), dim3(NTHREADS), 0, st, a); \
+    return hipGetLastError();
+#define LH_OBJ(KLV, G2)                       \
+    switch (obj) {                            \
+        case OBJ_NONE: LH(KLV, OBJ_NONE, G2)  \
+        case OBJ_FROB: LH(KLV, OBJ_FROB, G2)  \
+        case OBJ_KL: LH(KLV, OBJ_KL, G2)      \
+        case OBJ_QUALITY: LH(KLV, OBJ_QUALITY, G2) \
+        default: return hipErrorInvalidValue; \
+    }
+    if (!gemm2) {
+        // objective only: the rule does not enter
+        switch (obj) {
+            case OBJ_FROB: LH(false, OBJ_FROB, false)
+            case OBJ_KL: LH(false, OBJ_KL, false)
+            case OBJ_QUALITY: LH(false, OBJ_QUALITY, false)
+            default: return hipErrorInvalidValue;
+        }
+    }
+    if (kl) {
+        LH_OBJ(true, true)
+    } else {
+        LH_OBJ(false, true)
+    }
+#undef LH_OBJ
+#undef LH
+}
+
+hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st) {
+    const int rp = round_up(a.r, 32);
+    if (rp <= 32) return launch_h_mb<1>(a, kl, obj, gemm2, st);
+    if (rp <= 64) return launch_h_mb<2>(a, kl, obj, gemm2, st);
+    if (rp <= 128) return launch_h_mb<4>(a, kl, obj, gemm2, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass_w_f32(const PassArgs& a, int kl, hipStream_t st) {
+    const int rp = round_up(a.r, 32);
+    const int grid = xcd_grid(a.tiles * a.S);
+#define LW(K, MBV)                                                                      \
+    hipLaunchKernelGGL((K<MBV>), dim3(grid), dim3(NTHREADS), 0, st, a); \
+    return hipGetLastError();
+    if (kl) {
+        if (rp <= 32) { LW(k_pass_w_kl_f32, 1) }
+        if (rp <= 64) { LW(k_pass_w_kl_f32, 2) }
+        if (rp <= 128) { LW(k_pass_w_kl_f32, 4) }
+    } else {
+        if (rp <= 32) { LW(k_pass_w_euc_f32, 1) }
+        if (rp <= 64) { LW(k_pass_w_euc_f32, 2) }
+        if (rp <= 128) { LW(k_pass_w_euc_f32, 4) }
+    }
+#undef LW
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gpumlib_nmf

@@ -1,0 +1,36 @@
+// Workspace planning for the fused iteration: slice counts of the two passes and
+// the byte layout of the caller-provided workspace.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "nmf_common.hpp"
+
+namespace gpumlib_nmf {
+
+// eps of the reference: SMALL_VALUE_TO_ADD_DENOMINATOR = CUDA_VALUE(10e-9) = 1e-8f
+// (src/cuda/nmf/nmf_kernels.h:34).
+constexpr float kReferenceEpsilon = 10e-9f;
+// Fused kernels hold up to 4 MFMA row blocks of 32 rank rows.
+constexpr int kMaxFusedRank = 128;
+// Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
+constexpr int kTargetWorkgroups = 512;
+
+struct Plan {
+    int rule, precision, n, m, r;
+    int npad, rpad, ldr, ldp;
+    int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
+    int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
+    long pstride1, pstride3;
+    int nobjp;
+    size_t off_P, off_Psum, off_objp, bytes;
+};
+
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl);
+void pass_args(const Plan& pl, const float* V, const float* W, const float* H, void* ws, float eps, int which,
+               PassArgs* a);
+hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations, int update_w,
+                          int objective, double* d_objective, void* ws, hipStream_t st);
+hipError_t run_objective(const Plan& pl, const float* V, const float* W, const float* H, float eps, int objective,
+                         double* d_objective, void* ws, hipStream_t st);
+
+}  // namespace gpumlib_nmf

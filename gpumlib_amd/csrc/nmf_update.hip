@@ -1,0 +1,352 @@
+// Update ("epilogue") kernels and the reference-shaped element-wise / reduction
+// kernels.  All reductions are in a fixed order, so results do not depend on
+// timing or on what else runs on the device.
+#include "nmf_common.hpp"
+#include "nmf_internal.hpp"
+
+namespace gpumlib_nmf {
+
+constexpr int UPD_COLS = 64;  // columns of H per update workgroup
+constexpr int UPD_ROWS = 64;  // rows of W per update workgroup
+
+// ---------------------------------------------------------------------------
+// Euclidean H update, reference nmf_multiplicative_euclidian.cpp:36-45 and
+// nmf_kernels.cu:46:  H(a,u) *= WtV(a,u) / (WtWH(a,u) + eps), WtWH = WtW H.
+// WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = sum_s P[s][(m+b)*ldr + a].
+__global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H, const float* __restrict__ P,
+                                                           int m, int r, int ldr, int S, long pstride, float eps) {
+    extern __shared__ float sm[];
+    const int rs = r + 1;
+    float* sG = sm;           // sG[a*rs + b] = WtW(a,b)
+    float* sH = sm + r * rs;  // sH[uu*rs + b] = H(b, u0 + uu)
+    const int tid = threadIdx.x;
+    const int u0 = blockIdx.x * UPD_COLS;
+    const int ncol = min(UPD_COLS, m - u0);
+    for (int idx = tid; idx < r * r; idx += NTHREADS) {
+        const int a = idx % r, b = idx / r;
+        const float* src = P + (size_t)(m + b) * ldr + a;
+        float g = 0.f;
+        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
+        sG[a * rs + b] = g;
+    }
+    for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
+        const int uu = idx / r, a = idx - uu * r;
+        sH[uu * rs + a] = H[(size_t)(u0 + uu) * r + a];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
+        const int uu = idx / r, a = idx - uu * r;
+        const int u = u0 + uu;
+        const float* src = P + (size_t)u * ldr + a;
+        float wtv = 0.f;
+        for (int s = 0; s < S; s++) wtv += src[(size_t)s * pstride];
+        float d = 0.f;
+        for (int b = 0; b < r; b++) d += sG[a * rs + b] * sH[uu * rs + b];
+        H[(size_t)u * r + a] = sH[uu * rs + a] * (wtv / (d + eps));
+    }
+}
+
+// Euclidean W update, reference nmf_multiplicative_euclidian.cpp:53-66:
+// W(i,a) *= VHt(i,a) / (WHHt(i,a) + eps), WHHt = W HHt.
+// VHt(i,a) = sum_s P[s][a*ldp + i];  HHt(b,a) = sum_s P[s][a*ldp + npad + b].
+__global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W, const float* __restrict__ P,
+                                                           int n, int r, int ldp, int npad, int S, long pstride,
+                                                           float eps) {
+    extern __shared__ float sm[];
+    const int rs = r + 1;
+    float* sG = sm;           // sG[b*rs + a] = HHt(b,a)
+    float* sW = sm + r * rs;  // sW[ii*rs + b] = W(i0 + ii, b)
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * UPD_ROWS;
+    const int nrow = min(UPD_ROWS, n - i0);
+    for (int idx = tid; idx < r * r; idx += NTHREADS) {
+        const int b = idx % r, a = idx / r;
+        const float* src = P + (size_t)a * ldp + npad + b;
+        float g = 0.f;
+        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
+        sG[b * rs + a] = g;
+    }
+    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
+        const int ii = idx % UPD_ROWS, b = idx / UPD_ROWS;
+        if (ii < nrow) sW[ii * rs + b] = W[(size_t)b * n + i0 + ii];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
+        const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
+        if (ii >= nrow) continue;
+        const int i = i0 + ii;
+        const float* src = P + (size_t)a * ldp + i;
+        float vht = 0.f;
+        for (int s = 0; s < S; s++) vht += src[(size_t)s * pstride];
+        float d = 0.f;
+        for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
+        W[(size_t)a * n + i] = sW[ii * rs + a] * (vht / (d + eps));
+    }
+}
+
+// KL H update, reference nmf_kernels.cu:337-339 with the clamped sums of :104.
+__global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H, const float* __restrict__ P,
+                                                          const float* __restrict__ Psum, int m, int r, int ldr,
+                                                          int S, long pstride, float eps) {
+    __shared__ float ssum[512];
+    const int tid = threadIdx.x;
+    const int rpad = round_up(r, 4);
+    for (int a = tid; a < r; a += NTHREADS) {
+        float v = 0.f;
+        for (int s = 0; s < S; s++) v += Psum[(size_t)s * rpad + a];
+        ssum[a] = (v < eps) ? eps : v;
+    }
+    __syncthreads();
+    const int u0 = blockIdx.x * UPD_COLS;
+    const int ncol = min(UPD_COLS, m - u0);
+    for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
+        const int uu = idx / r, a = idx - uu * r;
+        const int u = u0 + uu;
+        const float* src = P + (size_t)u * ldr + a;
+        float num = 0.f;
+        for (int s = 0; s < S; s++) num += src[(size_t)s * pstride];
+        H[(size_t)u * r + a] *= (num / ssum[a]);
+    }
+}
+
+// KL W update, reference nmf_kernels.cu:289-291 with the clamped sums of :190.
+__global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W, const float* __restrict__ P,
+                                                          const float* __restrict__ Psum, int n, int r, int ldp,
+                                                          int S, long pstride, float eps) {
+    __shared__ float ssum[512];
+    const int tid = threadIdx.x;
+    const int rpad = round_up(r, 4);
+    for (int a = tid; a < r; a += NTHREADS) {
+        float v = 0.f;
+        for (int s = 0; s < S; s++) v += Psum[(size_t)s * rpad + a];
+        ssum[a] = (v < eps) ? eps : v;
+    }
+    __syncthreads();
+    const int i0 = blockIdx.x * UPD_ROWS;
+    const int nrow = min(UPD_ROWS, n - i0);
+    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
+        const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
+        if (ii >= nrow) continue;
+        const int i = i0 + ii;
+        const float* src = P + (size_t)a * ldp + i;
+        float num = 0.f;
+        for (int s = 0; s < S; s++) num += src[(size_t)s * pstride];
+        W[(size_t)a * n + i] *= (num / ssum[a]);
+    }
+}
+
+// Deterministic reduction of count doubles (fixed strided order, then a fixed tree).
+__global__ __launch_bounds__(NTHREADS) void k_reduce_f64(const double* __restrict__ in, int count, double* out) {
+    __shared__ double red[NTHREADS];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += NTHREADS) v += in[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = NTHREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Reference-shaped kernels over materialised operands.
+
+// nmf_kernels.cu:43-47
+__global__ void k_mu_update(const float* __restrict__ nm, const float* __restrict__ dm, float* __restrict__ mm,
+                            long elements, float eps) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < elements; k += (long)gridDim.x * blockDim.x)
+        mm[k] *= nm[k] / (dm[k] + eps);
+}
+
+// Column sums of W (one workgroup per column a), clamped at eps: nmf_kernels.cu:63-109.
+__global__ __launch_bounds__(NTHREADS) void k_sum_w(const float* __restrict__ W, int n, float eps, float* out) {
+    __shared__ float red[NTHREADS];
+    const int a = blockIdx.x;
+    float v = 0.f;
+    for (int i = threadIdx.x; i < n; i += NTHREADS) v += W[(size_t)a * n + i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = NTHREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[a] = (red[0] < eps) ? eps : red[0];
+}
+
+// Row sums of H (r x m, column-major), clamped at eps: nmf_kernels.cu:149-195.
+// A workgroup reads whole columns of H (contiguous) and keeps r partial sums.
+__global__ __launch_bounds__(NTHREADS) void k_sum_h(const float* __restrict__ H, int r, int m, float eps,
+                                                    float* out) {
+    __shared__ float red[NTHREADS];
+    // thread t owns rank row a = t % r of column group t / r
+    const int per = NTHREADS / r;  // columns processed per sweep (r <= 256)
+    const int a = threadIdx.x % r, g = threadIdx.x / r;
+    float v = 0.f;
+    if (g < per)
+        for (int u = g; u < m; u += per) v += H[(size_t)u * r + a];
+    red[threadIdx.x] = (g < per) ? v : 0.f;
+    __syncthreads();
+    if (threadIdx.x < r) {
+        float s = 0.f;
+        for (int j = 0; j < per; j++) s += red[j * r + threadIdx.x];
+        out[threadIdx.x] = (s < eps) ? eps : s;
+    }
+}
+
+// Reference quality sum_k V[k] log10(WH[k] + eps) - WH[k] (nmf_kernels_quality.cu:29-71),
+// accumulated in double over many workgroups.
+__global__ __launch_bounds__(NTHREADS) void k_quality(const float* __restrict__ V, const float* __restrict__ WH,
+                                                      long count, float eps, double* partials) {
+    __shared__ double red[NTHREADS / 64];
+    double acc = 0.0;
+    for (long k = (long)blockIdx.x * NTHREADS + threadIdx.x; k < count; k += (long)gridDim.x * NTHREADS) {
+        const float wh = WH[k];
+        acc += (double)V[k] * log10((double)(wh + eps)) - (double)wh;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < NTHREADS / 64; i++) t += red[i];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// KL H update with a materialised WH: the reference kernel contract of
+// nmf_kernels.cu:295-341.  One workgroup per column u of H.
+__global__ __launch_bounds__(NTHREADS) void k_kl_update_h_wh(float* __restrict__ H, const float* __restrict__ W,
+                                                             const float* __restrict__ V, const float* __restrict__ WH,
+                                                             const float* __restrict__ sumW, int n, int m, int r,
+                                                             float eps) {
+    __shared__ float sR[NTHREADS];
+    const int u = blockIdx.x, tid = threadIdx.x;
+    for (int a0 = 0; a0 < r; a0 += NTHREADS) {
+        const int a = a0 + tid;
+        float acc = 0.f;
+        for (int i0 = 0; i0 < n; i0 += NTHREADS) {
+            const int i = i0 + tid;
+            sR[tid] = (i < n) ? V[(size_t)u * n + i] / (WH[(size_t)u * n + i] + eps) : 0.f;
+            __syncthreads();
+            if (a < r) {
+                const int lim = min(NTHREADS, n - i0);
+                for (int j = 0; j < lim; j++) acc += W[(size_t)a * n + i0 + j] * sR[j];
+            }
+            __syncthreads();
+        }
+        if (a < r) H[(size_t)u * r + a] *= (acc / sumW[a]);
+    }
+}
+
+// KL W update with a materialised WH: the reference kernel contract of
+// nmf_kernels.cu:247-293.  One thread per row i, one rank column at a time.
+__global__ __launch_bounds__(NTHREADS) void k_kl_update_w_wh(float* __restrict__ W, const float* __restrict__ H,
+                                                             const float* __restrict__ V, const float* __restrict__ WH,
+                                                             const float* __restrict__ sumH, int n, int m, int r,
+                                                             float eps) {
+    const int i = blockIdx.x * NTHREADS + threadIdx.x;
+    if (i >= n) return;
+    for (int a = 0; a < r; a++) {
+        float acc = 0.f;
+        for (int u = 0; u < m; u++) {
+            const size_t k = (size_t)u * n + i;
+            acc += H[(size_t)u * r + a] * (V[k] / (WH[k] + eps));
+        }
+        W[(size_t)a * n + i] *= (acc / sumH[a]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+static size_t upd_smem(int r, int rows) { return (size_t)(r * (r + 1) + rows * (r + 1)) * sizeof(float); }
+
+hipError_t launch_update_h_euc(float* H, const float* P, int m, int r, int ldr, int S, long pstride, float eps,
+                               hipStream_t st) {
+    const size_t sm = upd_smem(r, UPD_COLS);
+    if (sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_update_h_euc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_update_h_euc, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), sm, st, H, P, m, r, ldr,
+                       S, pstride, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_w_euc(float* W, const float* P, int n, int r, int ldp, int npad, int S, long pstride,
+                               float eps, hipStream_t st) {
+    const size_t sm = upd_smem(r, UPD_ROWS);
+    if (sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_update_w_euc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_update_w_euc, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, n, r, ldp,
+                       npad, S, pstride, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr, int S, long pstride,
+                              float eps, hipStream_t st) {
+    if (r > 512) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_update_h_kl, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), 0, st, H, P, Psum, m, r,
+                       ldr, S, pstride, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_w_kl(float* W, const float* P, const float* Psum, int n, int r, int ldp, int S, long pstride,
+                              float eps, hipStream_t st) {
+    if (r > 512) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_update_w_kl, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), 0, st, W, P, Psum, n, r,
+                       ldp, S, pstride, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_f64(const double* in, int count, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_f64, dim3(1), dim3(NTHREADS), 0, st, in, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mu_update(const float* nm, const float* dm, float* mm, long elements, float eps, hipStream_t st) {
+    if (elements <= 0) return hipSuccess;
+    long blocks = (elements + NTHREADS - 1) / NTHREADS;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_mu_update, dim3((unsigned)blocks), dim3(NTHREADS), 0, st, nm, dm, mm, elements, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_w(const float* W, int n, int r, float eps, float* out, hipStream_t st) {
+    if (r <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_w, dim3(r), dim3(NTHREADS), 0, st, W, n, eps, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_h(const float* H, int r, int m, float eps, float* out, hipStream_t st) {
+    if (r <= 0) return hipSuccess;
+    if (r > NTHREADS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sum_h, dim3(1), dim3(NTHREADS), 0, st, H, r, m, eps, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_quality(const float* V, const float* WH, long count, float eps, double* partials, int npartials,
+                          double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_quality, dim3(npartials), dim3(NTHREADS), 0, st, V, WH, count, eps, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce_f64(partials, npartials, out, st);
+}
+
+hipError_t launch_kl_update_h_wh(float* H, const float* W, const float* V, const float* WH, const float* sumW, int n,
+                                 int m, int r, float eps, hipStream_t st) {
+    if (m <= 0 || r <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kl_update_h_wh, dim3(m), dim3(NTHREADS), 0, st, H, W, V, WH, sumW, n, m, r, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_kl_update_w_wh(float* W, const float* H, const float* V, const float* WH, const float* sumH, int n,
+                                 int m, int r, float eps, hipStream_t st) {
+    if (n <= 0 || r <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kl_update_w_wh, dim3((n + NTHREADS - 1) / NTHREADS), dim3(NTHREADS), 0, st, W, H, V, WH, sumH,
+                       n, m, r, eps);
+    return hipGetLastError();
+}
+
+}  // namespace gpumlib_nmf

@@ -1,0 +1,164 @@
+/*
+ * gpumlib_nmf.h — C-ABI of the MI355X (gfx950) NMF multiplicative-update library.
+ *
+ * Plain C types only: device pointers, sizes, a hipStream_t and hipError_t
+ * return codes.  Matrices are column-major, as in GPUMLib
+ * (src/algorithms/nmf/nmf_base.h:109-111):
+ *   V is n x m, V(i,u) = V[u*n + i];  W is n x r, W(i,a) = W[a*n + i];
+ *   H is r x m, H(a,u) = H[u*r + a].
+ * No entry point allocates device memory: scratch space is a caller-provided
+ * workspace whose size the *_workspace_bytes queries return.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the GPUMLib source tree).
+ */
+#ifndef GPUMLIB_NMF_H
+#define GPUMLIB_NMF_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1. The reference's own extern "C" launchers (src/cuda/nmf/nmf_kernels.h).  */
+/*    Same names and arguments; cudaError_t becomes hipError_t.  The grid and */
+/*    block arguments are accepted for drop-in compatibility and ignored: the */
+/*    launch geometry is chosen for gfx950.  Work goes to the null stream, as */
+/*    in the reference.  The elementwise rule uses eps = 1e-8f, the value of  */
+/*    SMALL_VALUE_TO_ADD_DENOMINATOR (nmf_kernels.h:34).                      */
+/* ------------------------------------------------------------------------ */
+
+/* replaces gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix, nmf_kernels.h:87 */
+hipError_t gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix(unsigned int gridDim, unsigned int blockDim,
+                                                                   float *nm, float *dm, float *m, int elements);
+/* replaces gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w, nmf_kernels.h:100 */
+hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w(dim3 gridDim, dim3 blockDim, float *W, float *H,
+                                                                      float *V, float *WH, float *sumH, int n, int m,
+                                                                      int r);
+/* replaces gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h, nmf_kernels.h:113 */
+hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(dim3 gridDim, dim3 blockDim, float *H, float *W,
+                                                                      float *V, float *WH, float *sumW, int n, int m,
+                                                                      int r);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Kernel-level entry points with an explicit stream and eps.              */
+/* ------------------------------------------------------------------------ */
+
+/* m[k] *= nm[k] / (dm[k] + eps), k < elements.  Kernel of nmf_kernels.cu:43-47. */
+hipError_t gpumlib_nmf_mu_update(const float *nm, const float *dm, float *m, long elements, float eps,
+                                 hipStream_t stream);
+
+/* sumW[a] = max(sum_i W(i,a), eps).  Replaces KernelSumW (nmf_kernels.h:64; nmf_kernels.cu:63-147). */
+hipError_t gpumlib_nmf_sum_w(const float *W, int n, int r, float eps, float *sumW, hipStream_t stream);
+
+/* sumH[a] = max(sum_u H(a,u), eps), r <= 256.  Replaces KernelSumH (nmf_kernels.h:73; nmf_kernels.cu:149-233). */
+hipError_t gpumlib_nmf_sum_h(const float *H, int r, int m, float eps, float *sumH, hipStream_t stream);
+
+/* KL updates over a materialised WH = W H (the reference kernel contract):
+ *   H(a,u) *= [sum_i W(i,a) V(i,u) / (WH(i,u) + eps)] / sumW[a]   nmf_kernels.cu:295-341
+ *   W(i,a) *= [sum_u H(a,u) V(i,u) / (WH(i,u) + eps)] / sumH[a]   nmf_kernels.cu:247-293 */
+hipError_t gpumlib_nmf_kl_update_h(float *H, const float *W, const float *V, const float *WH, const float *sumW,
+                                   int n, int m, int r, float eps, hipStream_t stream);
+hipError_t gpumlib_nmf_kl_update_w(float *W, const float *H, const float *V, const float *WH, const float *sumH,
+                                   int n, int m, int r, float eps, hipStream_t stream);
+
+/* Reference quality  sum_k V[k] log10(WH[k] + eps) - WH[k]  accumulated in FP64 into d_quality[0].
+ * Replaces KernelNMFquality (nmf_kernels.h:44; nmf_kernels_quality.cu:29-109), which reduced in
+ * FP32 on one workgroup. */
+size_t gpumlib_nmf_quality_workspace_bytes(long count);
+hipError_t gpumlib_nmf_quality(const float *V, const float *WH, long count, float eps, void *workspace,
+                               size_t workspace_bytes, double *d_quality, hipStream_t stream);
+
+/* C = alpha op(A) op(B) + beta C, column-major, op 'N' or 'T'.  Replaces the cublasSgemm calls of
+ * DeviceMatrix<float>::Multiply / MultiplyBySelfTranspose (src/memory/DeviceMatrix.h:195-227). */
+hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, float alpha, const float *A, int lda,
+                             const float *B, int ldb, float beta, float *C, int ldc, hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 3. Fused iterations.  One call runs whole multiplicative-update           */
+/*    iterations of NMF_MultiplicativeEuclidianDistance::DoIteration          */
+/*    (nmf_multiplicative_euclidian.cpp:29-67) or                             */
+/*    NMF_MultiplicativeDivergence::DoIteration                               */
+/*    (nmf_multiplicative_divergence.cpp:29-50) with the same operation order */
+/*    (H from the old W, then W from the new H; KL recomputes W H between).   */
+/* ------------------------------------------------------------------------ */
+
+enum { GPUMLIB_NMF_EUCLIDEAN = 0, GPUMLIB_NMF_KL = 1 };
+enum { GPUMLIB_NMF_FP32 = 0, GPUMLIB_NMF_BF16 = 1, GPUMLIB_NMF_FP8 = 2 };
+enum {
+    GPUMLIB_NMF_OBJ_NONE = 0,
+    GPUMLIB_NMF_OBJ_FROBENIUS = 1, /* sum (V - WH)^2 */
+    GPUMLIB_NMF_OBJ_KL = 2,        /* sum V ln(V/WH) - V + WH, 0 ln 0 = 0 */
+    GPUMLIB_NMF_OBJ_QUALITY = 3    /* reference quality, nmf_kernels_quality.cu:35 */
+};
+
+/* Device workspace needed by gpumlib_nmf_iterate / gpumlib_nmf_objective for one problem. */
+size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r);
+
+/* Runs `iterations` iterations in place on W and H (device pointers).  V is the device matrix in the
+ * storage format of `precision` (GPUMLIB_NMF_FP32: float n x m).  When objective != OBJ_NONE the
+ * objective of the state on entry is written, in FP64, to d_objective[0] (device memory); it is
+ * fused into the first iteration's pass over V.  update_w = 0 updates H only (DoIteration(false)).
+ * Ranks up to 128 are supported. */
+hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, const void *V, float *W, float *H,
+                               float eps, int iterations, int update_w, int objective, double *d_objective,
+                               void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Objective of the current state, FP64, written to d_objective[0] (device). */
+hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, const void *V, const float *W,
+                                 const float *H, float eps, int objective, double *d_objective, void *workspace,
+                                 size_t workspace_bytes, hipStream_t stream);
+
+/* Runs one contraction pass of the fused iteration alone (which = 0: the pass over the rows of V that
+ * feeds the H update; which = 1: the pass over the columns of V that feeds the W update), writing only
+ * workspace partials; W and H are unchanged.  For per-kernel timing and profiling. */
+hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void *V, const float *W,
+                            const float *H, float eps, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 4. Handle over the C++ class API (NMF_MultiplicativeEuclidianDistance /    */
+/*    NMF_MultiplicativeDivergence, constructed from host V, W, H as in the   */
+/*    (v, w, h) constructors of nmf_multiplicative_euclidian.h:62 and         */
+/*    nmf_multiplicative_divergence.h:57).  For foreign-language callers.     */
+/* ------------------------------------------------------------------------ */
+typedef struct gpumlib_nmf_object *gpumlib_nmf_handle;
+
+/* host_V (n x m), host_W (n x r), host_H (r x m) are column-major host arrays. */
+hipError_t gpumlib_nmf_object_create(int rule, int precision, int n, int m, int r, const float *host_V,
+                                     const float *host_W, const float *host_H, gpumlib_nmf_handle *out);
+hipError_t gpumlib_nmf_object_set_epsilon(gpumlib_nmf_handle h, float eps);
+hipError_t gpumlib_nmf_object_set_quality_interval(gpumlib_nmf_handle h, int interval);
+hipError_t gpumlib_nmf_object_do_iteration(gpumlib_nmf_handle h, int update_w);        /* NMF::DoIteration, nmf_base.h:145 */
+hipError_t gpumlib_nmf_object_do_iterations(gpumlib_nmf_handle h, int count, int update_w);
+float gpumlib_nmf_object_quality_improvement(gpumlib_nmf_handle h);                    /* nmf_base.h:148-150 */
+hipError_t gpumlib_nmf_object_get_w(gpumlib_nmf_handle h, float *host_W);              /* nmf_base.h:125-127 */
+hipError_t gpumlib_nmf_object_get_h(gpumlib_nmf_handle h, float *host_H);              /* nmf_base.h:131-133 */
+hipError_t gpumlib_nmf_object_get_wh(gpumlib_nmf_handle h, float *host_WH);            /* nmf_base.h:137-141 */
+hipError_t gpumlib_nmf_object_objective(gpumlib_nmf_handle h, int objective, double *host_value);
+void gpumlib_nmf_object_destroy(gpumlib_nmf_handle h);
+
+/* ------------------------------------------------------------------------ */
+/* 5. Seeded host-side input generators (no reference counterpart: the        */
+/*    reference draws W and H with rand(), nmf_base.h:56-61).  Counter-based  */
+/*    SplitMix64: value k of stream s under seed is                           */
+/*      mix(mix(seed ^ (s * 0xD1B54A32D192ED03)) + k)                         */
+/*    uniform float = (x >> 40) * 2^-24,  uniform double = (x >> 11) * 2^-53. */
+/* ------------------------------------------------------------------------ */
+uint64_t gpumlib_nmf_rng_u64(uint64_t seed, uint64_t stream, uint64_t k);
+void gpumlib_nmf_gen_uniform_f32(uint64_t seed, uint64_t stream, size_t count, float *out);
+/* V (n x m) = W0 (n x r0) H0 (r0 x m) + noise * U[0,1),  W0, H0 ~ U[0,1)  (streams 1, 2, 3). */
+void gpumlib_nmf_gen_euclidean_v(uint64_t seed, int n, int m, int r0, float noise, float *V);
+/* V (n x m) ~ Poisson(W0 H0), W0, H0 ~ Exp(1)  (streams 1, 2, 6). */
+void gpumlib_nmf_gen_poisson_v(uint64_t seed, int n, int m, int r0, float *V);
+/* Initial W (n x r) and H (r x m) ~ U[0,1)  (streams 4, 5). */
+void gpumlib_nmf_gen_init(uint64_t seed, int n, int m, int r, float *W, float *H);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUMLIB_NMF_H */

@@ -1,0 +1,174 @@
+"""Python access to the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker or as the timed CPU restatement; the product never does.
+
+Two implementations of the same rules:
+  * liborcnmf.so (nmf_oracle.c): textbook loops in FP64 / FP32, plus the
+    OpenMP host-cores FP32 baseline;
+  * the numpy FP64 functions below, for sizes where the textbook loops are too
+    slow (full-size parity runs).  They follow the same reference lines:
+      Euclidean  src/algorithms/nmf/nmf_multiplicative_euclidian.cpp:29-67,
+                 src/cuda/nmf/nmf_kernels.cu:43-47
+      KL         src/algorithms/nmf/nmf_multiplicative_divergence.cpp:29-50,
+                 src/cuda/nmf/nmf_kernels.cu:63-233, 247-341
+All matrices are column-major flat arrays (element (i, u) of an n x m matrix at
+u*n + i), i.e. numpy arrays X of logical shape (n, m) are passed as X.T.ravel().
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborcnmf.so")
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_i, _d, _f, _l, _c = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_long, ctypes.c_char
+
+_SIGS = {
+    "orc_gemm_f64": (None, [_c, _c, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+    "orc_gemm_f32": (None, [_c, _c, _i, _i, _i, _f, _fp, _i, _fp, _i, _f, _fp, _i]),
+    "orc_mu_update_f32": (None, [_fp, _fp, _fp, _l, _f]),
+    "orc_mu_update_f64": (None, [_dp, _dp, _dp, _l, _d]),
+    "orc_sum_w_f64": (None, [_i, _i, _dp, _d, _dp]),
+    "orc_sum_h_f64": (None, [_i, _i, _dp, _d, _dp]),
+    "orc_sum_w_f32": (None, [_i, _i, _fp, _f, _fp]),
+    "orc_sum_h_f32": (None, [_i, _i, _fp, _f, _fp]),
+    "orc_kl_update_h_f64": (None, [_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d]),
+    "orc_kl_update_w_f64": (None, [_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d]),
+    "orc_kl_update_h_f32": (None, [_fp, _fp, _fp, _fp, _fp, _i, _i, _i, _f]),
+    "orc_kl_update_w_f32": (None, [_fp, _fp, _fp, _fp, _fp, _i, _i, _i, _f]),
+    "orc_euc_iter_f64": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
+    "orc_euc_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_kl_iter_f64": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
+    "orc_kl_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_euc_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_kl_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_frob_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
+    "orc_kl_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
+    "orc_quality_f64": (_d, [_l, _dp, _dp, _d]),
+    "orc_omp_threads": (_i, []),
+}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def _p64(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _p32(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+# ---------------------------------------------------------------------------
+# Wrappers of the C oracle (arrays are flat, column-major; updated in place).
+def gemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    if C.dtype == np.float64:
+        lib().orc_gemm_f64(ta.encode(), tb.encode(), M, N, K, alpha, _p64(A), lda, _p64(B), ldb, beta, _p64(C), ldc)
+    else:
+        lib().orc_gemm_f32(ta.encode(), tb.encode(), M, N, K, alpha, _p32(A), lda, _p32(B), ldb, beta, _p32(C), ldc)
+    return C
+
+
+def euc_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
+    L = lib()
+    if V.dtype == np.float64:
+        fn, p, e = L.orc_euc_iter_f64, _p64, float(eps)
+    else:
+        fn, p, e = (L.orc_euc_iter_f32_omp if omp else L.orc_euc_iter_f32), _p32, float(np.float32(eps))
+    for _ in range(iters):
+        fn(n, m, r, p(V), p(W), p(H), e, 1 if update_w else 0)
+    return W, H
+
+
+def kl_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
+    L = lib()
+    if V.dtype == np.float64:
+        fn, p, e = L.orc_kl_iter_f64, _p64, float(eps)
+    else:
+        fn, p, e = (L.orc_kl_iter_f32_omp if omp else L.orc_kl_iter_f32), _p32, float(np.float32(eps))
+    for _ in range(iters):
+        fn(n, m, r, p(V), p(W), p(H), e, 1 if update_w else 0)
+    return W, H
+
+
+def frob(n, m, r, V, W, H):
+    return lib().orc_frob_f64(n, m, r, _p64(V), _p64(W), _p64(H))
+
+
+def kl_div(n, m, r, V, W, H):
+    return lib().orc_kl_f64(n, m, r, _p64(V), _p64(W), _p64(H))
+
+
+def quality(V, WH, eps):
+    return lib().orc_quality_f64(V.size, _p64(V), _p64(WH), float(eps))
+
+
+def omp_threads():
+    return lib().orc_omp_threads()
+
+
+# ---------------------------------------------------------------------------
+# numpy FP64 restatement of the same rules for large sizes.  Inputs are n x m
+# (V), n x r (W), r x m (H) 2-D arrays in float64; returns new arrays.
+def np_euc_iter(V, W, H, eps, update_w=True):
+    WtV = W.T @ V
+    WtW = W.T @ W
+    H = H * (WtV / (WtW @ H + eps))
+    if update_w:
+        VHt = V @ H.T
+        HHt = H @ H.T
+        W = W * (VHt / (W @ HHt + eps))
+    return W, H
+
+
+def np_kl_iter(V, W, H, eps, update_w=True):
+    WH = W @ H
+    sumW = np.maximum(W.sum(axis=0), eps)
+    H = H * ((W.T @ (V / (WH + eps))) / sumW[:, None])
+    if update_w:
+        WH = W @ H
+        sumH = np.maximum(H.sum(axis=1), eps)
+        W = W * (((V / (WH + eps)) @ H.T) / sumH[None, :])
+    return W, H
+
+
+def np_frob(V, W, H):
+    D = V - W @ H
+    return float(np.sum(D * D))
+
+
+def np_kl(V, W, H):
+    WH = W @ H
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(V > 0, V * np.log(np.where(V > 0, V, 1.0) / WH), 0.0)
+    return float(np.sum(t - V + WH))
+
+
+def np_quality(V, W, H, eps):
+    WH = W @ H
+    # the reference adds eps in float before log10 (nmf_kernels_quality.cu:34-35)
+    whe = (WH.astype(np.float32) + np.float32(eps)).astype(np.float64)
+    return float(np.sum(V * np.log10(whe) - WH))

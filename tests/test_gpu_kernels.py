@@ -1,0 +1,152 @@
+"""Kernel-level parity: the reference-shaped entry points against the CPU oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from gpumlib_amd import capi
+from gpumlib_amd import nmf as G
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def stream(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (64, 64, 64), (130, 257, 95), (1000, 3, 700)])
+def test_sgemm(native, orc, cuda, ta, tb, mnk):
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    Ar, Ac = (M, K) if ta == "N" else (K, M)
+    Br, Bc = (K, N) if tb == "N" else (N, K)
+    A = rng.random(Ar * Ac, dtype=np.float32)
+    B = rng.random(Br * Bc, dtype=np.float32)
+    C = rng.random(M * N, dtype=np.float32)
+    alpha, beta = 1.5, 0.25
+    dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
+    capi.check(native.gpumlib_nmf_sgemm(ta.encode(), tb.encode(), M, N, K, alpha, dA.data_ptr(), Ar,
+                                        dB.data_ptr(), Br, beta, dC.data_ptr(), M, stream(cuda)))
+    ref = C.astype(np.float64)
+    orc.gemm(ta, tb, M, N, K, alpha, A.astype(np.float64), Ar, B.astype(np.float64), Br, beta, ref, M)
+    assert rel_fro(dC.cpu().numpy(), ref) < 1e-5
+
+
+def test_sgemm_beta_zero_ignores_c(native, cuda):
+    """beta == 0 never reads C (cuBLAS contract): NaN in C does not leak."""
+    M = N = K = 40
+    A = np.ones(M * K, np.float32)
+    B = np.ones(K * N, np.float32)
+    C = np.full(M * N, np.nan, np.float32)
+    dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
+    capi.check(native.gpumlib_nmf_sgemm(b"N", b"N", M, N, K, 1.0, dA.data_ptr(), M, dB.data_ptr(), K, 0.0,
+                                        dC.data_ptr(), M, stream(cuda)))
+    assert np.all(dC.cpu().numpy() == K)
+
+
+def test_sgemm_rejects_bad_flags(native, cuda):
+    assert native.gpumlib_nmf_sgemm(b"X", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) != 0
+
+
+def test_mu_update_bit_exact(native, orc, cuda):
+    rng = np.random.default_rng(3)
+    E = 100_003
+    nm, dm, mm = (rng.random(E, dtype=np.float32) for _ in range(3))
+    dm[:10] = 0.0          # m * nm / eps
+    nm[10:20] = dm[10:20]  # m * dm / (dm + eps)
+    ref = mm.copy()
+    orc.lib().orc_mu_update_f32(orc._p32(nm), orc._p32(dm), orc._p32(ref), E, np.float32(1e-8))
+    dnm, ddm, dmm = dev(cuda, nm), dev(cuda, dm), dev(cuda, mm)
+    capi.check(native.gpumlib_nmf_mu_update(dnm.data_ptr(), ddm.data_ptr(), dmm.data_ptr(), E, 1e-8, stream(cuda)))
+    assert np.array_equal(dmm.cpu().numpy(), ref)
+    # the reference-named launcher (grid/block ignored, eps = 1e-8f, null stream)
+    dmm2 = dev(cuda, mm)
+    cuda.cuda.synchronize()
+    capi.check(native.gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix(1, 512, dnm.data_ptr(), ddm.data_ptr(),
+                                                                              dmm2.data_ptr(), E))
+    cuda.cuda.synchronize()
+    assert np.array_equal(dmm2.cpu().numpy(), ref)
+
+
+def test_sums_clamped(native, orc, cuda):
+    n, m, r = 1000, 777, 40
+    rng = np.random.default_rng(11)
+    W = rng.random(n * r, dtype=np.float32)
+    H = rng.random(r * m, dtype=np.float32)
+    W[3 * n:4 * n] = 0.0            # column 3 sums to 0 -> eps
+    H[5::r] = 0.0                   # row 5 sums to 0 -> eps
+    sw = np.zeros(r)
+    sh = np.zeros(r)
+    orc.lib().orc_sum_w_f64(n, r, orc._p64(W.astype(np.float64)), 1e-8, orc._p64(sw))
+    orc.lib().orc_sum_h_f64(r, m, orc._p64(H.astype(np.float64)), 1e-8, orc._p64(sh))
+    dW, dH = dev(cuda, W), dev(cuda, H)
+    osw = cuda.zeros(r, dtype=cuda.float32, device="cuda")
+    osh = cuda.zeros(r, dtype=cuda.float32, device="cuda")
+    capi.check(native.gpumlib_nmf_sum_w(dW.data_ptr(), n, r, 1e-8, osw.data_ptr(), stream(cuda)))
+    capi.check(native.gpumlib_nmf_sum_h(dH.data_ptr(), r, m, 1e-8, osh.data_ptr(), stream(cuda)))
+    assert np.allclose(osw.cpu().numpy(), sw, rtol=1e-5)
+    assert np.allclose(osh.cpu().numpy(), sh, rtol=1e-5)
+    assert osw.cpu().numpy()[3] == np.float32(1e-8) and osh.cpu().numpy()[5] == np.float32(1e-8)
+
+
+def test_quality(native, orc, cuda):
+    n = 300_001
+    rng = np.random.default_rng(12)
+    V = rng.random(n, dtype=np.float32) * 5
+    WH = rng.random(n, dtype=np.float32) * 5
+    WH[:7] = 0.0
+    ref = orc.quality(V.astype(np.float64), WH.astype(np.float64), 1e-8)
+    dV, dWH = dev(cuda, V), dev(cuda, WH)
+    wsb = int(native.gpumlib_nmf_quality_workspace_bytes(n))
+    ws = cuda.empty(wsb, dtype=cuda.uint8, device="cuda")
+    out = cuda.zeros(1, dtype=cuda.float64, device="cuda")
+    capi.check(native.gpumlib_nmf_quality(dV.data_ptr(), dWH.data_ptr(), n, 1e-8, ws.data_ptr(), wsb,
+                                          out.data_ptr(), stream(cuda)))
+    assert abs(float(out.cpu()[0]) - ref) / abs(ref) < 1e-6
+
+
+@pytest.mark.parametrize("nmr", [(96, 80, 16), (257, 131, 37)])
+def test_kl_updates_with_materialised_wh(native, orc, cuda, nmr):
+    """Reference kernel contract (nmf_kernels.cu:247-341) with WH given."""
+    n, m, r = nmr
+    V = G.gen_poisson(n, m, 8, seed=n)
+    W, H = G.gen_init(n, m, r, seed=m)
+    WH = np.zeros(n * m, np.float32)
+    orc.gemm("N", "N", n, m, r, 1.0, W, n, H, r, 0.0, WH, n)
+    sw = np.zeros(r, np.float32)
+    sh = np.zeros(r, np.float32)
+    orc.lib().orc_sum_w_f32(n, r, orc._p32(W), np.float32(1e-8), orc._p32(sw))
+    orc.lib().orc_sum_h_f32(r, m, orc._p32(H), np.float32(1e-8), orc._p32(sh))
+    h_ref, w_ref = H.astype(np.float64), W.astype(np.float64)
+    orc.lib().orc_kl_update_h_f64(orc._p64(h_ref), orc._p64(W.astype(np.float64)), orc._p64(V.astype(np.float64)),
+                                  orc._p64(WH.astype(np.float64)), orc._p64(sw.astype(np.float64)), n, m, r, 1e-8)
+    orc.lib().orc_kl_update_w_f64(orc._p64(w_ref), orc._p64(H.astype(np.float64)), orc._p64(V.astype(np.float64)),
+                                  orc._p64(WH.astype(np.float64)), orc._p64(sh.astype(np.float64)), n, m, r, 1e-8)
+    dV, dW, dH, dWH = dev(cuda, V), dev(cuda, W), dev(cuda, H), dev(cuda, WH)
+    dsw, dsh = dev(cuda, sw), dev(cuda, sh)
+    dH2 = dev(cuda, H)
+    dW2 = dev(cuda, W)
+    capi.check(native.gpumlib_nmf_kl_update_h(dH2.data_ptr(), dW.data_ptr(), dV.data_ptr(), dWH.data_ptr(),
+                                              dsw.data_ptr(), n, m, r, 1e-8, stream(cuda)))
+    capi.check(native.gpumlib_nmf_kl_update_w(dW2.data_ptr(), dH.data_ptr(), dV.data_ptr(), dWH.data_ptr(),
+                                              dsh.data_ptr(), n, m, r, 1e-8, stream(cuda)))
+    assert rel_fro(dH2.cpu().numpy(), h_ref) < 1e-5
+    assert rel_fro(dW2.cpu().numpy(), w_ref) < 1e-5
+    # reference-named launchers (dim3 arguments ignored; null stream)
+    g = capi.Dim3(1, 1, 1)
+    dH3, dW3 = dev(cuda, H), dev(cuda, W)
+    cuda.cuda.synchronize()
+    capi.check(native.gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(
+        g, g, dH3.data_ptr(), dW.data_ptr(), dV.data_ptr(), dWH.data_ptr(), dsw.data_ptr(), n, m, r))
+    capi.check(native.gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w(
+        g, g, dW3.data_ptr(), dH.data_ptr(), dV.data_ptr(), dWH.data_ptr(), dsh.data_ptr(), n, m, r))
+    cuda.cuda.synchronize()
+    assert np.array_equal(dH3.cpu().numpy(), dH2.cpu().numpy())
+    assert np.array_equal(dW3.cpu().numpy(), dW2.cpu().numpy())
