@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="euclid-16384x8192-r64", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"],
+                    help="precision of the headline value (the FP32 path is always measured beside it)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -56,7 +57,8 @@ def main():
     lib = capi.load()
     rule_name, n, m, r, r0, seed = WORKLOADS[args.workload]
     rule = G.EUCLIDEAN if rule_name == "euclidean" else G.KL
-    prec = G.FP32
+    if rule == G.KL:
+        args.precision = "fp32"  # the BF16 storage path runs the Euclidean rule
     eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
 
     t_gen = time.time()
@@ -64,39 +66,7 @@ def main():
     W0, H0 = G.gen_init(n, m, r, seed)
     t_gen = time.time() - t_gen
 
-    d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
-    stream = d.stream()
-
-    def run(k):
-        capi.check(lib.gpumlib_nmf_iterate(rule, prec, n, m, r, d.V.data_ptr(), d.W.data_ptr(), d.H.data_ptr(),
-                                           eps, k, 1, G.OBJ_NONE, None, d.ws.data_ptr(), d.ws_bytes, stream))
-
     obj_kind = G.OBJ_FROBENIUS if rule == G.EUCLIDEAN else G.OBJ_KL
-    obj0 = d.objective(obj_kind)
-    run(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run(args.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    obj1 = d.objective(obj_kind)
-
-    # Per-kernel durations, HIP events on the stream the kernels are launched on.
-    def time_pass(which):
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        for _ in range(3):
-            capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
-                                            d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
-        ev0.record()
-        for _ in range(args.kernel_reps):
-            capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
-                                            d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
-        ev1.record()
-        ev1.synchronize()
-        return ev0.elapsed_time(ev1) / args.kernel_reps  # ms per launch
-
-    ms_h, ms_w = time_pass(0), time_pass(1)
     if rule == G.EUCLIDEAN:
         flops_h = 2.0 * n * m * r + 2.0 * n * r * r
         flops_w = 2.0 * n * m * r + 2.0 * m * r * r
@@ -104,34 +74,96 @@ def main():
     else:
         flops_h = flops_w = 4.0 * n * m * r
         flops_iter = 8.0 * n * m * r
-    vbytes = 4.0 * n * m
-    dom_ms, dom_flops, dom_name = (ms_h, flops_h, "pass_h") if ms_h >= ms_w else (ms_w, flops_w, "pass_w")
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    iters_per_s = 1e3 / ms
 
+    def leg(prec_name):
+        prec = {"fp32": G.FP32, "bf16": G.BF16}[prec_name]
+        d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
+        stream = d.stream()
+        vbytes_el = 4 if prec == G.FP32 else 2
+        if prec != G.FP32:
+            vs_bytes = int(lib.gpumlib_nmf_vstore_bytes(prec, n, m))
+            store = torch.empty(vs_bytes, dtype=torch.uint8, device="cuda")
+            capi.check(lib.gpumlib_nmf_vstore_pack(prec, n, m, d.V.data_ptr(), store.data_ptr(), vs_bytes, stream))
+            d.V = store
+
+        def run(k):
+            capi.check(lib.gpumlib_nmf_iterate(rule, prec, n, m, r, d.V.data_ptr(), d.W.data_ptr(), d.H.data_ptr(),
+                                               eps, k, 1, G.OBJ_NONE, None, d.ws.data_ptr(), d.ws_bytes, stream))
+
+        obj0 = d.objective(obj_kind)
+        run(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(args.steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
+        obj1 = d.objective(obj_kind)
+
+        # Per-kernel durations: HIP events on the stream the kernels are launched on.
+        def time_pass(which):
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            for _ in range(3):
+                capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
+                                                d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
+            ev0.record()
+            for _ in range(args.kernel_reps):
+                capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
+                                                d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
+            ev1.record()
+            ev1.synchronize()
+            return ev0.elapsed_time(ev1) / args.kernel_reps
+
+        ms_h, ms_w = time_pass(0), time_pass(1)
+        dom = ("pass_h", ms_h, flops_h) if ms_h >= ms_w else ("pass_w", ms_w, flops_w)
+        out = {
+            "iters_per_s": 1e3 / ms, "ms_per_step": ms,
+            "mfma_peak_frac": flops_iter / (ms * 1e-3) / 1e12 / PEAK[f"{prec_name}_mfma_tflops"],
+            "v_stream_gbs": 2.0 * vbytes_el * n * m / (ms * 1e-3) / 1e9,
+            "objective_entry": obj0, "objective_after": obj1,
+            "kernels": {"pass_h_ms": ms_h, "pass_w_ms": ms_w,
+                        "pass_h_tflops": flops_h / (ms_h * 1e-3) / 1e12,
+                        "pass_w_tflops": flops_w / (ms_w * 1e-3) / 1e12},
+        }
+        if prec == G.FP32:
+            ach = dom[2] / (dom[1] * 1e-3) / 1e12
+            out["roofline"] = {"kernel": dom[0], "bound": "mfma", "achieved": ach, "peak": PEAK["fp32_mfma_tflops"],
+                               "unit": "TFLOP/s", "frac": ach / PEAK["fp32_mfma_tflops"], "traffic": None}
+        else:
+            vb = float(vbytes_el) * n * m  # algorithmic bytes of one pass: V streamed once
+            ach = vb / (dom[1] * 1e-3) / 1e9
+            out["roofline"] = {"kernel": dom[0], "bound": "hbm", "achieved": ach, "peak": PEAK["hbm_gbs"],
+                               "unit": "GB/s", "frac": ach / PEAK["hbm_gbs"], "traffic": None}
+        return out
+
+    legs = {"fp32": leg("fp32")}
+    if args.precision != "fp32":
+        legs[args.precision] = leg(args.precision)
+    head = legs[args.precision]
+    iters_total = args.warmup + args.steps
     line = {
         "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",
-        "value": iters_per_s,
+        "value": head["iters_per_s"],
         "unit": "iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": args.precision,
         "data": "synthetic (seeded SplitMix64 generator: V = W0 H0 + U[0,0.01) noise)" if rule == G.EUCLIDEAN
         else "synthetic (seeded SplitMix64 generator: V ~ Poisson(W0 H0), W0, H0 ~ Exp(1))",
-        "config": {"workload": args.workload, "rule": rule_name, "n": n, "m": m, "r": r, "precision": "fp32",
-                   "eps": eps, "parallelism": "single GPU"},
-        "mfma_peak_frac": flops_iter / (ms * 1e-3) / 1e12 / PEAK["fp32_mfma_tflops"],
-        "hbm_gbs_v_passes": 2 * vbytes / (ms * 1e-3) / 1e9,
-        "objective": {"entry": obj0, "after": obj1, "kind": "frobenius" if rule == G.EUCLIDEAN else "kl"},
-        "kernels": {"pass_h_ms": ms_h, "pass_w_ms": ms_w,
-                    "pass_h_tflops": flops_h / (ms_h * 1e-3) / 1e12, "pass_w_tflops": flops_w / (ms_w * 1e-3) / 1e12},
-        "roofline": {"kernel": dom_name, "bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": None},
+        "config": {"workload": args.workload, "rule": rule_name, "n": n, "m": m, "r": r,
+                   "precision": args.precision, "eps": eps, "parallelism": "single GPU"},
+        "mfma_peak_frac": head["mfma_peak_frac"],
+        "roofline": head["roofline"],
+        "legs": legs,
+        "speedup_vs_fp32_path": head["iters_per_s"] / legs["fp32"]["iters_per_s"],
+        "objective_rel_gap_vs_fp32_path": abs(head["objective_after"] - legs["fp32"]["objective_after"])
+        / legs["fp32"]["objective_after"],
+        "objective_note": f"objective after {iters_total} iterations from the same initial W, H",
         "gen_seconds": t_gen,
     }
 

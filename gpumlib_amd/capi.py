@@ -22,6 +22,22 @@ class Dim3(ctypes.Structure):
     _fields_ = [("x", _c.c_uint32), ("y", _c.c_uint32), ("z", _c.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", _c.c_size_t), ("max_active", _c.c_int), ("round_iterations", _c.c_int),
+                ("use_graph", _c.c_int), ("target_workgroups", _c.c_int)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("iterations", _c.c_int), ("converged", _c.c_int), ("objective", _c.c_double),
+                ("submit_s", _c.c_double), ("admit_s", _c.c_double), ("finish_s", _c.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", _c.c_longlong), ("completed", _c.c_longlong), ("rounds", _c.c_longlong),
+                ("iterations_batched", _c.c_longlong), ("peak_active", _c.c_int), ("pool_bytes", _c.c_size_t),
+                ("pool_high_water", _c.c_size_t)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix": (_i, [_c.c_uint, _c.c_uint, _vp, _vp, _vp, _i]),
@@ -36,6 +52,8 @@ SIGNATURES = {
     "gpumlib_nmf_quality": (_i, [_vp, _vp, _l, _f, _vp, _sz, _vp, _vp]),
     "gpumlib_nmf_sgemm": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "gpumlib_nmf_vstore_bytes": (_sz, [_i, _i, _i]),
+    "gpumlib_nmf_vstore_pack": (_i, [_i, _i, _i, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_iterate": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _i, _i, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_objective": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_pass": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _vp]),
@@ -50,6 +68,22 @@ SIGNATURES = {
     "gpumlib_nmf_object_get_wh": (_i, [_vp, _vp]),
     "gpumlib_nmf_object_objective": (_i, [_vp, _i, _c.POINTER(_d)]),
     "gpumlib_nmf_object_destroy": (None, [_vp]),
+    "gpumlib_nmf_batch_create": (_i, [_i, _c.POINTER(_vp)]),
+    "gpumlib_nmf_batch_destroy": (None, [_vp]),
+    "gpumlib_nmf_batch_set_target": (_i, [_vp, _i]),
+    "gpumlib_nmf_batch_workspace_bytes": (_sz, [_vp, _i, _i, _i, _i]),
+    "gpumlib_nmf_batch_add": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _c.POINTER(_i)]),
+    "gpumlib_nmf_batch_remove": (_i, [_vp, _i]),
+    "gpumlib_nmf_batch_active": (_i, [_vp]),
+    "gpumlib_nmf_batch_iterate": (_i, [_vp, _i, _i, _vp]),
+    "gpumlib_nmf_batch_graph_launch": (_i, [_vp, _i, _i, _vp]),
+    "gpumlib_nmf_batch_objectives": (_i, [_vp, _c.POINTER(_d), _vp]),
+    "gpumlib_nmf_batch_objective_device_ptr": (_vp, [_vp]),
+    "gpumlib_nmf_server_create": (_i, [_c.POINTER(ServerConfig), _c.POINTER(_vp)]),
+    "gpumlib_nmf_server_submit": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _f, _d, _i, _c.POINTER(_c.c_longlong)]),
+    "gpumlib_nmf_server_wait": (_i, [_vp, _c.c_longlong, _vp, _vp, _c.POINTER(JobResult)]),
+    "gpumlib_nmf_server_stats_get": (_i, [_vp, _c.POINTER(ServerStats)]),
+    "gpumlib_nmf_server_destroy": (None, [_vp]),
     "gpumlib_nmf_rng_u64": (_u64, [_u64, _u64, _u64]),
     "gpumlib_nmf_gen_uniform_f32": (None, [_u64, _u64, _sz, _vp]),
     "gpumlib_nmf_gen_euclidean_v": (None, [_u64, _i, _i, _i, _f, _vp]),

@@ -13,14 +13,19 @@ using namespace gpumlib_nmf;
 
 namespace gpumlib_nmf {
 
-static int choose_slices(int tiles, int K, int* kslice) {
-    const int target = kTargetWorkgroups;
+#define CK(x)                               \
+    do {                                    \
+        const hipError_t ck_e = (x);        \
+        if (ck_e != hipSuccess) return ck_e; \
+    } while (0)
+
+static int choose_slices(int tiles, int K, int kc, int target, int* kslice) {
     int S = (target + tiles - 1) / tiles;
-    const int maxS = (K + KC - 1) / KC;
+    const int maxS = (K + kc - 1) / kc;
     if (S > maxS) S = maxS;
     if (S < 1) S = 1;
-    int ks = round_up((K + S - 1) / S, KC);
-    if (ks < KC) ks = KC;
+    int ks = round_up((K + S - 1) / S, kc);
+    if (ks < kc) ks = kc;
     S = (K + ks - 1) / ks;
     if (S < 1) S = 1;
     *kslice = ks;
@@ -29,9 +34,19 @@ static int choose_slices(int tiles, int K, int* kslice) {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl) {
+size_t vstore_bf16_offset_t(int n, int m) { return align256((size_t)m * round_up(n, 8) * 2); }
+
+size_t vstore_bytes(int precision, int n, int m) {
+    if (n < 1 || m < 1) return 0;
+    if (precision == GPUMLIB_NMF_FP32) return (size_t)n * m * 4;
+    if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256((size_t)n * round_up(m, 8) * 2);
+    return 0;
+}
+
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
     if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return false;
-    if (precision != GPUMLIB_NMF_FP32) return false;
+    if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16) return false;
+    if (precision == GPUMLIB_NMF_BF16 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
     if (n < 1 || m < 1 || r < 1 || r > kMaxFusedRank) return false;
     memset(pl, 0, sizeof(*pl));
     pl->rule = rule;
@@ -43,23 +58,37 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl) {
     pl->rpad = round_up(r, 4);
     pl->ldr = pl->rpad;
     pl->ldp = pl->npad + pl->rpad;
+    pl->ldv8 = round_up(n, 8);
+    pl->ldt8 = round_up(m, 8);
     const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
+    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : 64;
     pl->tiles1 = (m + TILE - 1) / TILE + euc;
-    pl->S1 = choose_slices(pl->tiles1, n, &pl->ks1);
+    pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
     pl->tiles3 = (n + TILE - 1) / TILE + euc;
-    pl->S3 = choose_slices(pl->tiles3, m, &pl->ks3);
-    pl->pstride3 = (long)r * pl->ldp;
+    pl->S3 = choose_slices(pl->tiles3, m, kc, target, &pl->ks3);
+    // FP32 W pass writes P[s][a*ldp + i]; the BF16 pass over V^T writes P[s][i*ldr + a].
+    pl->pstride3 = (precision == GPUMLIB_NMF_FP32) ? (long)r * pl->ldp : (long)(n + r) * pl->ldr;
     const size_t p1 = (size_t)pl->S1 * pl->pstride1, p3 = (size_t)pl->S3 * pl->pstride3;
     const size_t pbytes = align256(sizeof(float) * (p1 > p3 ? p1 : p3));
     const int smax = pl->S1 > pl->S3 ? pl->S1 : pl->S3;
     const size_t sbytes = align256(sizeof(float) * (size_t)smax * pl->rpad);
     pl->nobjp = pl->tiles1 * pl->S1;  // one objective partial per H-pass workgroup
     const size_t obytes = align256(sizeof(double) * (size_t)pl->nobjp);
-    pl->off_P = 0;
-    pl->off_Psum = pbytes;
-    pl->off_objp = pbytes + sbytes;
-    pl->bytes = pbytes + sbytes + obytes;
+    size_t off = 0;
+    pl->off_P = off;
+    off += pbytes;
+    pl->off_Psum = off;
+    off += sbytes;
+    pl->off_objp = off;
+    off += obytes;
+    if (precision == GPUMLIB_NMF_BF16) {
+        pl->off_Wb = off;
+        off += align256((size_t)r * pl->ldv8 * 2);
+        pl->off_Hb = off;
+        off += align256((size_t)r * pl->ldt8 * 2);
+    }
+    pl->bytes = off;
     return true;
 }
 
@@ -83,6 +112,7 @@ void pass_args(const Plan& pl, const float* V, const float* W, const float* H, v
     a->npad = pl.npad;
     a->eps = eps;
     a->vec = (pl.n % 4 == 0) && (pl.r % 4 == 0) && aligned16(V) && aligned16(W) && aligned16(H);
+    a->ldv = (pl.precision == GPUMLIB_NMF_FP32) ? pl.n : pl.ldv8;
     if (which == 1) {
         a->kslice = pl.ks1;
         a->S = pl.S1;
@@ -102,11 +132,6 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
     PassArgs a1, a3;
     pass_args(pl, V, W, H, ws, eps, 1, &a1);
     pass_args(pl, V, W, H, ws, eps, 3, &a3);
-    hipError_t e;
-#define CK(x)                             \
-    do {                                  \
-        if ((e = (x)) != hipSuccess) return e; \
-    } while (0)
     for (int it = 0; it < iterations; it++) {
         const int obj = (it == 0 && d_objective) ? objective : GPUMLIB_NMF_OBJ_NONE;
         CK(launch_pass_h_f32(a1, kl, obj, 1, st));
@@ -114,7 +139,7 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
         if (kl)
             CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
         else
-            CK(launch_update_h_euc(H, a1.P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
+            CK(launch_update_h_euc(H, a1.P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
         if (!update_w) continue;
         CK(launch_pass_w_f32(a3, kl, st));
         if (kl)
@@ -125,15 +150,67 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
     return hipSuccess;
 }
 
-hipError_t run_objective(const Plan& pl, const float* V, const float* W, const float* H, float eps, int objective,
+hipError_t run_objective(const Plan& pl, const void* V, const float* W, const float* H, float eps, int objective,
                          double* d_objective, void* ws, hipStream_t st) {
     PassArgs a1;
-    pass_args(pl, V, W, H, ws, eps, 1, &a1);
+    pass_args(pl, (const float*)V, W, H, ws, eps, 1, &a1);
     a1.tiles = (pl.m + TILE - 1) / TILE;  // no gram tile
-    hipError_t e = launch_pass_h_f32(a1, 0, objective, 0, st);
-    if (e != hipSuccess) return e;
+    if (pl.precision == GPUMLIB_NMF_BF16) {
+        a1.kslice = round_up(pl.ks1, KC);
+        CK(launch_objective_bf16v(a1, objective, st));
+    } else {
+        CK(launch_pass_h_f32(a1, 0, objective, 0, st));
+    }
     return launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st);
-#undef CK
+}
+
+// BF16 storage: V store = [V (ld ldv8) | V^T (ld ldt8)] in BF16.  W and H stay FP32
+// (the updates and their ratios are FP32); BF16 shadows feed the MFMA passes.
+hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                               int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
+    char* base = (char*)ws;
+    const uint16_t* Vb = (const uint16_t*)Vstore;
+    const uint16_t* Vt = (const uint16_t*)((const char*)Vstore + vstore_bf16_offset_t(pl.n, pl.m));
+    uint16_t* Wb = (uint16_t*)(base + pl.off_Wb);
+    uint16_t* Hb = (uint16_t*)(base + pl.off_Hb);
+    float* P = (float*)(base + pl.off_P);
+    CK(launch_shadow_bf16(W, H, pl.n, pl.m, pl.r, Wb, pl.ldv8, Hb, pl.ldt8, st));
+    LpPassArgs ah, aw;
+    memset(&ah, 0, sizeof(ah));
+    ah.A = Wb;
+    ah.B = Vb;
+    ah.P = P;
+    ah.K = pl.n;
+    ah.N = pl.m;
+    ah.r = pl.r;
+    ah.lda = pl.ldv8;
+    ah.ldb = pl.ldv8;
+    ah.ldr = pl.ldr;
+    ah.kslice = pl.ks1;
+    ah.S = pl.S1;
+    ah.tiles = pl.tiles1;
+    ah.pstride = pl.pstride1;
+    aw = ah;
+    aw.A = Hb;
+    aw.B = Vt;
+    aw.K = pl.m;
+    aw.N = pl.n;
+    aw.lda = pl.ldt8;
+    aw.ldb = pl.ldt8;
+    aw.kslice = pl.ks3;
+    aw.S = pl.S3;
+    aw.tiles = pl.tiles3;
+    aw.pstride = pl.pstride3;
+    for (int it = 0; it < iterations; it++) {
+        if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
+            CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
+        CK(launch_pass_tn_bf16(ah, st));
+        CK(launch_update_h_euc(H, P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hb, pl.ldt8, st));
+        if (!update_w) continue;
+        CK(launch_pass_tn_bf16(aw, st));
+        CK(launch_update_w_euc_t(W, P, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st));
+    }
+    return hipSuccess;
 }
 
 }  // namespace gpumlib_nmf
@@ -224,8 +301,10 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
     if (objective != GPUMLIB_NMF_OBJ_NONE && !d_objective) return hipErrorInvalidValue;
     if (iterations == 0) {
         if (objective == GPUMLIB_NMF_OBJ_NONE) return hipSuccess;
-        return run_objective(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
+        return run_objective(pl, V, W, H, eps, objective, d_objective, workspace, stream);
     }
+    if (precision == GPUMLIB_NMF_BF16)
+        return run_iterations_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     return run_iterations(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective, workspace,
                           stream);
 }
@@ -237,7 +316,20 @@ hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, c
     if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || !d_objective) return hipErrorInvalidValue;
     if (objective < GPUMLIB_NMF_OBJ_FROBENIUS || objective > GPUMLIB_NMF_OBJ_QUALITY) return hipErrorInvalidValue;
-    return run_objective(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
+    return run_objective(pl, V, W, H, eps, objective, d_objective, workspace, stream);
+}
+
+size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m) { return vstore_bytes(precision, n, m); }
+
+hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float* V, void* vstore, size_t vstore_bytes_,
+                                   hipStream_t stream) {
+    const size_t need = vstore_bytes(precision, n, m);
+    if (!V || !vstore || need == 0 || vstore_bytes_ < need) return hipErrorInvalidValue;
+    if (precision == GPUMLIB_NMF_FP32)
+        return hipMemcpyAsync(vstore, V, need, hipMemcpyDeviceToDevice, stream);
+    uint16_t* Vb = (uint16_t*)vstore;
+    uint16_t* Vt = (uint16_t*)((char*)vstore + vstore_bf16_offset_t(n, m));
+    return launch_pack_v_bf16(V, n, m, Vb, round_up(n, 8), Vt, round_up(m, 8), stream);
 }
 
 hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void* V, const float* W,
@@ -247,6 +339,26 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
         return hipErrorInvalidValue;
     const int kl = (rule == GPUMLIB_NMF_KL);
+    if (precision == GPUMLIB_NMF_BF16) {
+        // one BF16 pass over the W/H shadows the last iterate call left in this workspace
+        char* base = (char*)workspace;
+        uint16_t* Wb = (uint16_t*)(base + pl.off_Wb);
+        uint16_t* Hb = (uint16_t*)(base + pl.off_Hb);
+        LpPassArgs a;
+        memset(&a, 0, sizeof(a));
+        a.P = (float*)(base + pl.off_P);
+        a.r = r;
+        a.ldr = pl.ldr;
+        if (which == 0) {
+            a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = a.ldb = pl.ldv8;
+            a.kslice = pl.ks1, a.S = pl.S1, a.tiles = pl.tiles1, a.pstride = pl.pstride1;
+        } else {
+            a.A = Hb, a.B = (const uint16_t*)((const char*)V + vstore_bf16_offset_t(n, m)), a.K = m, a.N = n;
+            a.lda = a.ldb = pl.ldt8;
+            a.kslice = pl.ks3, a.S = pl.S3, a.tiles = pl.tiles3, a.pstride = pl.pstride3;
+        }
+        return launch_pass_tn_bf16(a, stream);
+    }
     PassArgs a;
     pass_args(pl, (const float*)V, W, H, workspace, eps, which == 0 ? 1 : 3, &a);
     if (which == 0) return launch_pass_h_f32(a, kl, GPUMLIB_NMF_OBJ_NONE, 1, stream);

@@ -82,7 +82,40 @@ struct PassArgs {
     long pstride;     // floats per slice of P
     float eps;
     int vec;          // V, W, H rows/columns are 16-byte aligned (n % 4 == 0 and r % 4 == 0)
+    int ldv;          // leading dimension of V (n for FP32 V; a multiple of 8 for BF16 storage)
 };
+
+// Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
+// A: column a at A + a*lda; B: column u at B + u*ldb for u < N; the extra gram tile
+// (u in [N, N + r)) takes its columns from A.  P[s][u*ldr + a].
+struct LpPassArgs {
+    const uint16_t* A;
+    const uint16_t* B;
+    float* P;
+    int K, N, r;
+    int lda, ldb, ldr;
+    int kslice, S, tiles;
+    long pstride;
+};
+
+// One problem of a grouped (batched) launch: the arguments of its two passes and
+// of its updates.  P, Psum and objp point into the problem's own workspace, so
+// no value depends on which other problems share the launch.
+struct JobArgs {
+    PassArgs h;    // H pass (rows of V), also the H update's partials
+    PassArgs w;    // W pass (columns of V)
+    float* W;      // updated in place
+    float* H;
+    double* obj;   // objective outputs of the problem (two slots)
+    int rule;      // 0 Euclidean, 1 KL
+    int nobj;      // H-pass workgroups = objective partials
+};
+
+// Grouped launch table in device memory (capacity maxj problems):
+//   t[0] = problems J, t[1] = work items T, t[2 .. 2+J] = first item of each
+//   problem (t[2+J] = T), t[3+maxj .. 3+maxj+J) = index into the JobArgs array.
+constexpr int kGroupMaxJobs = 1024;
+inline int group_table_ints(int maxj) { return 3 + 2 * maxj; }
 
 // Objective selector for the fused H pass.
 enum ObjKind { OBJ_NONE = 0, OBJ_FROB = 1, OBJ_KL = 2, OBJ_QUALITY = 3 };

@@ -9,13 +9,27 @@ namespace gpumlib_nmf {
 hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st);
 hipError_t launch_pass_w_f32(const PassArgs& a, int kl, hipStream_t st);
 
+// BF16 storage path (nmf_pass_bf16.hip).
+hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st);
+hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ldv, uint16_t* Vt, int ldt,
+                              hipStream_t st);
+hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
+                              int ldh, hipStream_t st);
+// Objective of (W, H) against a BF16-stored V (nmf_pass_f32.hip).
+hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st);
+
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
+// Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
 hipError_t launch_update_h_euc(float* H, const float* P, int m, int r, int ldr, int S, long pstride,
-                               float eps, hipStream_t st);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st);
 // Euclidean W:  W(i,a) *= VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)
 hipError_t launch_update_w_euc(float* W, const float* P, int n, int r, int ldp, int npad, int S,
                                long pstride, float eps, hipStream_t st);
+// Euclidean W from partials laid out P[s][i*ldr + a] (gram rows at i >= n), as the
+// BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
+hipError_t launch_update_w_euc_t(float* W, const float* P, int n, int r, int ldr, int S, long pstride, float eps,
+                                 uint16_t* Wb, int ldwb, hipStream_t st);
 // KL H:  H(a,u) *= num(a,u) / max(sum_s Psum[s][a], eps)
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr,
                               int S, long pstride, float eps, hipStream_t st);
@@ -24,6 +38,14 @@ hipError_t launch_update_w_kl(float* W, const float* P, const float* Psum, int n
                               int S, long pstride, float eps, hipStream_t st);
 // Deterministic sum of count doubles into out[0].
 hipError_t launch_reduce_f64(const double* in, int count, double* out, hipStream_t st);
+
+// Grouped launches over many problems (nmf_pass_f32.hip, nmf_update.hip).  grid
+// is the persistent workgroup count; each workgroup strides over the table's items.
+hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int grid, hipStream_t st);
+hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st);
+hipError_t launch_group_update_h(const JobArgs* jobs, const int* table, int grid, hipStream_t st);
+hipError_t launch_group_update_w(const JobArgs* jobs, const int* table, int grid, hipStream_t st);
+hipError_t launch_group_reduce_obj(const JobArgs* jobs, const int* table, int slot, int grid, hipStream_t st);
 
 // Reference-shaped kernels over materialised operands.
 hipError_t launch_mu_update(const float* nm, const float* dm, float* mm, long elements, float eps,

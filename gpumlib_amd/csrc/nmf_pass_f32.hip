@@ -38,6 +38,24 @@ __device__ double block_sum_f64(double v, double* red) {
     return t;
 }
 
+// 4 consecutive elements of a BF16 column as floats (bf16 -> f32 is exact).
+__device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, int kend) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!ok) return v;
+    uint16_t e[4];
+    if (k + 3 < kend) {
+        const uint2 x = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned: leading dimension % 8 == 0
+        e[0] = x.x & 0xffff, e[1] = x.x >> 16, e[2] = x.y & 0xffff, e[3] = x.y >> 16;
+    } else {
+        for (int j = 0; j < 4; j++) e[j] = (k + j < kend) ? p[j] : (uint16_t)0;
+    }
+    v.x = __uint_as_float((uint32_t)e[0] << 16);
+    v.y = __uint_as_float((uint32_t)e[1] << 16);
+    v.z = __uint_as_float((uint32_t)e[2] << 16);
+    v.w = __uint_as_float((uint32_t)e[3] << 16);
+    return v;
+}
+
 template <int OBJ>
 __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
     if (OBJ == OBJ_FROB) {
@@ -58,23 +76,22 @@ __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
 // ---------------------------------------------------------------------------
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
 // entry; GEMM2 false: objective only (no partials written).
-template <int MB, bool KL, int OBJ, bool GEMM2>
-__global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
+// TV: element type of V (float, or uint16_t holding BF16 for the objective of a
+// BF16-stored V; that case never has a gram tile).
+template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
+__device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr bool NEED_WH = KL || (OBJ != OBJ_NONE);
     __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];    // W chunk, [a][i]
     __shared__ __attribute__((aligned(16))) float sB[2][TILE][LDK];  // V (or W) chunk, [u][i]
     __shared__ double red[NTHREADS / 64];
 
-    const int total = p.tiles * p.S;
-    const int w = xcd_work_index(blockIdx.x, gridDim.x);
-    if (w >= total) return;
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
     const int vt = (p.m + TILE - 1) / TILE;
     const bool gram = (!KL) && (t == vt);
-    const float* B = gram ? p.W : p.V;
+    const TV* Vt = reinterpret_cast<const TV*>(p.V);
     const int col0 = gram ? 0 : t * TILE;
     const int ncols = gram ? p.r : min(TILE, p.m - col0);
     const int kbeg = s * p.kslice;
@@ -97,7 +114,15 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
-            rb[j] = load4(B + (size_t)(col0 + col) * p.n + k0 + kq, col < ncols, k0 + kq, kend, vec);
+            if (gram) {
+                rb[j] = load4(p.W + (size_t)col * p.n + k0 + kq, col < ncols, k0 + kq, kend, vec);
+            } else if constexpr (sizeof(TV) == 2) {
+                rb[j] = load4_bf16(reinterpret_cast<const uint16_t*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,
+                                   col < ncols, k0 + kq, kend);
+            } else {
+                rb[j] = load4(reinterpret_cast<const float*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,
+                              col < ncols, k0 + kq, kend, vec);
+            }
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
@@ -147,7 +172,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
             f32x16 wh;
             for (int v = 0; v < 16; v++) wh[v] = 0.f;
 #pragma unroll
-            for (int kk = 0; kk < MB * 16; kk++) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
+            for (int kk = 0; kk < MB * 16; kk++)
+                if (kk < nb * 16) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
             // wh[v] is row i = acc_row(v, h) of the chunk, column c of the wave.
 #pragma unroll
             for (int g = 0; g < 4; g++) {
@@ -162,6 +188,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
                 if (KL && GEMM2) {
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++) {
+                        if (mb >= nb) continue;
                         const float4 wa = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][g * 8 + h * 4]);
                         acc[mb] = mfma32x2(wa.x, rj[0], acc[mb]);
                         acc[mb] = mfma32x2(wa.y, rj[1], acc[mb]);
@@ -177,6 +204,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
                 const float4 bv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
+                    if (mb >= nb) continue;
                     const float4 av = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][g * 8 + h * 4]);
                     acc[mb] = mfma32x2(av.x, bv.x, acc[mb]);
                     acc[mb] = mfma32x2(av.y, bv.y, acc[mb]);
@@ -218,16 +246,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
 // ---------------------------------------------------------------------------
 // W pass, Euclidean rule: P[s](i,a) = sum_u A(i,u) H(a,u), A = V (row tiles) or H (gram tile).
 template <int MB>
-__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_euc_f32(PassArgs p) {
+__device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr int PA = TILE + 32;  // rows k and k+1 land 32 banks apart
     constexpr int PB = RP + 32;
     __shared__ __attribute__((aligned(16))) float sA[2][KC][PA];  // [u][i]
     __shared__ __attribute__((aligned(16))) float sB[2][KC][PB];  // [u][a]
 
-    const int total = p.tiles * p.S;
-    const int w = xcd_work_index(blockIdx.x, gridDim.x);
-    if (w >= total) return;
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
@@ -285,7 +310,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_euc_f32(PassArgs p) {
         for (int kk = 0; kk < KC / 2; kk++) {
             const float av = sA[buf][2 * kk + h][wave * 32 + c];
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
+            for (int mb = 0; mb < MB; mb++)
+                if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
         }
         if (ch + 1 < nchunks) sstore(buf ^ 1);
         __syncthreads();
@@ -315,15 +341,12 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_euc_f32(PassArgs p) {
 // (W H)^T is formed per step as H^T W^T so that the ratio lands in the A-operand
 // layout of the second product without a shuffle.
 template <int MB>
-__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
+__device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr int PV = TILE + 8;  // rows u and u+4 land 32 banks apart
     __shared__ __attribute__((aligned(16))) float sV[2][KC][PV];   // [u][i]
     __shared__ __attribute__((aligned(16))) float sH[2][RP][LDK];  // [a][u]
 
-    const int total = p.tiles * p.S;
-    const int w = xcd_work_index(blockIdx.x, gridDim.x);
-    if (w >= total) return;
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
@@ -402,7 +425,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
         f32x16 wt;
         for (int v = 0; v < 16; v++) wt[v] = 0.f;
 #pragma unroll
-        for (int kk = 0; kk < MB * 16; kk++) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
+        for (int kk = 0; kk < MB * 16; kk++)
+            if (kk < nb * 16) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             float rj[4];
@@ -413,6 +437,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
             }
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
+                if (mb >= nb) continue;
                 const float4 hb = *reinterpret_cast<const float4*>(&sH[buf][mb * 32 + c][g * 8 + h * 4]);
                 acc[mb] = mfma32x2(rj[0], hb.x, acc[mb]);
                 acc[mb] = mfma32x2(rj[1], hb.y, acc[mb]);
@@ -440,6 +465,94 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
         }
     }
     if (do_sums && tid < p.r) p.Psum[(size_t)s * round_up(p.r, 4) + tid] = rowsum;
+}
+
+// ---------------------------------------------------------------------------
+// Single-problem kernels: one workgroup per work item, XCD-aware order.
+template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= p.tiles * p.S) return;
+    pass_h_body<MB, KL, OBJ, GEMM2, TV>(p, w, MB);
+}
+
+template <int MB>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_euc_f32(PassArgs p) {
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= p.tiles * p.S) return;
+    pass_w_euc_body<MB>(p, w, MB);
+}
+
+template <int MB>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_w_kl_f32(PassArgs p) {
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= p.tiles * p.S) return;
+    pass_w_kl_body<MB>(p, w, MB);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped kernels: a persistent grid strides over the work items of every
+// problem in the table.  Problems have r <= 64 (two rank blocks), and each runs
+// exactly the code of its single-problem launch on its own workspace.
+__device__ __forceinline__ int group_find(const int* first, int J, int item) {
+    int lo = 0, hi = J - 1;  // largest j with first[j] <= item
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool KL, int OBJ>
+__global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __restrict__ jobs, const int* table) {
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find(first, J, item);
+        const PassArgs p = jobs[idx[j]].h;
+        __syncthreads();
+        pass_h_body<2, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
+    }
+}
+
+template <bool KL>
+__global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_w(const JobArgs* __restrict__ jobs, const int* table) {
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find(first, J, item);
+        const PassArgs p = jobs[idx[j]].w;
+        __syncthreads();
+        if (KL)
+            pass_w_kl_body<2>(p, item - first[j], (p.r + 31) >> 5);
+        else
+            pass_w_euc_body<2>(p, item - first[j], (p.r + 31) >> 5);
+    }
+}
+
+hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int grid, hipStream_t st) {
+#define LG(K, O)                                                                                 \
+    hipLaunchKernelGGL((k_group_pass_h<K, O>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+    return hipGetLastError();
+    if (kl) {
+        if (obj == OBJ_NONE) { LG(true, OBJ_NONE) }
+        if (obj == OBJ_KL) { LG(true, OBJ_KL) }
+    } else {
+        if (obj == OBJ_NONE) { LG(false, OBJ_NONE) }
+        if (obj == OBJ_FROB) { LG(false, OBJ_FROB) }
+    }
+#undef LG
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
+    if (kl)
+        hipLaunchKernelGGL((k_group_pass_w<true>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else
+        hipLaunchKernelGGL((k_group_pass_w<false>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +594,29 @@ hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipS
     if (rp <= 32) return launch_h_mb<1>(a, kl, obj, gemm2, st);
     if (rp <= 64) return launch_h_mb<2>(a, kl, obj, gemm2, st);
     if (rp <= 128) return launch_h_mb<4>(a, kl, obj, gemm2, st);
+    return hipErrorInvalidValue;
+}
+
+template <int MB>
+static hipError_t launch_obj_bf16v_mb(const PassArgs& a, int obj, hipStream_t st) {
+    const int grid = xcd_grid(a.tiles * a.S);
+#define LO(OBJV)                                                                                              \
+    hipLaunchKernelGGL((k_pass_h_f32<MB, false, OBJV, false, uint16_t>), dim3(grid), dim3(NTHREADS), 0, st, a); \
+    return hipGetLastError();
+    switch (obj) {
+        case OBJ_FROB: LO(OBJ_FROB)
+        case OBJ_KL: LO(OBJ_KL)
+        case OBJ_QUALITY: LO(OBJ_QUALITY)
+        default: return hipErrorInvalidValue;
+    }
+#undef LO
+}
+
+hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st) {
+    const int rp = round_up(a.r, 32);
+    if (rp <= 32) return launch_obj_bf16v_mb<1>(a, obj, st);
+    if (rp <= 64) return launch_obj_bf16v_mb<2>(a, obj, st);
+    if (rp <= 128) return launch_obj_bf16v_mb<4>(a, obj, st);
     return hipErrorInvalidValue;
 }
 

@@ -18,19 +18,27 @@ constexpr int kTargetWorkgroups = 512;
 struct Plan {
     int rule, precision, n, m, r;
     int npad, rpad, ldr, ldp;
+    int ldv8, ldt8;       // BF16 storage: leading dimensions of V (n -> mult. of 8) and V^T (m -> mult. of 8)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;
     int nobjp;
-    size_t off_P, off_Psum, off_objp, bytes;
+    size_t off_P, off_Psum, off_objp, off_Wb, off_Hb, bytes;
 };
 
-bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl);
+// Byte layout of the BF16 V store: V (column-major, ld ldv8) then V^T (column-major, ld ldt8).
+size_t vstore_bf16_offset_t(int n, int m);
+size_t vstore_bytes(int precision, int n, int m);
+
+// target: workgroups aimed for per pass (K slices are chosen to reach it).
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target = kTargetWorkgroups);
 void pass_args(const Plan& pl, const float* V, const float* W, const float* H, void* ws, float eps, int which,
                PassArgs* a);
 hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations, int update_w,
                           int objective, double* d_objective, void* ws, hipStream_t st);
-hipError_t run_objective(const Plan& pl, const float* V, const float* W, const float* H, float eps, int objective,
+hipError_t run_objective(const Plan& pl, const void* V, const float* W, const float* H, float eps, int objective,
                          double* d_objective, void* ws, hipStream_t st);
+hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                               int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 
 }  // namespace gpumlib_nmf

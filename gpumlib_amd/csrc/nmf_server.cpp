@@ -1,0 +1,390 @@
+// Continuous-batching NMF server (no reference counterpart).
+//
+// Requests (one NMF problem each: rule, V, initial W and H, eps, tolerance,
+// iteration cap) are queued by any host thread.  One engine thread owns the
+// device work:
+//   * admission: a queued problem is admitted when a batch slot is free and its
+//     HBM footprint (V, W, H and its workspace) fits in the device pool; the
+//     queue is scanned in arrival order, skipping problems that do not fit yet;
+//   * rounds: every admitted problem advances `round_iterations` iterations with
+//     the grouped launches of the batch engine (replayed from one hipGraph),
+//     the FP64 objective being computed on entry to the first two iterations;
+//   * retirement: a problem whose relative objective change over one iteration
+//     is below its tolerance, or that reached its iteration cap, has W and H
+//     copied back and its memory returned to the pool; its slot is reused in
+//     the next round.
+// Problems admitted while others are in flight join the next round, so work of
+// different sizes and ages overlaps on the device.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/gpumlib_nmf.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// First-fit sub-allocator over one device allocation (offsets only).
+class Pool {
+  public:
+    void init(size_t bytes) {
+        total_ = bytes;
+        free_.clear();
+        free_[0] = bytes;
+        used_ = high_ = 0;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align256(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes, start = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            used_ += bytes;
+            high_ = std::max(high_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align256(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t total() const { return total_; }
+    size_t high_water() const { return high_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t total_ = 0, used_ = 0, high_ = 0;
+};
+
+struct Job {
+    long long id = 0;
+    int rule = 0, n = 0, m = 0, r = 0;
+    float eps = 0;
+    double tol = 0;
+    int max_iters = 0;
+    std::vector<float> V, W, H;  // inputs, then W and H results
+    // device placement
+    size_t off = 0, bytes = 0, offV = 0, offW = 0, offH = 0, offWs = 0, ws_bytes = 0;
+    int slot = -1;
+    int iters = 0;
+    int converged = 0;
+    double objective = 0;
+    double t_submit = 0, t_admit = 0, t_finish = 0;
+    hipError_t status = hipSuccess;
+    bool done = false;
+};
+
+}  // namespace
+
+struct gpumlib_nmf_server {
+    gpumlib_nmf_server_config cfg;
+    int device = 0;
+    Clock::time_point t0;
+    Pool pool;
+    char* d_pool = nullptr;
+    gpumlib_nmf_batch_handle batch = nullptr;
+    hipStream_t stream = nullptr;
+    double* h_obj = nullptr;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> pending;
+    std::map<long long, std::shared_ptr<Job>> jobs;  // submitted and not yet collected
+    std::vector<std::shared_ptr<Job>> active;        // by slot
+    long long next_id = 1;
+    bool stop = false;
+    hipError_t fatal = hipSuccess;
+    gpumlib_nmf_server_stats stats;
+    std::thread engine;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    void run();
+    bool admit_locked(std::vector<std::shared_ptr<Job>>& admitted);
+};
+
+static size_t job_footprint(gpumlib_nmf_batch_handle b, Job& j) {
+    j.ws_bytes = gpumlib_nmf_batch_workspace_bytes(b, j.rule, j.n, j.m, j.r);
+    if (j.ws_bytes == 0) return 0;
+    j.offV = 0;
+    j.offW = align256(sizeof(float) * (size_t)j.n * j.m);
+    j.offH = j.offW + align256(sizeof(float) * (size_t)j.n * j.r);
+    j.offWs = j.offH + align256(sizeof(float) * (size_t)j.r * j.m);
+    return j.offWs + align256(j.ws_bytes);
+}
+
+bool gpumlib_nmf_server::admit_locked(std::vector<std::shared_ptr<Job>>& admitted) {
+    int free_slots = cfg.max_active - (int)std::count_if(active.begin(), active.end(), [](auto& p) { return !!p; });
+    bool any = false;
+    for (auto it = pending.begin(); it != pending.end() && free_slots > 0;) {
+        Job& j = **it;
+        size_t off = 0;
+        if (pool.alloc(j.bytes, &off)) {
+            j.off = off;
+            admitted.push_back(*it);
+            it = pending.erase(it);
+            free_slots--;
+            any = true;
+        } else {
+            ++it;
+        }
+    }
+    return any;
+}
+
+void gpumlib_nmf_server::run() {
+    (void)hipSetDevice(device);
+    const int rounds_iters = cfg.round_iterations;
+    for (;;) {
+        std::vector<std::shared_ptr<Job>> admitted;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                admit_locked(admitted);
+                const bool have_active = std::any_of(active.begin(), active.end(), [](auto& p) { return !!p; });
+                if (!admitted.empty() || have_active) break;
+                if (stop) return;
+                cv_work.wait(lk);
+            }
+        }
+        // upload and register the admitted problems
+        hipError_t e = hipSuccess;
+        for (auto& jp : admitted) {
+            Job& j = *jp;
+            char* base = d_pool + j.off;
+            float* dV = (float*)(base + j.offV);
+            float* dW = (float*)(base + j.offW);
+            float* dH = (float*)(base + j.offH);
+            if (e == hipSuccess) e = hipMemcpyAsync(dV, j.V.data(), sizeof(float) * j.V.size(), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(dW, j.W.data(), sizeof(float) * j.W.size(), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(dH, j.H.data(), sizeof(float) * j.H.size(), hipMemcpyHostToDevice, stream);
+            int slot = -1;
+            if (e == hipSuccess)
+                e = gpumlib_nmf_batch_add(batch, j.rule, j.n, j.m, j.r, dV, dW, dH, j.eps, base + j.offWs, j.ws_bytes,
+                                          &slot);
+            j.slot = slot;
+            j.t_admit = now();
+            if (e == hipSuccess) {
+                active[slot] = jp;
+            } else {
+                // not admitted after all: return its memory and report the error to its waiter
+                std::lock_guard<std::mutex> lk(mu);
+                pool.release(j.off, j.bytes);
+                j.status = e;
+                j.done = true;
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);  // host input buffers may be released after this
+        for (auto& jp : admitted) {
+            std::vector<float>().swap(jp->V);
+        }
+        // one round
+        if (e == hipSuccess) {
+            if (cfg.use_graph)
+                e = gpumlib_nmf_batch_graph_launch(batch, rounds_iters, rounds_iters >= 2 ? 2 : 1, stream);
+            else
+                e = gpumlib_nmf_batch_iterate(batch, rounds_iters, rounds_iters >= 2 ? 2 : 1, stream);
+        }
+        if (e == hipSuccess) e = gpumlib_nmf_batch_objectives(batch, h_obj, stream);
+        std::vector<std::shared_ptr<Job>> finished;
+        int in_round = 0;
+        for (auto& jp : active) {
+            if (!jp) continue;
+            in_round++;
+            Job& j = *jp;
+            j.iters += rounds_iters;
+            const double f0 = h_obj[2 * j.slot], f1 = rounds_iters >= 2 ? h_obj[2 * j.slot + 1] : f0;
+            j.objective = f1;
+            const double rel = (f0 != 0) ? std::fabs(f0 - f1) / std::fabs(f0) : 0.0;
+            if (rounds_iters >= 2 && rel < j.tol) j.converged = 1;
+            if (e != hipSuccess || j.converged || j.iters >= j.max_iters) finished.push_back(jp);
+        }
+        for (auto& jp : finished) {
+            Job& j = *jp;
+            char* base = d_pool + j.off;
+            hipError_t e2 = e;
+            if (e2 == hipSuccess)
+                e2 = hipMemcpyAsync(j.W.data(), base + j.offW, sizeof(float) * j.W.size(), hipMemcpyDeviceToHost, stream);
+            if (e2 == hipSuccess)
+                e2 = hipMemcpyAsync(j.H.data(), base + j.offH, sizeof(float) * j.H.size(), hipMemcpyDeviceToHost, stream);
+            if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
+            j.status = e2;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stats.rounds++;
+            stats.iterations_batched += (long long)in_round * rounds_iters;
+            stats.peak_active = std::max(stats.peak_active, in_round);
+            for (auto& jp : finished) {
+                Job& j = *jp;
+                (void)gpumlib_nmf_batch_remove(batch, j.slot);
+                active[j.slot].reset();
+                pool.release(j.off, j.bytes);
+                j.t_finish = now();
+                j.done = true;
+                stats.completed++;
+            }
+            stats.pool_high_water = pool.high_water();
+            if (e != hipSuccess) fatal = e;
+        }
+        cv_done.notify_all();
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& jp : pending) {
+                jp->status = e;
+                jp->done = true;
+            }
+            pending.clear();
+            cv_done.notify_all();
+            return;
+        }
+    }
+}
+
+extern "C" {
+
+hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpumlib_nmf_server_handle* out) {
+    if (!cfg || !out || cfg->hbm_budget_bytes == 0 || cfg->max_active < 1 || cfg->max_active > 1024 ||
+        cfg->round_iterations < 1)
+        return hipErrorInvalidValue;
+    *out = nullptr;
+    gpumlib_nmf_server* s = new (std::nothrow) gpumlib_nmf_server;
+    if (!s) return hipErrorOutOfMemory;
+    s->cfg = *cfg;
+    s->t0 = Clock::now();
+    memset(&s->stats, 0, sizeof(s->stats));
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_pool, cfg->hbm_budget_bytes);
+    if (e == hipSuccess) e = gpumlib_nmf_batch_create(cfg->max_active, &s->batch);
+    if (e == hipSuccess && cfg->target_workgroups > 0) e = gpumlib_nmf_batch_set_target(s->batch, cfg->target_workgroups);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_obj, sizeof(double) * 2 * cfg->max_active, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        if (s->h_obj) (void)hipHostFree(s->h_obj);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->batch) gpumlib_nmf_batch_destroy(s->batch);
+        if (s->d_pool) (void)hipFree(s->d_pool);
+        delete s;
+        return e;
+    }
+    s->pool.init(cfg->hbm_budget_bytes);
+    s->active.resize(cfg->max_active);
+    s->engine = std::thread([s] { s->run(); });
+    *out = s;
+    return hipSuccess;
+}
+
+hipError_t gpumlib_nmf_server_submit(gpumlib_nmf_server_handle s, int rule, int n, int m, int r, const float* V,
+                                     const float* W, const float* H, float eps, double tol, int max_iterations,
+                                     long long* job_id) {
+    if (!s || !V || !W || !H || !job_id || n < 1 || m < 1 || r < 1 || r > 64 || max_iterations < 1 ||
+        (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL))
+        return hipErrorInvalidValue;
+    auto j = std::make_shared<Job>();
+    j->rule = rule;
+    j->n = n;
+    j->m = m;
+    j->r = r;
+    j->eps = eps;
+    j->tol = tol;
+    j->max_iters = max_iterations;
+    j->bytes = job_footprint(s->batch, *j);
+    if (j->bytes == 0) return hipErrorInvalidValue;
+    if (j->bytes > s->pool.total()) return hipErrorOutOfMemory;  // can never be admitted
+    j->V.assign(V, V + (size_t)n * m);
+    j->W.assign(W, W + (size_t)n * r);
+    j->H.assign(H, H + (size_t)r * m);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return hipErrorInvalidValue;
+        if (s->fatal != hipSuccess) return s->fatal;
+        j->id = s->next_id++;
+        j->t_submit = s->now();
+        s->jobs[j->id] = j;
+        s->pending.push_back(j);
+        s->stats.submitted++;
+        *job_id = j->id;
+    }
+    s->cv_work.notify_one();
+    return hipSuccess;
+}
+
+hipError_t gpumlib_nmf_server_wait(gpumlib_nmf_server_handle s, long long job_id, float* W_out, float* H_out,
+                                   gpumlib_nmf_job_result* res) {
+    if (!s) return hipErrorInvalidValue;
+    std::shared_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(job_id);
+        if (it == s->jobs.end()) return hipErrorInvalidValue;
+        j = it->second;
+        s->cv_done.wait(lk, [&] { return j->done; });
+        s->jobs.erase(it);
+    }
+    if (W_out) memcpy(W_out, j->W.data(), sizeof(float) * j->W.size());
+    if (H_out) memcpy(H_out, j->H.data(), sizeof(float) * j->H.size());
+    if (res) {
+        res->iterations = j->iters;
+        res->converged = j->converged;
+        res->objective = j->objective;
+        res->submit_s = j->t_submit;
+        res->admit_s = j->t_admit;
+        res->finish_s = j->t_finish;
+    }
+    return j->status;
+}
+
+hipError_t gpumlib_nmf_server_stats_get(gpumlib_nmf_server_handle s, gpumlib_nmf_server_stats* out) {
+    if (!s || !out) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->pool_bytes = s->pool.total();
+    return s->fatal;
+}
+
+void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->engine.joinable()) s->engine.join();  // finishes every admitted and queued problem first
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipHostFree(s->h_obj);
+    (void)hipStreamDestroy(s->stream);
+    gpumlib_nmf_batch_destroy(s->batch);
+    (void)hipFree(s->d_pool);
+    delete s;
+}
+
+}  // extern "C"

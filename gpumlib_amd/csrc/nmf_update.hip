@@ -13,14 +13,14 @@ constexpr int UPD_ROWS = 64;  // rows of W per update workgroup
 // Euclidean H update, reference nmf_multiplicative_euclidian.cpp:36-45 and
 // nmf_kernels.cu:46:  H(a,u) *= WtV(a,u) / (WtWH(a,u) + eps), WtWH = WtW H.
 // WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = sum_s P[s][(m+b)*ldr + a].
-__global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H, const float* __restrict__ P,
-                                                           int m, int r, int ldr, int S, long pstride, float eps) {
-    extern __shared__ float sm[];
+__device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const float* __restrict__ P, int m, int r,
+                                                  int ldr, int S, long pstride, float eps, uint16_t* __restrict__ Hs,
+                                                  int ldhs, int blk, float* sm) {
     const int rs = r + 1;
     float* sG = sm;           // sG[a*rs + b] = WtW(a,b)
     float* sH = sm + r * rs;  // sH[uu*rs + b] = H(b, u0 + uu)
     const int tid = threadIdx.x;
-    const int u0 = blockIdx.x * UPD_COLS;
+    const int u0 = blk * UPD_COLS;
     const int ncol = min(UPD_COLS, m - u0);
     for (int idx = tid; idx < r * r; idx += NTHREADS) {
         const int a = idx % r, b = idx / r;
@@ -44,20 +44,77 @@ __global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H
         for (int b = 0; b < r; b++) d += sG[a * rs + b] * sH[uu * rs + b];
         H[(size_t)u * r + a] = sH[uu * rs + a] * (wtv / (d + eps));
     }
+    if (Hs) {
+        __syncthreads();  // this workgroup's H columns are written
+        for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
+            const int a = idx / ncol, uu = idx - a * ncol;
+            Hs[(size_t)a * ldhs + u0 + uu] = __builtin_bit_cast(uint16_t, (__bf16)H[(size_t)(u0 + uu) * r + a]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H, const float* __restrict__ P,
+                                                           int m, int r, int ldr, int S, long pstride, float eps,
+                                                           uint16_t* __restrict__ Hs, int ldhs) {
+    extern __shared__ float sm[];
+    update_h_euc_body(H, P, m, r, ldr, S, pstride, eps, Hs, ldhs, blockIdx.x, sm);
+}
+
+// Euclidean W update from partials in the layout of the BF16 pass over V^T:
+// VHt(i,a) = sum_s P[s][i*ldr + a];  HHt(b,a) = sum_s P[s][(n+b)*ldr + a].
+__global__ __launch_bounds__(NTHREADS) void k_update_w_euc_t(float* __restrict__ W, const float* __restrict__ P,
+                                                             int n, int r, int ldr, int S, long pstride, float eps,
+                                                             uint16_t* __restrict__ Wb, int ldwb) {
+    extern __shared__ float sm[];
+    const int rs = r + 1;
+    float* sG = sm;                      // sG[b*rs + a] = HHt(b,a)
+    float* sW = sG + r * rs;             // sW[ii*rs + b] = W(i0 + ii, b)
+    float* sP = sW + UPD_ROWS * rs;      // sP[ii*rs + a] = VHt(i0 + ii, a)
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * UPD_ROWS;
+    const int nrow = min(UPD_ROWS, n - i0);
+    for (int idx = tid; idx < r * r; idx += NTHREADS) {
+        const int a = idx % r, b = idx / r;
+        const float* src = P + (size_t)(n + b) * ldr + a;
+        float g = 0.f;
+        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
+        sG[b * rs + a] = g;
+    }
+    for (int idx = tid; idx < nrow * r; idx += NTHREADS) {
+        const int ii = idx / r, a = idx - ii * r;
+        const float* src = P + (size_t)(i0 + ii) * ldr + a;
+        float v = 0.f;
+        for (int s = 0; s < S; s++) v += src[(size_t)s * pstride];
+        sP[ii * rs + a] = v;
+    }
+    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
+        const int ii = idx % UPD_ROWS, b = idx / UPD_ROWS;
+        if (ii < nrow) sW[ii * rs + b] = W[(size_t)b * n + i0 + ii];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
+        const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
+        if (ii >= nrow) continue;
+        const int i = i0 + ii;
+        float d = 0.f;
+        for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
+        const float w = sW[ii * rs + a] * (sP[ii * rs + a] / (d + eps));
+        W[(size_t)a * n + i] = w;
+        if (Wb) Wb[(size_t)a * ldwb + i] = __builtin_bit_cast(uint16_t, (__bf16)w);
+    }
 }
 
 // Euclidean W update, reference nmf_multiplicative_euclidian.cpp:53-66:
 // W(i,a) *= VHt(i,a) / (WHHt(i,a) + eps), WHHt = W HHt.
 // VHt(i,a) = sum_s P[s][a*ldp + i];  HHt(b,a) = sum_s P[s][a*ldp + npad + b].
-__global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W, const float* __restrict__ P,
-                                                           int n, int r, int ldp, int npad, int S, long pstride,
-                                                           float eps) {
-    extern __shared__ float sm[];
+__device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const float* __restrict__ P, int n, int r,
+                                                  int ldp, int npad, int S, long pstride, float eps, int blk,
+                                                  float* sm) {
     const int rs = r + 1;
     float* sG = sm;           // sG[b*rs + a] = HHt(b,a)
     float* sW = sm + r * rs;  // sW[ii*rs + b] = W(i0 + ii, b)
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * UPD_ROWS;
+    const int i0 = blk * UPD_ROWS;
     const int nrow = min(UPD_ROWS, n - i0);
     for (int idx = tid; idx < r * r; idx += NTHREADS) {
         const int b = idx % r, a = idx / r;
@@ -84,11 +141,17 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W
     }
 }
 
+__global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W, const float* __restrict__ P,
+                                                           int n, int r, int ldp, int npad, int S, long pstride,
+                                                           float eps) {
+    extern __shared__ float sm[];
+    update_w_euc_body(W, P, n, r, ldp, npad, S, pstride, eps, blockIdx.x, sm);
+}
+
 // KL H update, reference nmf_kernels.cu:337-339 with the clamped sums of :104.
-__global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H, const float* __restrict__ P,
-                                                          const float* __restrict__ Psum, int m, int r, int ldr,
-                                                          int S, long pstride, float eps) {
-    __shared__ float ssum[512];
+__device__ __forceinline__ void update_h_kl_body(float* __restrict__ H, const float* __restrict__ P,
+                                                 const float* __restrict__ Psum, int m, int r, int ldr, int S,
+                                                 long pstride, float eps, int blk, float* ssum) {
     const int tid = threadIdx.x;
     const int rpad = round_up(r, 4);
     for (int a = tid; a < r; a += NTHREADS) {
@@ -97,7 +160,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H,
         ssum[a] = (v < eps) ? eps : v;
     }
     __syncthreads();
-    const int u0 = blockIdx.x * UPD_COLS;
+    const int u0 = blk * UPD_COLS;
     const int ncol = min(UPD_COLS, m - u0);
     for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
         const int uu = idx / r, a = idx - uu * r;
@@ -109,11 +172,17 @@ __global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H,
     }
 }
 
-// KL W update, reference nmf_kernels.cu:289-291 with the clamped sums of :190.
-__global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W, const float* __restrict__ P,
-                                                          const float* __restrict__ Psum, int n, int r, int ldp,
+__global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H, const float* __restrict__ P,
+                                                          const float* __restrict__ Psum, int m, int r, int ldr,
                                                           int S, long pstride, float eps) {
     __shared__ float ssum[512];
+    update_h_kl_body(H, P, Psum, m, r, ldr, S, pstride, eps, blockIdx.x, ssum);
+}
+
+// KL W update, reference nmf_kernels.cu:289-291 with the clamped sums of :190.
+__device__ __forceinline__ void update_w_kl_body(float* __restrict__ W, const float* __restrict__ P,
+                                                 const float* __restrict__ Psum, int n, int r, int ldp, int S,
+                                                 long pstride, float eps, int blk, float* ssum) {
     const int tid = threadIdx.x;
     const int rpad = round_up(r, 4);
     for (int a = tid; a < r; a += NTHREADS) {
@@ -122,7 +191,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W,
         ssum[a] = (v < eps) ? eps : v;
     }
     __syncthreads();
-    const int i0 = blockIdx.x * UPD_ROWS;
+    const int i0 = blk * UPD_ROWS;
     const int nrow = min(UPD_ROWS, n - i0);
     for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
         const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
@@ -133,6 +202,13 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W,
         for (int s = 0; s < S; s++) num += src[(size_t)s * pstride];
         W[(size_t)a * n + i] *= (num / ssum[a]);
     }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W, const float* __restrict__ P,
+                                                          const float* __restrict__ Psum, int n, int r, int ldp,
+                                                          int S, long pstride, float eps) {
+    __shared__ float ssum[512];
+    update_w_kl_body(W, P, Psum, n, r, ldp, S, pstride, eps, blockIdx.x, ssum);
 }
 
 // Deterministic reduction of count doubles (fixed strided order, then a fixed tree).
@@ -258,17 +334,113 @@ __global__ __launch_bounds__(NTHREADS) void k_kl_update_w_wh(float* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Grouped updates (r <= 64 per problem) and per-problem objective reductions.
+__device__ __forceinline__ int group_find_u(const int* first, int J, int item) {
+    int lo = 0, hi = J - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+constexpr int kGroupSmemFloats = (64 + UPD_ROWS * 2) * 65;
+
+__global__ __launch_bounds__(NTHREADS) void k_group_update_h(const JobArgs* __restrict__ jobs, const int* table) {
+    __shared__ float sm[kGroupSmemFloats];
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find_u(first, J, item);
+        const JobArgs& ja = jobs[idx[j]];
+        const PassArgs& p = ja.h;
+        __syncthreads();
+        if (ja.rule == 0)
+            update_h_euc_body(ja.H, p.P, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, nullptr, 0, item - first[j], sm);
+        else
+            update_h_kl_body(ja.H, p.P, p.Psum, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm);
+    }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_group_update_w(const JobArgs* __restrict__ jobs, const int* table) {
+    __shared__ float sm[kGroupSmemFloats];
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find_u(first, J, item);
+        const JobArgs& ja = jobs[idx[j]];
+        const PassArgs& p = ja.w;
+        __syncthreads();
+        if (ja.rule == 0)
+            update_w_euc_body(ja.W, p.P, p.n, p.r, p.ldp, p.npad, p.S, p.pstride, p.eps, item - first[j], sm);
+        else
+            update_w_kl_body(ja.W, p.P, p.Psum, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm);
+    }
+}
+
+// One workgroup per problem: obj[slot] = sum of its H-pass objective partials (fixed order).
+__global__ __launch_bounds__(NTHREADS) void k_group_reduce_obj(const JobArgs* __restrict__ jobs, const int* table,
+                                                               int slot) {
+    __shared__ double red[NTHREADS];
+    const int J = table[0];
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int j = blockIdx.x; j < J; j += gridDim.x) {
+        const JobArgs& ja = jobs[idx[j]];
+        double v = 0.0;
+        for (int i = threadIdx.x; i < ja.nobj; i += NTHREADS) v += ja.h.objp[i];
+        __syncthreads();
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = NTHREADS / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) ja.obj[slot] = red[0];
+    }
+}
+
+hipError_t launch_group_update_h(const JobArgs* jobs, const int* table, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_update_h, dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_update_w(const JobArgs* jobs, const int* table, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_update_w, dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_reduce_obj(const JobArgs* jobs, const int* table, int slot, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_reduce_obj, dim3(grid), dim3(NTHREADS), 0, st, jobs, table, slot);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 static size_t upd_smem(int r, int rows) { return (size_t)(r * (r + 1) + rows * (r + 1)) * sizeof(float); }
 
 hipError_t launch_update_h_euc(float* H, const float* P, int m, int r, int ldr, int S, long pstride, float eps,
-                               hipStream_t st) {
+                               uint16_t* Hs, int ldhs, hipStream_t st) {
     const size_t sm = upd_smem(r, UPD_COLS);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_update_h_euc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_update_h_euc, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), sm, st, H, P, m, r, ldr,
-                       S, pstride, eps);
+                       S, pstride, eps, Hs, ldhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_w_euc_t(float* W, const float* P, int n, int r, int ldr, int S, long pstride, float eps,
+                                 uint16_t* Wb, int ldwb, hipStream_t st) {
+    const size_t sm = (size_t)(r * (r + 1) + 2 * UPD_ROWS * (r + 1)) * sizeof(float);
+    if (sm > 64 * 1024) {
+        hipError_t e =
+            hipFuncSetAttribute((const void*)k_update_w_euc_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_update_w_euc_t, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, n, r, ldr,
+                       S, pstride, eps, Wb, ldwb);
     return hipGetLastError();
 }
 

@@ -96,11 +96,20 @@ enum {
     GPUMLIB_NMF_OBJ_QUALITY = 3    /* reference quality, nmf_kernels_quality.cu:35 */
 };
 
+/* V store: the device form of V that gpumlib_nmf_iterate reads for a precision (no reference
+ * counterpart).  GPUMLIB_NMF_FP32: V itself (float, n x m column-major).  GPUMLIB_NMF_BF16: V and V^T in
+ * BF16 (round to nearest even), each column-major with a leading dimension rounded up to 8, V^T
+ * starting at a 256-byte boundary; the W pass then reads V^T with unit stride.  _pack builds it from
+ * an FP32 device V on `stream`.  Ranks, rules: BF16 storage runs the Euclidean rule. */
+size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
+hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
+                                   hipStream_t stream);
+
 /* Device workspace needed by gpumlib_nmf_iterate / gpumlib_nmf_objective for one problem. */
 size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r);
 
-/* Runs `iterations` iterations in place on W and H (device pointers).  V is the device matrix in the
- * storage format of `precision` (GPUMLIB_NMF_FP32: float n x m).  When objective != OBJ_NONE the
+/* Runs `iterations` iterations in place on W and H (device pointers, FP32 for every precision).  V is
+ * the V store of `precision` (GPUMLIB_NMF_FP32: the float n x m matrix itself).  When objective != OBJ_NONE the
  * objective of the state on entry is written, in FP64, to d_objective[0] (device memory); it is
  * fused into the first iteration's pass over V.  update_w = 0 updates H only (DoIteration(false)).
  * Ranks up to 128 are supported. */
@@ -115,7 +124,8 @@ hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, c
 
 /* Runs one contraction pass of the fused iteration alone (which = 0: the pass over the rows of V that
  * feeds the H update; which = 1: the pass over the columns of V that feeds the W update), writing only
- * workspace partials; W and H are unchanged.  For per-kernel timing and profiling. */
+ * workspace partials; W and H are unchanged.  For per-kernel timing and profiling.  With BF16 storage the
+ * pass reads the BF16 copies of W and H that the last gpumlib_nmf_iterate call left in `workspace`. */
 hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void *V, const float *W,
                             const float *H, float eps, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
@@ -142,7 +152,81 @@ hipError_t gpumlib_nmf_object_objective(gpumlib_nmf_handle h, int objective, dou
 void gpumlib_nmf_object_destroy(gpumlib_nmf_handle h);
 
 /* ------------------------------------------------------------------------ */
-/* 5. Seeded host-side input generators (no reference counterpart: the        */
+/* 5. Batches of independent problems (no reference counterpart: the          */
+/*    reference's only multi-problem workload, TrainUsingGroups in            */
+/*    src/examples/FacesNMF/FacesNMFDlg.cpp:582-656, runs its problems one    */
+/*    after another).  All registered problems (Euclidean or KL, FP32, r <=   */
+/*    64, each with its own V, W, H and workspace) advance together with six  */
+/*    grouped launches per iteration.  A problem's arithmetic depends only on */
+/*    its own shape, so its results are bitwise the same alone and in any     */
+/*    mix.  Objectives: the FP64 objective of the state on entry to           */
+/*    iteration 0 (and 1) is written to output slot 2*slot (and 2*slot+1).    */
+/* ------------------------------------------------------------------------ */
+typedef struct gpumlib_nmf_batch *gpumlib_nmf_batch_handle;
+
+hipError_t gpumlib_nmf_batch_create(int max_jobs, gpumlib_nmf_batch_handle *out); /* max_jobs <= 1024 */
+void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b);
+/* Work items per problem and pass that the K split aims for (default 32); applies to problems added later. */
+hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgroups_per_pass);
+size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r);
+hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r, const float *V, float *W,
+                                 float *H, float eps, void *workspace, size_t workspace_bytes, int *slot);
+hipError_t gpumlib_nmf_batch_remove(gpumlib_nmf_batch_handle b, int slot);
+int gpumlib_nmf_batch_active(gpumlib_nmf_batch_handle b);
+/* objectives in {0, 1, 2}, at most iterations. */
+hipError_t gpumlib_nmf_batch_iterate(gpumlib_nmf_batch_handle b, int iterations, int objectives, hipStream_t stream);
+/* The same work replayed from a hipGraph captured on the first call for (iterations, objectives);
+ * `stream` must not be the null stream.  The graph stays valid when problems are added or removed. */
+hipError_t gpumlib_nmf_batch_graph_launch(gpumlib_nmf_batch_handle b, int iterations, int objectives,
+                                          hipStream_t stream);
+/* Copies the 2 * max_jobs objective slots to the host (synchronises `stream`). */
+hipError_t gpumlib_nmf_batch_objectives(gpumlib_nmf_batch_handle b, double *host_out, hipStream_t stream);
+const double *gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b);
+
+/* ------------------------------------------------------------------------ */
+/* 6. Continuous-batching server (no reference counterpart).  Any host thread */
+/*    submits problems (Euclidean or KL, FP32, r <= 64; inputs are copied).   */
+/*    An engine thread admits a queued problem when a batch slot is free and  */
+/*    its HBM footprint (V, W, H, workspace) fits in the device pool, advances */
+/*    all admitted problems round_iterations iterations per round with the    */
+/*    grouped launches (optionally replayed from a hipGraph), and retires a   */
+/*    problem when its relative objective change over one iteration, measured */
+/*    each round, is below its tolerance or it reached max_iterations.        */
+/*    Times are seconds since the server was created.                          */
+/* ------------------------------------------------------------------------ */
+typedef struct gpumlib_nmf_server *gpumlib_nmf_server_handle;
+typedef struct {
+    size_t hbm_budget_bytes; /* device pool for resident problems */
+    int max_active;          /* batch slots, <= 1024 */
+    int round_iterations;    /* iterations per scheduling round */
+    int use_graph;           /* 1: replay each round from a captured hipGraph */
+    int target_workgroups;   /* K-split target per problem and pass, 0 = default */
+} gpumlib_nmf_server_config;
+typedef struct {
+    int iterations;
+    int converged;
+    double objective; /* FP64 objective after the last measured iteration */
+    double submit_s, admit_s, finish_s;
+} gpumlib_nmf_job_result;
+typedef struct {
+    long long submitted, completed, rounds, iterations_batched;
+    int peak_active;
+    size_t pool_bytes, pool_high_water;
+} gpumlib_nmf_server_stats;
+
+hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config *cfg, gpumlib_nmf_server_handle *out);
+hipError_t gpumlib_nmf_server_submit(gpumlib_nmf_server_handle s, int rule, int n, int m, int r, const float *V,
+                                     const float *W, const float *H, float eps, double tol, int max_iterations,
+                                     long long *job_id);
+/* Blocks until the problem is retired; copies W (n x r) and H (r x m) out when the pointers are non-null. */
+hipError_t gpumlib_nmf_server_wait(gpumlib_nmf_server_handle s, long long job_id, float *W_out, float *H_out,
+                                   gpumlib_nmf_job_result *result);
+hipError_t gpumlib_nmf_server_stats_get(gpumlib_nmf_server_handle s, gpumlib_nmf_server_stats *out);
+/* Finishes every submitted problem, then releases the server. */
+void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s);
+
+/* ------------------------------------------------------------------------ */
+/* 7. Seeded host-side input generators (no reference counterpart: the        */
 /*    reference draws W and H with rand(), nmf_base.h:56-61).  Counter-based  */
 /*    SplitMix64: value k of stream s under seed is                           */
 /*      mix(mix(seed ^ (s * 0xD1B54A32D192ED03)) + k)                         */

@@ -1,0 +1,84 @@
+"""The C-ABI library loads, exports every symbol the header declares, and rejects
+bad arguments before touching a device (CPU only: no compute calls)."""
+import os
+import re
+
+import pytest
+
+from gpumlib_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIP_INVALID_VALUE = 1
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "gpumlib_nmf.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(gpumlib_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = header_functions()
+    assert "gpumlib_nmf_iterate" in names and "gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix" in names
+    assert len(names) >= 30
+
+
+def test_every_declared_symbol_is_exported(native):
+    missing = [n for n in header_functions() if not hasattr(native, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert sorted(capi.SIGNATURES) == header_functions()
+
+
+def test_workspace_bytes(native):
+    ok = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, 16384, 8192, 64)
+    assert ok > 0 and ok < 64 << 20  # partials only: far below V itself (512 MiB)
+    assert native.gpumlib_nmf_workspace_bytes(capi.KL, capi.FP32, 1, 1, 1) > 0
+    for bad in [(capi.EUCLIDEAN, capi.FP32, 10, 10, 0), (capi.EUCLIDEAN, capi.FP32, 10, 10, 129),
+                (7, capi.FP32, 10, 10, 4), (capi.EUCLIDEAN, 9, 10, 10, 4), (capi.KL, capi.FP32, 0, 10, 4)]:
+        assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
+
+
+def test_iterate_rejects_bad_arguments(native):
+    f = native.gpumlib_nmf_iterate
+    ws = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, 64, 64, 8)
+    fake = 0x1000  # never dereferenced: validation fails first
+    # null V
+    assert f(0, 0, 64, 64, 8, None, fake, fake, 1e-8, 1, 1, 0, None, fake, ws, None) == HIP_INVALID_VALUE
+    # workspace too small
+    assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, ws - 1, None) == HIP_INVALID_VALUE
+    # objective requested without an output
+    assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, 1, 1, 1, None, fake, ws, None) == HIP_INVALID_VALUE
+    # unsupported rank
+    assert f(0, 0, 64, 64, 200, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, 1 << 30, None) == HIP_INVALID_VALUE
+    # negative iteration count
+    assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, -1, 1, 0, None, fake, ws, None) == HIP_INVALID_VALUE
+    # zero iterations, no objective: nothing to do
+    assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, 0, 1, 0, None, fake, ws, None) == 0
+
+
+def test_other_entry_points_reject_bad_arguments(native):
+    assert native.gpumlib_nmf_sgemm(b"Q", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_sgemm(b"N", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 2, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_quality(None, None, 10, 1e-8, None, 0, None, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_sum_h(None, 300, 10, 1e-8, None, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_mu_update(None, None, None, -1, 1e-8, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_objective(0, 0, 8, 8, 2, 0x10, 0x10, 0x10, 1e-8, 0, 0x10, 0x10, 1 << 20,
+                                        None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_pass(0, 0, 2, 8, 8, 2, 0x10, 0x10, 0x10, 1e-8, 0x10, 1 << 20, None) == HIP_INVALID_VALUE
+    import ctypes
+
+    h = ctypes.c_void_p()
+    assert native.gpumlib_nmf_object_create(0, 0, 0, 8, 2, None, None, None, ctypes.byref(h)) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_object_do_iteration(None, 1) == HIP_INVALID_VALUE
+    native.gpumlib_nmf_object_destroy(None)  # no-op
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(capi, "_lib", None)
+    monkeypatch.setattr(capi, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(capi.NativeLibraryMissing):
+        capi.load()

@@ -1,0 +1,227 @@
+"""Grouped (batched) execution and the continuous-batching server.
+
+* every problem of a mixed batch matches the FP64 oracle;
+* a problem's W and H are bitwise the same alone and in a mix;
+* hipGraph replays equal eager launches, replay after replay, with eager work
+  and synchronisations in between;
+* the server admits by HBM footprint, retires at convergence, and returns the
+  same bits for a problem alone and in a mix of concurrent problems.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from gpumlib_amd import capi
+from gpumlib_amd import nmf as G
+
+pytestmark = pytest.mark.gpu
+
+JOBS = [  # rule, n, m, r
+    (G.EUCLIDEAN, 300, 200, 16),
+    (G.KL, 250, 130, 8),
+    (G.EUCLIDEAN, 97, 61, 5),
+    (G.KL, 512, 384, 32),
+    (G.EUCLIDEAN, 1000, 700, 64),
+    (G.KL, 64, 2048, 64),
+]
+
+
+def make_job(k, rule, n, m, r):
+    V = G.gen_euclidean(n, m, r, 100 + k) if rule == G.EUCLIDEAN else G.gen_poisson(n, m, r, 100 + k)
+    W, H = G.gen_init(n, m, r, 200 + k)
+    return V, W, H
+
+
+class Batch:
+    def __init__(self, torch, lib, max_jobs=16):
+        self.torch, self.lib = torch, lib
+        h = ctypes.c_void_p()
+        capi.check(lib.gpumlib_nmf_batch_create(max_jobs, ctypes.byref(h)))
+        self.h = h
+        self.keep = []
+        self.stream = torch.cuda.Stream()
+
+    def add(self, rule, n, m, r, V, W, H, eps=G.REFERENCE_EPS):
+        t = self.torch
+        dV, dW, dH = (t.from_numpy(np.ascontiguousarray(x)).cuda() for x in (V, W, H))
+        wsb = int(self.lib.gpumlib_nmf_batch_workspace_bytes(self.h, rule, n, m, r))
+        ws = t.empty(wsb, dtype=t.uint8, device="cuda")
+        slot = ctypes.c_int()
+        capi.check(self.lib.gpumlib_nmf_batch_add(self.h, rule, n, m, r, dV.data_ptr(), dW.data_ptr(), dH.data_ptr(),
+                                                  eps, ws.data_ptr(), wsb, ctypes.byref(slot)))
+        self.keep.append((dV, dW, dH, ws))
+        return slot.value, dW, dH
+
+    def iterate(self, iters, objectives=0, graph=False):
+        fn = self.lib.gpumlib_nmf_batch_graph_launch if graph else self.lib.gpumlib_nmf_batch_iterate
+        with self.torch.cuda.stream(self.stream):
+            capi.check(fn(self.h, iters, objectives, ctypes.c_void_p(self.stream.cuda_stream)))
+        self.stream.synchronize()
+
+    def objectives(self, max_jobs=16):
+        out = (ctypes.c_double * (2 * max_jobs))()
+        capi.check(self.lib.gpumlib_nmf_batch_objectives(self.h, out, ctypes.c_void_p(self.stream.cuda_stream)))
+        return np.array(out)
+
+    def close(self):
+        self.torch.cuda.synchronize()
+        self.lib.gpumlib_nmf_batch_destroy(self.h)
+
+
+def test_batch_matches_oracle(native, orc, cuda):
+    b = Batch(cuda, native)
+    outs = []
+    for k, (rule, n, m, r) in enumerate(JOBS):
+        V, W, H = make_job(k, rule, n, m, r)
+        slot, dW, dH = b.add(rule, n, m, r, V, W, H)
+        outs.append((rule, n, m, r, V, W, H, dW, dH))
+    b.iterate(25)
+    for rule, n, m, r, V, W, H, dW, dH in outs:
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=25)
+        print(f"case rule={rule} n={n} m={m} r={r}", flush=True)
+        assert rel_fro(dW.cpu().numpy(), w) < 1e-4
+        assert rel_fro(dH.cpu().numpy(), h) < 1e-4
+    b.close()
+
+
+def test_alone_equals_mix(native, cuda):
+    k, (rule, n, m, r) = 3, JOBS[3]
+    V, W, H = make_job(k, rule, n, m, r)
+    b1 = Batch(cuda, native)
+    _, w1, h1 = b1.add(rule, n, m, r, V, W, H)
+    b1.iterate(15)
+    b2 = Batch(cuda, native)
+    for kk, job in enumerate(JOBS):
+        if kk == k:
+            _, w2, h2 = b2.add(rule, n, m, r, V, W, H)
+        else:
+            b2.add(*job, *make_job(kk, *job))
+    b2.iterate(15)
+    assert np.array_equal(w1.cpu().numpy(), w2.cpu().numpy())
+    assert np.array_equal(h1.cpu().numpy(), h2.cpu().numpy())
+    b1.close()
+    b2.close()
+
+
+def test_graph_replays_equal_eager(native, cuda):
+    eager, graphed = Batch(cuda, native), Batch(cuda, native)
+    pairs = []
+    for k, job in enumerate(JOBS[:4]):
+        V, W, H = make_job(k, *job)
+        _, we, he = eager.add(*job, V, W, H)
+        _, wg, hg = graphed.add(*job, V, W, H)
+        pairs.append((we, he, wg, hg))
+    for replay in range(4):
+        eager.iterate(5, objectives=2)
+        graphed.iterate(5, objectives=2, graph=True)
+        x = cuda.ones(1000, device="cuda") * replay  # eager work between replays
+        cuda.cuda.synchronize()
+        assert float(x.sum()) == 1000.0 * replay
+        for we, he, wg, hg in pairs:
+            assert np.array_equal(we.cpu().numpy(), wg.cpu().numpy()), replay
+            assert np.array_equal(he.cpu().numpy(), hg.cpu().numpy()), replay
+        assert np.array_equal(eager.objectives()[:8], graphed.objectives()[:8])
+    eager.close()
+    graphed.close()
+
+
+def test_batch_objectives_match_single(native, orc, cuda):
+    b = Batch(cuda, native)
+    slots = []
+    for k, job in enumerate(JOBS[:3]):
+        V, W, H = make_job(k, *job)
+        slots.append((b.add(*job, V, W, H)[0], job, V, W, H))
+    b.iterate(2, objectives=2)
+    obj = b.objectives()
+    for slot, (rule, n, m, r), V, W, H in slots:
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        f = orc.frob if rule == G.EUCLIDEAN else orc.kl_div
+        f0 = f(n, m, r, v, w, h)
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS)
+        f1 = f(n, m, r, v, w, h)
+        assert abs(obj[2 * slot] - f0) / f0 < 1e-5
+        assert abs(obj[2 * slot + 1] - f1) / f1 < 1e-5
+    b.close()
+
+
+# ------------------------------------------------------------------ server --
+class Server:
+    def __init__(self, lib, budget, max_active=64, round_iters=10, graph=1):
+        self.lib = lib
+        cfg = capi.ServerConfig(budget, max_active, round_iters, graph, 0)
+        h = ctypes.c_void_p()
+        capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, rule, n, m, r, V, W, H, tol=1e-4, max_iters=200):
+        jid = ctypes.c_longlong()
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        capi.check(self.lib.gpumlib_nmf_server_submit(self.h, rule, n, m, r, ptr(V), ptr(W), ptr(H), G.REFERENCE_EPS,
+                                                      tol, max_iters, ctypes.byref(jid)))
+        return jid.value
+
+    def wait(self, jid, n, m, r):
+        W = np.empty(n * r, np.float32)
+        H = np.empty(r * m, np.float32)
+        res = capi.JobResult()
+        capi.check(self.lib.gpumlib_nmf_server_wait(self.h, jid, W.ctypes.data_as(ctypes.c_void_p),
+                                                    H.ctypes.data_as(ctypes.c_void_p), ctypes.byref(res)))
+        return W, H, res
+
+    def stats(self):
+        st = capi.ServerStats()
+        capi.check(self.lib.gpumlib_nmf_server_stats_get(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        self.lib.gpumlib_nmf_server_destroy(self.h)
+
+
+def test_server_matches_oracle_and_retires(native, orc, cuda):
+    s = Server(native, budget=256 << 20)
+    ids = []
+    for k, job in enumerate(JOBS):
+        V, W, H = make_job(k, *job)
+        ids.append((s.submit(*job, V, W, H, tol=1e-4, max_iters=100), job, V, W, H))
+    for jid, (rule, n, m, r), V, W, H in ids:
+        gw, gh, res = s.wait(jid, n, m, r)
+        assert res.iterations % 10 == 0 and 10 <= res.iterations <= 100
+        assert res.converged or res.iterations == 100
+        assert res.finish_s >= res.admit_s >= res.submit_s
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS,
+                                                               iters=res.iterations)
+        assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
+    st = s.stats()
+    assert st.completed == len(JOBS)
+    s.close()
+
+
+def test_server_alone_equals_mix_and_admission(native, cuda):
+    target = (G.KL, 512, 384, 32)
+    V, W, H = make_job(3, *target)
+    s1 = Server(native, budget=64 << 20)
+    a = s1.wait(s1.submit(*target, V, W, H), 512, 384, 32)
+    s1.close()
+    # a pool that holds only a few problems at once: the rest wait for admission
+    rng = np.random.default_rng(5)
+    s2 = Server(native, budget=8 << 20, max_active=8)
+    others = []
+    for k in range(12):
+        rule = int(rng.integers(0, 2))
+        n, m, r = int(rng.integers(64, 400)), int(rng.integers(64, 400)), int(rng.choice([8, 16, 32]))
+        Vk, Wk, Hk = make_job(50 + k, rule, n, m, r)
+        others.append((s2.submit(rule, n, m, r, Vk, Wk, Hk), n, m, r))
+        if k == 5:
+            tid = s2.submit(*target, V, W, H)
+    b = s2.wait(tid, 512, 384, 32)
+    for jid, n, m, r in others:
+        s2.wait(jid, n, m, r)
+    st = s2.stats()
+    assert st.completed == 13 and st.pool_high_water <= 8 << 20 and st.peak_active <= 8
+    s2.close()
+    assert a[2].iterations == b[2].iterations
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

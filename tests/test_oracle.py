@@ -1,0 +1,249 @@
+"""Pins the CPU oracle (oracle/nmf_oracle.c, oracle/oracle.py) without the reference.
+
+The reference has no tests, golden vectors or fixtures for this path, and it
+cannot be built here (nvcc/cuBLAS absent; HostNMF.h does not compile with g++).
+So each oracle function is pinned by what the mathematics fixes: library
+routines (numpy matmul), closed forms, the Lee & Seung monotonicity theorem,
+fixed points, transpose duality, permutation equivariance, the rank-1 answer of
+the SVD, and an independent second implementation (numpy) of the same rules.
+Each check is chosen so that a plausible slip (a dropped eps, a transposed
+operand, the wrong update order, a missing clamp) fails it.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+
+
+def cm(x):
+    """2-D array -> flat column-major float64 (copy)."""
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).T).ravel()
+
+
+def mat(flat, n, m):
+    return flat.reshape(m, n).T
+
+
+def rand_problem(n, m, r, seed, exact=False):
+    rng = np.random.default_rng(seed)
+    W0, H0 = rng.random((n, r)), rng.random((r, m))
+    V = W0 @ H0 + (0.0 if exact else 0.01 * rng.random((n, m)))
+    W, H = rng.random((n, r)) + 0.1, rng.random((r, m)) + 0.1
+    return V, W, H, W0, H0
+
+
+def run(orc, rule, V, W, H, eps=1e-8, iters=1, update_w=True):
+    n, m = V.shape
+    r = W.shape[1]
+    v, w, h = cm(V), cm(W), cm(H)
+    (orc.euc_iter if rule == "euc" else orc.kl_iter)(n, m, r, v, w, h, eps, update_w=update_w, iters=iters)
+    return mat(w, n, r), mat(h, r, m)
+
+
+# ---------------------------------------------------------------- GEMM -----
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_gemm_matches_numpy(orc, ta, tb):
+    M, N, K = 7, 5, 9
+    rng = np.random.default_rng(1)
+    A = rng.random((M, K) if ta == "N" else (K, M))
+    B = rng.random((K, N) if tb == "N" else (N, K))
+    C = rng.random((M, N))
+    c = cm(C)
+    orc.gemm(ta, tb, M, N, K, 2.0, cm(A), A.shape[0], cm(B), B.shape[0], 0.5, c, M)
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    assert np.allclose(mat(c, M, N), 2.0 * opA @ opB + 0.5 * C, rtol=1e-13, atol=0)
+
+
+def test_gemm_beta_zero_ignores_c(orc):
+    c = np.full(4, np.nan)
+    orc.gemm("N", "N", 2, 2, 1, 1.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
+    assert np.all(c == 1.0)
+
+
+# ------------------------------------------------------- element-wise rule -
+def test_mu_update_closed_forms(orc):
+    eps = np.float32(1e-8)
+    m = np.array([1.0, 2.0, 3.0, 0.5], np.float32)
+    nm = np.array([2.0, 0.0, 4.0, 5.0], np.float32)
+    dm = np.array([2.0, 1.0, 0.0, 5.0], np.float32)
+    out = m.copy()
+    orc.lib().orc_mu_update_f32(orc._p32(nm), orc._p32(dm), orc._p32(out), 4, eps)
+    expect = m * (nm / (dm + eps))  # numpy float32: IEEE ops in the same order
+    assert np.array_equal(out, expect)
+    assert out[1] == 0.0                           # zero numerator
+    assert out[2] == np.float32(3.0) * (np.float32(4.0) / eps)  # dm = 0 -> m nm / eps
+    assert abs(out[0] - 1.0) < 1e-7 and abs(out[3] - 0.5) < 1e-7  # nm = dm -> ~m
+
+
+# ---------------------------------------------------------- Euclidean MU ---
+def test_euclidean_matches_numpy_restatement(orc):
+    V, W, H, _, _ = rand_problem(40, 30, 6, 2)
+    w1, h1 = run(orc, "euc", V, W, H, iters=3)
+    w2, h2 = W, H
+    for _ in range(3):
+        w2, h2 = orc.np_euc_iter(V, w2, h2, 1e-8)
+    assert rel_fro(w1, w2) < 1e-12 and rel_fro(h1, h2) < 1e-12
+
+
+def test_euclidean_monotone(orc):
+    V, W, H, _, _ = rand_problem(50, 40, 5, 3)
+    n, m = V.shape
+    prev = np.inf
+    w, h = W, H
+    for _ in range(60):
+        f = orc.frob(n, m, 5, cm(V), cm(w), cm(h))
+        assert f <= prev * (1 + 1e-12)
+        prev = f
+        w, h = run(orc, "euc", V, w, h)
+
+
+def test_euclidean_fixed_point(orc):
+    V, _, _, W0, H0 = rand_problem(30, 20, 4, 4, exact=True)
+    w, h = run(orc, "euc", V, W0, H0, iters=5)
+    assert rel_fro(w, W0) < 1e-7 and rel_fro(h, H0) < 1e-7
+
+
+def test_euclidean_transpose_duality(orc):
+    """U_W(V, W, H) == U_H(V^T, H^T, W^T)^T: catches swapped operands in either half."""
+    V, W, H, _, _ = rand_problem(25, 35, 4, 5)
+    w2, h2 = run(orc, "euc", V, W, H)                      # h2 = U_H(V,W,H), w2 = U_W(V,W,h2)
+    _, wt = run(orc, "euc", V.T, h2.T, W.T, update_w=False)  # U_H(V^T, h2^T, W^T)
+    assert rel_fro(wt.T, w2) < 1e-12
+
+
+def test_euclidean_update_order(orc):
+    """H is updated first from the old W, then W from the new H."""
+    V, W, H, _, _ = rand_problem(20, 15, 3, 6)
+    w2, h2 = run(orc, "euc", V, W, H)
+    _, h_only = run(orc, "euc", V, W, H, update_w=False)
+    assert np.array_equal(h2, h_only)
+    w_alt = W * ((V @ H.T) / (W @ (H @ H.T) + 1e-8))  # W from the OLD H: must differ
+    assert rel_fro(w2, w_alt) > 1e-6
+
+
+def test_euclidean_permutation_equivariance(orc):
+    V, W, H, _, _ = rand_problem(30, 22, 5, 7)
+    p = np.array([3, 0, 4, 1, 2])
+    w1, h1 = run(orc, "euc", V, W, H, iters=4)
+    w2, h2 = run(orc, "euc", V, W[:, p], H[p, :], iters=4)
+    assert rel_fro(w2, w1[:, p]) < 1e-12 and rel_fro(h2, h1[p, :]) < 1e-12
+
+
+def test_euclidean_rank1_converges_to_svd(orc):
+    """r = 1 on positive V: W H -> sigma_1 u_1 v_1^T (Perron-Frobenius + Eckart-Young)."""
+    rng = np.random.default_rng(8)
+    V = rng.random((12, 9)) + 0.5
+    w, h = run(orc, "euc", V, rng.random((12, 1)) + 0.1, rng.random((1, 9)) + 0.1, eps=0.0, iters=3000)
+    U, s, Vt = np.linalg.svd(V)
+    best = s[0] * np.outer(U[:, 0], Vt[0])
+    assert rel_fro(w @ h, best) < 1e-9
+
+
+def test_euclidean_f32_tracks_f64_c1(orc):
+    """FP32 restatement vs FP64 after 200 iterations at 256 x 128, r = 16."""
+    n, m, r = 256, 128, 16
+    V, W, H, _, _ = rand_problem(n, m, r, 9)
+    v32, w32, h32 = (cm(x).astype(np.float32) for x in (V, W, H))
+    v64, w64, h64 = (x.astype(np.float64) for x in (v32, w32, h32))
+    orc.euc_iter(n, m, r, v32, w32, h32, 1e-8, iters=200)
+    orc.euc_iter(n, m, r, v64, w64, h64, 1e-8, iters=200)
+    assert rel_fro(w32, w64) < 1e-4 and rel_fro(h32, h64) < 1e-4
+
+
+def test_euclidean_omp_baseline_matches(orc):
+    n, m, r = 300, 200, 12
+    V, W, H, _, _ = rand_problem(n, m, r, 10)
+    a = [cm(x).astype(np.float32) for x in (V, W, H)]
+    b = [x.copy() for x in a]
+    orc.euc_iter(n, m, r, *a, 1e-8, iters=10)
+    orc.euc_iter(n, m, r, *b, 1e-8, iters=10, omp=True)
+    assert rel_fro(a[1], b[1]) < 1e-5 and rel_fro(a[2], b[2]) < 1e-5
+
+
+# ---------------------------------------------------------------- KL MU ---
+def poisson_problem(n, m, r, seed):
+    rng = np.random.default_rng(seed)
+    W0, H0 = rng.exponential(size=(n, r)), rng.exponential(size=(r, m))
+    V = rng.poisson(W0 @ H0).astype(np.float64)
+    return V, rng.random((n, r)) + 0.1, rng.random((r, m)) + 0.1
+
+
+def test_kl_matches_numpy_restatement(orc):
+    V, W, H = poisson_problem(40, 30, 5, 11)
+    w1, h1 = run(orc, "kl", V, W, H, iters=3)
+    w2, h2 = W, H
+    for _ in range(3):
+        w2, h2 = orc.np_kl_iter(V, w2, h2, 1e-8)
+    assert rel_fro(w1, w2) < 1e-12 and rel_fro(h1, h2) < 1e-12
+
+
+def test_kl_monotone(orc):
+    V, W, H = poisson_problem(45, 35, 4, 12)
+    n, m = V.shape
+    prev = np.inf
+    w, h = W, H
+    for _ in range(60):
+        d = orc.kl_div(n, m, 4, cm(V), cm(w), cm(h))
+        assert d <= prev * (1 + 1e-12)
+        prev = d
+        w, h = run(orc, "kl", V, w, h)
+
+
+def test_kl_fixed_point(orc):
+    _, _, _, W0, H0 = rand_problem(25, 18, 3, 13, exact=True)
+    V = W0 @ H0
+    w, h = run(orc, "kl", V, W0, H0, iters=5)
+    assert rel_fro(w, W0) < 1e-7 and rel_fro(h, H0) < 1e-7
+
+
+def test_kl_transpose_duality(orc):
+    V, W, H = poisson_problem(28, 19, 3, 14)
+    w2, h2 = run(orc, "kl", V, W, H)
+    _, wt = run(orc, "kl", V.T, h2.T, W.T, update_w=False)
+    assert rel_fro(wt.T, w2) < 1e-12
+
+
+def test_kl_rank1_closed_form(orc):
+    """r = 1: one H update gives H(u) = colsum(V)(u) / sum(W); then W(i) = rowsum(V)(i) / sum(H)."""
+    V, _, _ = poisson_problem(16, 11, 2, 15)
+    V += 1.0
+    rng = np.random.default_rng(16)
+    W, H = rng.random((16, 1)) + 0.2, rng.random((1, 11)) + 0.2
+    w, h = run(orc, "kl", V, W, H, eps=0.0)
+    assert np.allclose(h[0], V.sum(axis=0) / W.sum(), rtol=1e-12)
+    assert np.allclose(w[:, 0], V.sum(axis=1) / h.sum(), rtol=1e-12)
+
+
+def test_kl_sums_are_clamped(orc):
+    """A zero column of W gives sumW = eps (nmf_kernels.cu:104), not 0 as in HostNMF."""
+    n, r = 10, 3
+    W = np.ones((n, r))
+    W[:, 1] = 0.0
+    out = np.zeros(r)
+    orc.lib().orc_sum_w_f64(n, r, orc._p64(cm(W)), 1e-8, orc._p64(out))
+    assert out[1] == 1e-8 and out[0] == n
+    H = np.ones((r, 7))
+    H[2] = 0.0
+    out = np.zeros(r)
+    orc.lib().orc_sum_h_f64(r, 7, orc._p64(cm(H)), 1e-8, orc._p64(out))
+    assert out[2] == 1e-8 and out[0] == 7
+
+
+# ------------------------------------------------------------ objectives ---
+def test_objectives_closed_forms(orc):
+    V = np.array([[1.0, 0.0], [2.0, 3.0]])
+    W = np.array([[1.0], [2.0]])
+    H = np.array([[1.0, 1.0]])
+    WH = W @ H  # [[1,1],[2,2]]
+    frob = orc.frob(2, 2, 1, cm(V), cm(W), cm(H))
+    assert frob == pytest.approx(0.0 + 1.0 + 0.0 + 1.0)
+    kl = orc.kl_div(2, 2, 1, cm(V), cm(W), cm(H))
+    # 0 ln 0 = 0 for V = 0: that entry contributes WH = 1
+    expect = (0 - 1 + 1) + (0 - 0 + 1) + (0 - 2 + 2) + (3 * np.log(3 / 2) - 3 + 2)
+    assert kl == pytest.approx(expect, rel=1e-14)
+    q = orc.quality(cm(V), cm(WH), 0.0)
+    assert q == pytest.approx(sum(V.ravel() * np.log10(WH.ravel()) - WH.ravel()), rel=1e-14)
+    assert orc.np_frob(V, W, H) == pytest.approx(frob)
+    assert orc.np_kl(V, W, H) == pytest.approx(kl)
