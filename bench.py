@@ -49,9 +49,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
-    from gpumlib_amd import capi
+    from gpumlib_amd import capi, hip
     from gpumlib_amd import nmf as G
 
     lib = capi.load()
@@ -78,41 +76,28 @@ def main():
     def leg(prec_name):
         prec = {"fp32": G.FP32, "bf16": G.BF16}[prec_name]
         d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
-        stream = d.stream()
         vbytes_el = 4 if prec == G.FP32 else 2
-        if prec != G.FP32:
-            vs_bytes = int(lib.gpumlib_nmf_vstore_bytes(prec, n, m))
-            store = torch.empty(vs_bytes, dtype=torch.uint8, device="cuda")
-            capi.check(lib.gpumlib_nmf_vstore_pack(prec, n, m, d.V.data_ptr(), store.data_ptr(), vs_bytes, stream))
-            d.V = store
-
-        def run(k):
-            capi.check(lib.gpumlib_nmf_iterate(rule, prec, n, m, r, d.V.data_ptr(), d.W.data_ptr(), d.H.data_ptr(),
-                                               eps, k, 1, G.OBJ_NONE, None, d.ws.data_ptr(), d.ws_bytes, stream))
-
         obj0 = d.objective(obj_kind)
-        run(args.warmup)
-        torch.cuda.synchronize()
+        d.launch(args.warmup)
+        d.stream.synchronize()
         t0 = time.perf_counter()
-        run(args.steps)
-        torch.cuda.synchronize()
+        d.launch(args.steps)
+        d.stream.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1e3 / args.steps
         obj1 = d.objective(obj_kind)
 
         # Per-kernel durations: HIP events on the stream the kernels are launched on.
         def time_pass(which):
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0, ev1 = hip.Event(), hip.Event()
             for _ in range(3):
-                capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
-                                                d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
-            ev0.record()
+                d.run_pass(which)
+            ev0.record(d.stream)
             for _ in range(args.kernel_reps):
-                capi.check(lib.gpumlib_nmf_pass(rule, prec, which, n, m, r, d.V.data_ptr(), d.W.data_ptr(),
-                                                d.H.data_ptr(), eps, d.ws.data_ptr(), d.ws_bytes, stream))
-            ev1.record()
+                d.run_pass(which)
+            ev1.record(d.stream)
             ev1.synchronize()
-            return ev0.elapsed_time(ev1) / args.kernel_reps
+            return ev0.elapsed_ms(ev1) / args.kernel_reps
 
         ms_h, ms_w = time_pass(0), time_pass(1)
         dom = ("pass_h", ms_h, flops_h) if ms_h >= ms_w else ("pass_w", ms_w, flops_w)

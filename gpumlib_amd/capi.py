@@ -89,6 +89,9 @@ SIGNATURES = {
     "gpumlib_nmf_gen_euclidean_v": (None, [_u64, _i, _i, _i, _f, _vp]),
     "gpumlib_nmf_gen_poisson_v": (None, [_u64, _i, _i, _i, _vp]),
     "gpumlib_nmf_gen_init": (None, [_u64, _i, _i, _i, _vp, _vp]),
+    "gpumlib_nmf_fp8_quantize": (None, [_vp, _sz, _i, _vp]),
+    "gpumlib_nmf_fp8_scale_for": (_i, [_f]),
+    "gpumlib_nmf_fp8_decode": (_f, [_c.c_uint8, _i]),
 }
 
 _lib = None

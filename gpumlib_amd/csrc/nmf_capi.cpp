@@ -36,18 +36,28 @@ static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 size_t vstore_bf16_offset_t(int n, int m) { return align256((size_t)m * round_up(n, 8) * 2); }
 
+Fp8Store fp8_store_layout(int n, int m) {
+    Fp8Store l;
+    l.off_t = align256((size_t)m * round_up(n, 16));
+    l.off_sc = l.off_t + align256((size_t)n * round_up(m, 16));
+    l.bytes = l.off_sc + align256((size_t)((n + 31) / 32) * ((m + 31) / 32));
+    return l;
+}
+
 size_t vstore_bytes(int precision, int n, int m) {
     if (n < 1 || m < 1) return 0;
     if (precision == GPUMLIB_NMF_FP32) return (size_t)n * m * 4;
     if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256((size_t)n * round_up(m, 8) * 2);
+    if (precision == GPUMLIB_NMF_FP8) return fp8_store_layout(n, m).bytes;
     return 0;
 }
 
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
     if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return false;
-    if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16) return false;
-    if (precision == GPUMLIB_NMF_BF16 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
+    if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8) return false;
+    if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
     if (n < 1 || m < 1 || r < 1 || r > kMaxFusedRank) return false;
+    if (precision == GPUMLIB_NMF_FP8 && r > 64) return false;
     memset(pl, 0, sizeof(*pl));
     pl->rule = rule;
     pl->precision = precision;
@@ -60,8 +70,12 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->ldp = pl->npad + pl->rpad;
     pl->ldv8 = round_up(n, 8);
     pl->ldt8 = round_up(m, 8);
+    pl->ld16n = round_up(n, 16);
+    pl->ld16m = round_up(m, 16);
+    pl->nbi = (n + 31) / 32;
+    pl->nbu = (m + 31) / 32;
     const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
-    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : 64;
+    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 64 : 128);
     pl->tiles1 = (m + TILE - 1) / TILE + euc;
     pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
@@ -87,6 +101,15 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         off += align256((size_t)r * pl->ldv8 * 2);
         pl->off_Hb = off;
         off += align256((size_t)r * pl->ldt8 * 2);
+    } else if (precision == GPUMLIB_NMF_FP8) {
+        pl->off_Wb = off;
+        off += align256((size_t)r * pl->ld16n);
+        pl->off_Hb = off;
+        off += align256((size_t)r * pl->ld16m);
+        pl->off_Wsc = off;
+        off += align256((size_t)r * pl->nbi);
+        pl->off_Hsc = off;
+        off += align256((size_t)r * pl->nbu);
     }
     pl->bytes = off;
     return true;
@@ -112,7 +135,11 @@ void pass_args(const Plan& pl, const float* V, const float* W, const float* H, v
     a->npad = pl.npad;
     a->eps = eps;
     a->vec = (pl.n % 4 == 0) && (pl.r % 4 == 0) && aligned16(V) && aligned16(W) && aligned16(H);
-    a->ldv = (pl.precision == GPUMLIB_NMF_FP32) ? pl.n : pl.ldv8;
+    a->ldv = (pl.precision == GPUMLIB_NMF_FP32) ? pl.n : (pl.precision == GPUMLIB_NMF_BF16 ? pl.ldv8 : pl.ld16n);
+    if (pl.precision == GPUMLIB_NMF_FP8) {
+        a->vsc = (const uint8_t*)V + fp8_store_layout(pl.n, pl.m).off_sc;
+        a->nbi = pl.nbi;
+    }
     if (which == 1) {
         a->kslice = pl.ks1;
         a->S = pl.S1;
@@ -156,8 +183,9 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
     pass_args(pl, (const float*)V, W, H, ws, eps, 1, &a1);
     a1.tiles = (pl.m + TILE - 1) / TILE;  // no gram tile
     if (pl.precision == GPUMLIB_NMF_BF16) {
-        a1.kslice = round_up(pl.ks1, KC);
         CK(launch_objective_bf16v(a1, objective, st));
+    } else if (pl.precision == GPUMLIB_NMF_FP8) {
+        CK(launch_objective_fp8v(a1, objective, st));
     } else {
         CK(launch_pass_h_f32(a1, 0, objective, 0, st));
     }
@@ -209,6 +237,59 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
         if (!update_w) continue;
         CK(launch_pass_tn_bf16(aw, st));
         CK(launch_update_w_euc_t(W, P, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st));
+    }
+    return hipSuccess;
+}
+
+static void fp8_args(const Plan& pl, const void* Vstore, void* ws, int which, Fp8PassArgs* a) {
+    char* base = (char*)ws;
+    const Fp8Store L = fp8_store_layout(pl.n, pl.m);
+    memset(a, 0, sizeof(*a));
+    a->P = (float*)(base + pl.off_P);
+    a->r = pl.r;
+    a->ldr = pl.ldr;
+    a->Bsc = (const uint8_t*)Vstore + L.off_sc;
+    a->bcol_shift = 5;
+    if (which == 0) {  // H pass: A = W (blocks along i), B = V (columns u, K = i)
+        a->A = (const uint8_t*)(base + pl.off_Wb);
+        a->Asc = (const uint8_t*)(base + pl.off_Wsc);
+        a->B = (const uint8_t*)Vstore;
+        a->K = pl.n, a->N = pl.m, a->lda = a->ldb = pl.ld16n, a->nbk = pl.nbi;
+        a->kslice = pl.ks1, a->S = pl.S1, a->tiles = pl.tiles1, a->pstride = pl.pstride1;
+        a->bs_col = pl.nbi, a->bs_k = 1;  // tile scale Vsc[ub * nbi + ib], column u -> ub
+    } else {  // W pass: A = H^T (blocks along u), B = V^T (columns i, K = u)
+        a->A = (const uint8_t*)(base + pl.off_Hb);
+        a->Asc = (const uint8_t*)(base + pl.off_Hsc);
+        a->B = (const uint8_t*)Vstore + L.off_t;
+        a->K = pl.m, a->N = pl.n, a->lda = a->ldb = pl.ld16m, a->nbk = pl.nbu;
+        a->kslice = pl.ks3, a->S = pl.S3, a->tiles = pl.tiles3, a->pstride = pl.pstride3;
+        a->bs_col = 1, a->bs_k = pl.nbi;  // column i -> ib, K block -> ub
+    }
+}
+
+hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                              int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
+    char* base = (char*)ws;
+    uint8_t* Wq = (uint8_t*)(base + pl.off_Wb);
+    uint8_t* Hq = (uint8_t*)(base + pl.off_Hb);
+    uint8_t* Wsc = (uint8_t*)(base + pl.off_Wsc);
+    uint8_t* Hsc = (uint8_t*)(base + pl.off_Hsc);
+    float* P = (float*)(base + pl.off_P);
+    Fp8PassArgs ah, aw;
+    fp8_args(pl, Vstore, ws, 0, &ah);
+    fp8_args(pl, Vstore, ws, 1, &aw);
+    CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
+    CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
+    for (int it = 0; it < iterations; it++) {
+        if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
+            CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
+        CK(launch_pass_tn_fp8(ah, st));
+        CK(launch_update_h_euc(H, P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
+        CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
+        if (!update_w) continue;
+        CK(launch_pass_tn_fp8(aw, st));
+        CK(launch_update_w_euc_t(W, P, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, nullptr, 0, st));
+        CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
     }
     return hipSuccess;
 }
@@ -305,6 +386,8 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
     }
     if (precision == GPUMLIB_NMF_BF16)
         return run_iterations_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
+    if (precision == GPUMLIB_NMF_FP8)
+        return run_iterations_fp8(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     return run_iterations(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective, workspace,
                           stream);
 }
@@ -327,6 +410,12 @@ hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float* V, 
     if (!V || !vstore || need == 0 || vstore_bytes_ < need) return hipErrorInvalidValue;
     if (precision == GPUMLIB_NMF_FP32)
         return hipMemcpyAsync(vstore, V, need, hipMemcpyDeviceToDevice, stream);
+    if (precision == GPUMLIB_NMF_FP8) {
+        const Fp8Store L = fp8_store_layout(n, m);
+        uint8_t* base = (uint8_t*)vstore;
+        return launch_pack_v_fp8(V, n, m, base, round_up(n, 16), base + L.off_t, round_up(m, 16), base + L.off_sc,
+                                 stream);
+    }
     uint16_t* Vb = (uint16_t*)vstore;
     uint16_t* Vt = (uint16_t*)((char*)vstore + vstore_bf16_offset_t(n, m));
     return launch_pack_v_bf16(V, n, m, Vb, round_up(n, 8), Vt, round_up(m, 8), stream);
@@ -339,6 +428,11 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
         return hipErrorInvalidValue;
     const int kl = (rule == GPUMLIB_NMF_KL);
+    if (precision == GPUMLIB_NMF_FP8) {
+        Fp8PassArgs a;
+        fp8_args(pl, V, workspace, which, &a);
+        return launch_pass_tn_fp8(a, stream);
+    }
     if (precision == GPUMLIB_NMF_BF16) {
         // one BF16 pass over the W/H shadows the last iterate call left in this workspace
         char* base = (char*)workspace;

@@ -83,6 +83,24 @@ struct PassArgs {
     float eps;
     int vec;          // V, W, H rows/columns are 16-byte aligned (n % 4 == 0 and r % 4 == 0)
     int ldv;          // leading dimension of V (n for FP32 V; a multiple of 8 for BF16 storage)
+    const uint8_t* vsc;  // FP8 storage: E8M0 scale of V's 32 x 32 tile (ub * nbi + ib)
+    int nbi;             // FP8 storage: 32-row blocks of V
+};
+
+// FP8 K-contiguous pass: as LpPassArgs, with E8M0 scales.  A's scale for column a
+// and K block kb is Asc[a * nbk + kb]; B's is Bsc[(col >> bcol_shift) * bs_col + kb * bs_k].
+struct Fp8PassArgs {
+    const uint8_t* A;
+    const uint8_t* B;
+    const uint8_t* Asc;
+    const uint8_t* Bsc;
+    float* P;
+    int K, N, r;
+    int lda, ldb, ldr;
+    int kslice, S, tiles;
+    long pstride;
+    int nbk;
+    int bcol_shift, bs_col, bs_k;
 };
 
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/gpumlib_nmf.h"
+#include "nmf_fp8.hpp"
 
 // Runs body(lo, hi) over [0, count) split into contiguous blocks on up to 16
 // host threads.  Every element is computed independently, so the split does
@@ -157,6 +158,16 @@ void gpumlib_nmf_gen_poisson_v(uint64_t seed, int n, int m, int r0, float* V) {
     });
     free(w0);
     free(h0);
+}
+
+void gpumlib_nmf_fp8_quantize(const float* x, size_t count, int scale, uint8_t* out) {
+    for (size_t k = 0; k < count; k++) out[k] = gpumlib_nmf::f32_to_e4m3(gpumlib_nmf::e8m0_unscale(x[k], scale));
+}
+
+int gpumlib_nmf_fp8_scale_for(float amax) { return gpumlib_nmf::e8m0_for_amax(amax); }
+
+float gpumlib_nmf_fp8_decode(uint8_t q, int scale) {
+    return gpumlib_nmf::e4m3_to_f32(q) * gpumlib_nmf::e8m0_value(scale);
 }
 
 void gpumlib_nmf_gen_init(uint64_t seed, int n, int m, int r, float* W, float* H) {

@@ -15,8 +15,15 @@ hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ld
                               hipStream_t st);
 hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
                               int ldh, hipStream_t st);
-// Objective of (W, H) against a BF16-stored V (nmf_pass_f32.hip).
+// FP8 storage path (nmf_pass_fp8.hip).
+hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st);
+hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv, uint8_t* Vtq, int ldt, uint8_t* Vsc,
+                             hipStream_t st);
+hipError_t launch_shadow_fp8(const float* X, int n, int m, int r, int which, uint8_t* Xq, int ldq, uint8_t* Xsc,
+                             hipStream_t st);
+// Objective of (W, H) against a BF16- or FP8-stored V (nmf_pass_f32.hip).
 hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st);
+hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
 
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)

@@ -20,6 +20,7 @@
 // with one barrier per step; the next step's global loads are in flight while
 // the current step's MFMAs run.
 #include "nmf_common.hpp"
+#include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
 
 namespace gpumlib_nmf {
@@ -116,6 +117,19 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
             if (gram) {
                 rb[j] = load4(p.W + (size_t)col * p.n + k0 + kq, col < ncols, k0 + kq, kend, vec);
+            } else if constexpr (sizeof(TV) == 1) {
+                // FP8 E4M3 with the E8M0 scale of V's 32 x 32 tile (k..k+3 share one tile)
+                const int kk = k0 + kq, u = col0 + col;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (col < ncols && kk < kend) {
+                    const uint8_t* q8 = reinterpret_cast<const uint8_t*>(Vt) + (size_t)u * p.ldv + kk;
+                    const float sc = e8m0_value(p.vsc[(size_t)(u >> 5) * p.nbi + (kk >> 5)]);
+                    v.x = e4m3_to_f32(q8[0]) * sc;
+                    if (kk + 1 < kend) v.y = e4m3_to_f32(q8[1]) * sc;
+                    if (kk + 2 < kend) v.z = e4m3_to_f32(q8[2]) * sc;
+                    if (kk + 3 < kend) v.w = e4m3_to_f32(q8[3]) * sc;
+                }
+                rb[j] = v;
             } else if constexpr (sizeof(TV) == 2) {
                 rb[j] = load4_bf16(reinterpret_cast<const uint16_t*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,
                                    col < ncols, k0 + kq, kend);
@@ -610,6 +624,29 @@ static hipError_t launch_obj_bf16v_mb(const PassArgs& a, int obj, hipStream_t st
         default: return hipErrorInvalidValue;
     }
 #undef LO
+}
+
+template <int MB>
+static hipError_t launch_obj_fp8v_mb(const PassArgs& a, int obj, hipStream_t st) {
+    const int grid = xcd_grid(a.tiles * a.S);
+#define LO(OBJV)                                                                                             \
+    hipLaunchKernelGGL((k_pass_h_f32<MB, false, OBJV, false, uint8_t>), dim3(grid), dim3(NTHREADS), 0, st, a); \
+    return hipGetLastError();
+    switch (obj) {
+        case OBJ_FROB: LO(OBJ_FROB)
+        case OBJ_KL: LO(OBJ_KL)
+        case OBJ_QUALITY: LO(OBJ_QUALITY)
+        default: return hipErrorInvalidValue;
+    }
+#undef LO
+}
+
+hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st) {
+    const int rp = round_up(a.r, 32);
+    if (rp <= 32) return launch_obj_fp8v_mb<1>(a, obj, st);
+    if (rp <= 64) return launch_obj_fp8v_mb<2>(a, obj, st);
+    if (rp <= 128) return launch_obj_fp8v_mb<4>(a, obj, st);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st) {

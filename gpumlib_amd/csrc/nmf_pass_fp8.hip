@@ -1,0 +1,252 @@
+// FP8 (OCP E4M3) storage path with block scales.
+//
+// V is quantised once with one E8M0 scale per 32 x 32 tile of V, shared by both
+// stored orientations (V and V^T), so the two passes see the same quantised
+// matrix.  The W and H operands are re-quantised after every update with one
+// scale per 32 consecutive elements along the contraction dimension.  Products
+// run on the scaled MFMA v_mfma_scale_f32_32x32x64_f8f6f4, which applies the
+// operand scales per 32-element K block and accumulates in FP32; updates, ratio
+// and eps guard stay FP32.  Like the BF16 path, both passes are K-contiguous
+// passes P[s](a, u) = sum_k A(k, a) B(k, u) (W half = H half of (V^T, H^T, W^T)).
+#include <hip/hip_runtime.h>
+
+#include "nmf_common.hpp"
+#include "nmf_fp8.hpp"
+#include "nmf_internal.hpp"
+
+namespace gpumlib_nmf {
+
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+constexpr int K8 = 128;        // K per pipeline step: 2 MFMAs of K = 64
+constexpr int LD8 = K8 + 16;   // LDS row pitch in bytes (36 words)
+
+__device__ __forceinline__ uint4 load16_u8(const uint8_t* p, bool ok, int k, int kend) {
+    if (!ok) return make_uint4(0, 0, 0, 0);
+    if (k + 15 < kend) return *reinterpret_cast<const uint4*>(p);
+    uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        if (k + j < kend) w[j >> 2] |= (uint32_t)p[j] << (8 * (j & 3));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+template <int MB>
+__global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
+    constexpr int RP = MB * 32;
+    __shared__ __attribute__((aligned(16))) uint8_t sA[2][RP][LD8];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[2][TILE][LD8];
+    __shared__ uint8_t sAs[2][RP][4];    // E8M0 scale per (row of A^T, 32-block of this step)
+    __shared__ uint8_t sBs[2][TILE][4];  // E8M0 scale per (column of B, 32-block of this step)
+
+    const int total = p.tiles * p.S;
+    const int w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= total) return;
+    const int s = w / p.tiles, t = w - s * p.tiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    const int vt = (p.N + TILE - 1) / TILE;
+    const bool gram = (t == vt);
+    const uint8_t* B = gram ? p.A : p.B;
+    const int ldb = gram ? p.lda : p.ldb;
+    const int col0 = gram ? 0 : t * TILE;
+    const int ncols = gram ? p.r : min(TILE, p.N - col0);
+    const int kbeg = s * p.kslice;
+    const int kend = min(p.K, kbeg + p.kslice);
+    const int nb = MB;
+
+    uint4 ra[MB], rb[4];
+    uint8_t rsa, rsb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
+            rb[j] = load16_u8(B + (size_t)(col0 + col) * ldb + k0 + kq, col < ncols, k0 + kq, kend);
+        }
+#pragma unroll
+        for (int j = 0; j < MB; j++) {
+            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
+            ra[j] = load16_u8(p.A + (size_t)a * p.lda + k0 + kq, a < p.r, k0 + kq, kend);
+        }
+        const int kb0 = k0 >> 5;
+        {
+            const int a = tid >> 2, kb = kb0 + (tid & 3);
+            rsa = (tid < RP * 4 && a < p.r && kb < p.nbk) ? p.Asc[(size_t)a * p.nbk + kb] : (uint8_t)127;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int e = tid + NTHREADS * jj, col = e >> 2, kb = kb0 + (e & 3);
+            uint8_t v = 127;
+            if (col < ncols && kb < p.nbk) {
+                v = gram ? p.Asc[(size_t)col * p.nbk + kb]
+                         : p.Bsc[(size_t)((col0 + col) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
+            }
+            rsb[jj] = v;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
+            *reinterpret_cast<uint4*>(&sB[buf][col][kq]) = rb[j];
+        }
+#pragma unroll
+        for (int j = 0; j < MB; j++) {
+            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
+            *reinterpret_cast<uint4*>(&sA[buf][a][kq]) = ra[j];
+        }
+        if (tid < RP * 4) sAs[buf][tid >> 2][tid & 3] = rsa;
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int e = tid + NTHREADS * jj;
+            sBs[buf][e >> 2][e & 3] = rsb[jj];
+        }
+    };
+
+    f32x16 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+
+    const int nchunks = (kend - kbeg + K8 - 1) / K8;
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * K8);
+        // lane (c, h) supplies the 32 values k = g*64 + h*32 + j of this step and their block scale
+#pragma unroll
+        for (int g = 0; g < K8 / 64; g++) {
+            const uint8_t* pb = &sB[buf][wave * 32 + c][g * 64 + h * 32];
+            const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 16);
+            const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+            const int sb = sBs[buf][wave * 32 + c][2 * g + h];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                if (mb >= nb) continue;
+                const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 32];
+                const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 16);
+                const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                                  (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                const int sa = sAs[buf][mb * 32 + c][2 * g + h];
+                acc[mb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc[mb], 0, 0, 0, sa, 0, sb);
+            }
+        }
+        if (ch + 1 < nchunks) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const int ul = wave * 32 + c;
+    if (ul < ncols) {
+        const int ucol = (gram ? p.N : col0) + ul;
+        float* dst = p.P + (size_t)s * p.pstride + (size_t)ucol * p.ldr;
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int a0 = mb * 32 + g * 8 + h * 4;
+                if (a0 < p.ldr) {
+                    float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
+                    *reinterpret_cast<float4*>(dst + a0) = o;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// V (n x m, FP32) -> Vq (column-major, ld ldv), Vtq (V^T column-major, ld ldt) and one
+// E8M0 scale per 32 x 32 tile: Vsc[ub * nbi + ib].  One workgroup per tile.
+__global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict__ V, int n, int m,
+                                                         uint8_t* __restrict__ Vq, int ldv, uint8_t* __restrict__ Vtq,
+                                                         int ldt, uint8_t* __restrict__ Vsc, int nbi) {
+    __shared__ float tile[32][33];
+    __shared__ float red[NTHREADS / 64];
+    __shared__ int sscale;
+    const int ib = blockIdx.x, ub = blockIdx.y;
+    const int i0 = ib * 32, u0 = ub * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    float amax = 0.f;
+    for (int uu = ty; uu < 32; uu += 8) {
+        const int i = i0 + tx, u = u0 + uu;
+        const float v = (i < n && u < m) ? V[(size_t)u * n + i] : 0.f;
+        tile[uu][tx] = v;
+        amax = fmaxf(amax, fabsf(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_down(amax, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = red[0];
+        for (int k = 1; k < NTHREADS / 64; k++) a = fmaxf(a, red[k]);
+        const int sc = e8m0_for_amax(a);
+        sscale = sc;
+        Vsc[(size_t)ub * nbi + ib] = (uint8_t)sc;
+    }
+    __syncthreads();
+    const int sc = sscale;
+    for (int uu = ty; uu < 32; uu += 8) {
+        const int i = i0 + tx, u = u0 + uu;
+        if (i < n && u < m) Vq[(size_t)u * ldv + i] = f32_to_e4m3(e8m0_unscale(tile[uu][tx], sc));
+    }
+    for (int ii = ty; ii < 32; ii += 8) {
+        const int u = u0 + tx, i = i0 + ii;
+        if (i < n && u < m) Vtq[(size_t)i * ldt + u] = f32_to_e4m3(e8m0_unscale(tile[tx][ii], sc));
+    }
+}
+
+// Operand shadows with one scale per 32 elements along the contraction dimension:
+//   which 0: W (n x r) -> Wq column-major (ld ldq), Wsc[a * nblk + ib]   (blocks along i)
+//   which 1: H (r x m) -> Hq = H^T column-major (ld ldq), Hsc[a * nblk + ub] (blocks along u)
+// One thread per (rank column a, block).
+__global__ __launch_bounds__(NTHREADS) void k_shadow_fp8(const float* __restrict__ X, int n, int m, int r, int which,
+                                                         uint8_t* __restrict__ Xq, int ldq, uint8_t* __restrict__ Xsc,
+                                                         int nblk) {
+    const int t = blockIdx.x * NTHREADS + threadIdx.x;
+    if (t >= r * nblk) return;
+    const int a = t / nblk, blk = t - a * nblk;
+    const int len = which == 0 ? n : m;
+    const int k0 = blk * 32, k1 = min(len, k0 + 32);
+    auto at = [&](int k) -> float { return which == 0 ? X[(size_t)a * n + k] : X[(size_t)k * r + a]; };
+    float amax = 0.f;
+    for (int k = k0; k < k1; k++) amax = fmaxf(amax, fabsf(at(k)));
+    const int sc = e8m0_for_amax(amax);
+    Xsc[t] = (uint8_t)sc;
+    for (int k = k0; k < k1; k++) Xq[(size_t)a * ldq + k] = f32_to_e4m3(e8m0_unscale(at(k), sc));
+}
+
+// ---------------------------------------------------------------------------
+template <int MB>
+static void launch_tn8(const Fp8PassArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_pass_tn_fp8<MB>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS), 0, st, a);
+}
+
+hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st) {
+    const int rp = round_up(a.r, 32);
+    if (rp <= 32)
+        launch_tn8<1>(a, st);
+    else if (rp <= 64)
+        launch_tn8<2>(a, st);
+    else
+        return hipErrorInvalidValue;  // two rank blocks at most on this path
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv, uint8_t* Vtq, int ldt, uint8_t* Vsc,
+                             hipStream_t st) {
+    const int nbi = (n + 31) / 32, nbu = (m + 31) / 32;
+    hipLaunchKernelGGL(k_pack_v_fp8, dim3(nbi, nbu), dim3(NTHREADS), 0, st, V, n, m, Vq, ldv, Vtq, ldt, Vsc, nbi);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow_fp8(const float* X, int n, int m, int r, int which, uint8_t* Xq, int ldq, uint8_t* Xsc,
+                             hipStream_t st) {
+    const int nblk = ((which == 0 ? n : m) + 31) / 32;
+    const int threads = r * nblk;
+    hipLaunchKernelGGL(k_shadow_fp8, dim3((threads + NTHREADS - 1) / NTHREADS), dim3(NTHREADS), 0, st, X, n, m, r,
+                       which, Xq, ldq, Xsc, nblk);
+    return hipGetLastError();
+}
+
+}  // namespace gpumlib_nmf

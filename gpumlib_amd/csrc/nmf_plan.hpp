@@ -23,8 +23,15 @@ struct Plan {
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;
     int nobjp;
-    size_t off_P, off_Psum, off_objp, off_Wb, off_Hb, bytes;
+    int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (bytes) and 32-blocks of n and m
+    size_t off_P, off_Psum, off_objp, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
 };
+
+// Byte layout of the FP8 V store: Vq (ld ld16n) | Vtq (ld ld16m) | tile scales (nbu x nbi).
+struct Fp8Store {
+    size_t off_t, off_sc, bytes;
+};
+Fp8Store fp8_store_layout(int n, int m);
 
 // Byte layout of the BF16 V store: V (column-major, ld ldv8) then V^T (column-major, ld ldt8).
 size_t vstore_bf16_offset_t(int n, int m);

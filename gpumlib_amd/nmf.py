@@ -1,6 +1,6 @@
 """Python harness over the C-ABI: seeded inputs on the host and NMF problems
-resident in HBM.  Device memory and streams come from PyTorch (plumbing only);
-all arithmetic runs in the native gfx950 library.
+resident in HBM.  Device memory and streams come from the library's own HIP
+runtime (gpumlib_amd.hip); all arithmetic runs in the native gfx950 library.
 
 Matrices are column-major, stored as flat float32 arrays: an n x m matrix X is
 held as X.T.ravel() (element (i, u) at index u*n + i), the layout of GPUMLib's
@@ -58,47 +58,69 @@ def gen_init(n, m, r, seed):
 
 
 class DeviceNMF:
-    """One NMF problem resident on the GPU, driven through gpumlib_nmf_iterate."""
+    """One NMF problem resident on the GPU, driven through gpumlib_nmf_iterate.
 
-    def __init__(self, rule, V, W, H, n, m, r, eps=REFERENCE_EPS, precision=FP32, device="cuda"):
-        import torch
+    precision BF16/FP8: V is packed into the V store of that precision on the device."""
 
-        self.torch = torch
+    def __init__(self, rule, V, W, H, n, m, r, eps=REFERENCE_EPS, precision=FP32, stream=None):
+        from . import hip
+
+        self.hip = hip
         self.lib = capi.load()
         self.rule, self.precision, self.n, self.m, self.r, self.eps = rule, precision, n, m, r, float(eps)
-        self.V = torch.from_numpy(np.ascontiguousarray(V, dtype=np.float32)).to(device)
-        self.W = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float32)).to(device)
-        self.H = torch.from_numpy(np.ascontiguousarray(H, dtype=np.float32)).to(device)
+        self.stream = stream or hip.Stream()
+        self.Vf = hip.Buffer.from_array(np.asarray(V, dtype=np.float32))
+        self.W = hip.Buffer.from_array(np.asarray(W, dtype=np.float32))
+        self.H = hip.Buffer.from_array(np.asarray(H, dtype=np.float32))
+        if precision == FP32:
+            self.V = self.Vf
+        else:
+            vsb = int(self.lib.gpumlib_nmf_vstore_bytes(precision, n, m))
+            if vsb == 0:
+                raise ValueError("unsupported precision")
+            self.V = hip.Buffer(vsb)
+            capi.check(self.lib.gpumlib_nmf_vstore_pack(precision, n, m, self.Vf.ptr, self.V.ptr, vsb,
+                                                        self.stream.handle), "gpumlib_nmf_vstore_pack")
+            self.stream.synchronize()
         self.ws_bytes = int(self.lib.gpumlib_nmf_workspace_bytes(rule, precision, n, m, r))
         if self.ws_bytes == 0:
             raise ValueError("unsupported problem shape / precision")
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
-        self.obj = torch.zeros(1, dtype=torch.float64, device=device)
+        self.ws = hip.Buffer(self.ws_bytes)
+        self.obj = hip.Buffer(8)
 
-    def stream(self):
-        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+    def launch(self, iterations, update_w=True, objective=OBJ_NONE):
+        """Enqueues iterations on the problem's stream (no synchronisation)."""
+        err = self.lib.gpumlib_nmf_iterate(
+            self.rule, self.precision, self.n, self.m, self.r, self.V.ptr, self.W.ptr, self.H.ptr, self.eps,
+            iterations, 1 if update_w else 0, objective, self.obj.ptr if objective else None, self.ws.ptr,
+            self.ws_bytes, self.stream.handle)
+        capi.check(err, "gpumlib_nmf_iterate")
 
     def iterate(self, iterations, update_w=True, objective=OBJ_NONE):
-        """Runs iterations; returns the objective of the entry state when requested (syncs)."""
-        err = self.lib.gpumlib_nmf_iterate(
-            self.rule, self.precision, self.n, self.m, self.r, self.V.data_ptr(), self.W.data_ptr(),
-            self.H.data_ptr(), self.eps, iterations, 1 if update_w else 0, objective,
-            self.obj.data_ptr() if objective else None, self.ws.data_ptr(), self.ws_bytes, self.stream())
-        capi.check(err, "gpumlib_nmf_iterate")
+        """Runs iterations; returns the objective of the entry state when requested."""
+        self.launch(iterations, update_w, objective)
+        self.stream.synchronize()
         if objective:
-            return float(self.obj.cpu()[0])
+            return float(self.obj.download(np.float64)[0])
         return None
+
+    def run_pass(self, which):
+        capi.check(self.lib.gpumlib_nmf_pass(self.rule, self.precision, which, self.n, self.m, self.r, self.V.ptr,
+                                             self.W.ptr, self.H.ptr, self.eps, self.ws.ptr, self.ws_bytes,
+                                             self.stream.handle), "gpumlib_nmf_pass")
 
     def objective(self, kind):
         err = self.lib.gpumlib_nmf_objective(
-            self.rule, self.precision, self.n, self.m, self.r, self.V.data_ptr(), self.W.data_ptr(),
-            self.H.data_ptr(), self.eps, kind, self.obj.data_ptr(), self.ws.data_ptr(), self.ws_bytes,
-            self.stream())
+            self.rule, self.precision, self.n, self.m, self.r, self.V.ptr, self.W.ptr, self.H.ptr, self.eps, kind,
+            self.obj.ptr, self.ws.ptr, self.ws_bytes, self.stream.handle)
         capi.check(err, "gpumlib_nmf_objective")
-        return float(self.obj.cpu()[0])
+        self.stream.synchronize()
+        return float(self.obj.download(np.float64)[0])
 
     def get_w(self):
-        return self.W.cpu().numpy().copy()
+        self.stream.synchronize()
+        return self.W.download(np.float32, self.n * self.r)
 
     def get_h(self):
-        return self.H.cpu().numpy().copy()
+        self.stream.synchronize()
+        return self.H.download(np.float32, self.r * self.m)

@@ -99,8 +99,11 @@ enum {
 /* V store: the device form of V that gpumlib_nmf_iterate reads for a precision (no reference
  * counterpart).  GPUMLIB_NMF_FP32: V itself (float, n x m column-major).  GPUMLIB_NMF_BF16: V and V^T in
  * BF16 (round to nearest even), each column-major with a leading dimension rounded up to 8, V^T
- * starting at a 256-byte boundary; the W pass then reads V^T with unit stride.  _pack builds it from
- * an FP32 device V on `stream`.  Ranks, rules: BF16 storage runs the Euclidean rule. */
+ * starting at a 256-byte boundary; the W pass then reads V^T with unit stride.  GPUMLIB_NMF_FP8: V and V^T
+ * in OCP E4M3 (leading dimensions rounded up to 16 bytes, 256-byte aligned parts) followed by one E8M0
+ * scale per 32 x 32 tile of V (scale of tile (i/32, u/32) at byte (u/32) * ceil(n/32) + i/32), shared by
+ * both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run
+ * the Euclidean rule; FP8 supports r <= 64. */
 size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
                                    hipStream_t stream);
@@ -240,6 +243,13 @@ void gpumlib_nmf_gen_euclidean_v(uint64_t seed, int n, int m, int r0, float nois
 void gpumlib_nmf_gen_poisson_v(uint64_t seed, int n, int m, int r0, float *V);
 /* Initial W (n x r) and H (r x m) ~ U[0,1)  (streams 4, 5). */
 void gpumlib_nmf_gen_init(uint64_t seed, int n, int m, int r, float *W, float *H);
+
+/* Host form of the FP8 storage conversion (the device code uses the same routines): OCP E4M3 of
+ * x / 2^(scale-127), round to nearest even, saturating at 448; the E8M0 scale for a block maximum;
+ * and the decoded value q * 2^(scale-127). */
+void gpumlib_nmf_fp8_quantize(const float *x, size_t count, int scale, uint8_t *out);
+int gpumlib_nmf_fp8_scale_for(float amax);
+float gpumlib_nmf_fp8_decode(uint8_t q, int scale);
 
 #ifdef __cplusplus
 }

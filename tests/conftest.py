@@ -34,11 +34,12 @@ def orc():
 
 
 @pytest.fixture(scope="session")
-def cuda():
-    import torch
+def cuda(native):
+    """The library's HIP runtime (gpumlib_amd.hip); fails loudly without a GPU."""
+    from gpumlib_amd import hip
 
-    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
-    return torch
+    assert hip.device_count() > 0, "GPU tests need a visible MI355X"
+    return hip
 
 
 def rel_fro(a, b):

@@ -35,38 +35,36 @@ def make_job(k, rule, n, m, r):
 
 
 class Batch:
-    def __init__(self, torch, lib, max_jobs=16):
-        self.torch, self.lib = torch, lib
+    def __init__(self, hip, lib, max_jobs=16):
+        self.hip, self.lib = hip, lib
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_batch_create(max_jobs, ctypes.byref(h)))
         self.h = h
         self.keep = []
-        self.stream = torch.cuda.Stream()
+        self.stream = hip.Stream()
 
     def add(self, rule, n, m, r, V, W, H, eps=G.REFERENCE_EPS):
-        t = self.torch
-        dV, dW, dH = (t.from_numpy(np.ascontiguousarray(x)).cuda() for x in (V, W, H))
+        dV, dW, dH = (self.hip.Buffer.from_array(np.ascontiguousarray(x)) for x in (V, W, H))
         wsb = int(self.lib.gpumlib_nmf_batch_workspace_bytes(self.h, rule, n, m, r))
-        ws = t.empty(wsb, dtype=t.uint8, device="cuda")
+        ws = self.hip.Buffer(wsb)
         slot = ctypes.c_int()
-        capi.check(self.lib.gpumlib_nmf_batch_add(self.h, rule, n, m, r, dV.data_ptr(), dW.data_ptr(), dH.data_ptr(),
-                                                  eps, ws.data_ptr(), wsb, ctypes.byref(slot)))
+        capi.check(self.lib.gpumlib_nmf_batch_add(self.h, rule, n, m, r, dV.ptr, dW.ptr, dH.ptr, eps, ws.ptr, wsb,
+                                                  ctypes.byref(slot)))
         self.keep.append((dV, dW, dH, ws))
         return slot.value, dW, dH
 
     def iterate(self, iters, objectives=0, graph=False):
         fn = self.lib.gpumlib_nmf_batch_graph_launch if graph else self.lib.gpumlib_nmf_batch_iterate
-        with self.torch.cuda.stream(self.stream):
-            capi.check(fn(self.h, iters, objectives, ctypes.c_void_p(self.stream.cuda_stream)))
+        capi.check(fn(self.h, iters, objectives, self.stream.handle))
         self.stream.synchronize()
 
     def objectives(self, max_jobs=16):
         out = (ctypes.c_double * (2 * max_jobs))()
-        capi.check(self.lib.gpumlib_nmf_batch_objectives(self.h, out, ctypes.c_void_p(self.stream.cuda_stream)))
+        capi.check(self.lib.gpumlib_nmf_batch_objectives(self.h, out, self.stream.handle))
         return np.array(out)
 
     def close(self):
-        self.torch.cuda.synchronize()
+        self.hip.synchronize()
         self.lib.gpumlib_nmf_batch_destroy(self.h)
 
 
@@ -82,8 +80,8 @@ def test_batch_matches_oracle(native, orc, cuda):
         v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
         (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=25)
         print(f"case rule={rule} n={n} m={m} r={r}", flush=True)
-        assert rel_fro(dW.cpu().numpy(), w) < 1e-4
-        assert rel_fro(dH.cpu().numpy(), h) < 1e-4
+        assert rel_fro(dW.download(np.float32), w) < 1e-4
+        assert rel_fro(dH.download(np.float32), h) < 1e-4
     b.close()
 
 
@@ -100,8 +98,8 @@ def test_alone_equals_mix(native, cuda):
         else:
             b2.add(*job, *make_job(kk, *job))
     b2.iterate(15)
-    assert np.array_equal(w1.cpu().numpy(), w2.cpu().numpy())
-    assert np.array_equal(h1.cpu().numpy(), h2.cpu().numpy())
+    assert np.array_equal(w1.download(np.float32), w2.download(np.float32))
+    assert np.array_equal(h1.download(np.float32), h2.download(np.float32))
     b1.close()
     b2.close()
 
@@ -117,12 +115,15 @@ def test_graph_replays_equal_eager(native, cuda):
     for replay in range(4):
         eager.iterate(5, objectives=2)
         graphed.iterate(5, objectives=2, graph=True)
-        x = cuda.ones(1000, device="cuda") * replay  # eager work between replays
-        cuda.cuda.synchronize()
-        assert float(x.sum()) == 1000.0 * replay
+        # eager work and synchronisations between replays
+        x = cuda.Buffer.from_array(np.full(1000, replay, np.float32))
+        y = cuda.Buffer(4000)
+        y.copy_from(x)
+        cuda.synchronize()
+        assert float(y.download(np.float32).sum()) == 1000.0 * replay
         for we, he, wg, hg in pairs:
-            assert np.array_equal(we.cpu().numpy(), wg.cpu().numpy()), replay
-            assert np.array_equal(he.cpu().numpy(), hg.cpu().numpy()), replay
+            assert np.array_equal(we.download(np.float32), wg.download(np.float32)), replay
+            assert np.array_equal(he.download(np.float32), hg.download(np.float32)), replay
         assert np.array_equal(eager.objectives()[:8], graphed.objectives()[:8])
     eager.close()
     graphed.close()

@@ -11,12 +11,24 @@ from gpumlib_amd import nmf as G
 pytestmark = pytest.mark.gpu
 
 
-def dev(torch, a):
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+def dev(hip, a):
+    return hip.Buffer.from_array(np.ascontiguousarray(a))
 
 
-def stream(torch):
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+_STREAM = None
+
+
+def stream(hip):
+    global _STREAM
+    if _STREAM is None:
+        _STREAM = hip.Stream()
+    return _STREAM.handle
+
+
+def get(buf, dtype=np.float32):
+    if _STREAM is not None:
+        _STREAM.synchronize()
+    return buf.download(dtype)
 
 
 @pytest.mark.parametrize("ta", ["N", "T"])
@@ -32,11 +44,11 @@ def test_sgemm(native, orc, cuda, ta, tb, mnk):
     C = rng.random(M * N, dtype=np.float32)
     alpha, beta = 1.5, 0.25
     dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
-    capi.check(native.gpumlib_nmf_sgemm(ta.encode(), tb.encode(), M, N, K, alpha, dA.data_ptr(), Ar,
-                                        dB.data_ptr(), Br, beta, dC.data_ptr(), M, stream(cuda)))
+    capi.check(native.gpumlib_nmf_sgemm(ta.encode(), tb.encode(), M, N, K, alpha, dA.ptr, Ar,
+                                        dB.ptr, Br, beta, dC.ptr, M, stream(cuda)))
     ref = C.astype(np.float64)
     orc.gemm(ta, tb, M, N, K, alpha, A.astype(np.float64), Ar, B.astype(np.float64), Br, beta, ref, M)
-    assert rel_fro(dC.cpu().numpy(), ref) < 1e-5
+    assert rel_fro(get(dC), ref) < 1e-5
 
 
 def test_sgemm_beta_zero_ignores_c(native, cuda):
@@ -46,9 +58,9 @@ def test_sgemm_beta_zero_ignores_c(native, cuda):
     B = np.ones(K * N, np.float32)
     C = np.full(M * N, np.nan, np.float32)
     dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
-    capi.check(native.gpumlib_nmf_sgemm(b"N", b"N", M, N, K, 1.0, dA.data_ptr(), M, dB.data_ptr(), K, 0.0,
-                                        dC.data_ptr(), M, stream(cuda)))
-    assert np.all(dC.cpu().numpy() == K)
+    capi.check(native.gpumlib_nmf_sgemm(b"N", b"N", M, N, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0,
+                                        dC.ptr, M, stream(cuda)))
+    assert np.all(get(dC) == K)
 
 
 def test_sgemm_rejects_bad_flags(native, cuda):
@@ -64,15 +76,15 @@ def test_mu_update_bit_exact(native, orc, cuda):
     ref = mm.copy()
     orc.lib().orc_mu_update_f32(orc._p32(nm), orc._p32(dm), orc._p32(ref), E, np.float32(1e-8))
     dnm, ddm, dmm = dev(cuda, nm), dev(cuda, dm), dev(cuda, mm)
-    capi.check(native.gpumlib_nmf_mu_update(dnm.data_ptr(), ddm.data_ptr(), dmm.data_ptr(), E, 1e-8, stream(cuda)))
-    assert np.array_equal(dmm.cpu().numpy(), ref)
+    capi.check(native.gpumlib_nmf_mu_update(dnm.ptr, ddm.ptr, dmm.ptr, E, 1e-8, stream(cuda)))
+    assert np.array_equal(get(dmm), ref)
     # the reference-named launcher (grid/block ignored, eps = 1e-8f, null stream)
     dmm2 = dev(cuda, mm)
-    cuda.cuda.synchronize()
-    capi.check(native.gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix(1, 512, dnm.data_ptr(), ddm.data_ptr(),
-                                                                              dmm2.data_ptr(), E))
-    cuda.cuda.synchronize()
-    assert np.array_equal(dmm2.cpu().numpy(), ref)
+    cuda.synchronize()
+    capi.check(native.gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix(1, 512, dnm.ptr, ddm.ptr,
+                                                                              dmm2.ptr, E))
+    cuda.synchronize()
+    assert np.array_equal(get(dmm2), ref)
 
 
 def test_sums_clamped(native, orc, cuda):
@@ -87,13 +99,13 @@ def test_sums_clamped(native, orc, cuda):
     orc.lib().orc_sum_w_f64(n, r, orc._p64(W.astype(np.float64)), 1e-8, orc._p64(sw))
     orc.lib().orc_sum_h_f64(r, m, orc._p64(H.astype(np.float64)), 1e-8, orc._p64(sh))
     dW, dH = dev(cuda, W), dev(cuda, H)
-    osw = cuda.zeros(r, dtype=cuda.float32, device="cuda")
-    osh = cuda.zeros(r, dtype=cuda.float32, device="cuda")
-    capi.check(native.gpumlib_nmf_sum_w(dW.data_ptr(), n, r, 1e-8, osw.data_ptr(), stream(cuda)))
-    capi.check(native.gpumlib_nmf_sum_h(dH.data_ptr(), r, m, 1e-8, osh.data_ptr(), stream(cuda)))
-    assert np.allclose(osw.cpu().numpy(), sw, rtol=1e-5)
-    assert np.allclose(osh.cpu().numpy(), sh, rtol=1e-5)
-    assert osw.cpu().numpy()[3] == np.float32(1e-8) and osh.cpu().numpy()[5] == np.float32(1e-8)
+    osw = cuda.Buffer(4 * r)
+    osh = cuda.Buffer(4 * r)
+    capi.check(native.gpumlib_nmf_sum_w(dW.ptr, n, r, 1e-8, osw.ptr, stream(cuda)))
+    capi.check(native.gpumlib_nmf_sum_h(dH.ptr, r, m, 1e-8, osh.ptr, stream(cuda)))
+    assert np.allclose(get(osw), sw, rtol=1e-5)
+    assert np.allclose(get(osh), sh, rtol=1e-5)
+    assert get(osw)[3] == np.float32(1e-8) and get(osh)[5] == np.float32(1e-8)
 
 
 def test_quality(native, orc, cuda):
@@ -105,11 +117,11 @@ def test_quality(native, orc, cuda):
     ref = orc.quality(V.astype(np.float64), WH.astype(np.float64), 1e-8)
     dV, dWH = dev(cuda, V), dev(cuda, WH)
     wsb = int(native.gpumlib_nmf_quality_workspace_bytes(n))
-    ws = cuda.empty(wsb, dtype=cuda.uint8, device="cuda")
-    out = cuda.zeros(1, dtype=cuda.float64, device="cuda")
-    capi.check(native.gpumlib_nmf_quality(dV.data_ptr(), dWH.data_ptr(), n, 1e-8, ws.data_ptr(), wsb,
-                                          out.data_ptr(), stream(cuda)))
-    assert abs(float(out.cpu()[0]) - ref) / abs(ref) < 1e-6
+    ws = cuda.Buffer(wsb)
+    out = cuda.Buffer(8)
+    capi.check(native.gpumlib_nmf_quality(dV.ptr, dWH.ptr, n, 1e-8, ws.ptr, wsb,
+                                          out.ptr, stream(cuda)))
+    assert abs(float(get(out, np.float64)[0]) - ref) / abs(ref) < 1e-6
 
 
 @pytest.mark.parametrize("nmr", [(96, 80, 16), (257, 131, 37)])
@@ -133,20 +145,20 @@ def test_kl_updates_with_materialised_wh(native, orc, cuda, nmr):
     dsw, dsh = dev(cuda, sw), dev(cuda, sh)
     dH2 = dev(cuda, H)
     dW2 = dev(cuda, W)
-    capi.check(native.gpumlib_nmf_kl_update_h(dH2.data_ptr(), dW.data_ptr(), dV.data_ptr(), dWH.data_ptr(),
-                                              dsw.data_ptr(), n, m, r, 1e-8, stream(cuda)))
-    capi.check(native.gpumlib_nmf_kl_update_w(dW2.data_ptr(), dH.data_ptr(), dV.data_ptr(), dWH.data_ptr(),
-                                              dsh.data_ptr(), n, m, r, 1e-8, stream(cuda)))
-    assert rel_fro(dH2.cpu().numpy(), h_ref) < 1e-5
-    assert rel_fro(dW2.cpu().numpy(), w_ref) < 1e-5
+    capi.check(native.gpumlib_nmf_kl_update_h(dH2.ptr, dW.ptr, dV.ptr, dWH.ptr,
+                                              dsw.ptr, n, m, r, 1e-8, stream(cuda)))
+    capi.check(native.gpumlib_nmf_kl_update_w(dW2.ptr, dH.ptr, dV.ptr, dWH.ptr,
+                                              dsh.ptr, n, m, r, 1e-8, stream(cuda)))
+    assert rel_fro(get(dH2), h_ref) < 1e-5
+    assert rel_fro(get(dW2), w_ref) < 1e-5
     # reference-named launchers (dim3 arguments ignored; null stream)
     g = capi.Dim3(1, 1, 1)
     dH3, dW3 = dev(cuda, H), dev(cuda, W)
-    cuda.cuda.synchronize()
+    cuda.synchronize()
     capi.check(native.gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(
-        g, g, dH3.data_ptr(), dW.data_ptr(), dV.data_ptr(), dWH.data_ptr(), dsw.data_ptr(), n, m, r))
+        g, g, dH3.ptr, dW.ptr, dV.ptr, dWH.ptr, dsw.ptr, n, m, r))
     capi.check(native.gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w(
-        g, g, dW3.data_ptr(), dH.data_ptr(), dV.data_ptr(), dWH.data_ptr(), dsh.data_ptr(), n, m, r))
-    cuda.cuda.synchronize()
-    assert np.array_equal(dH3.cpu().numpy(), dH2.cpu().numpy())
-    assert np.array_equal(dW3.cpu().numpy(), dW2.cpu().numpy())
+        g, g, dW3.ptr, dH.ptr, dV.ptr, dWH.ptr, dsh.ptr, n, m, r))
+    cuda.synchronize()
+    assert np.array_equal(get(dH3), get(dH2))
+    assert np.array_equal(get(dW3), get(dW2))
