@@ -82,8 +82,8 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
                 case T_PH_KL: items = sl.rule == GPUMLIB_NMF_KL ? sl.args.h.tiles * sl.args.h.S : 0; break;
                 case T_PW_EUC: items = sl.rule == GPUMLIB_NMF_EUCLIDEAN ? sl.args.w.tiles * sl.args.w.S : 0; break;
                 case T_PW_KL: items = sl.rule == GPUMLIB_NMF_KL ? sl.args.w.tiles * sl.args.w.S : 0; break;
-                case T_UH: items = (sl.m + 63) / 64; break;
-                case T_UW: items = (sl.n + 63) / 64; break;
+                case T_UH: items = (sl.m + UPD_COLS - 1) / UPD_COLS; break;
+                case T_UW: items = (sl.n + UPD_ROWS - 1) / UPD_ROWS; break;
                 case T_RED: items = 1; break;
             }
             if (items == 0) continue;
@@ -195,6 +195,12 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     memset(&sl.args, 0, sizeof(sl.args));
     pass_args(pl, V, W, H, workspace, eps, 1, &sl.args.h);
     pass_args(pl, V, W, H, workspace, eps, 3, &sl.args.w);
+    // gram arrival counters start at zero (outside any captured graph)
+    const hipError_t ze = hipMemset(sl.args.h.gcount, 0, 2 * sizeof(int));
+    if (ze != hipSuccess) {
+        sl.active = false;
+        return ze;
+    }
     sl.args.W = W;
     sl.args.H = H;
     sl.args.obj = b->d_obj + 2 * (size_t)s;

@@ -96,6 +96,10 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     off += sbytes;
     pl->off_objp = off;
     off += obytes;
+    pl->off_G = off;  // reduced grams of the two passes
+    off += align256(sizeof(float) * 2 * (size_t)r * pl->rpad);
+    pl->off_cnt = off;  // their arrival counters
+    off += 256;
     if (precision == GPUMLIB_NMF_BF16) {
         pl->off_Wb = off;
         off += align256((size_t)r * pl->ldv8 * 2);
@@ -140,6 +144,8 @@ void pass_args(const Plan& pl, const float* V, const float* W, const float* H, v
         a->vsc = (const uint8_t*)V + fp8_store_layout(pl.n, pl.m).off_sc;
         a->nbi = pl.nbi;
     }
+    a->G = (float*)(base + pl.off_G) + (which == 1 ? 0 : (size_t)pl.r * pl.rpad);
+    a->gcount = (int*)(base + pl.off_cnt) + (which == 1 ? 0 : 1);
     if (which == 1) {
         a->kslice = pl.ks1;
         a->S = pl.S1;
@@ -159,6 +165,7 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
     PassArgs a1, a3;
     pass_args(pl, V, W, H, ws, eps, 1, &a1);
     pass_args(pl, V, W, H, ws, eps, 3, &a3);
+    CK(launch_zero_i32(a1.gcount, 2, st));
     for (int it = 0; it < iterations; it++) {
         const int obj = (it == 0 && d_objective) ? objective : GPUMLIB_NMF_OBJ_NONE;
         CK(launch_pass_h_f32(a1, kl, obj, 1, st));
@@ -166,13 +173,13 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
         if (kl)
             CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
         else
-            CK(launch_update_h_euc(H, a1.P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
+            CK(launch_update_h_euc(H, a1.P, a1.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
         if (!update_w) continue;
         CK(launch_pass_w_f32(a3, kl, st));
         if (kl)
             CK(launch_update_w_kl(W, a3.P, a3.Psum, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st));
         else
-            CK(launch_update_w_euc(W, a3.P, pl.n, pl.r, pl.ldp, pl.npad, pl.S3, pl.pstride3, eps, st));
+            CK(launch_update_w_euc(W, a3.P, a3.G, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st));
     }
     return hipSuccess;
 }
@@ -203,8 +210,13 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     uint16_t* Hb = (uint16_t*)(base + pl.off_Hb);
     float* P = (float*)(base + pl.off_P);
     CK(launch_shadow_bf16(W, H, pl.n, pl.m, pl.r, Wb, pl.ldv8, Hb, pl.ldt8, st));
+    float* G = (float*)(base + pl.off_G);
+    int* cnt = (int*)(base + pl.off_cnt);
+    CK(launch_zero_i32(cnt, 2, st));
     LpPassArgs ah, aw;
     memset(&ah, 0, sizeof(ah));
+    ah.G = G;
+    ah.gcount = cnt;
     ah.A = Wb;
     ah.B = Vb;
     ah.P = P;
@@ -219,6 +231,8 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     ah.tiles = pl.tiles1;
     ah.pstride = pl.pstride1;
     aw = ah;
+    aw.G = G + (size_t)pl.r * pl.rpad;
+    aw.gcount = cnt + 1;
     aw.A = Hb;
     aw.B = Vt;
     aw.K = pl.m;
@@ -233,10 +247,10 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
         if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
             CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
         CK(launch_pass_tn_bf16(ah, st));
-        CK(launch_update_h_euc(H, P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hb, pl.ldt8, st));
+        CK(launch_update_h_euc(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hb, pl.ldt8, st));
         if (!update_w) continue;
         CK(launch_pass_tn_bf16(aw, st));
-        CK(launch_update_w_euc_t(W, P, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st));
+        CK(launch_update_w_euc_t(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st));
     }
     return hipSuccess;
 }
@@ -250,6 +264,8 @@ static void fp8_args(const Plan& pl, const void* Vstore, void* ws, int which, Fp
     a->ldr = pl.ldr;
     a->Bsc = (const uint8_t*)Vstore + L.off_sc;
     a->bcol_shift = 5;
+    a->G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)pl.r * pl.rpad);
+    a->gcount = (int*)(base + pl.off_cnt) + (which == 0 ? 0 : 1);
     if (which == 0) {  // H pass: A = W (blocks along i), B = V (columns u, K = i)
         a->A = (const uint8_t*)(base + pl.off_Wb);
         a->Asc = (const uint8_t*)(base + pl.off_Wsc);
@@ -280,15 +296,16 @@ hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, floa
     fp8_args(pl, Vstore, ws, 1, &aw);
     CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
     CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
+    CK(launch_zero_i32(ah.gcount, 2, st));
     for (int it = 0; it < iterations; it++) {
         if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
             CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
         CK(launch_pass_tn_fp8(ah, st));
-        CK(launch_update_h_euc(H, P, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
+        CK(launch_update_h_euc(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
         CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
         if (!update_w) continue;
         CK(launch_pass_tn_fp8(aw, st));
-        CK(launch_update_w_euc_t(W, P, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, nullptr, 0, st));
+        CK(launch_update_w_euc_t(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, nullptr, 0, st));
         CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
     }
     return hipSuccess;
@@ -428,6 +445,8 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
         return hipErrorInvalidValue;
     const int kl = (rule == GPUMLIB_NMF_KL);
+    hipError_t ze = launch_zero_i32((int*)((char*)workspace + pl.off_cnt), 2, stream);
+    if (ze != hipSuccess) return ze;
     if (precision == GPUMLIB_NMF_FP8) {
         Fp8PassArgs a;
         fp8_args(pl, V, workspace, which, &a);
@@ -443,6 +462,8 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         a.P = (float*)(base + pl.off_P);
         a.r = r;
         a.ldr = pl.ldr;
+        a.G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)r * pl.rpad);
+        a.gcount = (int*)(base + pl.off_cnt) + which;
         if (which == 0) {
             a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = a.ldb = pl.ldv8;
             a.kslice = pl.ks1, a.S = pl.S1, a.tiles = pl.tiles1, a.pstride = pl.pstride1;

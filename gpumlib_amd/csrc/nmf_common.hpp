@@ -24,6 +24,9 @@ constexpr int LDK = KC + 4;
 // Output columns (H pass) or rows (W pass) of V handled by one workgroup.
 constexpr int TILE = 128;
 constexpr int NTHREADS = 256;
+// Columns of H / rows of W handled by one update workgroup.
+constexpr int UPD_COLS = 32;
+constexpr int UPD_ROWS = 32;
 
 // Rank padding: the fused kernels hold r rounded up to 32-row MFMA blocks.
 __host__ __device__ inline int round_up(int x, int q) { return (x + q - 1) / q * q; }
@@ -62,6 +65,28 @@ __device__ __forceinline__ float4 load4(const float* p, bool ok, int k, int kend
     if (k + 3 < kend) v.w = p[3];
     return v;
 }
+// Gram reduction by the last arriving workgroup.  Each of the S workgroups of a
+// gram tile has written its slice partial; the one that arrives last sums the S
+// partials in slice order (the same order every time) into G and resets the
+// counter for the next pass.  off(e) is the offset of gram element e inside a
+// slice.  Device-scope fences make the other XCDs' partials visible.
+template <typename OffFn>
+__device__ void gram_finish(const float* P, long pstride, int S, int count, OffFn off, float* G, int* counter) {
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(counter, 1) == S - 1) ? 1 : 0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int e = threadIdx.x; e < count; e += blockDim.x) {
+        const long o = off(e);
+        float g = 0.f;
+        for (int s = 0; s < S; s++) g += P[(size_t)s * pstride + o];
+        G[e] = g;
+    }
+    if (threadIdx.x == 0) atomicExch(counter, 0);
+}
 #endif  // __HIPCC__
 
 // Launch description of one pass (H or W) of one problem.
@@ -85,6 +110,8 @@ struct PassArgs {
     int ldv;          // leading dimension of V (n for FP32 V; a multiple of 8 for BF16 storage)
     const uint8_t* vsc;  // FP8 storage: E8M0 scale of V's 32 x 32 tile (ub * nbi + ib)
     int nbi;             // FP8 storage: 32-row blocks of V
+    float* G;            // reduced gram of this pass's factor: gram(a,b) at G[b*rpad + a]
+    int* gcount;         // arrival counter of the gram tile's S workgroups (zero between passes)
 };
 
 // FP8 K-contiguous pass: as LpPassArgs, with E8M0 scales.  A's scale for column a
@@ -101,6 +128,8 @@ struct Fp8PassArgs {
     long pstride;
     int nbk;
     int bcol_shift, bs_col, bs_k;
+    float* G;
+    int* gcount;
 };
 
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
@@ -114,6 +143,8 @@ struct LpPassArgs {
     int lda, ldb, ldr;
     int kslice, S, tiles;
     long pstride;
+    float* G;
+    int* gcount;
 };
 
 // One problem of a grouped (batched) launch: the arguments of its two passes and

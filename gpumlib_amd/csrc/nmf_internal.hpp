@@ -28,15 +28,19 @@ hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
 // Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
-hipError_t launch_update_h_euc(float* H, const float* P, int m, int r, int ldr, int S, long pstride,
+// G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a].
+hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
                                float eps, uint16_t* Hs, int ldhs, hipStream_t st);
 // Euclidean W:  W(i,a) *= VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)
-hipError_t launch_update_w_euc(float* W, const float* P, int n, int r, int ldp, int npad, int S,
-                               long pstride, float eps, hipStream_t st);
+// G: the reduced gram H H^T written by the W pass, HHt(a,b) = G[b*rpad + a].
+hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
+                               float eps, hipStream_t st);
 // Euclidean W from partials laid out P[s][i*ldr + a] (gram rows at i >= n), as the
 // BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
-hipError_t launch_update_w_euc_t(float* W, const float* P, int n, int r, int ldr, int S, long pstride, float eps,
-                                 uint16_t* Wb, int ldwb, hipStream_t st);
+hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st);
+// Zeroes n ints (arrival counters) with a kernel, so that it is safe inside captured graphs.
+hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 // KL H:  H(a,u) *= num(a,u) / max(sum_s Psum[s][a], eps)
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr,
                               int S, long pstride, float eps, hipStream_t st);

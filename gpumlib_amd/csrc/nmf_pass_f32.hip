@@ -250,6 +250,11 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             }
         }
         if (do_sums && tid < p.r) p.Psum[(size_t)s * round_up(p.r, 4) + tid] = colsum;
+        if (gram) {
+            const int rpad = round_up(p.r, 4);
+            const long base = (long)p.m * p.ldr;  // gram columns u = m + b, ldr == rpad
+            gram_finish(p.P, p.pstride, p.S, p.r * rpad, [=](int e) { return base + e; }, p.G, p.gcount);
+        }
     }
     if (OBJ != OBJ_NONE) {
         const double tot = block_sum_f64(objacc, red);
@@ -347,6 +352,13 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
                 *reinterpret_cast<float4*>(dst + il) = o;
             }
         }
+    }
+    if (gram) {
+        // gram(a, b) = H H^T(a, b) = P[s][b*ldp + npad + a]; G[b*rpad + a]
+        const int rpad = round_up(p.r, 4);
+        const int ldp = p.ldp, npad = p.npad;
+        gram_finish(p.P, p.pstride, p.S, p.r * rpad,
+                    [=](int e) { return (long)(e / rpad) * ldp + npad + (e % rpad); }, p.G, p.gcount);
     }
 }
 

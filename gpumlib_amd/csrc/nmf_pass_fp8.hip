@@ -153,6 +153,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
             }
         }
     }
+    if (gram) {
+        const long base = (long)p.N * p.ldr;  // gram columns N + b, ldr == rpad
+        gram_finish(p.P, p.pstride, p.S, p.r * p.ldr, [=](int e) { return base + e; }, p.G, p.gcount);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -24,7 +24,7 @@ struct Plan {
     long pstride1, pstride3;
     int nobjp;
     int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (bytes) and 32-blocks of n and m
-    size_t off_P, off_Psum, off_objp, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
+    size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
 };
 
 // Byte layout of the FP8 V store: Vq (ld ld16n) | Vtq (ld ld16m) | tile scales (nbu x nbi).

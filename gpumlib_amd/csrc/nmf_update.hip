@@ -6,16 +6,31 @@
 
 namespace gpumlib_nmf {
 
-constexpr int UPD_COLS = 64;  // columns of H per update workgroup
-constexpr int UPD_ROWS = 64;  // rows of W per update workgroup
 
 // ---------------------------------------------------------------------------
+// Sum of S slice partials at stride pstride, in slice order (loads unrolled).
+__device__ __forceinline__ float sum_slices(const float* src, int S, long pstride) {
+    float v = 0.f;
+    int s = 0;
+    for (; s + 4 <= S; s += 4) {
+        const float x0 = src[(size_t)s * pstride], x1 = src[(size_t)(s + 1) * pstride];
+        const float x2 = src[(size_t)(s + 2) * pstride], x3 = src[(size_t)(s + 3) * pstride];
+        v += x0;
+        v += x1;
+        v += x2;
+        v += x3;
+    }
+    for (; s < S; s++) v += src[(size_t)s * pstride];
+    return v;
+}
+
 // Euclidean H update, reference nmf_multiplicative_euclidian.cpp:36-45 and
 // nmf_kernels.cu:46:  H(a,u) *= WtV(a,u) / (WtWH(a,u) + eps), WtWH = WtW H.
-// WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = sum_s P[s][(m+b)*ldr + a].
-__device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const float* __restrict__ P, int m, int r,
-                                                  int ldr, int S, long pstride, float eps, uint16_t* __restrict__ Hs,
-                                                  int ldhs, int blk, float* sm) {
+// WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = G[b*ldr + a] (reduced by the pass).
+__device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const float* __restrict__ P,
+                                                  const float* __restrict__ G, int m, int r, int ldr, int S,
+                                                  long pstride, float eps, uint16_t* __restrict__ Hs, int ldhs,
+                                                  int blk, float* sm) {
     const int rs = r + 1;
     float* sG = sm;           // sG[a*rs + b] = WtW(a,b)
     float* sH = sm + r * rs;  // sH[uu*rs + b] = H(b, u0 + uu)
@@ -23,11 +38,8 @@ __device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const f
     const int u0 = blk * UPD_COLS;
     const int ncol = min(UPD_COLS, m - u0);
     for (int idx = tid; idx < r * r; idx += NTHREADS) {
-        const int a = idx % r, b = idx / r;
-        const float* src = P + (size_t)(m + b) * ldr + a;
-        float g = 0.f;
-        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
-        sG[a * rs + b] = g;
+        const int b = idx / r, a = idx - b * r;
+        sG[a * rs + b] = G[(size_t)b * ldr + a];
     }
     for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
         const int uu = idx / r, a = idx - uu * r;
@@ -37,9 +49,7 @@ __device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const f
     for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
         const int uu = idx / r, a = idx - uu * r;
         const int u = u0 + uu;
-        const float* src = P + (size_t)u * ldr + a;
-        float wtv = 0.f;
-        for (int s = 0; s < S; s++) wtv += src[(size_t)s * pstride];
+        const float wtv = sum_slices(P + (size_t)u * ldr + a, S, pstride);
         float d = 0.f;
         for (int b = 0; b < r; b++) d += sG[a * rs + b] * sH[uu * rs + b];
         H[(size_t)u * r + a] = sH[uu * rs + a] * (wtv / (d + eps));
@@ -54,17 +64,19 @@ __device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const f
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H, const float* __restrict__ P,
-                                                           int m, int r, int ldr, int S, long pstride, float eps,
-                                                           uint16_t* __restrict__ Hs, int ldhs) {
+                                                           const float* __restrict__ G, int m, int r, int ldr, int S,
+                                                           long pstride, float eps, uint16_t* __restrict__ Hs,
+                                                           int ldhs) {
     extern __shared__ float sm[];
-    update_h_euc_body(H, P, m, r, ldr, S, pstride, eps, Hs, ldhs, blockIdx.x, sm);
+    update_h_euc_body(H, P, G, m, r, ldr, S, pstride, eps, Hs, ldhs, blockIdx.x, sm);
 }
 
-// Euclidean W update from partials in the layout of the BF16 pass over V^T:
-// VHt(i,a) = sum_s P[s][i*ldr + a];  HHt(b,a) = sum_s P[s][(n+b)*ldr + a].
+// Euclidean W update from partials in the layout of the BF16/FP8 pass over V^T:
+// VHt(i,a) = sum_s P[s][i*ldr + a];  HHt(b,a) = G[a*ldr + b].
 __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_t(float* __restrict__ W, const float* __restrict__ P,
-                                                             int n, int r, int ldr, int S, long pstride, float eps,
-                                                             uint16_t* __restrict__ Wb, int ldwb) {
+                                                             const float* __restrict__ G, int n, int r, int ldr, int S,
+                                                             long pstride, float eps, uint16_t* __restrict__ Wb,
+                                                             int ldwb) {
     extern __shared__ float sm[];
     const int rs = r + 1;
     float* sG = sm;                      // sG[b*rs + a] = HHt(b,a)
@@ -74,18 +86,12 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_t(float* __restrict__
     const int i0 = blockIdx.x * UPD_ROWS;
     const int nrow = min(UPD_ROWS, n - i0);
     for (int idx = tid; idx < r * r; idx += NTHREADS) {
-        const int a = idx % r, b = idx / r;
-        const float* src = P + (size_t)(n + b) * ldr + a;
-        float g = 0.f;
-        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
-        sG[b * rs + a] = g;
+        const int a = idx / r, b = idx - a * r;
+        sG[b * rs + a] = G[(size_t)a * ldr + b];
     }
     for (int idx = tid; idx < nrow * r; idx += NTHREADS) {
         const int ii = idx / r, a = idx - ii * r;
-        const float* src = P + (size_t)(i0 + ii) * ldr + a;
-        float v = 0.f;
-        for (int s = 0; s < S; s++) v += src[(size_t)s * pstride];
-        sP[ii * rs + a] = v;
+        sP[ii * rs + a] = sum_slices(P + (size_t)(i0 + ii) * ldr + a, S, pstride);
     }
     for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
         const int ii = idx % UPD_ROWS, b = idx / UPD_ROWS;
@@ -106,22 +112,19 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_t(float* __restrict__
 
 // Euclidean W update, reference nmf_multiplicative_euclidian.cpp:53-66:
 // W(i,a) *= VHt(i,a) / (WHHt(i,a) + eps), WHHt = W HHt.
-// VHt(i,a) = sum_s P[s][a*ldp + i];  HHt(b,a) = sum_s P[s][a*ldp + npad + b].
-__device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const float* __restrict__ P, int n, int r,
-                                                  int ldp, int npad, int S, long pstride, float eps, int blk,
-                                                  float* sm) {
-    const int rs = r + 1;
+// VHt(i,a) = sum_s P[s][a*ldp + i];  HHt(b,a) = G[a*rpad + b] (reduced by the pass).
+__device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const float* __restrict__ P,
+                                                  const float* __restrict__ G, int n, int r, int ldp, int S,
+                                                  long pstride, float eps, int blk, float* sm) {
+    const int rs = r + 1, rpad = round_up(r, 4);
     float* sG = sm;           // sG[b*rs + a] = HHt(b,a)
     float* sW = sm + r * rs;  // sW[ii*rs + b] = W(i0 + ii, b)
     const int tid = threadIdx.x;
     const int i0 = blk * UPD_ROWS;
     const int nrow = min(UPD_ROWS, n - i0);
     for (int idx = tid; idx < r * r; idx += NTHREADS) {
-        const int b = idx % r, a = idx / r;
-        const float* src = P + (size_t)a * ldp + npad + b;
-        float g = 0.f;
-        for (int s = 0; s < S; s++) g += src[(size_t)s * pstride];
-        sG[b * rs + a] = g;
+        const int a = idx / r, b = idx - a * r;
+        sG[b * rs + a] = G[(size_t)a * rpad + b];
     }
     for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
         const int ii = idx % UPD_ROWS, b = idx / UPD_ROWS;
@@ -132,9 +135,7 @@ __device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const f
         const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
         if (ii >= nrow) continue;
         const int i = i0 + ii;
-        const float* src = P + (size_t)a * ldp + i;
-        float vht = 0.f;
-        for (int s = 0; s < S; s++) vht += src[(size_t)s * pstride];
+        const float vht = sum_slices(P + (size_t)a * ldp + i, S, pstride);
         float d = 0.f;
         for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
         W[(size_t)a * n + i] = sW[ii * rs + a] * (vht / (d + eps));
@@ -142,10 +143,10 @@ __device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const f
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W, const float* __restrict__ P,
-                                                           int n, int r, int ldp, int npad, int S, long pstride,
-                                                           float eps) {
+                                                           const float* __restrict__ G, int n, int r, int ldp, int S,
+                                                           long pstride, float eps) {
     extern __shared__ float sm[];
-    update_w_euc_body(W, P, n, r, ldp, npad, S, pstride, eps, blockIdx.x, sm);
+    update_w_euc_body(W, P, G, n, r, ldp, S, pstride, eps, blockIdx.x, sm);
 }
 
 // KL H update, reference nmf_kernels.cu:337-339 with the clamped sums of :104.
@@ -165,9 +166,7 @@ __device__ __forceinline__ void update_h_kl_body(float* __restrict__ H, const fl
     for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
         const int uu = idx / r, a = idx - uu * r;
         const int u = u0 + uu;
-        const float* src = P + (size_t)u * ldr + a;
-        float num = 0.f;
-        for (int s = 0; s < S; s++) num += src[(size_t)s * pstride];
+        const float num = sum_slices(P + (size_t)u * ldr + a, S, pstride);
         H[(size_t)u * r + a] *= (num / ssum[a]);
     }
 }
@@ -197,9 +196,7 @@ __device__ __forceinline__ void update_w_kl_body(float* __restrict__ W, const fl
         const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
         if (ii >= nrow) continue;
         const int i = i0 + ii;
-        const float* src = P + (size_t)a * ldp + i;
-        float num = 0.f;
-        for (int s = 0; s < S; s++) num += src[(size_t)s * pstride];
+        const float num = sum_slices(P + (size_t)a * ldp + i, S, pstride);
         W[(size_t)a * n + i] *= (num / ssum[a]);
     }
 }
@@ -209,6 +206,15 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W,
                                                           int S, long pstride, float eps) {
     __shared__ float ssum[512];
     update_w_kl_body(W, P, Psum, n, r, ldp, S, pstride, eps, blockIdx.x, ssum);
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_zero_i32(int* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, p, n);
+    return hipGetLastError();
 }
 
 // Deterministic reduction of count doubles (fixed strided order, then a fixed tree).
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_update_h(const JobArgs* __re
         const PassArgs& p = ja.h;
         __syncthreads();
         if (ja.rule == 0)
-            update_h_euc_body(ja.H, p.P, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, nullptr, 0, item - first[j], sm);
+            update_h_euc_body(ja.H, p.P, p.G, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, nullptr, 0, item - first[j], sm);
         else
             update_h_kl_body(ja.H, p.P, p.Psum, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm);
     }
@@ -374,7 +380,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_update_w(const JobArgs* __re
         const PassArgs& p = ja.w;
         __syncthreads();
         if (ja.rule == 0)
-            update_w_euc_body(ja.W, p.P, p.n, p.r, p.ldp, p.npad, p.S, p.pstride, p.eps, item - first[j], sm);
+            update_w_euc_body(ja.W, p.P, p.G, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm);
         else
             update_w_kl_body(ja.W, p.P, p.Psum, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm);
     }
@@ -419,40 +425,40 @@ hipError_t launch_group_reduce_obj(const JobArgs* jobs, const int* table, int sl
 // ---------------------------------------------------------------------------
 static size_t upd_smem(int r, int rows) { return (size_t)(r * (r + 1) + rows * (r + 1)) * sizeof(float); }
 
-hipError_t launch_update_h_euc(float* H, const float* P, int m, int r, int ldr, int S, long pstride, float eps,
-                               uint16_t* Hs, int ldhs, hipStream_t st) {
+hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st) {
     const size_t sm = upd_smem(r, UPD_COLS);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_update_h_euc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_update_h_euc, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), sm, st, H, P, m, r, ldr,
-                       S, pstride, eps, Hs, ldhs);
+    hipLaunchKernelGGL(k_update_h_euc, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), sm, st, H, P, G, m, r,
+                       ldr, S, pstride, eps, Hs, ldhs);
     return hipGetLastError();
 }
 
-hipError_t launch_update_w_euc_t(float* W, const float* P, int n, int r, int ldr, int S, long pstride, float eps,
-                                 uint16_t* Wb, int ldwb, hipStream_t st) {
-    const size_t sm = (size_t)(r * (r + 1) + 2 * UPD_ROWS * (r + 1)) * sizeof(float);
+hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st) {
+    const size_t sm = (size_t)(r * (r + 1) + 2 * UPD_ROWS * (r + 1)) * sizeof(float);  // gram, W rows, VHt rows
     if (sm > 64 * 1024) {
         hipError_t e =
             hipFuncSetAttribute((const void*)k_update_w_euc_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_update_w_euc_t, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, n, r, ldr,
-                       S, pstride, eps, Wb, ldwb);
+    hipLaunchKernelGGL(k_update_w_euc_t, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, G, n, r,
+                       ldr, S, pstride, eps, Wb, ldwb);
     return hipGetLastError();
 }
 
-hipError_t launch_update_w_euc(float* W, const float* P, int n, int r, int ldp, int npad, int S, long pstride,
+hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
                                float eps, hipStream_t st) {
     const size_t sm = upd_smem(r, UPD_ROWS);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_update_w_euc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_update_w_euc, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, n, r, ldp,
-                       npad, S, pstride, eps);
+    hipLaunchKernelGGL(k_update_w_euc, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), sm, st, W, P, G, n, r,
+                       ldp, S, pstride, eps);
     return hipGetLastError();
 }
 

@@ -401,6 +401,90 @@ void orc_kl_iter_f32_omp(int n, int m, int r, const float *V, float *W, float *H
     free(Q);
 }
 
+/* FP64 KL iteration for full-size parity runs: the same rules as orc_kl_iter_f64
+ * (nmf_multiplicative_divergence.cpp:29-50), with W H, the ratio and the
+ * numerators fused per column of V and OpenMP over columns.  Summation inside
+ * each dot product is sequential, as in the textbook loops. */
+void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double *H, double eps, int updateW) {
+    double *sumW = (double *)malloc(sizeof(double) * (size_t)r);
+    orc_sum_w_f64(n, r, W, eps, sumW);
+    /* H half: per column u, q(i) = V(i,u) / ((W H)(i,u) + eps), H(a,u) *= sum_i W(i,a) q(i) / sumW[a] */
+#pragma omp parallel
+    {
+        double *q = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(static)
+        for (int u = 0; u < m; u++) {
+            const double *h = H + (size_t)u * r;
+            for (int i = 0; i < n; i++) q[i] = 0.0;
+            for (int a = 0; a < r; a++) {
+                const double ha = h[a];
+                const double *w = W + (size_t)a * n;
+                for (int i = 0; i < n; i++) q[i] += w[i] * ha;
+            }
+            const double *v = V + (size_t)u * n;
+            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps);
+            double *hn = H + (size_t)u * r;
+            double num[512];
+            for (int a = 0; a < r; a++) {
+                const double *w = W + (size_t)a * n;
+                double sacc = 0.0;
+                for (int i = 0; i < n; i++) sacc += w[i] * q[i];
+                num[a] = sacc;
+            }
+            for (int a = 0; a < r; a++) hn[a] *= (num[a] / sumW[a]);
+        }
+        free(q);
+    }
+    free(sumW);
+    if (!updateW) return;
+    double *sumH = (double *)malloc(sizeof(double) * (size_t)r);
+    orc_sum_h_f64(r, m, H, eps, sumH);
+    /* W half: num(i,a) = sum_u H(a,u) V(i,u) / ((W H)(i,u) + eps), per-thread accumulators in u order */
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    double *acc = (double *)calloc((size_t)nt * n * r, sizeof(double));
+    /* Each thread owns a contiguous range of columns u; the thread partials are then
+     * summed in thread order, so the result depends only on the thread count. */
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double *mine = acc + (size_t)t * n * r;
+        double *q = (double *)malloc(sizeof(double) * (size_t)n);
+        const int per = (m + nt - 1) / nt;
+        const int u0 = t * per, u1 = (u0 + per < m) ? u0 + per : m;
+        for (int u = u0; u < u1; u++) {
+            const double *h = H + (size_t)u * r;
+            for (int i = 0; i < n; i++) q[i] = 0.0;
+            for (int a = 0; a < r; a++) {
+                const double ha = h[a];
+                const double *w = W + (size_t)a * n;
+                for (int i = 0; i < n; i++) q[i] += w[i] * ha;
+            }
+            const double *v = V + (size_t)u * n;
+            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps);
+            for (int a = 0; a < r; a++) {
+                const double ha = h[a];
+                double *dst = mine + (size_t)a * n;
+                for (int i = 0; i < n; i++) dst[i] += ha * q[i];
+            }
+        }
+        free(q);
+    }
+    for (int a = 0; a < r; a++)
+        for (int i = 0; i < n; i++) {
+            double num = 0.0;
+            for (int t = 0; t < nt; t++) num += acc[(size_t)t * n * r + (size_t)a * n + i];
+            W[(size_t)a * n + i] *= (num / sumH[a]);
+        }
+    free(acc);
+    free(sumH);
+}
+
 int orc_omp_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

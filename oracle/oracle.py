@@ -48,6 +48,7 @@ _SIGS = {
     "orc_kl_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
     "orc_euc_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
     "orc_kl_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_kl_iter_f64_omp": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
     "orc_frob_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_kl_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_quality_f64": (_d, [_l, _dp, _dp, _d]),
@@ -106,7 +107,7 @@ def euc_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
 def kl_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
     L = lib()
     if V.dtype == np.float64:
-        fn, p, e = L.orc_kl_iter_f64, _p64, float(eps)
+        fn, p, e = (L.orc_kl_iter_f64_omp if omp else L.orc_kl_iter_f64), _p64, float(eps)
     else:
         fn, p, e = (L.orc_kl_iter_f32_omp if omp else L.orc_kl_iter_f32), _p32, float(np.float32(eps))
     for _ in range(iters):

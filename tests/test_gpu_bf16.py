@@ -63,7 +63,10 @@ def _gaps(n, m, r, iters, seed):
     ref = orc.np_frob(v, w, h)
     gw, gh = G.as_matrix(p.get_w(), n, r), G.as_matrix(p.get_h(), r, m)
     gobj_fp32v = orc.np_frob(v, gw.astype(np.float64), gh.astype(np.float64))
-    return (abs(gobj - ref_q) / ref_q, abs(gobj_fp32v - ref) / ref, rel_fro(gw, wq), rel_fro(gh, hq))
+    # relative to the objective, floored at 1e-6 of |V|^2 (a problem fitted exactly has objective ~ 0)
+    floor = 1e-6 * float(np.sum(vq * vq))
+    return (abs(gobj - ref_q) / max(ref_q, floor), abs(gobj_fp32v - ref) / max(ref, floor), rel_fro(gw, wq),
+            rel_fro(gh, hq))
 
 
 def test_bf16_objective_gap(native, orc, cuda):

@@ -179,6 +179,20 @@ def test_kl_matches_numpy_restatement(orc):
     assert rel_fro(w1, w2) < 1e-12 and rel_fro(h1, h2) < 1e-12
 
 
+def test_kl_f64_fused_omp_matches_textbook(orc):
+    """The fused OpenMP FP64 KL iteration (used at full size) equals the textbook loops."""
+    V, W, H = poisson_problem(130, 90, 7, 17)
+    n, m = V.shape
+    a = [cm(x) for x in (V, W, H)]
+    b = [x.copy() for x in a]
+    orc.kl_iter(n, m, 7, *a, 1e-9, iters=5)
+    orc.kl_iter(n, m, 7, *b, 1e-9, iters=5, omp=True)
+    assert rel_fro(a[1], b[1]) < 1e-12 and rel_fro(a[2], b[2]) < 1e-12
+    c = [cm(x) for x in (V, W, H)]
+    orc.kl_iter(n, m, 7, *c, 1e-9, iters=1, update_w=False, omp=True)
+    assert np.array_equal(c[1], cm(W))  # W untouched
+
+
 def test_kl_monotone(orc):
     V, W, H = poisson_problem(45, 35, 4, 12)
     n, m = V.shape
