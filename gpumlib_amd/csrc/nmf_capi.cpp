@@ -76,10 +76,13 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->nbu = (m + 31) / 32;
     const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
     const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 64 : 128);
-    pl->tiles1 = (m + TILE - 1) / TILE + euc;
+    // FP32 passes give the gram its own tile; the BF16 / FP8 passes fold it into
+    // the first column tile of every slice.
+    const int gram_tile = (precision == GPUMLIB_NMF_FP32) ? euc : 0;
+    pl->tiles1 = (m + TILE - 1) / TILE + gram_tile;
     pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
-    pl->tiles3 = (n + TILE - 1) / TILE + euc;
+    pl->tiles3 = (n + TILE - 1) / TILE + gram_tile;
     pl->S3 = choose_slices(pl->tiles3, m, kc, target, &pl->ks3);
     // FP32 W pass writes P[s][a*ldp + i]; the BF16 pass over V^T writes P[s][i*ldr + a].
     pl->pstride3 = (precision == GPUMLIB_NMF_FP32) ? (long)r * pl->ldp : (long)(n + r) * pl->ldr;

@@ -6,7 +6,7 @@
 // problem (V^T, H^T, W^T): U_W(V, W, H) = U_H(V^T, H^T, W^T)^T.  So both
 // contractions use one kernel, a K-contiguous ("TN") BF16 MFMA pass:
 //   P[s](a, u) = sum_{k in slice s} A(k, a) B(k, u),
-//   H pass:  A = W (n x r), B = V   -> W^T V, and one gram tile W^T W
+//   H pass:  A = W (n x r), B = V   -> W^T V, and the gram W^T W
 //   W pass:  A = H^T (m x r), B = V^T -> H V^T = (V H^T)^T, and the gram H H^T
 // Both passes stream V once at 2 bytes per element.
 #include <hip/hip_runtime.h>
@@ -38,9 +38,17 @@ __device__ __forceinline__ uint4 load8_bf16(const uint16_t* p, bool ok, int k, i
                       e[6] | ((uint32_t)e[7] << 16));
 }
 
+// One workgroup per (slice s, column tile t); 4 waves, wave w owns columns
+// w*32 .. w*32+31 of the 128-column tile and all MB row blocks.  The first
+// column tile of every slice also accumulates the slice's gram A^T A from the
+// A rows already staged in LDS (gram block (mb, nb) on wave mb*MB + nb, modulo
+// 4), so the grid is exactly tiles x S workgroups of one shape and the gram
+// costs no extra pass over A.  The last of the S gram workgroups reduces the
+// partial grams into G.
 template <int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     constexpr int RP = MB * 32;
+    constexpr int GB = (MB * MB + 3) / 4;  // gram blocks per wave
     __shared__ __attribute__((aligned(16))) uint16_t sA[2][RP][LDB];
     __shared__ __attribute__((aligned(16))) uint16_t sB[2][TILE][LDB];
 
@@ -50,12 +58,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
-    const int vt = (p.N + TILE - 1) / TILE;
-    const bool gram = (t == vt);
-    const uint16_t* B = gram ? p.A : p.B;
-    const int ldb = gram ? p.lda : p.ldb;
-    const int col0 = gram ? 0 : t * TILE;
-    const int ncols = gram ? p.r : min(TILE, p.N - col0);
+    const bool gram = (t == 0) && p.G != nullptr;
+    const int col0 = t * TILE;
+    const int ncols = min(TILE, p.N - col0);
     const int kbeg = s * p.kslice;
     const int kend = min(p.K, kbeg + p.kslice);
 
@@ -64,7 +69,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 8;
-            rb[j] = load8_bf16(B + (size_t)(col0 + col) * ldb + k0 + kq, col < ncols, k0 + kq, kend);
+            rb[j] = load8_bf16(p.B + (size_t)(col0 + col) * p.ldb + k0 + kq, col < ncols, k0 + kq, kend);
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
@@ -85,10 +90,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
         }
     };
 
-    f32x16 acc[MB];
+    f32x16 acc[MB], gacc[GB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++)
         for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+#pragma unroll
+    for (int q = 0; q < GB; q++)
+        for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
     const int nchunks = (kend - kbeg + KB - 1) / KB;
     gload(kbeg);
@@ -107,15 +115,30 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
                 acc[mb] = mfma_bf16(av, bv, acc[mb]);
             }
         }
+        if (gram) {
+#pragma unroll
+            for (int q = 0; q < GB; q++) {
+                const int blk = wave + 4 * q;
+                if (blk < MB * MB) {
+                    const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+#pragma unroll
+                    for (int g = 0; g < KB / 16; g++) {
+                        const uint4 av = *reinterpret_cast<const uint4*>(&sA[buf][mb * 32 + c][g * 16 + h * 8]);
+                        const uint4 bv = *reinterpret_cast<const uint4*>(&sA[buf][nb * 32 + c][g * 16 + h * 8]);
+                        gacc[q] = mfma_bf16(av, bv, gacc[q]);
+                    }
+                }
+            }
+        }
         if (ch + 1 < nchunks) sstore(buf ^ 1);
         __syncthreads();
     }
 
     // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
+    float* slice = p.P + (size_t)s * p.pstride;
     const int ul = wave * 32 + c;
     if (ul < ncols) {
-        const int ucol = (gram ? p.N : col0) + ul;
-        float* dst = p.P + (size_t)s * p.pstride + (size_t)ucol * p.ldr;
+        float* dst = slice + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
@@ -129,8 +152,23 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
         }
     }
     if (gram) {
-        const long base = (long)p.N * p.ldr;  // gram columns N + b, ldr == rpad
-        gram_finish(p.P, p.pstride, p.S, p.r * p.ldr, [=](int e) { return base + e; }, p.G, p.gcount);
+        // gacc[q][v] = gram(a = mb*32 + acc_row(v,h), b = nb*32 + c), stored at column N + b
+#pragma unroll
+        for (int q = 0; q < GB; q++) {
+            const int blk = wave + 4 * q;
+            if (blk >= MB * MB) continue;
+            const int mb = blk / MB, b = (blk - (blk / MB) * MB) * 32 + c;
+            if (b >= p.r) continue;
+            float* dst = slice + (size_t)(p.N + b) * p.ldr;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int a0 = mb * 32 + g * 8 + h * 4;
+                if (a0 < p.ldr)
+                    *reinterpret_cast<float4*>(dst + a0) =
+                        make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
+            }
+        }
+        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.r * p.ldr, p.G, p.gcount);
     }
 }
 

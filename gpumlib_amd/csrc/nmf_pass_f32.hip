@@ -253,7 +253,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         if (gram) {
             const int rpad = round_up(p.r, 4);
             const long base = (long)p.m * p.ldr;  // gram columns u = m + b, ldr == rpad
-            gram_finish(p.P, p.pstride, p.S, p.r * rpad, [=](int e) { return base + e; }, p.G, p.gcount);
+            gram_finish4(p.P + base, p.pstride, p.S, p.r * rpad, p.G, p.gcount);
         }
     }
     if (OBJ != OBJ_NONE) {

@@ -45,15 +45,14 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
-    const int vt = (p.N + TILE - 1) / TILE;
-    const bool gram = (t == vt);
-    const uint8_t* B = gram ? p.A : p.B;
-    const int ldb = gram ? p.lda : p.ldb;
-    const int col0 = gram ? 0 : t * TILE;
-    const int ncols = gram ? p.r : min(TILE, p.N - col0);
+    // the first column tile of every slice also accumulates the slice's gram A^T A
+    // from the A rows staged in LDS (block (mb, nb) on wave mb*MB + nb, modulo 4)
+    const bool gram = (t == 0) && p.G != nullptr;
+    const int col0 = t * TILE;
+    const int ncols = min(TILE, p.N - col0);
     const int kbeg = s * p.kslice;
     const int kend = min(p.K, kbeg + p.kslice);
-    const int nb = MB;
+    constexpr int GB = (MB * MB + 3) / 4;
 
     uint4 ra[MB], rb[4];
     uint8_t rsa, rsb[2];
@@ -61,7 +60,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
-            rb[j] = load16_u8(B + (size_t)(col0 + col) * ldb + k0 + kq, col < ncols, k0 + kq, kend);
+            rb[j] = load16_u8(p.B + (size_t)(col0 + col) * p.ldb + k0 + kq, col < ncols, k0 + kq, kend);
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
@@ -77,10 +76,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
         for (int jj = 0; jj < 2; jj++) {
             const int e = tid + NTHREADS * jj, col = e >> 2, kb = kb0 + (e & 3);
             uint8_t v = 127;
-            if (col < ncols && kb < p.nbk) {
-                v = gram ? p.Asc[(size_t)col * p.nbk + kb]
-                         : p.Bsc[(size_t)((col0 + col) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
-            }
+            if (col < ncols && kb < p.nbk)
+                v = p.Bsc[(size_t)((col0 + col) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
             rsb[jj] = v;
         }
     };
@@ -103,10 +100,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
         }
     };
 
-    f32x16 acc[MB];
+    f32x16 acc[MB], gacc[GB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++)
         for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+#pragma unroll
+    for (int q = 0; q < GB; q++)
+        for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
     const int nchunks = (kend - kbeg + K8 - 1) / K8;
     gload(kbeg);
@@ -115,32 +115,55 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     for (int ch = 0; ch < nchunks; ch++) {
         const int buf = ch & 1;
         if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * K8);
-        // lane (c, h) supplies the 32 values k = g*64 + h*32 + j of this step and their block scale
+        // The instruction takes K block 0 of a row from bytes 0..15 of both lane halves and
+        // its scale from lane c, K block 1 from bytes 16..31 and its scale from lane c + 32.
+        // So lane (c, h) supplies k = g*64 + 16h + j in bytes j < 16 and k = g*64 + 32 + 16h + j
+        // in bytes 16 + j, and the scale of block 2g + h of this step.
 #pragma unroll
         for (int g = 0; g < K8 / 64; g++) {
-            const uint8_t* pb = &sB[buf][wave * 32 + c][g * 64 + h * 32];
-            const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 16);
+            const uint8_t* pb = &sB[buf][wave * 32 + c][g * 64 + h * 16];
+            const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
             const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
             const int sb = sBs[buf][wave * 32 + c][2 * g + h];
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
-                if (mb >= nb) continue;
-                const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 32];
-                const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 16);
+                const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 16];
+                const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
                 const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
                                   (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
                 const int sa = sAs[buf][mb * 32 + c][2 * g + h];
                 acc[mb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc[mb], 0, 0, 0, sa, 0, sb);
             }
         }
+        if (gram) {
+#pragma unroll
+            for (int q = 0; q < GB; q++) {
+                const int blk = wave + 4 * q;
+                if (blk >= MB * MB) continue;
+                const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+#pragma unroll
+                for (int g = 0; g < K8 / 64; g++) {
+                    const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 16];
+                    const uint8_t* pb = &sA[buf][nb * 32 + c][g * 64 + h * 16];
+                    const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
+                    const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
+                    const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                                      (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                    const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                                      (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                    const int sa = sAs[buf][mb * 32 + c][2 * g + h], sb = sAs[buf][nb * 32 + c][2 * g + h];
+                    gacc[q] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, gacc[q], 0, 0, 0, sa, 0, sb);
+                }
+            }
+        }
         if (ch + 1 < nchunks) sstore(buf ^ 1);
         __syncthreads();
     }
 
+    float* slice = p.P + (size_t)s * p.pstride;
     const int ul = wave * 32 + c;
     if (ul < ncols) {
-        const int ucol = (gram ? p.N : col0) + ul;
-        float* dst = p.P + (size_t)s * p.pstride + (size_t)ucol * p.ldr;
+        float* dst = slice + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
@@ -154,8 +177,23 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
         }
     }
     if (gram) {
-        const long base = (long)p.N * p.ldr;  // gram columns N + b, ldr == rpad
-        gram_finish(p.P, p.pstride, p.S, p.r * p.ldr, [=](int e) { return base + e; }, p.G, p.gcount);
+        // gacc[q][v] = gram(a = mb*32 + acc_row(v,h), b = nb*32 + c), stored at column N + b
+#pragma unroll
+        for (int q = 0; q < GB; q++) {
+            const int blk = wave + 4 * q;
+            if (blk >= MB * MB) continue;
+            const int mb = blk / MB, b = (blk - (blk / MB) * MB) * 32 + c;
+            if (b >= p.r) continue;
+            float* dst = slice + (size_t)(p.N + b) * p.ldr;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int a0 = mb * 32 + g * 8 + h * 4;
+                if (a0 < p.ldr)
+                    *reinterpret_cast<float4*>(dst + a0) =
+                        make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
+            }
+        }
+        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.r * p.ldr, p.G, p.gcount);
     }
 }
 

@@ -117,6 +117,26 @@ class DeviceNMF:
         self.stream.synchronize()
         return float(self.obj.download(np.float64)[0])
 
+    def objective_fp32v(self, kind):
+        """Objective of the current W, H against the FP32 V (whatever the V store's precision)."""
+        if self.precision == FP32:
+            return self.objective(kind)
+        if getattr(self, "ws32", None) is None:
+            self.ws32_bytes = int(self.lib.gpumlib_nmf_workspace_bytes(self.rule, FP32, self.n, self.m, self.r))
+            self.ws32 = self.hip.Buffer(self.ws32_bytes)
+        err = self.lib.gpumlib_nmf_objective(
+            self.rule, FP32, self.n, self.m, self.r, self.Vf.ptr, self.W.ptr, self.H.ptr, self.eps, kind,
+            self.obj.ptr, self.ws32.ptr, self.ws32_bytes, self.stream.handle)
+        capi.check(err, "gpumlib_nmf_objective")
+        self.stream.synchronize()
+        return float(self.obj.download(np.float64)[0])
+
+    def set_wh(self, W, H):
+        """Replaces W and H (the V store stays)."""
+        self.stream.synchronize()
+        self.W.upload(np.asarray(W, dtype=np.float32))
+        self.H.upload(np.asarray(H, dtype=np.float32))
+
     def get_w(self):
         self.stream.synchronize()
         return self.W.download(np.float32, self.n * self.r)

@@ -156,6 +156,74 @@ def np_kl_iter(V, W, H, eps, update_w=True):
     return W, H
 
 
+# ---------------------------------------------------------------------------
+# Low-precision operand restatement.  The BF16 and FP8 storage paths of the
+# library are additions with no counterpart in the reference: they keep V in a
+# narrow format and feed the MFMA passes narrow "shadow" copies of W and H,
+# while the updates run on FP32 masters.  The functions below restate that
+# arithmetic in FP64 (exact products and sums of the rounded operands), so that
+# a test can separate the rounding the format implies (stated as an objective
+# gap to the FP64 oracle) from the accumulation error of the kernels.
+def bf16_round(x):
+    """float32 -> nearest-even bfloat16 value, as float64 (finite inputs)."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def e8m0_for_amax(amax):
+    """Smallest s with amax / 2^(s-127) <= 448 (E4M3 maximum), s in [0, 254]; 127 for amax = 0."""
+    amax = np.asarray(amax, dtype=np.float64)
+    mant, ex = np.frexp(np.where(amax > 0, amax, 1.0))  # amax = mant * 2^ex, mant in [0.5, 1)
+    s = (ex - 1) - 8 + (mant * 2 > 1.75) + 127
+    return np.where(amax > 0, np.clip(s, 0, 254), 127).astype(np.int64)
+
+
+def e4m3_round(y):
+    """Non-negative finite y -> nearest OCP E4M3 value (ties to even), saturating at 448."""
+    y = np.asarray(y, dtype=np.float64)
+    _, ex = np.frexp(np.where(y > 0, y, 1.0))
+    e = np.maximum(ex - 1, -6)  # binade exponent; below 2^-6 the spacing stays 2^-9
+    step = np.ldexp(1.0, e - 3)
+    return np.minimum(np.round(y / step) * step, 448.0)
+
+
+def fp8_block_round(X, axis):
+    """Dequantised E4M3 copy of X (2-D, >= 0) with one E8M0 scale per 32 entries along axis."""
+    X = np.asarray(X, dtype=np.float32).astype(np.float64)
+    Xt = X if axis == 0 else X.T
+    out = np.empty_like(Xt)
+    for k0 in range(0, Xt.shape[0], 32):
+        blk = Xt[k0:k0 + 32]
+        unit = np.ldexp(1.0, e8m0_for_amax(np.abs(blk).max(axis=0)) - 127)
+        out[k0:k0 + 32] = e4m3_round(blk / unit) * unit
+    return out if axis == 0 else out.T
+
+
+def fp8_tile_round(V):
+    """Dequantised E4M3 copy of V (n x m) with one E8M0 scale per 32 x 32 tile."""
+    V = np.asarray(V, dtype=np.float32).astype(np.float64)
+    out = np.empty_like(V)
+    for i0 in range(0, V.shape[0], 32):
+        for u0 in range(0, V.shape[1], 32):
+            blk = V[i0:i0 + 32, u0:u0 + 32]
+            unit = np.ldexp(1.0, int(e8m0_for_amax(np.abs(blk).max())) - 127)
+            out[i0:i0 + 32, u0:u0 + 32] = e4m3_round(blk / unit) * unit
+    return out
+
+
+def np_euc_iter_shadow(Vq, W, H, eps, shadow_w, shadow_h, update_w=True):
+    """Euclidean MU with the passes fed shadow operands: W^T V and W^T W from
+    shadow_w(W), V H^T and H H^T from shadow_h(H_new); ratios and the masters
+    in full precision (the same update rules as np_euc_iter)."""
+    Ws = shadow_w(W)
+    H = H * ((Ws.T @ Vq) / ((Ws.T @ Ws) @ H + eps))
+    if update_w:
+        Hs = shadow_h(H)
+        W = W * ((Vq @ Hs.T) / (W @ (Hs @ Hs.T) + eps))
+    return W, H
+
+
 def np_frob(V, W, H):
     D = V - W @ H
     return float(np.sum(D * D))

@@ -42,6 +42,16 @@ def cuda(native):
     return hip
 
 
+def fp32_objective_allowance(ref, v_sq_sum, r):
+    """Absolute error the GPU's FP32 residuals may add to a Frobenius objective near ref.
+
+    Each residual d = v - (W H)(i, u) is formed in FP32 from r products, so it carries an
+    error |delta| <= 4 r 2^-24 |v| (a loose bound on r-term FP32 dot-product rounding); the
+    objective sum d^2 then moves by at most 2 |d| |delta| <= 2 sqrt(ref) 4 r 2^-24 |V|_F
+    (Cauchy-Schwarz).  It matters only when the fit is nearly exact (ref ~ 0)."""
+    return 2.0 * np.sqrt(max(ref, 0.0)) * 4.0 * r * 2.0 ** -24 * np.sqrt(v_sq_sum)
+
+
 def rel_fro(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

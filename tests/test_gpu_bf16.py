@@ -6,7 +6,7 @@ FP32 V (storage error included)."""
 import numpy as np
 import pytest
 
-from conftest import rel_fro
+from conftest import fp32_objective_allowance, rel_fro
 from gpumlib_amd import nmf as G
 
 pytestmark = pytest.mark.gpu
@@ -44,6 +44,14 @@ def test_pack_bit_exact(native, cuda, nm):
 
 
 def _gaps(n, m, r, iters, seed):
+    """Run the BF16 path and return its errors against three FP64 references:
+    same_v  the oracle on the BF16-rounded V (the format's effect on W and H operands, stated);
+    fp32_v  the oracle on the FP32 V, objectives of both results taken against the FP32 V;
+    emu_*   the restatement of the path's own arithmetic (BF16 V and BF16 shadows of W and the
+            new H feeding the products, FP32 masters in the update) in FP64: only the FP32
+            accumulation of the kernels separates the two."""
+    from oracle import oracle as orc
+
     V, W, H = G.gen_euclidean(n, m, r, seed), *G.gen_init(n, m, r, seed)
     p = BF16Problem(V, W, H, n, m, r)
     p.iterate(iters)
@@ -52,36 +60,42 @@ def _gaps(n, m, r, iters, seed):
     v = G.as_matrix(V, n, m).astype(np.float64)
     w0 = G.as_matrix(W, n, r).astype(np.float64)
     h0 = G.as_matrix(H, r, m).astype(np.float64)
-    from oracle import oracle as orc
-
     wq, hq = w0, h0
     w, h = w0, h0
+    we, he = w0, h0
     for _ in range(iters):
         wq, hq = orc.np_euc_iter(vq, wq, hq, G.REFERENCE_EPS)
         w, h = orc.np_euc_iter(v, w, h, G.REFERENCE_EPS)
+        we, he = orc.np_euc_iter_shadow(vq, we, he, G.REFERENCE_EPS, orc.bf16_round, orc.bf16_round)
     ref_q = orc.np_frob(vq, wq, hq)
     ref = orc.np_frob(v, w, h)
+    ref_e = orc.np_frob(vq, we, he)
     gw, gh = G.as_matrix(p.get_w(), n, r), G.as_matrix(p.get_h(), r, m)
     gobj_fp32v = orc.np_frob(v, gw.astype(np.float64), gh.astype(np.float64))
-    # relative to the objective, floored at 1e-6 of |V|^2 (a problem fitted exactly has objective ~ 0)
-    floor = 1e-6 * float(np.sum(vq * vq))
-    return (abs(gobj - ref_q) / max(ref_q, floor), abs(gobj_fp32v - ref) / max(ref, floor), rel_fro(gw, wq),
-            rel_fro(gh, hq))
+    floor = 1e-9 * float(np.sum(vq * vq))  # an exactly fitted problem has objective ~ 0
+    # emu_obj: error beyond what FP32 residuals may add near an exact fit, relative to the objective
+    excess = max(0.0, abs(gobj - ref_e) - fp32_objective_allowance(ref_e, float(np.sum(vq * vq)), r))
+    out = dict(same_v=abs(gobj - ref_q) / max(ref_q, floor), fp32_v=abs(gobj_fp32v - ref) / max(ref, floor),
+               emu_obj=excess / max(ref_e, floor), emu_w=rel_fro(gw, we), emu_h=rel_fro(gh, he))
+    print(f"bf16 {n}x{m} r={r} after {iters} iterations: " + "  ".join(f"{k} {v:.2e}" for k, v in out.items()))
+    return out
 
 
 def test_bf16_objective_gap(native, orc, cuda):
-    gap_same_v, gap_fp32_v, ew, eh = _gaps(2048, 1024, 64, 100, seed=0x5EED + 5)
-    print(f"bf16 gaps: same-V {gap_same_v:.2e}  fp32-V {gap_fp32_v:.2e}  W {ew:.2e}  H {eh:.2e}")
-    assert gap_same_v < 1e-3
-    assert gap_fp32_v < 1e-3
-    assert ew < 3e-2 and eh < 3e-2
+    g = _gaps(2048, 1024, 64, 100, seed=0x5EED + 5)
+    assert g["same_v"] < 1e-3
+    assert g["fp32_v"] < 1e-3
+    # the kernels against the restatement of their own arithmetic: FP32 accumulation only
+    assert g["emu_obj"] < 1e-4
+    assert g["emu_w"] < 1e-3 and g["emu_h"] < 1e-3
 
 
 @pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 100), (1, 9, 1)])
 def test_bf16_shapes(native, orc, cuda, shape):
     n, m, r = shape
-    gap_same_v, _, _, _ = _gaps(n, m, r, 20, seed=n + m)
-    assert gap_same_v < 1e-3
+    g = _gaps(n, m, r, 20, seed=n + m)
+    assert g["emu_obj"] < 1e-4
+    assert g["emu_w"] < 1e-3 and g["emu_h"] < 1e-3
 
 
 def test_bf16_deterministic(native, cuda):

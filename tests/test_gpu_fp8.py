@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import rel_fro
+from conftest import fp32_objective_allowance, rel_fro
 from gpumlib_amd import nmf as G
 
 pytestmark = pytest.mark.gpu
@@ -57,10 +57,17 @@ def test_fp8_pack_bit_exact(native, cuda, nm):
 
 
 def test_fp8_objective_and_gaps(native, orc, cuda):
+    """Against the restatement of the path's own arithmetic (E4M3 V with 32 x 32 tile scales,
+    E4M3 shadows of W and the new H with one scale per 32 entries along the contraction, FP32
+    masters) the kernels differ by FP32 accumulation, and by the rare element whose master lies
+    within that error of an E4M3 rounding boundary and rounds to the neighbouring value (a step
+    of 2^-3 or 2^-4 relative): hence looser bounds than the BF16 path.  The gaps to the FP64
+    oracle without operand rounding are stated, with a loose bound."""
     n, m, r, iters = 1024, 768, 32, 50
     V, W, H = G.gen_euclidean(n, m, r, 0x5EED + 6), *G.gen_init(n, m, r, 7)
     p = G.DeviceNMF(G.EUCLIDEAN, V, W, H, n, m, r, precision=G.FP8)
     _, _, vq = host_store(native, V, n, m)
+    assert np.array_equal(vq, orc.fp8_tile_round(G.as_matrix(V, n, m)))
     w0, h0 = G.as_matrix(W, n, r).astype(np.float64), G.as_matrix(H, r, m).astype(np.float64)
     # objective of the entry state against the dequantised store: FP32-level agreement
     f0 = p.objective(G.OBJ_FROBENIUS)
@@ -68,18 +75,27 @@ def test_fp8_objective_and_gaps(native, orc, cuda):
     assert abs(f0 - ref0) / ref0 < 1e-5
     p.iterate(iters)
     gobj = p.objective(G.OBJ_FROBENIUS)
-    wq, hq = w0, h0
     v = G.as_matrix(V, n, m).astype(np.float64)
+    wq, hq = w0, h0
     w, h = w0, h0
+    we, he = w0, h0
+    sw = lambda X: orc.fp8_block_round(X, 0)  # W: blocks along i
+    sh = lambda X: orc.fp8_block_round(X, 1)  # H: blocks along u
     for _ in range(iters):
         wq, hq = orc.np_euc_iter(vq, wq, hq, G.REFERENCE_EPS)
         w, h = orc.np_euc_iter(v, w, h, G.REFERENCE_EPS)
-    gap_same_v = abs(gobj - orc.np_frob(vq, wq, hq)) / orc.np_frob(vq, wq, hq)
+        we, he = orc.np_euc_iter_shadow(vq, we, he, G.REFERENCE_EPS, sw, sh)
     gw, gh = G.as_matrix(p.get_w(), n, r).astype(np.float64), G.as_matrix(p.get_h(), r, m).astype(np.float64)
+    ref_e = orc.np_frob(vq, we, he)
+    emu_obj = abs(gobj - ref_e) / ref_e
+    emu_w, emu_h = rel_fro(gw, we), rel_fro(gh, he)
+    gap_same_v = abs(gobj - orc.np_frob(vq, wq, hq)) / orc.np_frob(vq, wq, hq)
     gap_fp32_v = abs(orc.np_frob(v, gw, gh) - orc.np_frob(v, w, h)) / orc.np_frob(v, w, h)
-    print(f"fp8 gaps after {iters} iterations: same-V {gap_same_v:.3e}  fp32-V {gap_fp32_v:.3e}")
-    assert gap_same_v < 5e-2  # FP8 operands: stated, loose bound
-    assert np.isfinite(gobj) and gobj < f0  # still descending
+    print(f"fp8 after {iters} iterations: emu obj {emu_obj:.2e} W {emu_w:.2e} H {emu_h:.2e}; "
+          f"gaps same-V {gap_same_v:.3e}  fp32-V {gap_fp32_v:.3e}")
+    assert emu_obj < 1e-3
+    assert emu_w < 1e-2 and emu_h < 1e-2
+    assert gap_same_v < 1e-2 and gap_fp32_v < 1e-2
 
 
 def test_fp8_deterministic(native, cuda):
@@ -91,3 +107,28 @@ def test_fp8_deterministic(native, cuda):
         p.iterate(5)
         outs.append((p.get_w(), p.get_h()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 64), (1, 40, 1)])
+def test_fp8_shapes(native, orc, cuda, shape):
+    """Ragged shapes against the restatement of the FP8 arithmetic (bounds as above)."""
+    n, m, r = shape
+    iters = 20
+    V, W, H = G.gen_euclidean(n, m, r, n + m), *G.gen_init(n, m, r, n + m)
+    p = G.DeviceNMF(G.EUCLIDEAN, V, W, H, n, m, r, precision=G.FP8)
+    p.iterate(iters)
+    gobj = p.objective(G.OBJ_FROBENIUS)
+    vq = orc.fp8_tile_round(G.as_matrix(V, n, m))
+    we, he = G.as_matrix(W, n, r).astype(np.float64), G.as_matrix(H, r, m).astype(np.float64)
+    for _ in range(iters):
+        we, he = orc.np_euc_iter_shadow(vq, we, he, G.REFERENCE_EPS, lambda X: orc.fp8_block_round(X, 0),
+                                        lambda X: orc.fp8_block_round(X, 1))
+    gw, gh = G.as_matrix(p.get_w(), n, r), G.as_matrix(p.get_h(), r, m)
+    ref_e = orc.np_frob(vq, we, he)
+    floor = 1e-9 * float(np.sum(vq * vq))  # an exactly fitted problem has objective ~ 0
+    excess = max(0.0, abs(gobj - ref_e) - fp32_objective_allowance(ref_e, float(np.sum(vq * vq)), r))
+    emu_obj = excess / max(ref_e, floor)
+    print(f"fp8 {n}x{m} r={r} after {iters} iterations: emu obj {emu_obj:.2e} W {rel_fro(gw, we):.2e} "
+          f"H {rel_fro(gh, he):.2e}")
+    assert emu_obj < 1e-3
+    assert rel_fro(gw, we) < 1e-2 and rel_fro(gh, he) < 1e-2

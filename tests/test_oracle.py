@@ -261,3 +261,82 @@ def test_objectives_closed_forms(orc):
     assert q == pytest.approx(sum(V.ravel() * np.log10(WH.ravel()) - WH.ravel()), rel=1e-14)
     assert orc.np_frob(V, W, H) == pytest.approx(frob)
     assert orc.np_kl(V, W, H) == pytest.approx(kl)
+
+
+# ------------------------------------------ low-precision operand restatement -
+def _e4m3_values():
+    """All non-negative finite OCP E4M3 values, by code (0x00 .. 0x7e)."""
+    out = []
+    for q in range(0x7F):
+        E, mnt = q >> 3, q & 7
+        out.append(mnt / 8 * 2.0 ** -6 if E == 0 else (1 + mnt / 8) * 2.0 ** (E - 7))
+    return np.array(out)
+
+
+def test_bf16_round_matches_torch(orc):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.random(5000) * 100, rng.random(1000) * 1e-30, [0.0, 1.0, 3.0]]).astype(np.float32)
+    # exact midpoints between neighbouring bf16 values: ties go to the even value
+    b = (x.view(np.uint32) & 0xFFFF0000).view(np.float32)
+    mid = (b.view(np.uint32) | 0x8000).view(np.float32)
+    x = np.concatenate([x, mid])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(orc.bf16_round(x), ref)
+
+
+def test_e4m3_round_brute_force(orc):
+    vals = _e4m3_values()
+    rng = np.random.default_rng(22)
+    ys = np.concatenate([rng.random(2000) * 500, rng.random(500) * 0.02, vals, (vals[1:] + vals[:-1]) / 2,
+                         [0.0, 448.0, 449.0, 464.0, 500.0, 2.0 ** -10, 3 * 2.0 ** -10]])
+    got = orc.e4m3_round(ys)
+    for y, g in zip(ys, got):
+        d = np.abs(vals - y)
+        ties = np.nonzero(d == d.min())[0]
+        code = ties[ties % 2 == 0][0] if np.any(ties % 2 == 0) else ties[0]
+        assert g == vals[code], (y, g, vals[code])
+
+
+@pytest.mark.parametrize("amax", [1e-3, 0.7, 1.0, 447.0, 448.0, 449.0, 3e4, 2.5e7])
+def test_e8m0_smallest_fitting_power(orc, amax):
+    s = int(orc.e8m0_for_amax(amax))
+    assert amax / 2.0 ** (s - 127) <= 448.0 < amax / 2.0 ** (s - 128)
+
+
+def test_fp8_tile_and_block_rounding(orc):
+    """Every 32 x 32 tile (or 32-block) is rounded with its own smallest fitting power-of-two scale."""
+    rng = np.random.default_rng(23)
+    V = (rng.random((70, 45)) * np.exp(rng.normal(0, 3, (70, 45)))).astype(np.float32)
+    q = orc.fp8_tile_round(V)
+    vals = _e4m3_values()
+    for i0 in (0, 32, 64):
+        for u0 in (0, 32):
+            blk = V[i0:i0 + 32, u0:u0 + 32].astype(np.float64)
+            s = int(orc.e8m0_for_amax(blk.max()))
+            unit = 2.0 ** (s - 127)
+            y = (blk / unit).ravel()
+            near = vals[np.abs(vals[None, :] - y[:, None]).argmin(axis=1)]
+            # nearest value (ties are measure-zero for random data)
+            assert np.array_equal(q[i0:i0 + 32, u0:u0 + 32].ravel(), near * unit)
+    W = rng.random((70, 3)).astype(np.float32)
+    qb = orc.fp8_block_round(W, 0)
+    for a in range(3):
+        for i0 in (0, 32, 64):
+            blk = W[i0:i0 + 32, a].astype(np.float64)
+            unit = 2.0 ** (int(orc.e8m0_for_amax(blk.max())) - 127)
+            assert np.array_equal(qb[i0:i0 + 32, a], orc.e4m3_round(blk / unit) * unit)
+    assert np.array_equal(orc.fp8_block_round(W.T, 1), qb.T)
+
+
+def test_shadow_iteration_order(orc):
+    """W is shadowed before the H half, the new H before the W half; identity shadows give np_euc_iter."""
+    V, W, H, _, _ = rand_problem(20, 15, 3, 24)
+    seen = []
+    ident_w = lambda X: (seen.append(("w", X.copy())), X)[1]
+    ident_h = lambda X: (seen.append(("h", X.copy())), X)[1]
+    w1, h1 = orc.np_euc_iter_shadow(V, W, H, 1e-8, ident_w, ident_h)
+    w2, h2 = orc.np_euc_iter(V, W, H, 1e-8)
+    assert np.array_equal(w1, w2) and np.array_equal(h1, h2)
+    assert [k for k, _ in seen] == ["w", "h"]
+    assert np.array_equal(seen[0][1], W) and np.array_equal(seen[1][1], h2)
