@@ -89,19 +89,26 @@ def main():
         ms = (t1 - t0) * 1e3 / args.steps
         obj1 = d.objective(obj_kind)
 
-        # Per-kernel durations: HIP events on the stream the kernels are launched on.
-        def time_pass(which):
-            ev0, ev1 = hip.Event(), hip.Event()
-            for _ in range(3):
-                d.run_pass(which)
-            ev0.record(d.stream)
-            for _ in range(args.kernel_reps):
-                d.run_pass(which)
-            ev1.record(d.stream)
-            ev1.synchronize()
-            return ev0.elapsed_ms(ev1) / args.kernel_reps
+        # Per-pass durations: HIP events on the stream the kernels are launched on, with
+        # the H and W passes alternating as they do in an iteration (each streams its own
+        # copy of V, so neither finds the other's data in the last-level cache).
+        def time_passes():
+            for _ in range(2):
+                d.run_pass(0)
+                d.run_pass(1)
+            evs = [hip.Event() for _ in range(2 * args.kernel_reps + 1)]
+            evs[0].record(d.stream)
+            for k in range(args.kernel_reps):
+                d.run_pass(0)
+                evs[2 * k + 1].record(d.stream)
+                d.run_pass(1)
+                evs[2 * k + 2].record(d.stream)
+            evs[-1].synchronize()
+            th = sum(evs[2 * k].elapsed_ms(evs[2 * k + 1]) for k in range(args.kernel_reps)) / args.kernel_reps
+            tw = sum(evs[2 * k + 1].elapsed_ms(evs[2 * k + 2]) for k in range(args.kernel_reps)) / args.kernel_reps
+            return th, tw
 
-        ms_h, ms_w = time_pass(0), time_pass(1)
+        ms_h, ms_w = time_passes()
 
         # Objective (against the FP32 V, FP64 reduction on the GPU) after fixed iteration
         # counts from the initial W, H: the precision gap is stated for every result.

@@ -34,20 +34,20 @@ static int choose_slices(int tiles, int K, int kc, int target, int* kslice) {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-size_t vstore_bf16_offset_t(int n, int m) { return align256((size_t)m * round_up(n, 8) * 2); }
+size_t vstore_bf16_offset_t(int n, int m) { return align256((size_t)m * bf16_ld(n) * 2); }
 
 Fp8Store fp8_store_layout(int n, int m) {
     Fp8Store l;
-    l.off_t = align256((size_t)m * round_up(n, 16));
-    l.off_sc = l.off_t + align256((size_t)n * round_up(m, 16));
-    l.bytes = l.off_sc + align256((size_t)((n + 31) / 32) * ((m + 31) / 32));
+    l.off_t = align256((size_t)m * fp8_ld(n));
+    l.off_sc = l.off_t + align256((size_t)n * fp8_ld(m));
+    l.bytes = l.off_sc + align256((size_t)(fp8_ld(n) / 32) * (fp8_ld(m) / 32));
     return l;
 }
 
 size_t vstore_bytes(int precision, int n, int m) {
     if (n < 1 || m < 1) return 0;
     if (precision == GPUMLIB_NMF_FP32) return (size_t)n * m * 4;
-    if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256((size_t)n * round_up(m, 8) * 2);
+    if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256((size_t)n * bf16_ld(m) * 2);
     if (precision == GPUMLIB_NMF_FP8) return fp8_store_layout(n, m).bytes;
     return 0;
 }
@@ -68,21 +68,24 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->rpad = round_up(r, 4);
     pl->ldr = pl->rpad;
     pl->ldp = pl->npad + pl->rpad;
-    pl->ldv8 = round_up(n, 8);
-    pl->ldt8 = round_up(m, 8);
-    pl->ld16n = round_up(n, 16);
-    pl->ld16m = round_up(m, 16);
-    pl->nbi = (n + 31) / 32;
-    pl->nbu = (m + 31) / 32;
+    pl->ldv8 = bf16_ld(n);
+    pl->ldt8 = bf16_ld(m);
+    pl->ld16n = fp8_ld(n);
+    pl->ld16m = fp8_ld(m);
+    pl->nbi = pl->ld16n / 32;
+    pl->nbu = pl->ld16m / 32;
     const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
-    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 64 : 128);
-    // FP32 passes give the gram its own tile; the BF16 / FP8 passes fold it into
-    // the first column tile of every slice.
-    const int gram_tile = (precision == GPUMLIB_NMF_FP32) ? euc : 0;
-    pl->tiles1 = (m + TILE - 1) / TILE + gram_tile;
+    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 128);
+    // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
+    // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
+    // the BF16 / FP8 passes fold the whole gram into the first column tile.
+    const int nbk = (r + 31) / 32, gram_tiles = (precision == GPUMLIB_NMF_FP32 && euc) ? nbk * nbk : 0;
+    const int tw = (precision == GPUMLIB_NMF_BF16) ? TILE_LP : TILE;  // columns (rows) per workgroup
+    const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
+    pl->tiles1 = vt1 > gram_tiles ? vt1 : gram_tiles;
     pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
-    pl->tiles3 = (n + TILE - 1) / TILE + gram_tile;
+    pl->tiles3 = vt3 > gram_tiles ? vt3 : gram_tiles;
     pl->S3 = choose_slices(pl->tiles3, m, kc, target, &pl->ks3);
     // FP32 W pass writes P[s][a*ldp + i]; the BF16 pass over V^T writes P[s][i*ldr + a].
     pl->pstride3 = (precision == GPUMLIB_NMF_FP32) ? (long)r * pl->ldp : (long)(n + r) * pl->ldr;
@@ -90,7 +93,10 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     const size_t pbytes = align256(sizeof(float) * (p1 > p3 ? p1 : p3));
     const int smax = pl->S1 > pl->S3 ? pl->S1 : pl->S3;
     const size_t sbytes = align256(sizeof(float) * (size_t)smax * pl->rpad);
-    pl->nobjp = pl->tiles1 * pl->S1;  // one objective partial per H-pass workgroup
+    pl->tiles_obj = (m + TILE - 1) / TILE;
+    pl->S_obj = choose_slices(pl->tiles_obj, n, KC, target, &pl->ks_obj);
+    // one objective partial per workgroup of the H pass (FP32) or of the objective pass
+    pl->nobjp = pl->tiles1 * pl->S1 > pl->tiles_obj * pl->S_obj ? pl->tiles1 * pl->S1 : pl->tiles_obj * pl->S_obj;
     const size_t obytes = align256(sizeof(double) * (size_t)pl->nobjp);
     size_t off = 0;
     pl->off_P = off;
@@ -191,7 +197,9 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
                          double* d_objective, void* ws, hipStream_t st) {
     PassArgs a1;
     pass_args(pl, (const float*)V, W, H, ws, eps, 1, &a1);
-    a1.tiles = (pl.m + TILE - 1) / TILE;  // no gram tile
+    a1.tiles = pl.tiles_obj;  // V's column tiles only, slices chosen for this pass
+    a1.S = pl.S_obj;
+    a1.kslice = pl.ks_obj;
     if (pl.precision == GPUMLIB_NMF_BF16) {
         CK(launch_objective_bf16v(a1, objective, st));
     } else if (pl.precision == GPUMLIB_NMF_FP8) {
@@ -433,12 +441,11 @@ hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float* V, 
     if (precision == GPUMLIB_NMF_FP8) {
         const Fp8Store L = fp8_store_layout(n, m);
         uint8_t* base = (uint8_t*)vstore;
-        return launch_pack_v_fp8(V, n, m, base, round_up(n, 16), base + L.off_t, round_up(m, 16), base + L.off_sc,
-                                 stream);
+        return launch_pack_v_fp8(V, n, m, base, fp8_ld(n), base + L.off_t, fp8_ld(m), base + L.off_sc, stream);
     }
     uint16_t* Vb = (uint16_t*)vstore;
     uint16_t* Vt = (uint16_t*)((char*)vstore + vstore_bf16_offset_t(n, m));
-    return launch_pack_v_bf16(V, n, m, Vb, round_up(n, 8), Vt, round_up(m, 8), stream);
+    return launch_pack_v_bf16(V, n, m, Vb, bf16_ld(n), Vt, bf16_ld(m), stream);
 }
 
 hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void* V, const float* W,

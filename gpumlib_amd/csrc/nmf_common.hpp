@@ -25,6 +25,7 @@ constexpr int LDK = KC + 4;
 constexpr int TILE = 128;
 constexpr int NTHREADS = 256;
 // Columns of H / rows of W handled by one update workgroup.
+constexpr int TILE_LP = 64;   // columns per workgroup of the BF16 / FP8 passes
 constexpr int UPD_COLS = 32;
 constexpr int UPD_ROWS = 32;
 
@@ -65,17 +66,19 @@ __device__ __forceinline__ float4 load4(const float* p, bool ok, int k, int kend
     if (k + 3 < kend) v.w = p[3];
     return v;
 }
-// Gram reduction by the last arriving workgroup.  Each of the S workgroups of a
-// gram tile has written its slice partial; the one that arrives last sums the S
-// partials in slice order (the same order every time) into G and resets the
-// counter for the next pass.  off(e) is the offset of gram element e inside a
-// slice.  Device-scope fences make the other XCDs' partials visible.
+// Gram reduction by the last arriving workgroup.  The workgroups that compute
+// parts of the gram (arrivals of them per launch) have written their parts of
+// the S slice partials; the one that arrives last sums the S partials in slice
+// order (the same order every time) into G and resets the counter for the next
+// pass.  off(e) is the offset of gram element e inside a slice.  Device-scope
+// fences make the other XCDs' partials visible.
 template <typename OffFn>
-__device__ void gram_finish(const float* P, long pstride, int S, int count, OffFn off, float* G, int* counter) {
+__device__ void gram_finish(const float* P, long pstride, int S, int arrivals, int count, OffFn off, float* G,
+                            int* counter) {
     __shared__ int last;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(counter, 1) == S - 1) ? 1 : 0;
+    if (threadIdx.x == 0) last = (atomicAdd(counter, 1) == arrivals - 1) ? 1 : 0;
     __syncthreads();
     if (!last) return;
     __threadfence();
@@ -116,11 +119,12 @@ __device__ void gram_finish(const float* P, long pstride, int S, int count, OffF
 // gram, count floats (a multiple of 4, 16-byte aligned) per slice.  Each thread
 // sums two float4 groups, 4 slices per step (8 loads of 16 bytes in flight),
 // adding the slices in order.
-__device__ inline void gram_finish4(const float* P, long pstride, int S, int count, float* G, int* counter) {
+__device__ inline void gram_finish4(const float* P, long pstride, int S, int arrivals, int count, float* G,
+                                    int* counter) {
     __shared__ int last;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(counter, 1) == S - 1) ? 1 : 0;
+    if (threadIdx.x == 0) last = (atomicAdd(counter, 1) == arrivals - 1) ? 1 : 0;
     __syncthreads();
     if (!last) return;
     __threadfence();

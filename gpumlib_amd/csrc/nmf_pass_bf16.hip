@@ -1,5 +1,5 @@
 // BF16 storage path: V held in BF16 in both orientations (V and V^T, each
-// column-major with a 16-byte-aligned leading dimension), BF16 shadows of W and
+// column-major, leading dimensions whole 256-element chunks, zero padded), BF16 shadows of W and
 // H as MFMA operands, FP32 accumulation and FP32 updates.
 //
 // With V^T stored, the W half of an iteration is the H half of the transposed
@@ -14,141 +14,165 @@
 #include "nmf_common.hpp"
 #include "nmf_internal.hpp"
 
+#include <type_traits>
+
 namespace gpumlib_nmf {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // a native vector: staging stays in registers
 
-constexpr int KB = 64;          // K per pipeline step (4 MFMA k-steps of 16)
-constexpr int LDB = KB + 8;     // LDS row pitch in bf16 elements (144 bytes)
+constexpr int KB = 256;         // K per chunk: 512 contiguous bytes of every column
+constexpr int LDB = KB + 8;     // LDS row pitch in bf16 elements (528 bytes)
 
-__device__ __forceinline__ f32x16 mfma_bf16(uint4 a, uint4 b, f32x16 acc) {
+__device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
                                                    0, 0, 0);
 }
 
-// Guarded load of 8 bf16 p[0..7]; element j exists iff ok && k + j < kend.  Rows
-// start 16-byte aligned (leading dimensions are multiples of 8).
-__device__ __forceinline__ uint4 load8_bf16(const uint16_t* p, bool ok, int k, int kend) {
-    if (!ok) return make_uint4(0, 0, 0, 0);
-    if (k + 7 < kend) return *reinterpret_cast<const uint4*>(p);
-    uint16_t e[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) e[j] = (k + j < kend) ? p[j] : (uint16_t)0;
-    return make_uint4(e[0] | ((uint32_t)e[1] << 16), e[2] | ((uint32_t)e[3] << 16), e[4] | ((uint32_t)e[5] << 16),
-                      e[6] | ((uint32_t)e[7] << 16));
-}
-
-// One workgroup per (slice s, column tile t); 4 waves, wave w owns columns
-// w*32 .. w*32+31 of the 128-column tile and all MB row blocks.  The first
-// column tile of every slice also accumulates the slice's gram A^T A from the
-// A rows already staged in LDS (gram block (mb, nb) on wave mb*MB + nb, modulo
-// 4), so the grid is exactly tiles x S workgroups of one shape and the gram
-// costs no extra pass over A.  The last of the S gram workgroups reduces the
-// partial grams into G.
+// One workgroup per (slice s, column tile t) of TILE_LP = 64 columns.  Every chunk is
+// 256 elements deep, so each column is read in 512-byte runs (long runs keep HBM pages
+// open; 128-byte runs over 128 columns measured markedly slower).  The chunk's B
+// columns and A rows are staged through LDS; the next chunk's loads are in flight in
+// registers while the current one is multiplied.  The 2*MB output blocks of 32 x 32
+// (rank block mb, column block nb) are spread over the 4 waves.
+//
+// The first column tile of every slice also accumulates the slice's gram A^T A (gram
+// block (mb, nb) on wave mb*MB + nb, modulo 4); the last of the S gram workgroups
+// reduces the partial grams into G.
+//
+// V, V^T and the shadows are padded to whole chunks with zeros, so no load needs a
+// bounds check; lanes past the last column or rank row re-read the last valid one:
+// those products land only in output columns that are not stored and in rank rows
+// >= r of P and of the gram, which no update reads.
 template <int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     constexpr int RP = MB * 32;
-    constexpr int GB = (MB * MB + 3) / 4;  // gram blocks per wave
-    __shared__ __attribute__((aligned(16))) uint16_t sA[2][RP][LDB];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[2][TILE][LDB];
+    constexpr int NBLK = 2 * MB;                      // output blocks of 32 x 32
+    constexpr int BPW = NBLK >= 4 ? NBLK / 4 : 1;     // blocks per wave
+    constexpr int GB = (MB * MB + 3) / 4;             // gram blocks per wave
+    constexpr int BL = TILE_LP * KB / 8 / NTHREADS;   // 16-byte B loads per thread per chunk
+    constexpr int AL = RP * KB / 8 / NTHREADS;        // 16-byte A loads per thread per chunk
+    constexpr int LPC = KB / 8;                       // lanes per column (or row) in a load
+    __shared__ __attribute__((aligned(16))) uint16_t sA[RP][LDB];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[TILE_LP][LDB];
 
     const int total = p.tiles * p.S;
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
     if (w >= total) return;
     const int s = w / p.tiles, t = w - s * p.tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const bool gram = (t == 0) && p.G != nullptr;
-    const int col0 = t * TILE;
-    const int ncols = min(TILE, p.N - col0);
+    const int col0 = t * TILE_LP;
+    const int ncols = min(TILE_LP, p.N - col0);
     const int kbeg = s * p.kslice;
-    const int kend = min(p.K, kbeg + p.kslice);
+    const int kend = min(round_up(p.K, KB), kbeg + p.kslice);
 
-    uint4 ra[MB], rb[4];
-    auto gload = [&](int k0) {
+    u32x4 rb[BL], ra[AL];
+    auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 8;
-            rb[j] = load8_bf16(p.B + (size_t)(col0 + col) * p.ldb + k0 + kq, col < ncols, k0 + kq, kend);
+        for (int j = 0; j < BL; j++) {
+            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 8;
+            const int cc = col < ncols ? col : ncols - 1;
+            rb[j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
         }
 #pragma unroll
-        for (int j = 0; j < MB; j++) {
-            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 8;
-            ra[j] = load8_bf16(p.A + (size_t)a * p.lda + k0 + kq, a < p.r, k0 + kq, kend);
+        for (int j = 0; j < AL; j++) {
+            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 8;
+            const int aa = a < p.r ? a : p.r - 1;
+            ra[j] = *reinterpret_cast<const u32x4*>(p.A + (size_t)aa * p.lda + k0 + kq);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the multiply
+    };
+    auto sstore = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < BL; j++) {
+            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 8;
+            *reinterpret_cast<u32x4*>(&sB[col][kq]) = rb[j];
+        }
+#pragma unroll
+        for (int j = 0; j < AL; j++) {
+            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 8;
+            *reinterpret_cast<u32x4*>(&sA[a][kq]) = ra[j];
         }
     };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 8;
-            *reinterpret_cast<uint4*>(&sB[buf][col][kq]) = rb[j];
-        }
-#pragma unroll
-        for (int j = 0; j < MB; j++) {
-            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 8;
-            *reinterpret_cast<uint4*>(&sA[buf][a][kq]) = ra[j];
-        }
-    };
 
-    f32x16 acc[MB], gacc[GB];
+    f32x16 acc[BPW], gacc[GB];
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++)
-        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    for (int q = 0; q < BPW; q++)
+        for (int v = 0; v < 16; v++) acc[q][v] = 0.f;
 #pragma unroll
     for (int q = 0; q < GB; q++)
         for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
-    const int nchunks = (kend - kbeg + KB - 1) / KB;
-    gload(kbeg);
-    sstore(0);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ch++) {
-        const int buf = ch & 1;
-        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KB);
-        // lane (c, h) supplies k = g*16 + h*8 + j (j < 8) for both operands
+    // lane (c, h) supplies k = g*16 + h*8 + j (j < 8) for both operands; the gram-carrying
+    // and the plain workgroups run separate copies of the loop (no branch between MFMAs)
+    auto compute = [&](auto gram_tag) __attribute__((always_inline)) {
+        constexpr bool GR = decltype(gram_tag)::value;
 #pragma unroll
         for (int g = 0; g < KB / 16; g++) {
-            const uint4 bv = *reinterpret_cast<const uint4*>(&sB[buf][wave * 32 + c][g * 16 + h * 8]);
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++) {
-                const uint4 av = *reinterpret_cast<const uint4*>(&sA[buf][mb * 32 + c][g * 16 + h * 8]);
-                acc[mb] = mfma_bf16(av, bv, acc[mb]);
+            for (int q = 0; q < BPW; q++) {
+                const int blk = wave * BPW + q;
+                if (NBLK % 4 == 0 || blk < NBLK) {
+                    const int mb = blk >> 1, nb = blk & 1;
+                    const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
+                    const u32x4 bv = *reinterpret_cast<const u32x4*>(&sB[nb * 32 + c][g * 16 + h * 8]);
+                    acc[q] = mfma_bf16(av, bv, acc[q]);
+                }
             }
-        }
-        if (gram) {
+            if constexpr (GR) {
 #pragma unroll
-            for (int q = 0; q < GB; q++) {
-                const int blk = wave + 4 * q;
-                if (blk < MB * MB) {
-                    const int mb = blk / MB, nb = blk - (blk / MB) * MB;
-#pragma unroll
-                    for (int g = 0; g < KB / 16; g++) {
-                        const uint4 av = *reinterpret_cast<const uint4*>(&sA[buf][mb * 32 + c][g * 16 + h * 8]);
-                        const uint4 bv = *reinterpret_cast<const uint4*>(&sA[buf][nb * 32 + c][g * 16 + h * 8]);
+                for (int q = 0; q < GB; q++) {
+                    const int blk = wave + 4 * q;
+                    if ((MB * MB) % 4 == 0 || blk < MB * MB) {
+                        const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+                        const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
+                        const u32x4 bv = *reinterpret_cast<const u32x4*>(&sA[nb * 32 + c][g * 16 + h * 8]);
                         gacc[q] = mfma_bf16(av, bv, gacc[q]);
                     }
                 }
             }
         }
-        if (ch + 1 < nchunks) sstore(buf ^ 1);
-        __syncthreads();
-    }
+    };
 
-    // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
+    const int nchunks = (kend - kbeg) / KB;  // whole chunks (padded operands)
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+        gload(kbeg);
+        sstore();
+        __syncthreads();
+        for (int ch = 0; ch < nchunks; ch++) {
+            const bool more = ch + 1 < nchunks;
+            if (more) gload(kbeg + (ch + 1) * KB);
+            compute(gram_tag);
+            __syncthreads();
+            if (more) {
+                sstore();
+                __syncthreads();
+            }
+        }
+    };
+    if (gram)
+        mainloop(std::true_type{});
+    else
+        mainloop(std::false_type{});
+
+    // acc[q][v] = P(a = mb*32 + acc_row(v,h), u = nb*32 + c) for block wave*BPW + q
     float* slice = p.P + (size_t)s * p.pstride;
-    const int ul = wave * 32 + c;
-    if (ul < ncols) {
+#pragma unroll
+    for (int q = 0; q < BPW; q++) {
+        const int blk = wave * BPW + q;
+        if (blk >= NBLK) continue;
+        const int mb = blk >> 1, ul = (blk & 1) * 32 + c;
+        if (ul >= ncols) continue;
         float* dst = slice + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const int a0 = mb * 32 + g * 8 + h * 4;
-                if (a0 < p.ldr) {
-                    float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
-                    *reinterpret_cast<float4*>(dst + a0) = o;
-                }
-            }
+        for (int g = 0; g < 4; g++) {
+            const int a0 = mb * 32 + g * 8 + h * 4;
+            if (a0 < p.ldr)
+                *reinterpret_cast<float4*>(dst + a0) =
+                    make_float4(acc[q][g * 4], acc[q][g * 4 + 1], acc[q][g * 4 + 2], acc[q][g * 4 + 3]);
         }
     }
     if (gram) {
@@ -168,13 +192,14 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
                         make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
             }
         }
-        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.r * p.ldr, p.G, p.gcount);
+        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G, p.gcount);
     }
 }
 
 // ---------------------------------------------------------------------------
 // V (n x m, FP32 column-major) -> Vb (column-major, ld ldv) and Vt (V^T column-major,
 // ld ldt), both BF16 (round to nearest even).  64 x 64 tiles through LDS.
+// The grid covers the padded stores (ldv and ldt multiples of 256): padding gets zeros.
 __global__ __launch_bounds__(NTHREADS) void k_pack_v_bf16(const float* __restrict__ V, int n, int m,
                                                           uint16_t* __restrict__ Vb, int ldv,
                                                           uint16_t* __restrict__ Vt, int ldt) {
@@ -183,32 +208,30 @@ __global__ __launch_bounds__(NTHREADS) void k_pack_v_bf16(const float* __restric
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     for (int uu = ty; uu < 64; uu += 4) {
         const int i = i0 + tx, u = u0 + uu;
-        uint16_t b = 0;
-        if (i < n && u < m) {
-            b = __builtin_bit_cast(uint16_t, (__bf16)V[(size_t)u * n + i]);
-            Vb[(size_t)u * ldv + i] = b;
-        }
+        const uint16_t b = (i < n && u < m) ? __builtin_bit_cast(uint16_t, (__bf16)V[(size_t)u * n + i]) : 0;
+        if (u < m) Vb[(size_t)u * ldv + i] = b;
         tile[uu][tx] = b;
     }
     __syncthreads();
     for (int ii = ty; ii < 64; ii += 4) {
         const int u = u0 + tx, i = i0 + ii;
-        if (i < n && u < m) Vt[(size_t)i * ldt + u] = tile[tx][ii];
+        if (i < n) Vt[(size_t)i * ldt + u] = tile[tx][ii];
     }
 }
 
 // W (n x r, FP32) -> Wb (column-major, ld ldw);  H (r x m, FP32) -> Ht (H^T column-major, ld ldh).
+// Padding (i in [n, ldw), u in [m, ldh)) gets zeros.
 __global__ __launch_bounds__(NTHREADS) void k_shadow_bf16(const float* __restrict__ W, const float* __restrict__ H,
                                                           int n, int m, int r, uint16_t* __restrict__ Wb, int ldw,
                                                           uint16_t* __restrict__ Ht, int ldh) {
-    const long nw = (long)n * r, nh = (long)r * m;
+    const long nw = (long)ldw * r, nh = (long)r * ldh;
     for (long e = (long)blockIdx.x * NTHREADS + threadIdx.x; e < nw + nh; e += (long)gridDim.x * NTHREADS) {
         if (e < nw) {
-            const long a = e / n, i = e - a * n;
-            Wb[a * ldw + i] = __builtin_bit_cast(uint16_t, (__bf16)W[e]);
+            const long a = e / ldw, i = e - a * ldw;
+            Wb[e] = i < n ? __builtin_bit_cast(uint16_t, (__bf16)W[a * n + i]) : (uint16_t)0;
         } else {
-            const long f = e - nw, a = f / m, u = f - a * m;
-            Ht[a * ldh + u] = __builtin_bit_cast(uint16_t, (__bf16)H[u * r + a]);
+            const long f = e - nw, a = f / ldh, u = f - a * ldh;
+            Ht[f] = u < m ? __builtin_bit_cast(uint16_t, (__bf16)H[u * r + a]) : (uint16_t)0;
         }
     }
 }
@@ -234,14 +257,15 @@ hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st) {
 
 hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ldv, uint16_t* Vt, int ldt,
                               hipStream_t st) {
-    dim3 grid((n + 63) / 64, (m + 63) / 64);
+    if (ldv % 64 || ldt % 64 || ldv < n || ldt < m) return hipErrorInvalidValue;
+    dim3 grid(ldv / 64, ldt / 64);
     hipLaunchKernelGGL(k_pack_v_bf16, grid, dim3(NTHREADS), 0, st, V, n, m, Vb, ldv, Vt, ldt);
     return hipGetLastError();
 }
 
 hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
                               int ldh, hipStream_t st) {
-    const long total = (long)n * r + (long)r * m;
+    const long total = (long)ldw * r + (long)r * ldh;
     long blocks = (total + NTHREADS - 1) / NTHREADS;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_shadow_bf16, dim3((unsigned)blocks), dim3(NTHREADS), 0, st, W, H, n, m, r, Wb, ldw, Ht, ldh);

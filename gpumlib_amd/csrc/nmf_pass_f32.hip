@@ -23,6 +23,8 @@
 #include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
 
+#include <type_traits>
+
 namespace gpumlib_nmf {
 
 // ---------------------------------------------------------------------------
@@ -78,7 +80,12 @@ __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
 // entry; GEMM2 false: objective only (no partials written).
 // TV: element type of V (float, or uint16_t holding BF16 for the objective of a
-// BF16-stored V; that case never has a gram tile).
+// BF16-stored V, or uint8_t holding FP8; those run with GEMM2 false).
+// Euclidean gram W^T W: its nb x nb blocks of 32 x 32 are spread over the first
+// tiles of every slice (block t on tile t; tiles past the last column tile of V
+// carry a gram block only), and inside a tile the 4 waves each take a quarter of
+// every chunk's k range, so no workgroup carries more than an eighth extra MFMA
+// work and the grid stays one shape.
 template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
 __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
@@ -88,13 +95,15 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     __shared__ double red[NTHREADS / 64];
 
     const int s = w / p.tiles, t = w - s * p.tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const int vt = (p.m + TILE - 1) / TILE;
-    const bool gram = (!KL) && (t == vt);
     const TV* Vt = reinterpret_cast<const TV*>(p.V);
-    const int col0 = gram ? 0 : t * TILE;
-    const int ncols = gram ? p.r : min(TILE, p.m - col0);
+    const int col0 = t * TILE;
+    const int ncols = (t < vt) ? min(TILE, p.m - col0) : 0;
+    const int nblk = nb * nb;
+    const bool gcar = (!KL) && GEMM2 && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
     const int kend = min(p.n, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -106,18 +115,16 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
             const int b = 2 * kk + h;
-            hreg[kk] = (!gram && u < p.m && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
+            hreg[kk] = (u < p.m && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
         }
     }
 
     float4 ra[MB], rb[4];
-    auto gload = [&](int k0) {
+    auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
-            if (gram) {
-                rb[j] = load4(p.W + (size_t)col * p.n + k0 + kq, col < ncols, k0 + kq, kend, vec);
-            } else if constexpr (sizeof(TV) == 1) {
+            if constexpr (sizeof(TV) == 1) {
                 // FP8 E4M3 with the E8M0 scale of V's 32 x 32 tile (k..k+3 share one tile)
                 const int kk = k0 + kq, u = col0 + col;
                 float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -144,7 +151,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             ra[j] = load4(p.W + (size_t)a * p.n + k0 + kq, a < p.r, k0 + kq, kend, vec);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
@@ -157,15 +164,21 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         }
     };
 
-    f32x16 acc[MB];
+    f32x16 acc[MB], gacc;
 #pragma unroll
     for (int mb = 0; mb < MB; mb++)
         for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    for (int v = 0; v < 16; v++) gacc[v] = 0.f;
+    const int gmb = gcar ? t / nb : 0, gnb = gcar ? t - (t / nb) * nb : 0;  // carried gram block
     double objacc = 0.0;
     float colsum = 0.f;  // KL: sum over this slice of W(i, tid) (tile 0 only)
     const bool do_sums = KL && GEMM2 && (t == 0) && (tid < RP);
 
+    // The gram-carrying and the plain workgroups run separate copies of the loop, so
+    // that no MFMA accumulator sits under a data-dependent branch.
     const int nchunks = (kend - kbeg + KC - 1) / KC;
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+    constexpr bool GR = decltype(gram_tag)::value;
     gload(kbeg);
     sstore(0);
     __syncthreads();
@@ -181,7 +194,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             }
         }
 
-        if (NEED_WH && !gram) {
+        if (NEED_WH) {
             // wh(i, u) for the 32 rows of this step and the wave's 32 columns.
             f32x16 wh;
             for (int v = 0; v < 16; v++) wh[v] = 0.f;
@@ -227,16 +240,28 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 }
             }
         }
+        if constexpr (GR) {  // this wave's k group (wave*8 .. wave*8+7) of the chunk for the carried block
+            const float4 av = *reinterpret_cast<const float4*>(&sA[buf][gmb * 32 + c][wave * 8 + h * 4]);
+            const float4 bv = *reinterpret_cast<const float4*>(&sA[buf][gnb * 32 + c][wave * 8 + h * 4]);
+            gacc = mfma32x2(av.x, bv.x, gacc);
+            gacc = mfma32x2(av.y, bv.y, gacc);
+            gacc = mfma32x2(av.z, bv.z, gacc);
+            gacc = mfma32x2(av.w, bv.w, gacc);
+        }
         if (ch + 1 < nchunks) sstore(buf ^ 1);
         __syncthreads();
     }
+    };
+    if (gcar)
+        mainloop(std::true_type{});
+    else
+        mainloop(std::false_type{});
 
     if (GEMM2) {
         // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
         const int ul = wave * 32 + c;
         if (ul < ncols) {
-            const int ucol = (gram ? p.m : col0) + ul;
-            float* dst = p.P + (size_t)s * p.pstride + (size_t)ucol * p.ldr;
+            float* dst = p.P + (size_t)s * p.pstride + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
@@ -250,10 +275,22 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             }
         }
         if (do_sums && tid < p.r) p.Psum[(size_t)s * round_up(p.r, 4) + tid] = colsum;
-        if (gram) {
-            const int rpad = round_up(p.r, 4);
-            const long base = (long)p.m * p.ldr;  // gram columns u = m + b, ldr == rpad
-            gram_finish4(p.P + base, p.pstride, p.S, p.r * rpad, p.G, p.gcount);
+        if (gcar) {
+            // gram(a, b) at column u = m + b of the slice (ldr == rpad); the 4 waves' k quarters
+            // are summed in wave order through LDS (free after the loop)
+            float* gred = &sB[0][0][0];
+            float* gdst = p.P + (size_t)s * p.pstride + (size_t)p.m * p.ldr;
+#pragma unroll
+            for (int v = 0; v < 16; v++) gred[(wave * 64 + lane) * 16 + v] = gacc[v];
+            __syncthreads();
+            for (int e = tid; e < 64 * 16; e += NTHREADS) {
+                const int le = e >> 4, v = e & 15;
+                const float g = ((gred[le * 16 + v] + gred[(64 + le) * 16 + v]) + gred[(128 + le) * 16 + v]) +
+                                gred[(192 + le) * 16 + v];
+                const int a = gmb * 32 + acc_row(v, le >> 5), b = gnb * 32 + (le & 31);
+                if (a < p.ldr && b < p.r) gdst[(size_t)b * p.ldr + a] = g;
+            }
+            gram_finish4(gdst - (size_t)s * p.pstride, p.pstride, p.S, p.S * nblk, p.r * p.ldr, p.G, p.gcount);
         }
     }
     if (OBJ != OBJ_NONE) {
@@ -263,7 +300,8 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 }
 
 // ---------------------------------------------------------------------------
-// W pass, Euclidean rule: P[s](i,a) = sum_u A(i,u) H(a,u), A = V (row tiles) or H (gram tile).
+// W pass, Euclidean rule: P[s](i,a) = sum_u V(i,u) H(a,u), and the gram H H^T spread
+// over the first tiles of every slice as in the H pass (k quarters per wave).
 template <int MB>
 __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
@@ -273,14 +311,16 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     __shared__ __attribute__((aligned(16))) float sB[2][KC][PB];  // [u][a]
 
     const int s = w / p.tiles, t = w - s * p.tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const int vt = (p.n + TILE - 1) / TILE;
-    const bool gram = (t == vt);
-    const float* A = gram ? p.H : p.V;
-    const int lda = gram ? p.r : p.n;
-    const int row0 = gram ? 0 : t * TILE;
-    const int nrows = gram ? p.r : min(TILE, p.n - row0);
+    const float* A = p.V;
+    const int lda = p.n;
+    const int row0 = t * TILE;
+    const int nrows = (t < vt) ? min(TILE, p.n - row0) : 0;
+    const int nblk = nb * nb;
+    const bool gcar = p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
     const int kend = min(p.m, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -288,7 +328,7 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
 
     float4 ra[4], rb[BLOADS];
-    auto gload = [&](int k0) {
+    auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -300,7 +340,7 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -313,32 +353,49 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
         }
     };
 
-    f32x16 acc[MB];
+    f32x16 acc[MB], gacc;
 #pragma unroll
     for (int mb = 0; mb < MB; mb++)
         for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    for (int v = 0; v < 16; v++) gacc[v] = 0.f;
+    const int gmb = gcar ? t / nb : 0, gnb = gcar ? t - (t / nb) * nb : 0;  // carried gram block
 
+    // separate loop copies for gram-carrying and plain workgroups (no MFMA under a branch)
     const int nchunks = (kend - kbeg + KC - 1) / KC;
-    gload(kbeg);
-    sstore(0);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ch++) {
-        const int buf = ch & 1;
-        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
-#pragma unroll
-        for (int kk = 0; kk < KC / 2; kk++) {
-            const float av = sA[buf][2 * kk + h][wave * 32 + c];
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-                if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
-        }
-        if (ch + 1 < nchunks) sstore(buf ^ 1);
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+        constexpr bool GR = decltype(gram_tag)::value;
+        gload(kbeg);
+        sstore(0);
         __syncthreads();
-    }
+        for (int ch = 0; ch < nchunks; ch++) {
+            const int buf = ch & 1;
+            if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
+#pragma unroll
+            for (int kk = 0; kk < KC / 2; kk++) {
+                const float av = sA[buf][2 * kk + h][wave * 32 + c];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+                    if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
+            }
+            if constexpr (GR) {  // this wave's k quarter (steps wave*4 .. wave*4+3) for the carried block
+#pragma unroll
+                for (int j = 0; j < KC / 8; j++) {
+                    const int kk = wave * (KC / 8) + j;
+                    gacc = mfma32x2(sB[buf][2 * kk + h][gmb * 32 + c], sB[buf][2 * kk + h][gnb * 32 + c], gacc);
+                }
+            }
+            if (ch + 1 < nchunks) sstore(buf ^ 1);
+            __syncthreads();
+        }
+    };
+    if (gcar)
+        mainloop(std::true_type{});
+    else
+        mainloop(std::false_type{});
 
     // acc[mb][v] = P(i = wave*32 + acc_row(v,h), a = mb*32 + c)
-    const int orow0 = gram ? p.npad : row0;
-    const int rlim = gram ? round_up(p.r, 4) : (p.npad - row0);
+    const int orow0 = row0;
+    const int rlim = p.npad - row0;
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) {
         const int a = mb * 32 + c;
@@ -353,11 +410,23 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             }
         }
     }
-    if (gram) {
-        // gram(a, b) = H H^T(a, b) = P[s][b*ldp + npad + a]; G[b*rpad + a]
+    if (gcar) {
+        // gram(a, b) = H H^T(a, b) at P[s][b*ldp + npad + a]; reduced into G[b*rpad + a]
         const int rpad = round_up(p.r, 4);
         const int ldp = p.ldp, npad = p.npad;
-        gram_finish(p.P, p.pstride, p.S, p.r * rpad,
+        float* gred = &sA[0][0][0];  // free after the loop
+        float* gdst = p.P + (size_t)s * p.pstride + npad;
+#pragma unroll
+        for (int v = 0; v < 16; v++) gred[(wave * 64 + lane) * 16 + v] = gacc[v];
+        __syncthreads();
+        for (int e = tid; e < 64 * 16; e += NTHREADS) {
+            const int le = e >> 4, v = e & 15;
+            const float g = ((gred[le * 16 + v] + gred[(64 + le) * 16 + v]) + gred[(128 + le) * 16 + v]) +
+                            gred[(192 + le) * 16 + v];
+            const int a = gmb * 32 + acc_row(v, le >> 5), b = gnb * 32 + (le & 31);
+            if (a < rpad && b < p.r) gdst[(size_t)b * ldp + a] = g;
+        }
+        gram_finish(p.P, p.pstride, p.S, p.S * nblk, p.r * rpad,
                     [=](int e) { return (long)(e / rpad) * ldp + npad + (e % rpad); }, p.G, p.gcount);
     }
 }
@@ -374,7 +443,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     __shared__ __attribute__((aligned(16))) float sH[2][RP][LDK];  // [a][u]
 
     const int s = w / p.tiles, t = w - s * p.tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const int row0 = t * TILE;
     const int nrows = min(TILE, p.n - row0);
@@ -396,7 +466,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     }
 
     float4 ra[4], rb[BLOADS];
-    auto gload = [&](int k0) {
+    auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -408,7 +478,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
             rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -499,7 +569,7 @@ template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
     if (w >= p.tiles * p.S) return;
-    pass_h_body<MB, KL, OBJ, GEMM2, TV>(p, w, MB);
+    pass_h_body<MB, KL, OBJ, GEMM2, TV>(p, w, MB);  // MB = ceil(r/32)
 }
 
 template <int MB>
@@ -619,6 +689,7 @@ hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipS
     const int rp = round_up(a.r, 32);
     if (rp <= 32) return launch_h_mb<1>(a, kl, obj, gemm2, st);
     if (rp <= 64) return launch_h_mb<2>(a, kl, obj, gemm2, st);
+    if (rp <= 96) return launch_h_mb<3>(a, kl, obj, gemm2, st);
     if (rp <= 128) return launch_h_mb<4>(a, kl, obj, gemm2, st);
     return hipErrorInvalidValue;
 }
@@ -657,6 +728,7 @@ hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st) {
     const int rp = round_up(a.r, 32);
     if (rp <= 32) return launch_obj_fp8v_mb<1>(a, obj, st);
     if (rp <= 64) return launch_obj_fp8v_mb<2>(a, obj, st);
+    if (rp <= 96) return launch_obj_fp8v_mb<3>(a, obj, st);
     if (rp <= 128) return launch_obj_fp8v_mb<4>(a, obj, st);
     return hipErrorInvalidValue;
 }
@@ -665,6 +737,7 @@ hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st) {
     const int rp = round_up(a.r, 32);
     if (rp <= 32) return launch_obj_bf16v_mb<1>(a, obj, st);
     if (rp <= 64) return launch_obj_bf16v_mb<2>(a, obj, st);
+    if (rp <= 96) return launch_obj_bf16v_mb<3>(a, obj, st);
     if (rp <= 128) return launch_obj_bf16v_mb<4>(a, obj, st);
     return hipErrorInvalidValue;
 }
@@ -678,10 +751,12 @@ hipError_t launch_pass_w_f32(const PassArgs& a, int kl, hipStream_t st) {
     if (kl) {
         if (rp <= 32) { LW(k_pass_w_kl_f32, 1) }
         if (rp <= 64) { LW(k_pass_w_kl_f32, 2) }
+        if (rp <= 96) { LW(k_pass_w_kl_f32, 3) }
         if (rp <= 128) { LW(k_pass_w_kl_f32, 4) }
     } else {
         if (rp <= 32) { LW(k_pass_w_euc_f32, 1) }
         if (rp <= 64) { LW(k_pass_w_euc_f32, 2) }
+        if (rp <= 96) { LW(k_pass_w_euc_f32, 3) }
         if (rp <= 128) { LW(k_pass_w_euc_f32, 4) }
     }
 #undef LW

@@ -14,22 +14,15 @@
 #include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
 
+#include <type_traits>
+
 namespace gpumlib_nmf {
 
 typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int K8 = 128;        // K per pipeline step: 2 MFMAs of K = 64
 constexpr int LD8 = K8 + 16;   // LDS row pitch in bytes (36 words)
-
-__device__ __forceinline__ uint4 load16_u8(const uint8_t* p, bool ok, int k, int kend) {
-    if (!ok) return make_uint4(0, 0, 0, 0);
-    if (k + 15 < kend) return *reinterpret_cast<const uint4*>(p);
-    uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 16; j++)
-        if (k + j < kend) w[j >> 2] |= (uint32_t)p[j] << (8 * (j & 3));
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
 
 template <int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
@@ -43,60 +36,71 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
     if (w >= total) return;
     const int s = w / p.tiles, t = w - s * p.tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     // the first column tile of every slice also accumulates the slice's gram A^T A
     // from the A rows staged in LDS (block (mb, nb) on wave mb*MB + nb, modulo 4)
     const bool gram = (t == 0) && p.G != nullptr;
     const int col0 = t * TILE;
     const int ncols = min(TILE, p.N - col0);
+    // The stores and shadows are padded to whole chunks with zeros (and scales 127), so
+    // every chunk is loaded without bounds checks; lanes past the last column or rank
+    // row re-read the last valid one (their products land only in output columns that
+    // are not stored and in rank rows >= r, which no update reads).
     const int kbeg = s * p.kslice;
-    const int kend = min(p.K, kbeg + p.kslice);
+    const int kend = min(round_up(p.K, K8), kbeg + p.kslice);
     constexpr int GB = (MB * MB + 3) / 4;
 
-    uint4 ra[MB], rb[4];
-    uint8_t rsa, rsb[2];
-    auto gload = [&](int k0) {
+    // Two register sets of one chunk each (native vectors, selected by a compile-time
+    // tag, so they stay in registers).
+    u32x4 ra[2][MB], rb[2][4];
+    uint8_t rsa[2], rsb[2][2];
+    using Set0 = std::integral_constant<int, 0>;
+    using Set1 = std::integral_constant<int, 1>;
+    auto gload = [&](auto set, int k0) __attribute__((always_inline)) {
+        constexpr int R = decltype(set)::value;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
-            rb[j] = load16_u8(p.B + (size_t)(col0 + col) * p.ldb + k0 + kq, col < ncols, k0 + kq, kend);
+            const int cc = col < ncols ? col : ncols - 1;
+            rb[R][j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
             const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
-            ra[j] = load16_u8(p.A + (size_t)a * p.lda + k0 + kq, a < p.r, k0 + kq, kend);
+            const int aa = a < p.r ? a : p.r - 1;
+            ra[R][j] = *reinterpret_cast<const u32x4*>(p.A + (size_t)aa * p.lda + k0 + kq);
         }
         const int kb0 = k0 >> 5;
         {
-            const int a = tid >> 2, kb = kb0 + (tid & 3);
-            rsa = (tid < RP * 4 && a < p.r && kb < p.nbk) ? p.Asc[(size_t)a * p.nbk + kb] : (uint8_t)127;
+            const int a = (tid >> 2) < p.r ? (tid >> 2) : p.r - 1, kb = kb0 + (tid & 3);
+            rsa[R] = p.Asc[(size_t)a * p.nbk + kb];
         }
 #pragma unroll
         for (int jj = 0; jj < 2; jj++) {
             const int e = tid + NTHREADS * jj, col = e >> 2, kb = kb0 + (e & 3);
-            uint8_t v = 127;
-            if (col < ncols && kb < p.nbk)
-                v = p.Bsc[(size_t)((col0 + col) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
-            rsb[jj] = v;
+            const int cc = col < ncols ? col : ncols - 1;
+            rsb[R][jj] = p.Bsc[(size_t)((col0 + cc) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](auto set, int buf) __attribute__((always_inline)) {
+        constexpr int R = decltype(set)::value;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
-            *reinterpret_cast<uint4*>(&sB[buf][col][kq]) = rb[j];
+            *reinterpret_cast<u32x4*>(&sB[buf][col][kq]) = rb[R][j];
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
             const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
-            *reinterpret_cast<uint4*>(&sA[buf][a][kq]) = ra[j];
+            *reinterpret_cast<u32x4*>(&sA[buf][a][kq]) = ra[R][j];
         }
-        if (tid < RP * 4) sAs[buf][tid >> 2][tid & 3] = rsa;
+        if (tid < RP * 4) sAs[buf][tid >> 2][tid & 3] = rsa[R];
 #pragma unroll
         for (int jj = 0; jj < 2; jj++) {
             const int e = tid + NTHREADS * jj;
-            sBs[buf][e >> 2][e & 3] = rsb[jj];
+            sBs[buf][e >> 2][e & 3] = rsb[R][jj];
         }
     };
 
@@ -108,13 +112,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     for (int q = 0; q < GB; q++)
         for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
-    const int nchunks = (kend - kbeg + K8 - 1) / K8;
-    gload(kbeg);
-    sstore(0);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ch++) {
-        const int buf = ch & 1;
-        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * K8);
+    auto compute = [&](int buf) __attribute__((always_inline)) {
         // The instruction takes K block 0 of a row from bytes 0..15 of both lane halves and
         // its scale from lane c, K block 1 from bytes 16..31 and its scale from lane c + 32.
         // So lane (c, h) supplies k = g*64 + 16h + j in bytes j < 16 and k = g*64 + 32 + 16h + j
@@ -156,7 +154,26 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                 }
             }
         }
-        if (ch + 1 < nchunks) sstore(buf ^ 1);
+    };
+
+    // Stages of two chunks: the loads of chunks ch+2 and ch+3 are issued, chunks ch and
+    // ch+1 are multiplied out of LDS buffers 0 and 1 while those loads are in flight, and
+    // after a barrier the new chunks are stored (no loop-carried staging registers).
+    const int nchunks = (kend - kbeg + K8 - 1) / K8;
+    gload(Set0{}, kbeg);
+    if (nchunks > 1) gload(Set1{}, kbeg + K8);
+    sstore(Set0{}, 0);
+    if (nchunks > 1) sstore(Set1{}, 1);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch += 2) {
+        const bool more0 = ch + 2 < nchunks, more1 = ch + 3 < nchunks;
+        if (more0) gload(Set0{}, kbeg + (ch + 2) * K8);
+        if (more1) gload(Set1{}, kbeg + (ch + 3) * K8);
+        compute(0);
+        if (ch + 1 < nchunks) compute(1);
+        __syncthreads();
+        if (more0) sstore(Set0{}, 0);
+        if (more1) sstore(Set1{}, 1);
         __syncthreads();
     }
 
@@ -193,13 +210,14 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                         make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
             }
         }
-        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.r * p.ldr, p.G, p.gcount);
+        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G, p.gcount);
     }
 }
 
 // ---------------------------------------------------------------------------
 // V (n x m, FP32) -> Vq (column-major, ld ldv), Vtq (V^T column-major, ld ldt) and one
-// E8M0 scale per 32 x 32 tile: Vsc[ub * nbi + ib].  One workgroup per tile.
+// E8M0 scale per 32 x 32 tile: Vsc[ub * nbi + ib] (nbi = ldv / 32).  One workgroup per
+// tile of the padded stores; tiles past V's edges get zeros and the scale 127 (2^0).
 __global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict__ V, int n, int m,
                                                          uint8_t* __restrict__ Vq, int ldv, uint8_t* __restrict__ Vtq,
                                                          int ldt, uint8_t* __restrict__ Vsc, int nbi) {
@@ -228,19 +246,21 @@ __global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict
     }
     __syncthreads();
     const int sc = sscale;
+    // the grid covers the padded stores (ldv x ldt): padding gets zeros (tile values 0)
     for (int uu = ty; uu < 32; uu += 8) {
         const int i = i0 + tx, u = u0 + uu;
-        if (i < n && u < m) Vq[(size_t)u * ldv + i] = f32_to_e4m3(e8m0_unscale(tile[uu][tx], sc));
+        if (u < m) Vq[(size_t)u * ldv + i] = f32_to_e4m3(e8m0_unscale(tile[uu][tx], sc));
     }
     for (int ii = ty; ii < 32; ii += 8) {
         const int u = u0 + tx, i = i0 + ii;
-        if (i < n && u < m) Vtq[(size_t)i * ldt + u] = f32_to_e4m3(e8m0_unscale(tile[tx][ii], sc));
+        if (i < n) Vtq[(size_t)i * ldt + u] = f32_to_e4m3(e8m0_unscale(tile[tx][ii], sc));
     }
 }
 
 // Operand shadows with one scale per 32 elements along the contraction dimension:
 //   which 0: W (n x r) -> Wq column-major (ld ldq), Wsc[a * nblk + ib]   (blocks along i)
 //   which 1: H (r x m) -> Hq = H^T column-major (ld ldq), Hsc[a * nblk + ub] (blocks along u)
+// nblk = ldq / 32 covers the padded length; entries past n (or m) are zeros.
 // One thread per (rank column a, block).
 __global__ __launch_bounds__(NTHREADS) void k_shadow_fp8(const float* __restrict__ X, int n, int m, int r, int which,
                                                          uint8_t* __restrict__ Xq, int ldq, uint8_t* __restrict__ Xsc,
@@ -255,7 +275,7 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_fp8(const float* __restrict
     for (int k = k0; k < k1; k++) amax = fmaxf(amax, fabsf(at(k)));
     const int sc = e8m0_for_amax(amax);
     Xsc[t] = (uint8_t)sc;
-    for (int k = k0; k < k1; k++) Xq[(size_t)a * ldq + k] = f32_to_e4m3(e8m0_unscale(at(k), sc));
+    for (int k = k0; k < k0 + 32; k++) Xq[(size_t)a * ldq + k] = k < k1 ? f32_to_e4m3(e8m0_unscale(at(k), sc)) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -277,14 +297,16 @@ hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st) {
 
 hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv, uint8_t* Vtq, int ldt, uint8_t* Vsc,
                              hipStream_t st) {
-    const int nbi = (n + 31) / 32, nbu = (m + 31) / 32;
+    if (ldv % 32 || ldt % 32 || ldv < n || ldt < m) return hipErrorInvalidValue;
+    const int nbi = ldv / 32, nbu = ldt / 32;
     hipLaunchKernelGGL(k_pack_v_fp8, dim3(nbi, nbu), dim3(NTHREADS), 0, st, V, n, m, Vq, ldv, Vtq, ldt, Vsc, nbi);
     return hipGetLastError();
 }
 
 hipError_t launch_shadow_fp8(const float* X, int n, int m, int r, int which, uint8_t* Xq, int ldq, uint8_t* Xsc,
                              hipStream_t st) {
-    const int nblk = ((which == 0 ? n : m) + 31) / 32;
+    if (ldq % 32 || ldq < (which == 0 ? n : m)) return hipErrorInvalidValue;
+    const int nblk = ldq / 32;
     const int threads = r * nblk;
     hipLaunchKernelGGL(k_shadow_fp8, dim3((threads + NTHREADS - 1) / NTHREADS), dim3(NTHREADS), 0, st, X, n, m, r,
                        which, Xq, ldq, Xsc, nblk);

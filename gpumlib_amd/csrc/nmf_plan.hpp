@@ -18,22 +18,30 @@ constexpr int kTargetWorkgroups = 512;
 struct Plan {
     int rule, precision, n, m, r;
     int npad, rpad, ldr, ldp;
-    int ldv8, ldt8;       // BF16 storage: leading dimensions of V (n -> mult. of 8) and V^T (m -> mult. of 8)
+    int ldv8, ldt8;       // BF16 storage: leading dimensions of V and V^T (bf16_ld of n and of m)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;
+    int tiles_obj, S_obj, ks_obj;  // objective-only pass (FP32-style H pass over V's column tiles)
     int nobjp;
-    int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (bytes) and 32-blocks of n and m
+    int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (fp8_ld of n, m) and their 32-blocks
     size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
 };
 
-// Byte layout of the FP8 V store: Vq (ld ld16n) | Vtq (ld ld16m) | tile scales (nbu x nbi).
+// Leading dimensions of the narrow V stores and operand shadows: whole pass chunks
+// (256 BF16 or 128 FP8 elements along the contraction), the padding zero-filled,
+// so the streaming passes load every chunk without bounds checks.
+inline int bf16_ld(int k) { return (k + 255) / 256 * 256; }
+inline int fp8_ld(int k) { return (k + 127) / 128 * 128; }
+
+// Byte layout of the FP8 V store: Vq (m columns, ld fp8_ld(n)) | Vtq (n columns, ld fp8_ld(m)) |
+// tile scales (fp8_ld(m)/32 x fp8_ld(n)/32, padding tiles 127 = 2^0).
 struct Fp8Store {
     size_t off_t, off_sc, bytes;
 };
 Fp8Store fp8_store_layout(int n, int m);
 
-// Byte layout of the BF16 V store: V (column-major, ld ldv8) then V^T (column-major, ld ldt8).
+// Byte layout of the BF16 V store: V (column-major, ld bf16_ld(n)) then V^T (ld bf16_ld(m)).
 size_t vstore_bf16_offset_t(int n, int m);
 size_t vstore_bytes(int precision, int n, int m);
 

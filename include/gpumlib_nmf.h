@@ -98,10 +98,12 @@ enum {
 
 /* V store: the device form of V that gpumlib_nmf_iterate reads for a precision (no reference
  * counterpart).  GPUMLIB_NMF_FP32: V itself (float, n x m column-major).  GPUMLIB_NMF_BF16: V and V^T in
- * BF16 (round to nearest even), each column-major with a leading dimension rounded up to 8, V^T
- * starting at a 256-byte boundary; the W pass then reads V^T with unit stride.  GPUMLIB_NMF_FP8: V and V^T
- * in OCP E4M3 (leading dimensions rounded up to 16 bytes, 256-byte aligned parts) followed by one E8M0
- * scale per 32 x 32 tile of V (scale of tile (i/32, u/32) at byte (u/32) * ceil(n/32) + i/32), shared by
+ * BF16 (round to nearest even), each column-major with its leading dimension rounded up to a multiple of
+ * 256 (ldv = 256*ceil(n/256) for V, ldt = 256*ceil(m/256) for V^T), padding zero, V^T starting at a 256-byte
+ * boundary; the W pass then reads V^T with unit stride.  GPUMLIB_NMF_FP8: V and V^T in OCP E4M3 with
+ * leading dimensions rounded up to multiples of 128 bytes (ldv = 128*ceil(n/128), ldt = 128*ceil(m/128)),
+ * padding zero, 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V
+ * (scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding tiles hold 127 = 2^0), shared by
  * both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run
  * the Euclidean rule; FP8 supports r <= 64. */
 size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);

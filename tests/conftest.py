@@ -42,14 +42,17 @@ def cuda(native):
     return hip
 
 
-def fp32_objective_allowance(ref, v_sq_sum, r):
-    """Absolute error the GPU's FP32 residuals may add to a Frobenius objective near ref.
+def objective_allowance(ref, v_sq_sum, r, d_wh):
+    """Absolute difference allowed between a GPU Frobenius objective and a reference one, ref.
 
-    Each residual d = v - (W H)(i, u) is formed in FP32 from r products, so it carries an
-    error |delta| <= 4 r 2^-24 |v| (a loose bound on r-term FP32 dot-product rounding); the
-    objective sum d^2 then moves by at most 2 |d| |delta| <= 2 sqrt(ref) 4 r 2^-24 |V|_F
-    (Cauchy-Schwarz).  It matters only when the fit is nearly exact (ref ~ 0)."""
-    return 2.0 * np.sqrt(max(ref, 0.0)) * 4.0 * r * 2.0 ** -24 * np.sqrt(v_sq_sum)
+    The two are |V - W H|^2 for factors whose products differ by d_wh = |W H - W' H'|_F
+    (measured), with the GPU's residuals formed in FP32 from r products each, an error of at
+    most 4 r 2^-24 |v| per entry (a loose bound on r-term FP32 dot-product rounding).  With
+    D = d_wh + 4 r 2^-24 |V|_F the objectives differ by at most 2 sqrt(ref) D + D^2
+    (Cauchy-Schwarz).  This matters only near an exact fit (ref ~ 0), where the relative
+    difference of the objectives says nothing about the factors."""
+    D = d_wh + 4.0 * r * 2.0 ** -24 * np.sqrt(v_sq_sum)
+    return 2.0 * np.sqrt(max(ref, 0.0)) * D + D * D
 
 
 def rel_fro(a, b):
