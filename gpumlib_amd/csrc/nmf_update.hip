@@ -182,29 +182,31 @@ __device__ __forceinline__ void stage_lds(int total, Ld ld, St st) {
 // W update from a pass over V^T (nmf_multiplicative_euclidian.cpp:53-66): j = i,
 //   X(i,a) = W(i,a) at W[a*n + i], C(b,a) = HHt(b,a) = G[a*ldg + b]  (gt = 1)
 // Sh (optional) receives the BF16 shadow of the new X: X(j,a) at Sh[a*ldsh + j].
-// Each thread owns (row jj, 4 consecutive a); JT rows per workgroup.
-template <int JT>
+// Each thread owns (row jj, 4 consecutive a); JT rows per workgroup.  RP is r rounded
+// up to 32: the staged C and X are zero beyond r, so the inner product runs over RP
+// terms, fully unrolled (the added terms are exact zeros).
+template <int JT, int RP>
 __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict__ X, long xs_j, long xs_a,
                                                               const float* __restrict__ P, const float* __restrict__ G,
                                                               int ldg, int gt, int count, int r, int ldr, int S,
                                                               long pstride, float eps, uint16_t* __restrict__ Sh,
                                                               int ldsh) {
     extern __shared__ __attribute__((aligned(16))) float smr[];
-    const int rp4 = round_up(r, 4), rs4 = rp4 + 4, nq = rp4 / 4;
-    float* sG = smr;               // sG[b*rs4 + a] = C(b,a)
-    float* sX = sG + r * rs4;      // sX[jj*rs4 + b] = X(j0+jj, b)
-    float* sN = sX + JT * rs4;     // new X(j0+jj, a)
+    constexpr int RS = RP + 4, NQ = RP / 4, ITEMS = JT * NQ, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
+    float* sG = smr;            // sG[b*RS + a] = C(b,a), zero for a or b >= r
+    float* sX = sG + RP * RS;   // sX[jj*RS + b] = X(j0+jj, b), zero for b >= r
+    float* sN = sX + JT * RS;   // new X(j0+jj, a)
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * JT, nj = min(JT, count - j0);
+    const int rq = round_up(r, 4) / 4;  // float4 groups of a row of P (ldr = 4*rq)
 
-    // 1. issue the slice loads of this thread's items (at most 2 items of 4 values)
-    constexpr int MAXI = 2;
+    // 1. issue the slice loads of this thread's items
     float4 num[MAXI];
 #pragma unroll
     for (int k = 0; k < MAXI; k++) {
         num[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const int item = tid + k * NTHREADS, jj = item / nq, q = item - jj * nq;
-        if (item >= JT * nq || jj >= nj) continue;
+        const int item = tid + k * NTHREADS, jj = item / NQ, q = item - jj * NQ;
+        if (item >= ITEMS || jj >= nj || q >= rq) continue;
         const float4* src = reinterpret_cast<const float4*>(P + (size_t)(j0 + jj) * ldr) + q;
         const long ps4 = pstride >> 2;
         int s = 0;
@@ -221,41 +223,40 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
             num[k].x += x0.x, num[k].y += x0.y, num[k].z += x0.z, num[k].w += x0.w;
         }
     }
-    // 2. stage C and the X rows (zero padded to rp4 columns), loads batched
-    if (gt) {  // sG[b][a] = G[a*ldg + b]: read along b (coalesced), write transposed
-        stage_lds<8>(r * rp4, [&](int idx) { const int a = idx / rp4, b = idx - a * rp4;
-                                             return (a < r && b < r) ? G[(size_t)a * ldg + b] : 0.f; },
-                     [&](int idx, float v) { const int a = idx / rp4, b = idx - a * rp4;
-                                             if (b < r) sG[b * rs4 + a] = v; });
-        for (int idx = tid; idx < r * (rp4 - r); idx += NTHREADS) {
-            const int b = idx / (rp4 - r), a = r + idx - b * (rp4 - r);
-            sG[b * rs4 + a] = 0.f;
-        }
+    // 2. stage C and the X rows, zero padded to RP, loads batched
+    if (gt) {  // C(b,a) = G[a*ldg + b]: read along b (coalesced), write transposed
+        stage_lds<8>(RP * RP,
+                     [&](int idx) { const int a = idx / RP, b = idx - (idx / RP) * RP;
+                                    return (a < r && b < r) ? G[(size_t)a * ldg + b] : 0.f; },
+                     [&](int idx, float v) { const int a = idx / RP, b = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
     } else {
-        stage_lds<8>(r * rp4, [&](int idx) { const int b = idx / rp4, a = idx - b * rp4;
-                                             return a < r ? G[(size_t)b * ldg + a] : 0.f; },
-                     [&](int idx, float v) { const int b = idx / rp4, a = idx - b * rp4; sG[b * rs4 + a] = v; });
+        stage_lds<8>(RP * RP,
+                     [&](int idx) { const int b = idx / RP, a = idx - (idx / RP) * RP;
+                                    return (a < r && b < r) ? G[(size_t)b * ldg + a] : 0.f; },
+                     [&](int idx, float v) { const int b = idx / RP, a = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
     }
     if (xs_a == 1) {
-        stage_lds<8>(JT * rp4, [&](int idx) { const int jj = idx / rp4, b = idx - jj * rp4;
-                                              return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + b] : 0.f; },
-                     [&](int idx, float v) { const int jj = idx / rp4, b = idx - jj * rp4; sX[jj * rs4 + b] = v; });
+        stage_lds<8>(JT * RP,
+                     [&](int idx) { const int jj = idx / RP, b = idx - (idx / RP) * RP;
+                                    return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + b] : 0.f; },
+                     [&](int idx, float v) { const int jj = idx / RP, b = idx - (idx / RP) * RP; sX[jj * RS + b] = v; });
     } else {
-        stage_lds<8>(JT * rp4,
+        stage_lds<8>(JT * RP,
                      [&](int idx) { const int jj = idx % JT, b = idx / JT;
                                     return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + (size_t)b * xs_a] : 0.f; },
-                     [&](int idx, float v) { const int jj = idx % JT, b = idx / JT; sX[jj * rs4 + b] = v; });
+                     [&](int idx, float v) { const int jj = idx % JT, b = idx / JT; sX[jj * RS + b] = v; });
     }
     __syncthreads();
     // 3. denominators and new values
 #pragma unroll
     for (int k = 0; k < MAXI; k++) {
-        const int item = tid + k * NTHREADS, jj = item / nq, q = item - jj * nq;
-        if (item >= JT * nq || jj >= nj) continue;
+        const int item = tid + k * NTHREADS, jj = item / NQ, q = item - jj * NQ;
+        if (item >= ITEMS || jj >= nj) continue;
         float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float* xr = sX + jj * rs4;
-        for (int b = 0; b < r; b++) {
-            const float4 g = *reinterpret_cast<const float4*>(sG + b * rs4 + 4 * q);
+        const float* xr = sX + jj * RS;
+#pragma unroll 16
+        for (int b = 0; b < RP; b++) {
+            const float4 g = *reinterpret_cast<const float4*>(sG + b * RS + 4 * q);
             const float xb = xr[b];
             d.x += xb * g.x, d.y += xb * g.y, d.z += xb * g.z, d.w += xb * g.w;
         }
@@ -265,25 +266,25 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         o.y = xo.y * (num[k].y / (d.y + eps));
         o.z = xo.z * (num[k].z / (d.z + eps));
         o.w = xo.w * (num[k].w / (d.w + eps));
-        *reinterpret_cast<float4*>(sN + jj * rs4 + 4 * q) = o;
+        *reinterpret_cast<float4*>(sN + jj * RS + 4 * q) = o;
     }
     __syncthreads();
     // 4. coalesced writes of X and of its shadow
     if (xs_a == 1) {
         for (int idx = tid; idx < nj * r; idx += NTHREADS) {
             const int jj = idx / r, a = idx - jj * r;
-            X[(size_t)(j0 + jj) * xs_j + a] = sN[jj * rs4 + a];
+            X[(size_t)(j0 + jj) * xs_j + a] = sN[jj * RS + a];
         }
     } else {
         for (int idx = tid; idx < JT * r; idx += NTHREADS) {
             const int jj = idx % JT, a = idx / JT;
-            if (jj < nj) X[(size_t)(j0 + jj) * xs_j + (size_t)a * xs_a] = sN[jj * rs4 + a];
+            if (jj < nj) X[(size_t)(j0 + jj) * xs_j + (size_t)a * xs_a] = sN[jj * RS + a];
         }
     }
     if (Sh) {
         for (int idx = tid; idx < JT * r; idx += NTHREADS) {
             const int jj = idx % JT, a = idx / JT;
-            if (jj < nj) Sh[(size_t)a * ldsh + j0 + jj] = __builtin_bit_cast(uint16_t, (__bf16)sN[jj * rs4 + a]);
+            if (jj < nj) Sh[(size_t)a * ldsh + j0 + jj] = __builtin_bit_cast(uint16_t, (__bf16)sN[jj * RS + a]);
         }
     }
 }
@@ -291,63 +292,83 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
 // Column form, for the FP32 W pass layout P[s][a*ldp + i] (i contiguous per a):
 //   W(i,a) <- W(i,a) * (VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)),  HHt(b,a) = G[a*rpad + b]
 // (nmf_multiplicative_euclidian.cpp:53-66).  JT rows i per workgroup; each thread owns
-// up to 4 (i, a) elements, i fastest, and issues all their slice loads before waiting.
-template <int JT>
+// 4 consecutive rows i of one rank column a (i fastest across lanes), issues all its
+// slice loads before waiting, and runs the inner product over RP (r rounded up to 32)
+// zero-padded terms, fully unrolled.  W rows are staged transposed (sWt[b][i]) so the
+// 4 rows of a thread are one 16-byte LDS read.
+template <int JT, int RP>
 __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restrict__ W, const float* __restrict__ P,
                                                                 const float* __restrict__ G, int n, int r, int ldp,
                                                                 int S, long pstride, float eps) {
     extern __shared__ __attribute__((aligned(16))) float smc[];
-    const int rs = r + 1, rpad = round_up(r, 4);
-    float* sG = smc;            // sG[b*rs + a] = HHt(b,a)
-    float* sW = sG + r * rs;    // sW[ii*rs + b] = W(i0+ii, b)
+    constexpr int RS = RP + 4, JS = JT + 4, NI4 = JT / 4, ITEMS = NI4 * RP, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
+    float* sG = smc;             // sG[b*RS + a] = HHt(b,a), zero for a or b >= r
+    float* sWt = sG + RP * RS;   // sWt[b*JS + ii] = W(i0+ii, b), zero outside
     const int tid = threadIdx.x;
+    const int rpad = round_up(r, 4);
     const int i0 = blockIdx.x * JT, nrow = min(JT, n - i0);
-    const int total = JT * r;
-    for (int base = 0; base < total; base += 4 * NTHREADS) {
-        float num[4];
+
+    float4 num[MAXI];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            num[k] = 0.f;
-            const int idx = base + k * NTHREADS + tid, ii = idx % JT, a = idx / JT;
-            if (idx >= total || ii >= nrow) continue;
-            const float* src = P + (size_t)a * ldp + i0 + ii;
-            int s = 0;
-            for (; s + 4 <= S; s += 4) {
-                const float x0 = src[(size_t)s * pstride], x1 = src[(size_t)(s + 1) * pstride];
-                const float x2 = src[(size_t)(s + 2) * pstride], x3 = src[(size_t)(s + 3) * pstride];
-                num[k] += x0;
-                num[k] += x1;
-                num[k] += x2;
-                num[k] += x3;
-            }
-            for (; s < S; s++) num[k] += src[(size_t)s * pstride];
+    for (int k = 0; k < MAXI; k++) {
+        num[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int item = tid + k * NTHREADS, i4 = item % NI4, a = item / NI4;
+        if (item >= ITEMS || a >= r || 4 * i4 >= nrow) continue;
+        // rows up to round_up(n, 4) exist in every slice (i0 and ldp are multiples of 4)
+        const float4* src = reinterpret_cast<const float4*>(P + (size_t)a * ldp + i0 + 4 * i4);
+        const long ps4 = pstride >> 2;
+        int s = 0;
+        for (; s + 4 <= S; s += 4) {
+            const float4 x0 = src[(size_t)s * ps4], x1 = src[(size_t)(s + 1) * ps4];
+            const float4 x2 = src[(size_t)(s + 2) * ps4], x3 = src[(size_t)(s + 3) * ps4];
+            num[k].x += x0.x, num[k].y += x0.y, num[k].z += x0.z, num[k].w += x0.w;
+            num[k].x += x1.x, num[k].y += x1.y, num[k].z += x1.z, num[k].w += x1.w;
+            num[k].x += x2.x, num[k].y += x2.y, num[k].z += x2.z, num[k].w += x2.w;
+            num[k].x += x3.x, num[k].y += x3.y, num[k].z += x3.z, num[k].w += x3.w;
         }
-        if (base == 0) {
-            stage_lds<8>(r * r, [&](int idx) { const int a = idx / r, b = idx - a * r; return G[(size_t)a * rpad + b]; },
-                         [&](int idx, float v) { const int a = idx / r, b = idx - a * r; sG[b * rs + a] = v; });
-            stage_lds<8>(JT * r,
-                         [&](int idx) { const int ii = idx % JT, b = idx / JT;
-                                        return ii < nrow ? W[(size_t)b * n + i0 + ii] : 0.f; },
-                         [&](int idx, float v) { const int ii = idx % JT, b = idx / JT; sW[ii * rs + b] = v; });
+        for (; s < S; s++) {
+            const float4 x0 = src[(size_t)s * ps4];
+            num[k].x += x0.x, num[k].y += x0.y, num[k].z += x0.z, num[k].w += x0.w;
         }
-        __syncthreads();
-        float out[4];
+    }
+    stage_lds<8>(RP * RP,
+                 [&](int idx) { const int a = idx / RP, b = idx - (idx / RP) * RP;
+                                return (a < r && b < r) ? G[(size_t)a * rpad + b] : 0.f; },
+                 [&](int idx, float v) { const int a = idx / RP, b = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
+    stage_lds<8>(RP * JT,
+                 [&](int idx) { const int ii = idx % JT, b = idx / JT;
+                                return (ii < nrow && b < r) ? W[(size_t)b * n + i0 + ii] : 0.f; },
+                 [&](int idx, float v) { const int ii = idx % JT, b = idx / JT; sWt[b * JS + ii] = v; });
+    __syncthreads();
+    float4 out[MAXI];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int idx = base + k * NTHREADS + tid, ii = idx % JT, a = idx / JT;
-            out[k] = 0.f;
-            if (idx >= total || ii >= nrow) continue;
-            float d = 0.f;
-            for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
-            out[k] = sW[ii * rs + a] * (num[k] / (d + eps));
+    for (int k = 0; k < MAXI; k++) {
+        const int item = tid + k * NTHREADS, i4 = item % NI4, a = item / NI4;
+        out[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (item >= ITEMS || a >= r || 4 * i4 >= nrow) continue;
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 16
+        for (int b = 0; b < RP; b++) {
+            const float4 w4 = *reinterpret_cast<const float4*>(sWt + b * JS + 4 * i4);
+            const float g = sG[b * RS + a];
+            d.x += w4.x * g, d.y += w4.y * g, d.z += w4.z * g, d.w += w4.w * g;
         }
-        // every thread has read the W rows of this round: the new values may go out
-        // (other rounds read only sW, which is never overwritten)
+        const float4 wo = *reinterpret_cast<const float4*>(sWt + a * JS + 4 * i4);
+        out[k].x = wo.x * (num[k].x / (d.x + eps));
+        out[k].y = wo.y * (num[k].y / (d.y + eps));
+        out[k].z = wo.z * (num[k].z / (d.z + eps));
+        out[k].w = wo.w * (num[k].w / (d.w + eps));
+    }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int idx = base + k * NTHREADS + tid, ii = idx % JT, a = idx / JT;
-            if (idx < total && ii < nrow) W[(size_t)a * n + i0 + ii] = out[k];
-        }
+    for (int k = 0; k < MAXI; k++) {
+        const int item = tid + k * NTHREADS, i4 = item % NI4, a = item / NI4;
+        if (item >= ITEMS || a >= r) continue;
+        const int i = i0 + 4 * i4;
+        float* dst = W + (size_t)a * n + i;
+        if (4 * i4 < nrow) dst[0] = out[k].x;
+        if (4 * i4 + 1 < nrow) dst[1] = out[k].y;
+        if (4 * i4 + 2 < nrow) dst[2] = out[k].z;
+        if (4 * i4 + 3 < nrow) dst[3] = out[k].w;
     }
 }
 
@@ -633,28 +654,30 @@ static hipError_t set_smem(K kernel, size_t sm) {
     return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
 }
 
-// Rows per workgroup of the row-form update: about 512 workgroups or more for large counts.
+// Row-form update: JT = 32 rows per workgroup for long counts (about 512 or more
+// workgroups), else 16; RP = r rounded up to 32.
+template <int JT, int RP>
+static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
+                                int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
+                                hipStream_t st) {
+    const size_t sm = (size_t)(RP + 2 * JT) * (RP + 4) * sizeof(float);
+    hipError_t e = set_smem(k_update_rows_euc<JT, RP>, sm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_update_rows_euc<JT, RP>), dim3((count + JT - 1) / JT), dim3(NTHREADS), sm, st, X, xs_j, xs_a,
+                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh);
+    return hipGetLastError();
+}
+
 static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                               int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
                               hipStream_t st) {
-    const int rp4 = round_up(r, 4);
-    if (r < 1 || rp4 > 128) return hipErrorInvalidValue;  // at most 2 items of 4 per thread
-    const bool wide = count >= 16384 && rp4 <= 64;  // 32 rows x 16 groups = 2 items per thread
-    const size_t sm = (size_t)(r + 64) * (rp4 + 4) * sizeof(float);
-    const dim3 grid((count + 31) / 32);
-    hipError_t e;
-    if (wide) {
-        if ((e = set_smem(k_update_rows_euc<32>, sm)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_update_rows_euc<32>, grid, dim3(NTHREADS), sm, st, X, xs_j, xs_a, P, G, ldg, gt, count,
-                           r, ldr, S, pstride, eps, Sh, ldsh);
-    } else {
-        const dim3 g16((count + 15) / 16);
-        const size_t sm16 = (size_t)(r + 32) * (rp4 + 4) * sizeof(float);
-        if ((e = set_smem(k_update_rows_euc<16>, sm16)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_update_rows_euc<16>, g16, dim3(NTHREADS), sm16, st, X, xs_j, xs_a, P, G, ldg, gt, count,
-                           r, ldr, S, pstride, eps, Sh, ldsh);
-    }
-    return hipGetLastError();
+    if (r < 1 || r > 128) return hipErrorInvalidValue;
+    const bool wide = count >= 16384;
+#define LR(JT, RP) return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, st)
+    if (r <= 32) { if (wide) LR(32, 32); LR(16, 32); }
+    if (r <= 64) { if (wide) LR(32, 64); LR(16, 64); }
+    LR(16, 128);
+#undef LR
 }
 
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
@@ -667,15 +690,26 @@ hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n
     return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, st);
 }
 
+template <int JT, int RP>
+static hipError_t launch_cols_t(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
+                                float eps, hipStream_t st) {
+    const size_t sm = (size_t)(RP * (RP + 4) + RP * (JT + 4)) * sizeof(float);
+    hipError_t e = set_smem(k_update_w_euc_cols<JT, RP>, sm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_update_w_euc_cols<JT, RP>), dim3((n + JT - 1) / JT), dim3(NTHREADS), sm, st, W, P, G, n, r,
+                       ldp, S, pstride, eps);
+    return hipGetLastError();
+}
+
 hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
                                float eps, hipStream_t st) {
-    constexpr int JT = 16;
-    const size_t sm = (size_t)(r + JT) * (r + 1) * sizeof(float);
-    hipError_t e = set_smem(k_update_w_euc_cols<JT>, sm);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_update_w_euc_cols<JT>, dim3((n + JT - 1) / JT), dim3(NTHREADS), sm, st, W, P, G, n, r, ldp,
-                       S, pstride, eps);
-    return hipGetLastError();
+    if (r < 1 || r > 128) return hipErrorInvalidValue;
+    const bool wide = n >= 16384;
+    if (r <= 32) return wide ? launch_cols_t<32, 32>(W, P, G, n, r, ldp, S, pstride, eps, st)
+                             : launch_cols_t<16, 32>(W, P, G, n, r, ldp, S, pstride, eps, st);
+    if (r <= 64) return wide ? launch_cols_t<32, 64>(W, P, G, n, r, ldp, S, pstride, eps, st)
+                             : launch_cols_t<16, 64>(W, P, G, n, r, ldp, S, pstride, eps, st);
+    return launch_cols_t<16, 128>(W, P, G, n, r, ldp, S, pstride, eps, st);
 }
 
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr, int S, long pstride,
