@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--serve-jobs", type=int, default=512,
+                    help="jobs of the batched-serving mix run through the server (0 = skip serving)")
+    ap.add_argument("--serve-seq-jobs", type=int, default=512,
+                    help="how many of those jobs are also run one at a time for the comparison")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the jobs (per s)")
     ap.add_argument("--gap-iterations", default="100,500,1000",
                     help="iteration counts at which the objective gap to the FP32 path is reported ('' = none)")
     return ap.parse_args()
@@ -174,6 +179,25 @@ def main():
                           "to the FP64 host oracle within 1e-4 by the parity tests",
         "gen_seconds": t_gen,
     }
+
+    if args.serve_jobs > 0:
+        from gpumlib_amd import serving
+
+        print(f"serving: generating {args.serve_jobs} jobs", file=sys.stderr, flush=True)
+        jobs = serving.make_jobs(args.serve_jobs)
+        data = [serving.job_data(j) for j in jobs]
+        print("serving: batched run", file=sys.stderr, flush=True)
+        srv = serving.run_server(lib, jobs, data, rate=args.serve_rate)
+        nseq = min(args.serve_seq_jobs, len(jobs))
+        print(f"serving: one-at-a-time run of {nseq} jobs", file=sys.stderr, flush=True)
+        seq = serving.run_one_at_a_time(jobs[:nseq], data[:nseq], rate=args.serve_rate)
+        line["serving"] = {
+            "workload": f"{len(jobs)} jobs, Euclidean/KL 50/50, m log-uniform [64,4096], n log-uniform [64,2048], "
+                        f"r in {{8,16,32,64}}, to relative objective change < 1e-4 or 500 iterations, "
+                        f"Poisson arrivals at {args.serve_rate:g}/s",
+            "batched": srv, "one_at_a_time": seq,
+            "jobs_per_s_speedup": srv["jobs_per_s"] / seq["jobs_per_s"],
+        }
 
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
