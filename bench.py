@@ -27,6 +27,20 @@ WORKLOADS = {
 }
 
 
+def load_traffic(workload):
+    """HBM bytes per launch of the pass kernels from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy of known size).
+    Counters cannot be read live; an absent summary gives traffic null."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        d = json.load(f)
+    out = {k: v["bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
+    out["_source"] = f"profiles/pmc_traffic_{workload}.json"
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,6 +57,9 @@ def parse():
     ap.add_argument("--serve-seq-jobs", type=int, default=512,
                     help="how many of those jobs are also run one at a time for the comparison")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the jobs (per s)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device to device three times (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
     ap.add_argument("--gap-iterations", default="100,500,1000",
                     help="iteration counts at which the objective gap to the FP32 path is reported ('' = none)")
     return ap.parse_args()
@@ -60,6 +77,15 @@ def main():
     from gpumlib_amd import nmf as G
 
     lib = capi.load()
+    if args.copy_calibration:
+        nbytes = 512 << 20
+        src, dst = hip.Buffer(nbytes), hip.Buffer(nbytes)
+        src.fill_zero()
+        for _ in range(3):
+            dst.copy_from(src, nbytes)
+        hip.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+        return
     rule_name, n, m, r, r0, seed = WORKLOADS[args.workload]
     rule = G.EUCLIDEAN if rule_name == "euclidean" else G.KL
     if rule == G.KL:
@@ -135,18 +161,22 @@ def main():
                         "pass_h_tflops": flops_h / (ms_h * 1e-3) / 1e12,
                         "pass_w_tflops": flops_w / (ms_w * 1e-3) / 1e12},
         }
+        tr = traffic.get(f"{prec_name}.{dom[0]}")
         if prec == G.FP32:
             ach = dom[2] / (dom[1] * 1e-3) / 1e12
             out["roofline"] = {"kernel": dom[0], "bound": "mfma", "achieved": ach, "peak": PEAK["fp32_mfma_tflops"],
-                               "unit": "TFLOP/s", "frac": ach / PEAK["fp32_mfma_tflops"], "traffic": None}
+                               "unit": "TFLOP/s", "frac": ach / PEAK["fp32_mfma_tflops"], "traffic": tr}
         else:
             vb = float(vbytes_el) * n * m  # algorithmic bytes of one pass: V streamed once
             ach = vb / (dom[1] * 1e-3) / 1e9
             out["roofline"] = {"kernel": dom[0], "bound": "hbm", "achieved": ach, "peak": PEAK["hbm_gbs"],
-                               "unit": "GB/s", "frac": ach / PEAK["hbm_gbs"], "traffic": None}
+                               "unit": "GB/s", "frac": ach / PEAK["hbm_gbs"], "traffic": tr}
+        if tr is not None:
+            out["roofline"]["traffic_source"] = traffic["_source"]
         return out
 
     gap_iters = sorted(int(x) for x in args.gap_iterations.split(",") if x.strip())
+    traffic = load_traffic(args.workload)
     legs = {"fp32": leg("fp32")}
     if args.precision != "fp32":
         legs[args.precision] = leg(args.precision)
