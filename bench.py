@@ -47,8 +47,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="euclid-16384x8192-r64", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"],
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp8"],
                     help="precision of the headline value (the FP32 path is always measured beside it)")
+    ap.add_argument("--extra-legs", default="fp8",
+                    help="further precisions measured and reported beside the headline (comma list, '' = none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -89,7 +91,7 @@ def main():
     rule_name, n, m, r, r0, seed = WORKLOADS[args.workload]
     rule = G.EUCLIDEAN if rule_name == "euclidean" else G.KL
     if rule == G.KL:
-        args.precision = "fp32"  # the BF16 storage path runs the Euclidean rule
+        args.precision = "fp32"  # the narrow storage paths run the Euclidean rule
     eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
 
     t_gen = time.time()
@@ -107,9 +109,9 @@ def main():
         flops_iter = 8.0 * n * m * r
 
     def leg(prec_name):
-        prec = {"fp32": G.FP32, "bf16": G.BF16}[prec_name]
+        prec = {"fp32": G.FP32, "bf16": G.BF16, "fp8": G.FP8}[prec_name]
         d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
-        vbytes_el = 4 if prec == G.FP32 else 2
+        vbytes_el = {G.FP32: 4, G.BF16: 2, G.FP8: 1}[prec]
         obj0 = d.objective(obj_kind)
         d.launch(args.warmup)
         d.stream.synchronize()
@@ -180,6 +182,15 @@ def main():
     legs = {"fp32": leg("fp32")}
     if args.precision != "fp32":
         legs[args.precision] = leg(args.precision)
+    if rule == G.EUCLIDEAN:  # the narrow storage paths run the Euclidean rule
+        for extra in (x.strip() for x in args.extra_legs.split(",")):
+            if extra and extra not in legs:
+                legs[extra] = leg(extra)
+                legs[extra]["speedup_vs_fp32_path"] = legs[extra]["iters_per_s"] / legs["fp32"]["iters_per_s"]
+                if gap_iters:
+                    legs[extra]["objective_rel_gap_vs_fp32_path"] = {
+                        str(t): abs(legs[extra]["objective_at"][t] - legs["fp32"]["objective_at"][t])
+                        / legs["fp32"]["objective_at"][t] for t in gap_iters}
     head = legs[args.precision]
     line = {
         "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",

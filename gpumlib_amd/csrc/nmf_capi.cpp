@@ -75,12 +75,12 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->nbi = pl->ld16n / 32;
     pl->nbu = pl->ld16m / 32;
     const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
-    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 128);
+    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
     // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
     // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
     // the BF16 / FP8 passes fold the whole gram into the first column tile.
     const int nbk = (r + 31) / 32, gram_tiles = (precision == GPUMLIB_NMF_FP32 && euc) ? nbk * nbk : 0;
-    const int tw = (precision == GPUMLIB_NMF_BF16) ? TILE_LP : TILE;  // columns (rows) per workgroup
+    const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP;  // columns (rows) per workgroup
     const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
     pl->tiles1 = vt1 > gram_tiles ? vt1 : gram_tiles;
     pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);

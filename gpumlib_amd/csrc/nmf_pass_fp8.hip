@@ -21,16 +21,48 @@ namespace gpumlib_nmf {
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int K8 = 128;        // K per pipeline step: 2 MFMAs of K = 64
-constexpr int LD8 = K8 + 16;   // LDS row pitch in bytes (36 words)
+constexpr int K8 = 512;        // K per chunk: 512 contiguous bytes of every column (8 MFMA steps of 64)
+constexpr int LD8 = K8 + 16;   // LDS row pitch in bytes
+constexpr int NKB = K8 / 32;   // E8M0 scale blocks per chunk
 
+__device__ __forceinline__ f32x16 mfma_fp8(u32x4 a0, u32x4 a1, u32x4 b0, u32x4 b1, int sa, int sb, f32x16 acc) {
+    const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+    const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, sa, 0, sb);
+}
+
+// Same structure as the BF16 pass: one workgroup per (slice s, column tile t) of 64
+// columns, chunks 512 elements (bytes) deep so every column is read in 512-byte runs,
+// the chunk's B columns and A rows (and their block scales) staged through LDS, the next
+// chunk's loads in flight in registers while the current one is multiplied, the 2*MB
+// output blocks spread over the 4 waves, the slice's gram on the first column tile.
+//
+// B scales come from the 32 x 32 tile scales of V: one scale per (32-column block, K
+// block), so the 64-column tile needs two rows of 16 scales per chunk; A scales are per
+// (rank row, K block).
+//
+// The instruction takes K block 0 of a row from bytes 0..15 of both lane halves and its
+// scale from lane c, K block 1 from bytes 16..31 and its scale from lane c + 32.  So lane
+// (c, h) supplies k = g*64 + 16h + j in bytes j < 16 and k = g*64 + 32 + 16h + j in bytes
+// 16 + j, and the scale of block 2g + h of the chunk.
+//
+// The stores and shadows are padded to whole chunks with zeros (scales 127), so no load
+// needs a bounds check; lanes past the last column or rank row re-read the last valid one
+// (their products land only in output columns that are not stored and in rank rows >= r,
+// which no update reads).
 template <int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     constexpr int RP = MB * 32;
-    __shared__ __attribute__((aligned(16))) uint8_t sA[2][RP][LD8];
-    __shared__ __attribute__((aligned(16))) uint8_t sB[2][TILE][LD8];
-    __shared__ uint8_t sAs[2][RP][4];    // E8M0 scale per (row of A^T, 32-block of this step)
-    __shared__ uint8_t sBs[2][TILE][4];  // E8M0 scale per (column of B, 32-block of this step)
+    constexpr int NBLK = 2 * MB;
+    constexpr int BPW = NBLK >= 4 ? NBLK / 4 : 1;
+    constexpr int GB = (MB * MB + 3) / 4;
+    constexpr int BL = TILE_LP * K8 / 16 / NTHREADS;  // 16-byte B loads per thread per chunk
+    constexpr int AL = RP * K8 / 16 / NTHREADS;       // 16-byte A loads per thread per chunk
+    constexpr int LPC = K8 / 16;                      // lanes per column (or row) in a load
+    __shared__ __attribute__((aligned(16))) uint8_t sA[RP][LD8];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[TILE_LP][LD8];
+    __shared__ __attribute__((aligned(16))) uint8_t sAs[RP][NKB];  // scale of (rank row, K block)
+    __shared__ __attribute__((aligned(16))) uint8_t sBs[2][NKB];   // scale of (32-column block, K block)
 
     const int total = p.tiles * p.S;
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
@@ -39,158 +71,132 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
-    // the first column tile of every slice also accumulates the slice's gram A^T A
-    // from the A rows staged in LDS (block (mb, nb) on wave mb*MB + nb, modulo 4)
     const bool gram = (t == 0) && p.G != nullptr;
-    const int col0 = t * TILE;
-    const int ncols = min(TILE, p.N - col0);
-    // The stores and shadows are padded to whole chunks with zeros (and scales 127), so
-    // every chunk is loaded without bounds checks; lanes past the last column or rank
-    // row re-read the last valid one (their products land only in output columns that
-    // are not stored and in rank rows >= r, which no update reads).
+    const int col0 = t * TILE_LP;
+    const int ncols = min(TILE_LP, p.N - col0);
     const int kbeg = s * p.kslice;
     const int kend = min(round_up(p.K, K8), kbeg + p.kslice);
-    constexpr int GB = (MB * MB + 3) / 4;
 
-    // Two register sets of one chunk each (native vectors, selected by a compile-time
-    // tag, so they stay in registers).
-    u32x4 ra[2][MB], rb[2][4];
-    uint8_t rsa[2], rsb[2][2];
-    using Set0 = std::integral_constant<int, 0>;
-    using Set1 = std::integral_constant<int, 1>;
-    auto gload = [&](auto set, int k0) __attribute__((always_inline)) {
-        constexpr int R = decltype(set)::value;
+    u32x4 rb[BL], ra[AL];
+    uint32_t rsa = 0;   // 4 A scales (row tid/4, blocks 4*(tid%4) .. +3)
+    uint8_t rsb = 127;  // one B scale (column block tid/16, block tid%16), threads < 32
+    auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
+        for (int j = 0; j < BL; j++) {
+            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 16;
             const int cc = col < ncols ? col : ncols - 1;
-            rb[R][j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
+            rb[j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
         }
 #pragma unroll
-        for (int j = 0; j < MB; j++) {
-            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
+        for (int j = 0; j < AL; j++) {
+            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 16;
             const int aa = a < p.r ? a : p.r - 1;
-            ra[R][j] = *reinterpret_cast<const u32x4*>(p.A + (size_t)aa * p.lda + k0 + kq);
+            ra[j] = *reinterpret_cast<const u32x4*>(p.A + (size_t)aa * p.lda + k0 + kq);
         }
         const int kb0 = k0 >> 5;
-        {
-            const int a = (tid >> 2) < p.r ? (tid >> 2) : p.r - 1, kb = kb0 + (tid & 3);
-            rsa[R] = p.Asc[(size_t)a * p.nbk + kb];
+        if (tid < RP * 4) {
+            const int a = tid >> 2, aa = a < p.r ? a : p.r - 1;
+            rsa = *reinterpret_cast<const uint32_t*>(p.Asc + (size_t)aa * p.nbk + kb0 + 4 * (tid & 3));
         }
-#pragma unroll
-        for (int jj = 0; jj < 2; jj++) {
-            const int e = tid + NTHREADS * jj, col = e >> 2, kb = kb0 + (e & 3);
-            const int cc = col < ncols ? col : ncols - 1;
-            rsb[R][jj] = p.Bsc[(size_t)((col0 + cc) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
+        if (tid < 2 * NKB) {
+            const int cb = tid / NKB, kb = kb0 + tid % NKB;
+            const int cc = cb * 32 < ncols ? cb * 32 : ncols - 1;
+            rsb = p.Bsc[(size_t)((col0 + cc) >> p.bcol_shift) * p.bs_col + (size_t)kb * p.bs_k];
         }
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the multiply
     };
-    auto sstore = [&](auto set, int buf) __attribute__((always_inline)) {
-        constexpr int R = decltype(set)::value;
+    auto sstore = [&]() __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 16;
-            *reinterpret_cast<u32x4*>(&sB[buf][col][kq]) = rb[R][j];
+        for (int j = 0; j < BL; j++) {
+            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 16;
+            *reinterpret_cast<u32x4*>(&sB[col][kq]) = rb[j];
         }
 #pragma unroll
-        for (int j = 0; j < MB; j++) {
-            const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 16;
-            *reinterpret_cast<u32x4*>(&sA[buf][a][kq]) = ra[R][j];
+        for (int j = 0; j < AL; j++) {
+            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 16;
+            *reinterpret_cast<u32x4*>(&sA[a][kq]) = ra[j];
         }
-        if (tid < RP * 4) sAs[buf][tid >> 2][tid & 3] = rsa[R];
-#pragma unroll
-        for (int jj = 0; jj < 2; jj++) {
-            const int e = tid + NTHREADS * jj;
-            sBs[buf][e >> 2][e & 3] = rsb[R][jj];
-        }
+        if (tid < RP * 4) *reinterpret_cast<uint32_t*>(&sAs[tid >> 2][4 * (tid & 3)]) = rsa;
+        if (tid < 2 * NKB) sBs[tid / NKB][tid % NKB] = rsb;
     };
 
-    f32x16 acc[MB], gacc[GB];
+    f32x16 acc[BPW], gacc[GB];
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++)
-        for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
+    for (int q = 0; q < BPW; q++)
+        for (int v = 0; v < 16; v++) acc[q][v] = 0.f;
 #pragma unroll
     for (int q = 0; q < GB; q++)
         for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
-    auto compute = [&](int buf) __attribute__((always_inline)) {
-        // The instruction takes K block 0 of a row from bytes 0..15 of both lane halves and
-        // its scale from lane c, K block 1 from bytes 16..31 and its scale from lane c + 32.
-        // So lane (c, h) supplies k = g*64 + 16h + j in bytes j < 16 and k = g*64 + 32 + 16h + j
-        // in bytes 16 + j, and the scale of block 2g + h of this step.
+    auto compute = [&](auto gram_tag) __attribute__((always_inline)) {
+        constexpr bool GR = decltype(gram_tag)::value;
 #pragma unroll
         for (int g = 0; g < K8 / 64; g++) {
-            const uint8_t* pb = &sB[buf][wave * 32 + c][g * 64 + h * 16];
-            const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
-            const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-            const int sb = sBs[buf][wave * 32 + c][2 * g + h];
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++) {
-                const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 16];
-                const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
-                const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
-                                  (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-                const int sa = sAs[buf][mb * 32 + c][2 * g + h];
-                acc[mb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc[mb], 0, 0, 0, sa, 0, sb);
+            for (int q = 0; q < BPW; q++) {
+                const int blk = wave * BPW + q;
+                if (NBLK % 4 == 0 || blk < NBLK) {
+                    const int mb = blk >> 1, nb = blk & 1;
+                    const uint8_t* pa = &sA[mb * 32 + c][g * 64 + h * 16];
+                    const uint8_t* pb = &sB[nb * 32 + c][g * 64 + h * 16];
+                    acc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
+                                      *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
+                                      sAs[mb * 32 + c][2 * g + h], sBs[nb][2 * g + h], acc[q]);
+                }
             }
-        }
-        if (gram) {
+            if constexpr (GR) {
 #pragma unroll
-            for (int q = 0; q < GB; q++) {
-                const int blk = wave + 4 * q;
-                if (blk >= MB * MB) continue;
-                const int mb = blk / MB, nb = blk - (blk / MB) * MB;
-#pragma unroll
-                for (int g = 0; g < K8 / 64; g++) {
-                    const uint8_t* pa = &sA[buf][mb * 32 + c][g * 64 + h * 16];
-                    const uint8_t* pb = &sA[buf][nb * 32 + c][g * 64 + h * 16];
-                    const uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
-                    const uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
-                    const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
-                                      (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-                    const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
-                                      (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-                    const int sa = sAs[buf][mb * 32 + c][2 * g + h], sb = sAs[buf][nb * 32 + c][2 * g + h];
-                    gacc[q] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, gacc[q], 0, 0, 0, sa, 0, sb);
+                for (int q = 0; q < GB; q++) {
+                    const int blk = wave + 4 * q;
+                    if ((MB * MB) % 4 == 0 || blk < MB * MB) {
+                        const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+                        const uint8_t* pa = &sA[mb * 32 + c][g * 64 + h * 16];
+                        const uint8_t* pb = &sA[nb * 32 + c][g * 64 + h * 16];
+                        gacc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
+                                           *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
+                                           sAs[mb * 32 + c][2 * g + h], sAs[nb * 32 + c][2 * g + h], gacc[q]);
+                    }
                 }
             }
         }
     };
 
-    // Stages of two chunks: the loads of chunks ch+2 and ch+3 are issued, chunks ch and
-    // ch+1 are multiplied out of LDS buffers 0 and 1 while those loads are in flight, and
-    // after a barrier the new chunks are stored (no loop-carried staging registers).
-    const int nchunks = (kend - kbeg + K8 - 1) / K8;
-    gload(Set0{}, kbeg);
-    if (nchunks > 1) gload(Set1{}, kbeg + K8);
-    sstore(Set0{}, 0);
-    if (nchunks > 1) sstore(Set1{}, 1);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ch += 2) {
-        const bool more0 = ch + 2 < nchunks, more1 = ch + 3 < nchunks;
-        if (more0) gload(Set0{}, kbeg + (ch + 2) * K8);
-        if (more1) gload(Set1{}, kbeg + (ch + 3) * K8);
-        compute(0);
-        if (ch + 1 < nchunks) compute(1);
+    const int nchunks = (kend - kbeg) / K8;  // whole chunks (padded operands)
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+        gload(kbeg);
+        sstore();
         __syncthreads();
-        if (more0) sstore(Set0{}, 0);
-        if (more1) sstore(Set1{}, 1);
-        __syncthreads();
-    }
+        for (int ch = 0; ch < nchunks; ch++) {
+            const bool more = ch + 1 < nchunks;
+            if (more) gload(kbeg + (ch + 1) * K8);
+            compute(gram_tag);
+            __syncthreads();
+            if (more) {
+                sstore();
+                __syncthreads();
+            }
+        }
+    };
+    if (gram)
+        mainloop(std::true_type{});
+    else
+        mainloop(std::false_type{});
 
+    // acc[q][v] = P(a = mb*32 + acc_row(v,h), u = nb*32 + c) for block wave*BPW + q
     float* slice = p.P + (size_t)s * p.pstride;
-    const int ul = wave * 32 + c;
-    if (ul < ncols) {
+#pragma unroll
+    for (int q = 0; q < BPW; q++) {
+        const int blk = wave * BPW + q;
+        if (blk >= NBLK) continue;
+        const int mb = blk >> 1, ul = (blk & 1) * 32 + c;
+        if (ul >= ncols) continue;
         float* dst = slice + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const int a0 = mb * 32 + g * 8 + h * 4;
-                if (a0 < p.ldr) {
-                    float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
-                    *reinterpret_cast<float4*>(dst + a0) = o;
-                }
-            }
+        for (int g = 0; g < 4; g++) {
+            const int a0 = mb * 32 + g * 8 + h * 4;
+            if (a0 < p.ldr)
+                *reinterpret_cast<float4*>(dst + a0) =
+                    make_float4(acc[q][g * 4], acc[q][g * 4 + 1], acc[q][g * 4 + 2], acc[q][g * 4 + 3]);
         }
     }
     if (gram) {

@@ -29,10 +29,10 @@ struct Plan {
 };
 
 // Leading dimensions of the narrow V stores and operand shadows: whole pass chunks
-// (256 BF16 or 128 FP8 elements along the contraction), the padding zero-filled,
+// (256 BF16 or 512 FP8 elements along the contraction), the padding zero-filled,
 // so the streaming passes load every chunk without bounds checks.
 inline int bf16_ld(int k) { return (k + 255) / 256 * 256; }
-inline int fp8_ld(int k) { return (k + 127) / 128 * 128; }
+inline int fp8_ld(int k) { return (k + 511) / 512 * 512; }
 
 // Byte layout of the FP8 V store: Vq (m columns, ld fp8_ld(n)) | Vtq (n columns, ld fp8_ld(m)) |
 // tile scales (fp8_ld(m)/32 x fp8_ld(n)/32, padding tiles 127 = 2^0).

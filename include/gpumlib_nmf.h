@@ -101,7 +101,7 @@ enum {
  * BF16 (round to nearest even), each column-major with its leading dimension rounded up to a multiple of
  * 256 (ldv = 256*ceil(n/256) for V, ldt = 256*ceil(m/256) for V^T), padding zero, V^T starting at a 256-byte
  * boundary; the W pass then reads V^T with unit stride.  GPUMLIB_NMF_FP8: V and V^T in OCP E4M3 with
- * leading dimensions rounded up to multiples of 128 bytes (ldv = 128*ceil(n/128), ldt = 128*ceil(m/128)),
+ * leading dimensions rounded up to multiples of 512 bytes (ldv = 512*ceil(n/512), ldt = 512*ceil(m/512)),
  * padding zero, 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V
  * (scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding tiles hold 127 = 2^0), shared by
  * both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run

@@ -46,8 +46,8 @@ def test_fp8_pack_bit_exact(native, cuda, nm):
     W, H = G.gen_init(n, m, 4, 1)
     p = G.DeviceNMF(G.EUCLIDEAN, V, W, H, n, m, 4, precision=G.FP8)
     raw = p.V.download(np.uint8)
-    # leading dimensions are whole 128-byte chunks: zero padding, padding tiles scaled 127 (2^0)
-    ldn, ldm = (n + 127) // 128 * 128, (m + 127) // 128 * 128
+    # leading dimensions are whole 512-byte chunks: zero padding, padding tiles scaled 127 (2^0)
+    ldn, ldm = (n + 511) // 512 * 512, (m + 511) // 512 * 512
     off_t = (m * ldn + 255) // 256 * 256
     off_sc = off_t + (n * ldm + 255) // 256 * 256
     nbi, nbu = ldn // 32, ldm // 32
