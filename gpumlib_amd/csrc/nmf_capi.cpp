@@ -312,12 +312,10 @@ hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, floa
         if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
             CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
         CK(launch_pass_tn_fp8(ah, st));
-        CK(launch_update_h_euc(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
-        CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
+        CK(launch_update_h_euc_q8(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hq, pl.ld16m, Hsc, st));
         if (!update_w) continue;
         CK(launch_pass_tn_fp8(aw, st));
-        CK(launch_update_w_euc_t(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, nullptr, 0, st));
-        CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
+        CK(launch_update_w_euc_t_q8(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wq, pl.ld16n, Wsc, st));
     }
     return hipSuccess;
 }

@@ -39,6 +39,12 @@ hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, 
 // BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
                                  float eps, uint16_t* Wb, int ldwb, hipStream_t st);
+// FP8 storage path: the same updates, also writing the E4M3 shadow of the new factor
+// (one E8M0 scale per 32 rows along the contraction).
+hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
+                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st);
+hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, int n, int r, int ldr, int S,
+                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st);
 // Zeroes n ints (arrival counters) with a kernel, so that it is safe inside captured graphs.
 hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 // KL H:  H(a,u) *= num(a,u) / max(sum_s Psum[s][a], eps)

@@ -2,6 +2,7 @@
 // kernels.  All reductions are in a fixed order, so results do not depend on
 // timing or on what else runs on the device.
 #include "nmf_common.hpp"
+#include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
 
 namespace gpumlib_nmf {
@@ -190,7 +191,8 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
                                                               const float* __restrict__ P, const float* __restrict__ G,
                                                               int ldg, int gt, int count, int r, int ldr, int S,
                                                               long pstride, float eps, uint16_t* __restrict__ Sh,
-                                                              int ldsh) {
+                                                              int ldsh, uint8_t* __restrict__ Q, int ldq,
+                                                              uint8_t* __restrict__ Qsc, int nblk) {
     extern __shared__ __attribute__((aligned(16))) float smr[];
     constexpr int RS = RP + 4, NQ = RP / 4, ITEMS = JT * NQ, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
     float* sG = smr;            // sG[b*RS + a] = C(b,a), zero for a or b >= r
@@ -285,6 +287,33 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         for (int idx = tid; idx < JT * r; idx += NTHREADS) {
             const int jj = idx % JT, a = idx / JT;
             if (jj < nj) Sh[(size_t)a * ldsh + j0 + jj] = __builtin_bit_cast(uint16_t, (__bf16)sN[jj * RS + a]);
+        }
+    }
+    if constexpr (JT == 32) {
+        // FP8 shadow: the workgroup's 32 rows are one scale block of every rank column a
+        // (X(j,a) at Q[a*ldq + j], scale at Qsc[a*nblk + j/32]); rows past count are zeros
+        if (Q) {
+            for (int a = tid; a < r; a += NTHREADS) {
+                float amax = 0.f;
+                for (int jj = 0; jj < nj; jj++) amax = fmaxf(amax, fabsf(sN[jj * RS + a]));
+                const int sc = e8m0_for_amax(amax);
+                Qsc[(size_t)a * nblk + j0 / 32] = (uint8_t)sc;
+                uint32_t word[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int jj = 4 * q + b;
+                        const uint32_t e = jj < nj ? f32_to_e4m3(e8m0_unscale(sN[jj * RS + a], sc)) : 0u;
+                        v |= e << (8 * b);
+                    }
+                    word[q] = v;
+                }
+                uint4* dst = reinterpret_cast<uint4*>(Q + (size_t)a * ldq + j0);
+                dst[0] = make_uint4(word[0], word[1], word[2], word[3]);
+                dst[1] = make_uint4(word[4], word[5], word[6], word[7]);
+            }
         }
     }
 }
@@ -659,21 +688,24 @@ static hipError_t set_smem(K kernel, size_t sm) {
 template <int JT, int RP>
 static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                                 int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                                hipStream_t st) {
+                                uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, hipStream_t st) {
     const size_t sm = (size_t)(RP + 2 * JT) * (RP + 4) * sizeof(float);
     hipError_t e = set_smem(k_update_rows_euc<JT, RP>, sm);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_update_rows_euc<JT, RP>), dim3((count + JT - 1) / JT), dim3(NTHREADS), sm, st, X, xs_j, xs_a,
-                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh);
+                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk);
     return hipGetLastError();
 }
 
+// Q (FP8 shadow) needs 32 rows per workgroup (one scale block); it is given for r <= 64.
 static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                               int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                              hipStream_t st) {
-    if (r < 1 || r > 128) return hipErrorInvalidValue;
-    const bool wide = count >= 16384;
-#define LR(JT, RP) return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, st)
+                              uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, hipStream_t st) {
+    if (r < 1 || r > 128 || (Q && (r > 64 || ldq % 32 || ldq < count))) return hipErrorInvalidValue;
+    const bool wide = count >= 16384 || Q != nullptr;
+#define LR(JT, RP)                                                                                                \
+    return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, \
+                                 Qsc, nblk, st)
     if (r <= 32) { if (wide) LR(32, 32); LR(16, 32); }
     if (r <= 64) { if (wide) LR(32, 64); LR(16, 64); }
     LR(16, 128);
@@ -682,12 +714,22 @@ static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, co
 
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
                                float eps, uint16_t* Hs, int ldhs, hipStream_t st) {
-    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, st);
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, st);
 }
 
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
                                  float eps, uint16_t* Wb, int ldwb, hipStream_t st) {
-    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, st);
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, st);
+}
+
+hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
+                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st) {
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, nullptr, 0, Hq, ldq, Hsc, ldq / 32, st);
+}
+
+hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, int n, int r, int ldr, int S,
+                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st) {
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, nullptr, 0, Wq, ldq, Wsc, ldq / 32, st);
 }
 
 template <int JT, int RP>
