@@ -60,6 +60,8 @@ SHAPES = [
     (1, 7, 1),
     (4100, 36, 12),   # tall: many K slices in the H pass
     (36, 4100, 12),   # wide: many K slices in the W pass
+    (16500, 90, 64),  # n >= 16384: the 32-row update workgroups, a partial last one
+    (16390, 70, 20),  # the same for one rank block, n not a multiple of 4
 ]
 
 
