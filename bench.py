@@ -230,15 +230,19 @@ def main():
         print("serving: batched run", file=sys.stderr, flush=True)
         srv = serving.run_server(lib, jobs, data, rate=args.serve_rate)
         nseq = min(args.serve_seq_jobs, len(jobs))
-        print(f"serving: one-at-a-time run of {nseq} jobs", file=sys.stderr, flush=True)
-        seq = serving.run_one_at_a_time(jobs[:nseq], data[:nseq], rate=args.serve_rate)
         line["serving"] = {
             "workload": f"{len(jobs)} jobs, Euclidean/KL 50/50, m log-uniform [64,4096], n log-uniform [64,2048], "
                         f"r in {{8,16,32,64}}, to relative objective change < 1e-4 or 500 iterations, "
                         f"Poisson arrivals at {args.serve_rate:g}/s",
-            "batched": srv, "one_at_a_time": seq,
-            "jobs_per_s_speedup": srv["jobs_per_s"] / seq["jobs_per_s"],
+            "batched": srv,
         }
+        if nseq > 0:
+            print(f"serving: one-at-a-time run of {nseq} jobs", file=sys.stderr, flush=True)
+            seq = serving.run_one_at_a_time(
+                jobs[:nseq], data[:nseq], rate=args.serve_rate,
+                progress=lambda k: print(f"serving: {k} jobs done one at a time", file=sys.stderr, flush=True))
+            line["serving"]["one_at_a_time"] = seq
+            line["serving"]["jobs_per_s_speedup"] = srv["jobs_per_s"] / seq["jobs_per_s"]
 
     if not args.no_cpu_baseline:
         from oracle import oracle as orc

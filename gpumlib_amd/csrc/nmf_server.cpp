@@ -27,6 +27,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -106,6 +107,27 @@ struct Job {
 
 }  // namespace
 
+struct gpumlib_nmf_server;
+
+namespace {
+// Servers still alive when the library is unloaded, i.e. the process exits without
+// destroying them (an exception in the caller, say).  Their engine threads must not
+// keep launching work while the HIP runtime is being torn down (in-flight kernels would
+// then touch memory that is going away), so the library's exit path stops each engine
+// after its round in flight and waits for its stream.  The registry is created after
+// the HIP runtime has been initialised (on the first server), so it is destroyed
+// before the runtime.
+struct LiveServers {
+    std::mutex mu;
+    std::set<gpumlib_nmf_server*> set;
+    ~LiveServers();
+};
+LiveServers& live_servers() {
+    static LiveServers l;
+    return l;
+}
+}  // namespace
+
 struct gpumlib_nmf_server {
     gpumlib_nmf_server_config cfg;
     int device = 0;
@@ -123,6 +145,7 @@ struct gpumlib_nmf_server {
     std::vector<std::shared_ptr<Job>> active;        // by slot
     long long next_id = 1;
     bool stop = false;
+    bool abandon = false;  // process exit: leave after the round in flight, do not finish queued work
     hipError_t fatal = hipSuccess;
     gpumlib_nmf_server_stats stats;
     std::thread engine;
@@ -169,6 +192,7 @@ void gpumlib_nmf_server::run() {
         {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
+                if (abandon) return;
                 admit_locked(admitted);
                 const bool have_active = std::any_of(active.begin(), active.end(), [](auto& p) { return !!p; });
                 if (!admitted.empty() || have_active) break;
@@ -270,6 +294,20 @@ void gpumlib_nmf_server::run() {
     }
 }
 
+LiveServers::~LiveServers() {
+    std::lock_guard<std::mutex> g(mu);
+    for (gpumlib_nmf_server* s : set) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->abandon = true;
+        }
+        s->cv_work.notify_all();
+        if (s->engine.joinable()) s->engine.join();
+        (void)hipStreamSynchronize(s->stream);
+    }
+    set.clear();
+}
+
 extern "C" {
 
 hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpumlib_nmf_server_handle* out) {
@@ -298,6 +336,11 @@ hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpuml
     }
     s->pool.init(cfg->hbm_budget_bytes);
     s->active.resize(cfg->max_active);
+    {
+        LiveServers& l = live_servers();
+        std::lock_guard<std::mutex> g(l.mu);
+        l.set.insert(s);
+    }
     s->engine = std::thread([s] { s->run(); });
     *out = s;
     return hipSuccess;
@@ -373,6 +416,11 @@ hipError_t gpumlib_nmf_server_stats_get(gpumlib_nmf_server_handle s, gpumlib_nmf
 
 void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s) {
     if (!s) return;
+    {
+        LiveServers& l = live_servers();
+        std::lock_guard<std::mutex> g(l.mu);
+        l.set.erase(s);
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;

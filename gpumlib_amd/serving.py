@@ -60,30 +60,32 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
     gaps = rng.exponential(1.0 / rate, size=len(jobs)) if rate > 0 else np.zeros(len(jobs))
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     ids = []
-    t0 = time.perf_counter()
-    due = t0
-    for job, (V, W, H), gap in zip(jobs, data, gaps):
-        due += gap
-        while time.perf_counter() < due:
-            time.sleep(min(0.001, due - time.perf_counter()))
-        rule, n, m, r, _ = job
-        eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
-        jid = ctypes.c_longlong()
-        capi.check(lib.gpumlib_nmf_server_submit(h, rule, n, m, r, ptr(V), ptr(W), ptr(H), eps, tol, max_iters,
-                                                 ctypes.byref(jid)), "gpumlib_nmf_server_submit")
-        ids.append(jid.value)
     lat, iters, conv, first, last = [], [], 0, None, None
-    for jid in ids:
-        res = capi.JobResult()
-        capi.check(lib.gpumlib_nmf_server_wait(h, jid, None, None, ctypes.byref(res)), "gpumlib_nmf_server_wait")
-        lat.append(res.finish_s - res.submit_s)
-        iters.append(res.iterations)
-        conv += res.converged
-        first = res.submit_s if first is None else min(first, res.submit_s)
-        last = res.finish_s if last is None else max(last, res.finish_s)
     st = capi.ServerStats()
-    capi.check(lib.gpumlib_nmf_server_stats_get(h, ctypes.byref(st)), "gpumlib_nmf_server_stats_get")
-    lib.gpumlib_nmf_server_destroy(h)
+    try:
+        due = time.perf_counter()
+        for job, (V, W, H), gap in zip(jobs, data, gaps):
+            due += gap
+            wait = due - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            rule, n, m, r, _ = job
+            eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
+            jid = ctypes.c_longlong()
+            capi.check(lib.gpumlib_nmf_server_submit(h, rule, n, m, r, ptr(V), ptr(W), ptr(H), eps, tol, max_iters,
+                                                     ctypes.byref(jid)), "gpumlib_nmf_server_submit")
+            ids.append(jid.value)
+        for jid in ids:
+            res = capi.JobResult()
+            capi.check(lib.gpumlib_nmf_server_wait(h, jid, None, None, ctypes.byref(res)), "gpumlib_nmf_server_wait")
+            lat.append(res.finish_s - res.submit_s)
+            iters.append(res.iterations)
+            conv += res.converged
+            first = res.submit_s if first is None else min(first, res.submit_s)
+            last = res.finish_s if last is None else max(last, res.finish_s)
+        capi.check(lib.gpumlib_nmf_server_stats_get(h, ctypes.byref(st)), "gpumlib_nmf_server_stats_get")
+    finally:
+        lib.gpumlib_nmf_server_destroy(h)  # finishes what was submitted, then releases the server
     span = last - first
     return {"jobs": len(ids), "jobs_per_s": len(ids) / span, "span_s": span,
             "latency_p50_s": _percentile(lat, 50), "latency_p99_s": _percentile(lat, 99),
@@ -92,7 +94,7 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
             "arrival_rate_per_s": rate, "round_iterations": round_iters, "graph": bool(use_graph)}
 
 
-def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=0.0):
+def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=0.0, progress=None):
     """The same jobs launched one after another through gpumlib_nmf_iterate (one
     problem, its own buffers), with the server's stopping rule and cadence.
 
@@ -119,6 +121,8 @@ def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=
         d.get_h()
         lat.append(time.perf_counter() - s0)
         iters.append(done)
+        if progress and len(lat) % 256 == 0:
+            progress(len(lat))
     span = time.perf_counter() - t0
     rng = np.random.default_rng(12345)
     arrive = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs))) if rate > 0 else np.zeros(len(jobs))

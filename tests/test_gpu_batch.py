@@ -226,3 +226,55 @@ def test_server_alone_equals_mix_and_admission(native, cuda):
     s2.close()
     assert a[2].iterations == b[2].iterations
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_serving_stream_error_still_destroys_server(native, cuda):
+    """A submit that fails part-way through a serving stream (here a rank above 64)
+    raises, and the server is destroyed on the way out: the jobs already submitted
+    finish and a new server runs normally afterwards."""
+    from gpumlib_amd import serving
+
+    jobs = serving.make_jobs(6, seed=77)
+    jobs[3] = (G.EUCLIDEAN, 80, 90, 65, 1)
+    data = [serving.job_data(j if j[3] <= 64 else (j[0], j[1], j[2], 8, j[4])) for j in jobs]
+    with pytest.raises(RuntimeError, match="gpumlib_nmf_server_submit"):
+        serving.run_server(native, jobs, data, rate=0.0, max_iters=20)
+    ok = serving.run_server(native, jobs[:3], data[:3], rate=0.0, max_iters=20)
+    assert ok["jobs"] == 3
+
+
+_EXIT_CHILD = r"""
+import ctypes, sys
+sys.path.insert(0, {root!r})
+from gpumlib_amd import capi
+from gpumlib_amd import nmf as G
+lib = capi.load()
+cfg = capi.ServerConfig(64 << 20, 16, 10, 1, 0)
+h = ctypes.c_void_p()
+capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+keep = []
+for k in range(12):
+    V = G.gen_euclidean(400, 300, 16, k)
+    W, H = G.gen_init(400, 300, 16, k)
+    keep.append((V, W, H))
+    jid = ctypes.c_longlong()
+    capi.check(lib.gpumlib_nmf_server_submit(h, G.EUCLIDEAN, 400, 300, 16, ptr(V), ptr(W), ptr(H),
+                                             G.REFERENCE_EPS, 0.0, 500, ctypes.byref(jid)))
+print("submitted; leaving without destroying the server", flush=True)
+"""
+
+
+def test_server_left_running_at_process_exit(native, cuda):
+    """A process that ends while its server still has work (never destroyed) exits
+    cleanly: the library stops the engine after its round in flight and waits for its
+    stream before the HIP runtime goes away."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-c", _EXIT_CHILD.format(root=root)], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    assert "submitted" in p.stdout
