@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libgpumlib_nmf.so")
 
 EUCLIDEAN, KL = 0, 1
+ADDITIVE_EUCLIDEAN, ADDITIVE_KL, ADDITIVE_EUCLIDEAN_WITH_H = 2, 3, 4
 FP32, BF16, FP8 = 0, 1, 2
 OBJ_NONE, OBJ_FROBENIUS, OBJ_KL, OBJ_QUALITY = 0, 1, 2, 3
 REFERENCE_EPS = 1e-8  # float(10e-9f) == float(1e-8f)
@@ -43,6 +44,12 @@ SIGNATURES = {
     "gpumlib_cuda_nmf_multiplicative_euclidean_update_matrix": (_i, [_c.c_uint, _c.c_uint, _vp, _vp, _vp, _i]),
     "gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_w": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
     "gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+    "gpumlib_cuda_nmf_additive_euclidean_update_matrix": (_i, [_c.c_uint, _c.c_uint, _vp, _vp, _vp, _i]),
+    "gpumlib_cuda_nmf_additive_divergence_update_matrix_h": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+    "gpumlib_cuda_nmf_additive_divergence_update_matrix_w": (_i, [Dim3, Dim3, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+    "gpumlib_nmf_additive_update": (_i, [_vp, _vp, _vp, _l, _vp]),
+    "gpumlib_nmf_additive_kl_update_h": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
+    "gpumlib_nmf_additive_kl_update_w": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "gpumlib_nmf_mu_update": (_i, [_vp, _vp, _vp, _l, _f, _vp]),
     "gpumlib_nmf_sum_w": (_i, [_vp, _i, _i, _f, _vp, _vp]),
     "gpumlib_nmf_sum_h": (_i, [_vp, _i, _i, _f, _vp, _vp]),

@@ -172,7 +172,9 @@ hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgrou
 
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r) {
     Plan pl;
-    if (!b || r > 64 || !make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->target)) return 0;
+    if (!b || r > 64 || (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) ||
+        !make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->target))
+        return 0;
     return pl.bytes;
 }
 
@@ -180,6 +182,7 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
                                  float* H, float eps, void* workspace, size_t workspace_bytes, int* slot) {
     Plan pl;
     if (!b || !slot || !V || !W || !H || !workspace || r > 64) return hipErrorInvalidValue;
+    if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return hipErrorInvalidValue;  // multiplicative only
     if (!make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->target) || workspace_bytes < pl.bytes)
         return hipErrorInvalidValue;
     int s = 0;

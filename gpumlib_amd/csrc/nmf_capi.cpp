@@ -53,7 +53,7 @@ size_t vstore_bytes(int precision, int n, int m) {
 }
 
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
-    if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return false;
+    if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
     if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8) return false;
     if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
     if (n < 1 || m < 1 || r < 1 || r > kMaxFusedRank) return false;
@@ -74,7 +74,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->ld16m = fp8_ld(m);
     pl->nbi = pl->ld16n / 32;
     pl->nbu = pl->ld16m / 32;
-    const int euc = (rule == GPUMLIB_NMF_EUCLIDEAN);
+    const int euc = !rule_is_kl(rule);
     const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
     // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
     // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
@@ -168,27 +168,39 @@ void pass_args(const Plan& pl, const float* V, const float* W, const float* H, v
     }
 }
 
+// FP32 iterations of every rule.  The additive rules run the same passes with the
+// additive update; GPUMLIB_NMF_ADDITIVE_EUCLIDEAN leaves H unchanged, as
+// NMF_AdditiveEuclidian::DoIteration does (nmf_additive_euclidian.cpp:35-42 hands H to
+// the multiplicative launcher as its numerator, so only its scratch deltaH2 changes),
+// so its H half is not run at all.
 hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations, int update_w,
                           int objective, double* d_objective, void* ws, hipStream_t st) {
-    const int kl = (pl.rule == GPUMLIB_NMF_KL);
+    const int kl = rule_is_kl(pl.rule);
+    const int add = rule_is_additive(pl.rule);
+    const int h_half = (pl.rule != GPUMLIB_NMF_ADDITIVE_EUCLIDEAN);
     PassArgs a1, a3;
     pass_args(pl, V, W, H, ws, eps, 1, &a1);
     pass_args(pl, V, W, H, ws, eps, 3, &a3);
     CK(launch_zero_i32(a1.gcount, 2, st));
     for (int it = 0; it < iterations; it++) {
         const int obj = (it == 0 && d_objective) ? objective : GPUMLIB_NMF_OBJ_NONE;
-        CK(launch_pass_h_f32(a1, kl, obj, 1, st));
-        if (obj != GPUMLIB_NMF_OBJ_NONE) CK(launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st));
-        if (kl)
-            CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st));
-        else
-            CK(launch_update_h_euc(H, a1.P, a1.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st));
+        if (h_half) {
+            CK(launch_pass_h_f32(a1, kl, obj, 1, st));
+            if (obj != GPUMLIB_NMF_OBJ_NONE) CK(launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st));
+            if (kl)
+                CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st, add));
+            else
+                CK(launch_update_h_euc(H, a1.P, a1.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, nullptr, 0, st,
+                                       add));
+        } else if (obj != GPUMLIB_NMF_OBJ_NONE) {
+            CK(run_objective(pl, V, W, H, eps, obj, d_objective, ws, st));
+        }
         if (!update_w) continue;
         CK(launch_pass_w_f32(a3, kl, st));
         if (kl)
-            CK(launch_update_w_kl(W, a3.P, a3.Psum, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st));
+            CK(launch_update_w_kl(W, a3.P, a3.Psum, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st, add));
         else
-            CK(launch_update_w_euc(W, a3.P, a3.G, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st));
+            CK(launch_update_w_euc(W, a3.P, a3.G, pl.n, pl.r, pl.ldp, pl.S3, pl.pstride3, eps, st, add));
     }
     return hipSuccess;
 }
@@ -340,6 +352,39 @@ hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(dim3, dim3
     return launch_kl_update_h_wh(H, W, V, WH, sumW, n, m, r, kReferenceEpsilon, nullptr);
 }
 
+hipError_t gpumlib_cuda_nmf_additive_euclidean_update_matrix(unsigned int, unsigned int, float* X, float* deltaX1,
+                                                             float* deltaX2, int elements) {
+    return launch_add_update(X, deltaX1, deltaX2, elements, nullptr);
+}
+
+hipError_t gpumlib_cuda_nmf_additive_divergence_update_matrix_h(dim3, dim3, float* H, float* W, float* V, float* WH,
+                                                                float* sumW, int n, int m, int r) {
+    return launch_kl_update_h_wh(H, W, V, WH, sumW, n, m, r, kReferenceEpsilon, nullptr, 1);
+}
+
+hipError_t gpumlib_cuda_nmf_additive_divergence_update_matrix_w(dim3, dim3, float* W, float* H, float* V, float* WH,
+                                                                float* sumH, int n, int m, int r) {
+    return launch_kl_update_w_wh(W, H, V, WH, sumH, n, m, r, kReferenceEpsilon, nullptr, 1);
+}
+
+hipError_t gpumlib_nmf_additive_update(float* X, const float* d1, const float* d2, long elements,
+                                       hipStream_t stream) {
+    if (elements < 0) return hipErrorInvalidValue;
+    return launch_add_update(X, d1, d2, elements, stream);
+}
+
+hipError_t gpumlib_nmf_additive_kl_update_h(float* H, const float* W, const float* V, const float* WH,
+                                            const float* sumW, int n, int m, int r, float eps, hipStream_t stream) {
+    if (n < 0 || m < 0 || r < 0) return hipErrorInvalidValue;
+    return launch_kl_update_h_wh(H, W, V, WH, sumW, n, m, r, eps, stream, 1);
+}
+
+hipError_t gpumlib_nmf_additive_kl_update_w(float* W, const float* H, const float* V, const float* WH,
+                                            const float* sumH, int n, int m, int r, float eps, hipStream_t stream) {
+    if (n < 0 || m < 0 || r < 0) return hipErrorInvalidValue;
+    return launch_kl_update_w_wh(W, H, V, WH, sumH, n, m, r, eps, stream, 1);
+}
+
 hipError_t gpumlib_nmf_mu_update(const float* nm, const float* dm, float* m, long elements, float eps,
                                  hipStream_t stream) {
     if (elements < 0) return hipErrorInvalidValue;
@@ -452,7 +497,7 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
     if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
         return hipErrorInvalidValue;
-    const int kl = (rule == GPUMLIB_NMF_KL);
+    const int kl = rule_is_kl(rule);
     hipError_t ze = launch_zero_i32((int*)((char*)workspace + pl.off_cnt), 2, stream);
     if (ze != hipSuccess) return ze;
     if (precision == GPUMLIB_NMF_FP8) {

@@ -29,16 +29,18 @@ hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
 // Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
 // G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a].
+// add != 0 (every Euclidean / KL update below): the additive-update rule of
+// NMF_AdditiveEuclidian / NMF_AdditiveDivergence in place of the multiplicative one.
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add = 0);
 // Euclidean W:  W(i,a) *= VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)
 // G: the reduced gram H H^T written by the W pass, HHt(a,b) = G[b*rpad + a].
 hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
-                               float eps, hipStream_t st);
+                               float eps, hipStream_t st, int add = 0);
 // Euclidean W from partials laid out P[s][i*ldr + a] (gram rows at i >= n), as the
 // BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st);
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add = 0);
 // FP8 storage path: the same updates, also writing the E4M3 shadow of the new factor
 // (one E8M0 scale per 32 rows along the contraction).
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
@@ -49,10 +51,10 @@ hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, in
 hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 // KL H:  H(a,u) *= num(a,u) / max(sum_s Psum[s][a], eps)
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr,
-                              int S, long pstride, float eps, hipStream_t st);
+                              int S, long pstride, float eps, hipStream_t st, int add = 0);
 // KL W:  W(i,a) *= num(i,a) / max(sum_s Psum[s][a], eps)
 hipError_t launch_update_w_kl(float* W, const float* P, const float* Psum, int n, int r, int ldp,
-                              int S, long pstride, float eps, hipStream_t st);
+                              int S, long pstride, float eps, hipStream_t st, int add = 0);
 // Deterministic sum of count doubles into out[0].
 hipError_t launch_reduce_f64(const double* in, int count, double* out, hipStream_t st);
 
@@ -67,14 +69,15 @@ hipError_t launch_group_reduce_obj(const JobArgs* jobs, const int* table, int sl
 // Reference-shaped kernels over materialised operands.
 hipError_t launch_mu_update(const float* nm, const float* dm, float* mm, long elements, float eps,
                             hipStream_t st);
+hipError_t launch_add_update(float* X, const float* d1, const float* d2, long elements, hipStream_t st);
 hipError_t launch_sum_w(const float* W, int n, int r, float eps, float* out, hipStream_t st);
 hipError_t launch_sum_h(const float* H, int r, int m, float eps, float* out, hipStream_t st);
 hipError_t launch_quality(const float* V, const float* WH, long count, float eps, double* partials,
                           int npartials, double* out, hipStream_t st);
 hipError_t launch_kl_update_h_wh(float* H, const float* W, const float* V, const float* WH,
-                                 const float* sumW, int n, int m, int r, float eps, hipStream_t st);
+                                 const float* sumW, int n, int m, int r, float eps, hipStream_t st, int add = 0);
 hipError_t launch_kl_update_w_wh(float* W, const float* H, const float* V, const float* WH,
-                                 const float* sumH, int n, int m, int r, float eps, hipStream_t st);
+                                 const float* sumH, int n, int m, int r, float eps, hipStream_t st, int add = 0);
 
 // General FP32 GEMM, column-major, cuBLAS-style operation flags (nmf_gemm.hip).
 hipError_t launch_sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,

@@ -1,11 +1,14 @@
-// C-ABI handle over the C++ class API (NMF_MultiplicativeEuclidianDistance /
-// NMF_MultiplicativeDivergence), for callers that bind C rather than C++.
+// C-ABI handle over the C++ class API (NMF_MultiplicativeEuclidianDistance,
+// NMF_MultiplicativeDivergence, NMF_AdditiveEuclidian, NMF_AdditiveDivergence), for
+// callers that bind C rather than C++.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
 #include <new>
 
 #include "../../include/gpumlib_nmf.h"
+#include "../src/algorithms/nmf/nmf_additive_divergence.h"
+#include "../src/algorithms/nmf/nmf_additive_euclidian.h"
 #include "../src/algorithms/nmf/nmf_multiplicative_divergence.h"
 #include "../src/algorithms/nmf/nmf_multiplicative_euclidian.h"
 
@@ -33,7 +36,9 @@ extern "C" {
 hipError_t gpumlib_nmf_object_create(int rule, int precision, int n, int m, int r, const float* host_V,
                                      const float* host_W, const float* host_H, gpumlib_nmf_handle* out) {
     if (!out || !host_V || !host_W || !host_H || n < 1 || m < 1 || r < 1) return hipErrorInvalidValue;
-    if (gpumlib_nmf_workspace_bytes(rule, precision, n, m, r) == 0) return hipErrorInvalidValue;
+    // the C++ classes run the reference's FP32 arithmetic
+    if (precision != GPUMLIB_NMF_FP32 || gpumlib_nmf_workspace_bytes(rule, precision, n, m, r) == 0)
+        return hipErrorInvalidValue;
     *out = nullptr;
     HostMatrix<cudafloat> v = host_matrix(host_V, n, m);
     HostMatrix<cudafloat> w = host_matrix(host_W, n, r);
@@ -44,10 +49,16 @@ hipError_t gpumlib_nmf_object_create(int rule, int precision, int n, int m, int 
     obj->n = n;
     obj->m = m;
     obj->r = r;
-    if (rule == GPUMLIB_NMF_EUCLIDEAN)
-        obj->algo = new (std::nothrow) NMF_MultiplicativeEuclidianDistance(v, w, h);
-    else
-        obj->algo = new (std::nothrow) NMF_MultiplicativeDivergence(v, w, h);
+    switch (rule) {
+        case GPUMLIB_NMF_EUCLIDEAN: obj->algo = new (std::nothrow) NMF_MultiplicativeEuclidianDistance(v, w, h); break;
+        case GPUMLIB_NMF_KL: obj->algo = new (std::nothrow) NMF_MultiplicativeDivergence(v, w, h); break;
+        case GPUMLIB_NMF_ADDITIVE_KL: obj->algo = new (std::nothrow) NMF_AdditiveDivergence(v, w, h); break;
+        default: {
+            NMF_AdditiveEuclidian* a = new (std::nothrow) NMF_AdditiveEuclidian(v, w, h);
+            if (a) a->SetUpdateH(rule == GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H);
+            obj->algo = a;
+        }
+    }
     if (!obj->algo) {
         delete obj;
         return hipErrorOutOfMemory;

@@ -15,6 +15,11 @@ constexpr int kMaxFusedRank = 128;
 // Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
 constexpr int kTargetWorkgroups = 512;
 
+// Update rules (GPUMLIB_NMF_* rule constants of the C-ABI): which passes a rule runs
+// (the KL ones or the Euclidean ones) and whether its update is the additive form.
+inline bool rule_is_kl(int rule) { return rule == 1 /* KL */ || rule == 3 /* ADDITIVE_KL */; }
+inline bool rule_is_additive(int rule) { return rule >= 2; }
+
 struct Plan {
     int rule, precision, n, m, r;
     int npad, rpad, ldr, ldp;

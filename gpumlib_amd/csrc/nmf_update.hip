@@ -25,6 +25,20 @@ __device__ __forceinline__ float sum_slices(const float* src, int S, long pstrid
     return v;
 }
 
+// Additive Euclidean element rule (nmf_kernels.cu:31-39): x + (x / d2) * (d1 - d2),
+// set to 0 when negative (a NaN is kept: `v < 0` is false for it).  No eps.
+__device__ __forceinline__ float additive_rule(float x, float d1, float d2) {
+    const float v = x + (x / d2) * (d1 - d2);
+    return v < 0.f ? 0.f : v;
+}
+
+// Additive KL rule (nmf_kernels_additive_divergence.cu:109-121, 178-190):
+// x + (x / sum) * t, t = sum_k F(k) (ratio(k) - 1), set to 0 when negative.
+__device__ __forceinline__ float additive_kl_rule(float x, float sum, float t) {
+    const float v = x + (x / sum) * t;
+    return v < 0.f ? 0.f : v;
+}
+
 // Euclidean H update, reference nmf_multiplicative_euclidian.cpp:36-45 and
 // nmf_kernels.cu:46:  H(a,u) *= WtV(a,u) / (WtWH(a,u) + eps), WtWH = WtW H.
 // WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = G[b*ldr + a] (reduced by the pass).
@@ -192,7 +206,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
                                                               int ldg, int gt, int count, int r, int ldr, int S,
                                                               long pstride, float eps, uint16_t* __restrict__ Sh,
                                                               int ldsh, uint8_t* __restrict__ Q, int ldq,
-                                                              uint8_t* __restrict__ Qsc, int nblk) {
+                                                              uint8_t* __restrict__ Qsc, int nblk, int add) {
     extern __shared__ __attribute__((aligned(16))) float smr[];
     constexpr int RS = RP + 4, NQ = RP / 4, ITEMS = JT * NQ, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
     float* sG = smr;            // sG[b*RS + a] = C(b,a), zero for a or b >= r
@@ -264,10 +278,17 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         }
         const float4 xo = *reinterpret_cast<const float4*>(xr + 4 * q);
         float4 o;
-        o.x = xo.x * (num[k].x / (d.x + eps));
-        o.y = xo.y * (num[k].y / (d.y + eps));
-        o.z = xo.z * (num[k].z / (d.z + eps));
-        o.w = xo.w * (num[k].w / (d.w + eps));
+        if (add) {
+            o.x = additive_rule(xo.x, num[k].x, d.x);
+            o.y = additive_rule(xo.y, num[k].y, d.y);
+            o.z = additive_rule(xo.z, num[k].z, d.z);
+            o.w = additive_rule(xo.w, num[k].w, d.w);
+        } else {
+            o.x = xo.x * (num[k].x / (d.x + eps));
+            o.y = xo.y * (num[k].y / (d.y + eps));
+            o.z = xo.z * (num[k].z / (d.z + eps));
+            o.w = xo.w * (num[k].w / (d.w + eps));
+        }
         *reinterpret_cast<float4*>(sN + jj * RS + 4 * q) = o;
     }
     __syncthreads();
@@ -328,7 +349,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
 template <int JT, int RP>
 __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restrict__ W, const float* __restrict__ P,
                                                                 const float* __restrict__ G, int n, int r, int ldp,
-                                                                int S, long pstride, float eps) {
+                                                                int S, long pstride, float eps, int add) {
     extern __shared__ __attribute__((aligned(16))) float smc[];
     constexpr int RS = RP + 4, JS = JT + 4, NI4 = JT / 4, ITEMS = NI4 * RP, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
     float* sG = smc;             // sG[b*RS + a] = HHt(b,a), zero for a or b >= r
@@ -383,10 +404,17 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restric
             d.x += w4.x * g, d.y += w4.y * g, d.z += w4.z * g, d.w += w4.w * g;
         }
         const float4 wo = *reinterpret_cast<const float4*>(sWt + a * JS + 4 * i4);
-        out[k].x = wo.x * (num[k].x / (d.x + eps));
-        out[k].y = wo.y * (num[k].y / (d.y + eps));
-        out[k].z = wo.z * (num[k].z / (d.z + eps));
-        out[k].w = wo.w * (num[k].w / (d.w + eps));
+        if (add) {
+            out[k].x = additive_rule(wo.x, num[k].x, d.x);
+            out[k].y = additive_rule(wo.y, num[k].y, d.y);
+            out[k].z = additive_rule(wo.z, num[k].z, d.z);
+            out[k].w = additive_rule(wo.w, num[k].w, d.w);
+        } else {
+            out[k].x = wo.x * (num[k].x / (d.x + eps));
+            out[k].y = wo.y * (num[k].y / (d.y + eps));
+            out[k].z = wo.z * (num[k].z / (d.z + eps));
+            out[k].w = wo.w * (num[k].w / (d.w + eps));
+        }
     }
 #pragma unroll
     for (int k = 0; k < MAXI; k++) {
@@ -402,15 +430,19 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restric
 }
 
 // KL H update, reference nmf_kernels.cu:337-339 with the clamped sums of :104.
+// add != 0: the additive rule of nmf_kernels_additive_divergence.cu:178-190,
+//   H <- max(0, H + (H / sum) * (num - rawsum)),  num - rawsum = sum_i W(i,a) (ratio - 1).
+// ssum holds [clamped sums | raw sums] (2r floats).
 __device__ __forceinline__ void update_h_kl_body(float* __restrict__ H, const float* __restrict__ P,
                                                  const float* __restrict__ Psum, int m, int r, int ldr, int S,
-                                                 long pstride, float eps, int blk, float* ssum) {
+                                                 long pstride, float eps, int blk, float* ssum, int add) {
     const int tid = threadIdx.x;
     const int rpad = round_up(r, 4);
     for (int a = tid; a < r; a += NTHREADS) {
         float v = 0.f;
         for (int s = 0; s < S; s++) v += Psum[(size_t)s * rpad + a];
         ssum[a] = (v < eps) ? eps : v;
+        ssum[r + a] = v;
     }
     __syncthreads();
     const int u0 = blk * UPD_COLS;
@@ -419,27 +451,33 @@ __device__ __forceinline__ void update_h_kl_body(float* __restrict__ H, const fl
         const int uu = idx / r, a = idx - uu * r;
         const int u = u0 + uu;
         const float num = sum_slices(P + (size_t)u * ldr + a, S, pstride);
-        H[(size_t)u * r + a] *= (num / ssum[a]);
+        float* h = H + (size_t)u * r + a;
+        if (add)
+            *h = additive_kl_rule(*h, ssum[a], num - ssum[r + a]);
+        else
+            *h *= (num / ssum[a]);
     }
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_update_h_kl(float* __restrict__ H, const float* __restrict__ P,
                                                           const float* __restrict__ Psum, int m, int r, int ldr,
-                                                          int S, long pstride, float eps) {
-    __shared__ float ssum[512];
-    update_h_kl_body(H, P, Psum, m, r, ldr, S, pstride, eps, blockIdx.x, ssum);
+                                                          int S, long pstride, float eps, int add) {
+    __shared__ float ssum[1024];
+    update_h_kl_body(H, P, Psum, m, r, ldr, S, pstride, eps, blockIdx.x, ssum, add);
 }
 
 // KL W update, reference nmf_kernels.cu:289-291 with the clamped sums of :190.
+// add != 0: the additive rule of nmf_kernels_additive_divergence.cu:109-121.
 __device__ __forceinline__ void update_w_kl_body(float* __restrict__ W, const float* __restrict__ P,
                                                  const float* __restrict__ Psum, int n, int r, int ldp, int S,
-                                                 long pstride, float eps, int blk, float* ssum) {
+                                                 long pstride, float eps, int blk, float* ssum, int add) {
     const int tid = threadIdx.x;
     const int rpad = round_up(r, 4);
     for (int a = tid; a < r; a += NTHREADS) {
         float v = 0.f;
         for (int s = 0; s < S; s++) v += Psum[(size_t)s * rpad + a];
         ssum[a] = (v < eps) ? eps : v;
+        ssum[r + a] = v;
     }
     __syncthreads();
     const int i0 = blk * UPD_ROWS;
@@ -449,15 +487,19 @@ __device__ __forceinline__ void update_w_kl_body(float* __restrict__ W, const fl
         if (ii >= nrow) continue;
         const int i = i0 + ii;
         const float num = sum_slices(P + (size_t)a * ldp + i, S, pstride);
-        W[(size_t)a * n + i] *= (num / ssum[a]);
+        float* w = W + (size_t)a * n + i;
+        if (add)
+            *w = additive_kl_rule(*w, ssum[a], num - ssum[r + a]);
+        else
+            *w *= (num / ssum[a]);
     }
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_update_w_kl(float* __restrict__ W, const float* __restrict__ P,
                                                           const float* __restrict__ Psum, int n, int r, int ldp,
-                                                          int S, long pstride, float eps) {
-    __shared__ float ssum[512];
-    update_w_kl_body(W, P, Psum, n, r, ldp, S, pstride, eps, blockIdx.x, ssum);
+                                                          int S, long pstride, float eps, int add) {
+    __shared__ float ssum[1024];
+    update_w_kl_body(W, P, Psum, n, r, ldp, S, pstride, eps, blockIdx.x, ssum, add);
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -491,6 +533,13 @@ __global__ void k_mu_update(const float* __restrict__ nm, const float* __restric
                             long elements, float eps) {
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < elements; k += (long)gridDim.x * blockDim.x)
         mm[k] *= nm[k] / (dm[k] + eps);
+}
+
+// nmf_kernels.cu:31-39
+__global__ void k_add_update(float* __restrict__ X, const float* __restrict__ d1, const float* __restrict__ d2,
+                             long elements) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < elements; k += (long)gridDim.x * blockDim.x)
+        X[k] = additive_rule(X[k], d1[k], d2[k]);
 }
 
 // Column sums of W (one workgroup per column a), clamped at eps: nmf_kernels.cu:63-109.
@@ -553,7 +602,7 @@ __global__ __launch_bounds__(NTHREADS) void k_quality(const float* __restrict__ 
 __global__ __launch_bounds__(NTHREADS) void k_kl_update_h_wh(float* __restrict__ H, const float* __restrict__ W,
                                                              const float* __restrict__ V, const float* __restrict__ WH,
                                                              const float* __restrict__ sumW, int n, int m, int r,
-                                                             float eps) {
+                                                             float eps, int add) {
     __shared__ float sR[NTHREADS];
     const int u = blockIdx.x, tid = threadIdx.x;
     for (int a0 = 0; a0 < r; a0 += NTHREADS) {
@@ -561,7 +610,8 @@ __global__ __launch_bounds__(NTHREADS) void k_kl_update_h_wh(float* __restrict__
         float acc = 0.f;
         for (int i0 = 0; i0 < n; i0 += NTHREADS) {
             const int i = i0 + tid;
-            sR[tid] = (i < n) ? V[(size_t)u * n + i] / (WH[(size_t)u * n + i] + eps) : 0.f;
+            // additive form (nmf_kernels_additive_divergence.cu:160-169): the term is ratio - 1
+            sR[tid] = (i < n) ? V[(size_t)u * n + i] / (WH[(size_t)u * n + i] + eps) - (add ? 1.f : 0.f) : 0.f;
             __syncthreads();
             if (a < r) {
                 const int lim = min(NTHREADS, n - i0);
@@ -569,7 +619,13 @@ __global__ __launch_bounds__(NTHREADS) void k_kl_update_h_wh(float* __restrict__
             }
             __syncthreads();
         }
-        if (a < r) H[(size_t)u * r + a] *= (acc / sumW[a]);
+        if (a < r) {
+            float* h = H + (size_t)u * r + a;
+            if (add)
+                *h = additive_kl_rule(*h, sumW[a], acc);
+            else
+                *h *= (acc / sumW[a]);
+        }
     }
 }
 
@@ -578,16 +634,20 @@ __global__ __launch_bounds__(NTHREADS) void k_kl_update_h_wh(float* __restrict__
 __global__ __launch_bounds__(NTHREADS) void k_kl_update_w_wh(float* __restrict__ W, const float* __restrict__ H,
                                                              const float* __restrict__ V, const float* __restrict__ WH,
                                                              const float* __restrict__ sumH, int n, int m, int r,
-                                                             float eps) {
+                                                             float eps, int add) {
     const int i = blockIdx.x * NTHREADS + threadIdx.x;
     if (i >= n) return;
     for (int a = 0; a < r; a++) {
         float acc = 0.f;
         for (int u = 0; u < m; u++) {
             const size_t k = (size_t)u * n + i;
-            acc += H[(size_t)u * r + a] * (V[k] / (WH[k] + eps));
+            acc += H[(size_t)u * r + a] * (V[k] / (WH[k] + eps) - (add ? 1.f : 0.f));
         }
-        W[(size_t)a * n + i] *= (acc / sumH[a]);
+        float* w = W + (size_t)a * n + i;
+        if (add)
+            *w = additive_kl_rule(*w, sumH[a], acc);
+        else
+            *w *= (acc / sumH[a]);
     }
 }
 
@@ -617,7 +677,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_update_h(const JobArgs* __re
         if (ja.rule == 0)
             update_h_euc_body(ja.H, p.P, p.G, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, nullptr, 0, item - first[j], sm);
         else
-            update_h_kl_body(ja.H, p.P, p.Psum, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm);
+            update_h_kl_body(ja.H, p.P, p.Psum, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm, 0);
     }
 }
 
@@ -634,7 +694,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_update_w(const JobArgs* __re
         if (ja.rule == 0)
             update_w_euc_body(ja.W, p.P, p.G, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm);
         else
-            update_w_kl_body(ja.W, p.P, p.Psum, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm);
+            update_w_kl_body(ja.W, p.P, p.Psum, p.n, p.r, p.ldp, p.S, p.pstride, p.eps, item - first[j], sm, 0);
     }
 }
 
@@ -688,24 +748,24 @@ static hipError_t set_smem(K kernel, size_t sm) {
 template <int JT, int RP>
 static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                                 int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                                uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, hipStream_t st) {
+                                uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, hipStream_t st) {
     const size_t sm = (size_t)(RP + 2 * JT) * (RP + 4) * sizeof(float);
     hipError_t e = set_smem(k_update_rows_euc<JT, RP>, sm);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_update_rows_euc<JT, RP>), dim3((count + JT - 1) / JT), dim3(NTHREADS), sm, st, X, xs_j, xs_a,
-                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk);
+                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk, add);
     return hipGetLastError();
 }
 
 // Q (FP8 shadow) needs 32 rows per workgroup (one scale block); it is given for r <= 64.
 static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                               int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                              uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, hipStream_t st) {
+                              uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, hipStream_t st) {
     if (r < 1 || r > 128 || (Q && (r > 64 || ldq % 32 || ldq < count))) return hipErrorInvalidValue;
     const bool wide = count >= 16384 || Q != nullptr;
 #define LR(JT, RP)                                                                                                \
     return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, \
-                                 Qsc, nblk, st)
+                                 Qsc, nblk, add, st)
     if (r <= 32) { if (wide) LR(32, 32); LR(16, 32); }
     if (r <= 64) { if (wide) LR(32, 64); LR(16, 64); }
     LR(16, 128);
@@ -713,60 +773,60 @@ static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, co
 }
 
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st) {
-    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, st);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add) {
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, add, st);
 }
 
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st) {
-    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, st);
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add) {
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, add, st);
 }
 
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
                                   long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st) {
-    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, nullptr, 0, Hq, ldq, Hsc, ldq / 32, st);
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, nullptr, 0, Hq, ldq, Hsc, ldq / 32, 0, st);
 }
 
 hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, int n, int r, int ldr, int S,
                                     long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st) {
-    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, nullptr, 0, Wq, ldq, Wsc, ldq / 32, st);
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, nullptr, 0, Wq, ldq, Wsc, ldq / 32, 0, st);
 }
 
 template <int JT, int RP>
 static hipError_t launch_cols_t(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
-                                float eps, hipStream_t st) {
+                                float eps, int add, hipStream_t st) {
     const size_t sm = (size_t)(RP * (RP + 4) + RP * (JT + 4)) * sizeof(float);
     hipError_t e = set_smem(k_update_w_euc_cols<JT, RP>, sm);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_update_w_euc_cols<JT, RP>), dim3((n + JT - 1) / JT), dim3(NTHREADS), sm, st, W, P, G, n, r,
-                       ldp, S, pstride, eps);
+                       ldp, S, pstride, eps, add);
     return hipGetLastError();
 }
 
 hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
-                               float eps, hipStream_t st) {
+                               float eps, hipStream_t st, int add) {
     if (r < 1 || r > 128) return hipErrorInvalidValue;
     const bool wide = n >= 16384;
-    if (r <= 32) return wide ? launch_cols_t<32, 32>(W, P, G, n, r, ldp, S, pstride, eps, st)
-                             : launch_cols_t<16, 32>(W, P, G, n, r, ldp, S, pstride, eps, st);
-    if (r <= 64) return wide ? launch_cols_t<32, 64>(W, P, G, n, r, ldp, S, pstride, eps, st)
-                             : launch_cols_t<16, 64>(W, P, G, n, r, ldp, S, pstride, eps, st);
-    return launch_cols_t<16, 128>(W, P, G, n, r, ldp, S, pstride, eps, st);
+    if (r <= 32) return wide ? launch_cols_t<32, 32>(W, P, G, n, r, ldp, S, pstride, eps, add, st)
+                             : launch_cols_t<16, 32>(W, P, G, n, r, ldp, S, pstride, eps, add, st);
+    if (r <= 64) return wide ? launch_cols_t<32, 64>(W, P, G, n, r, ldp, S, pstride, eps, add, st)
+                             : launch_cols_t<16, 64>(W, P, G, n, r, ldp, S, pstride, eps, add, st);
+    return launch_cols_t<16, 128>(W, P, G, n, r, ldp, S, pstride, eps, add, st);
 }
 
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr, int S, long pstride,
-                              float eps, hipStream_t st) {
+                              float eps, hipStream_t st, int add) {
     if (r > 512) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_update_h_kl, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), 0, st, H, P, Psum, m, r,
-                       ldr, S, pstride, eps);
+                       ldr, S, pstride, eps, add);
     return hipGetLastError();
 }
 
 hipError_t launch_update_w_kl(float* W, const float* P, const float* Psum, int n, int r, int ldp, int S, long pstride,
-                              float eps, hipStream_t st) {
+                              float eps, hipStream_t st, int add) {
     if (r > 512) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_update_w_kl, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), 0, st, W, P, Psum, n, r,
-                       ldp, S, pstride, eps);
+                       ldp, S, pstride, eps, add);
     return hipGetLastError();
 }
 
@@ -780,6 +840,14 @@ hipError_t launch_mu_update(const float* nm, const float* dm, float* mm, long el
     long blocks = (elements + NTHREADS - 1) / NTHREADS;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_mu_update, dim3((unsigned)blocks), dim3(NTHREADS), 0, st, nm, dm, mm, elements, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_update(float* X, const float* d1, const float* d2, long elements, hipStream_t st) {
+    if (elements <= 0) return hipSuccess;
+    long blocks = (elements + NTHREADS - 1) / NTHREADS;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_add_update, dim3((unsigned)blocks), dim3(NTHREADS), 0, st, X, d1, d2, elements);
     return hipGetLastError();
 }
 
@@ -805,17 +873,17 @@ hipError_t launch_quality(const float* V, const float* WH, long count, float eps
 }
 
 hipError_t launch_kl_update_h_wh(float* H, const float* W, const float* V, const float* WH, const float* sumW, int n,
-                                 int m, int r, float eps, hipStream_t st) {
+                                 int m, int r, float eps, hipStream_t st, int add) {
     if (m <= 0 || r <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kl_update_h_wh, dim3(m), dim3(NTHREADS), 0, st, H, W, V, WH, sumW, n, m, r, eps);
+    hipLaunchKernelGGL(k_kl_update_h_wh, dim3(m), dim3(NTHREADS), 0, st, H, W, V, WH, sumW, n, m, r, eps, add);
     return hipGetLastError();
 }
 
 hipError_t launch_kl_update_w_wh(float* W, const float* H, const float* V, const float* WH, const float* sumH, int n,
-                                 int m, int r, float eps, hipStream_t st) {
+                                 int m, int r, float eps, hipStream_t st, int add) {
     if (n <= 0 || r <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_kl_update_w_wh, dim3((n + NTHREADS - 1) / NTHREADS), dim3(NTHREADS), 0, st, W, H, V, WH, sumH,
-                       n, m, r, eps);
+                       n, m, r, eps, add);
     return hipGetLastError();
 }
 

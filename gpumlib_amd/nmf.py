@@ -11,11 +11,12 @@ import ctypes
 import numpy as np
 
 from . import capi
-from .capi import BF16, EUCLIDEAN, FP8, FP32, KL, OBJ_FROBENIUS, OBJ_KL, OBJ_NONE, OBJ_QUALITY, REFERENCE_EPS
+from .capi import (ADDITIVE_EUCLIDEAN, ADDITIVE_EUCLIDEAN_WITH_H, ADDITIVE_KL, BF16, EUCLIDEAN, FP8, FP32, KL,
+                   OBJ_FROBENIUS, OBJ_KL, OBJ_NONE, OBJ_QUALITY, REFERENCE_EPS)
 
 __all__ = ["colmajor", "as_matrix", "gen_euclidean", "gen_poisson", "gen_init", "DeviceNMF",
-           "EUCLIDEAN", "KL", "FP32", "BF16", "FP8", "OBJ_NONE", "OBJ_FROBENIUS", "OBJ_KL", "OBJ_QUALITY",
-           "REFERENCE_EPS"]
+           "EUCLIDEAN", "KL", "ADDITIVE_EUCLIDEAN", "ADDITIVE_KL", "ADDITIVE_EUCLIDEAN_WITH_H",
+           "FP32", "BF16", "FP8", "OBJ_NONE", "OBJ_FROBENIUS", "OBJ_KL", "OBJ_QUALITY", "REFERENCE_EPS"]
 
 
 def colmajor(x):

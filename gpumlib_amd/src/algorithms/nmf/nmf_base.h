@@ -1,4 +1,4 @@
-// Base class of the multiplicative-update NMF algorithms.  Drop-in for
+// Base class of the NMF algorithms (multiplicative and additive updates).  Drop-in for
 // GPUMLib::NMF (reference src/algorithms/nmf/nmf_base.h:45-151).
 //
 // Given a non-negative V (n x m, column-major), find W (n x r) and H (r x m) with
@@ -118,7 +118,10 @@ class NMF {
         Setup(ruleKind);
     }
 
-    // One multiplicative-update iteration (H, then W unless updateW is false).
+    // Switches between update rules that run the same passes (hence the same workspace).
+    void SetRule(int ruleKind) { rule = ruleKind; }
+
+    // One iteration of the object's update rule (H, then W unless updateW is false).
     void Iterate(bool updateW) {
         const bool qualityPoint = qualityInterval > 0 && (iterationsDone % qualityInterval) == 0;
         if (qualityPoint) ConsumeQuality();

@@ -44,6 +44,19 @@ hipError_t gpumlib_cuda_nmf_multiplicative_divergence_update_matrix_h(dim3 gridD
                                                                       float *V, float *WH, float *sumW, int n, int m,
                                                                       int r);
 
+/* replaces gpumlib_cuda_nmf_additive_euclidean_update_matrix, nmf_kernels.h:116 (kernel nmf_kernels.cu:31-39):
+ *   X[k] = X[k] + (X[k] / deltaX2[k]) * (deltaX1[k] - deltaX2[k]), 0 if negative (no eps) */
+hipError_t gpumlib_cuda_nmf_additive_euclidean_update_matrix(unsigned int gridDim, unsigned int blockDim, float *X,
+                                                             float *deltaX1, float *deltaX2, int elements);
+/* replaces gpumlib_cuda_nmf_additive_divergence_update_matrix_h, nmf_kernels.h:129
+ * (kernel nmf_kernels_additive_divergence.cu:126-193) */
+hipError_t gpumlib_cuda_nmf_additive_divergence_update_matrix_h(dim3 gridDim, dim3 blockDim, float *H, float *W,
+                                                                float *V, float *WH, float *sumW, int n, int m, int r);
+/* replaces gpumlib_cuda_nmf_additive_divergence_update_matrix_w, nmf_kernels.h:142
+ * (kernel nmf_kernels_additive_divergence.cu:57-124) */
+hipError_t gpumlib_cuda_nmf_additive_divergence_update_matrix_w(dim3 gridDim, dim3 blockDim, float *W, float *H,
+                                                                float *V, float *WH, float *sumH, int n, int m, int r);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Kernel-level entry points with an explicit stream and eps.              */
 /* ------------------------------------------------------------------------ */
@@ -66,6 +79,18 @@ hipError_t gpumlib_nmf_kl_update_h(float *H, const float *W, const float *V, con
 hipError_t gpumlib_nmf_kl_update_w(float *W, const float *H, const float *V, const float *WH, const float *sumH,
                                    int n, int m, int r, float eps, hipStream_t stream);
 
+/* Additive-update element rule (kernel of nmf_kernels.cu:31-39), explicit stream:
+ *   X[k] = X[k] + (X[k] / d2[k]) * (d1[k] - d2[k]), set to 0 when negative. */
+hipError_t gpumlib_nmf_additive_update(float *X, const float *d1, const float *d2, long elements, hipStream_t stream);
+
+/* Additive KL updates over a materialised WH (nmf_kernels_additive_divergence.cu):
+ *   H(a,u) = max(0, H + (H / sumW[a]) sum_i W(i,a) (V(i,u) / (WH(i,u) + eps) - 1))   :126-193
+ *   W(i,a) = max(0, W + (W / sumH[a]) sum_u H(a,u) (V(i,u) / (WH(i,u) + eps) - 1))   :57-124 */
+hipError_t gpumlib_nmf_additive_kl_update_h(float *H, const float *W, const float *V, const float *WH,
+                                            const float *sumW, int n, int m, int r, float eps, hipStream_t stream);
+hipError_t gpumlib_nmf_additive_kl_update_w(float *W, const float *H, const float *V, const float *WH,
+                                            const float *sumH, int n, int m, int r, float eps, hipStream_t stream);
+
 /* Reference quality  sum_k V[k] log10(WH[k] + eps) - WH[k]  accumulated in FP64 into d_quality[0].
  * Replaces KernelNMFquality (nmf_kernels.h:44; nmf_kernels_quality.cu:29-109), which reduced in
  * FP32 on one workgroup. */
@@ -87,7 +112,23 @@ hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, floa
 /*    (H from the old W, then W from the new H; KL recomputes W H between).   */
 /* ------------------------------------------------------------------------ */
 
-enum { GPUMLIB_NMF_EUCLIDEAN = 0, GPUMLIB_NMF_KL = 1 };
+/* Update rules.  The additive ones follow NMF_AdditiveEuclidian::DoIteration
+ * (nmf_additive_euclidian.cpp:31-56) and NMF_AdditiveDivergence::DoIteration
+ * (nmf_additive_divergence.cpp:30-51); they run in FP32 (precision GPUMLIB_NMF_FP32)
+ * through gpumlib_nmf_iterate / the object API, not through the batch or the server.
+ *   GPUMLIB_NMF_ADDITIVE_EUCLIDEAN: as the reference, H is left unchanged (its H step
+ *     hands H to the multiplicative launcher as the numerator, :42, so only scratch is
+ *     written); W <- max(0, W + (W / WHHt) (VHt - WHHt)).
+ *   GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H: also H <- max(0, H + (H / WtWH) (WtV - WtWH))
+ *     (no reference counterpart: the H step the reference's scratch buffers were for).
+ *   GPUMLIB_NMF_ADDITIVE_KL: H and W by the additive KL kernels, sums clamped at eps. */
+enum {
+    GPUMLIB_NMF_EUCLIDEAN = 0,
+    GPUMLIB_NMF_KL = 1,
+    GPUMLIB_NMF_ADDITIVE_EUCLIDEAN = 2,
+    GPUMLIB_NMF_ADDITIVE_KL = 3,
+    GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H = 4
+};
 enum { GPUMLIB_NMF_FP32 = 0, GPUMLIB_NMF_BF16 = 1, GPUMLIB_NMF_FP8 = 2 };
 enum {
     GPUMLIB_NMF_OBJ_NONE = 0,
@@ -135,10 +176,13 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
                             const float *H, float eps, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
-/* 4. Handle over the C++ class API (NMF_MultiplicativeEuclidianDistance /    */
-/*    NMF_MultiplicativeDivergence, constructed from host V, W, H as in the   */
-/*    (v, w, h) constructors of nmf_multiplicative_euclidian.h:62 and         */
-/*    nmf_multiplicative_divergence.h:57).  For foreign-language callers.     */
+/* 4. Handle over the C++ class API (NMF_MultiplicativeEuclidianDistance,     */
+/*    NMF_MultiplicativeDivergence, NMF_AdditiveEuclidian,                    */
+/*    NMF_AdditiveDivergence; constructed from host V, W, H as in the         */
+/*    (v, w, h) constructors of nmf_multiplicative_euclidian.h:62,            */
+/*    nmf_multiplicative_divergence.h:57, nmf_additive_euclidian.h:57 and     */
+/*    nmf_additive_divergence.h:56).  rule: any GPUMLIB_NMF_* rule; precision */
+/*    must be GPUMLIB_NMF_FP32.  For foreign-language callers.                */
 /* ------------------------------------------------------------------------ */
 typedef struct gpumlib_nmf_object *gpumlib_nmf_handle;
 

@@ -230,6 +230,128 @@ void orc_kl_iter_f32(int n, int m, int r, const float *V, float *W, float *H, fl
 }
 
 /* ------------------------------------------------------------------------ */
+/* Additive-update variants.                                                  */
+/*                                                                            */
+/* Element rule of NMF_AdditiveEuclidian (src/cuda/nmf/nmf_kernels.cu:31-39):  */
+/*   X[k] = X[k] + (X[k] / d2[k]) * (d1[k] - d2[k]);  X[k] = 0 if negative     */
+/* (no eps; a NaN stays NaN, as `v < 0` is false for it).                     */
+/* ------------------------------------------------------------------------ */
+#define ADD_UPDATE_BODY(T)                                                    \
+    for (long k = 0; k < elements; k++) {                                     \
+        T v = X[k] + (X[k] / d2[k]) * (d1[k] - d2[k]);                         \
+        if (v < 0) v = 0;                                                     \
+        X[k] = v;                                                             \
+    }
+void orc_add_update_f64(const double *d1, const double *d2, double *X, long elements) { ADD_UPDATE_BODY(double) }
+void orc_add_update_f32(const float *d1, const float *d2, float *X, long elements) { ADD_UPDATE_BODY(float) }
+
+/* Additive KL kernels over a materialised WH                                 */
+/* (src/cuda/nmf/nmf_kernels_additive_divergence.cu):                         */
+/*  H(a,u) = H + (H / sumW[a]) * sum_i W(i,a) * (V(i,u)/(WH(i,u)+eps) - 1)    */
+/*      :126-193 (term :169, update and clamp :178-190)                       */
+/*  W(i,a) = W + (W / sumH[a]) * sum_u H(a,u) * (V(i,u)/(WH(i,u)+eps) - 1)    */
+/*      :57-124 (term :100, update and clamp :109-121)                        */
+/* each clamped at 0.  sumW / sumH are the clamped sums of SumW / SumH.       */
+#define ADD_KL_H_BODY(T)                                                      \
+    for (int u = 0; u < m; u++) {                                             \
+        for (int a = 0; a < r; a++) {                                         \
+            T s = 0;                                                          \
+            for (int i = 0; i < n; i++) {                                     \
+                size_t k = (size_t)u * n + i;                                 \
+                s += W[(size_t)a * n + i] * (V[k] / (WH[k] + eps) - 1);       \
+            }                                                                 \
+            T h = H[(size_t)u * r + a];                                       \
+            T v = h + (h / sumW[a]) * s;                                      \
+            if (v < 0) v = 0;                                                 \
+            H[(size_t)u * r + a] = v;                                         \
+        }                                                                     \
+    }
+#define ADD_KL_W_BODY(T)                                                      \
+    for (int a = 0; a < r; a++) {                                             \
+        for (int i = 0; i < n; i++) {                                         \
+            T s = 0;                                                          \
+            for (int u = 0; u < m; u++) {                                     \
+                size_t k = (size_t)u * n + i;                                 \
+                s += H[(size_t)u * r + a] * (V[k] / (WH[k] + eps) - 1);       \
+            }                                                                 \
+            T w = W[(size_t)a * n + i];                                       \
+            T v = w + (w / sumH[a]) * s;                                      \
+            if (v < 0) v = 0;                                                 \
+            W[(size_t)a * n + i] = v;                                         \
+        }                                                                     \
+    }
+void orc_add_kl_update_h_f64(double *H, const double *W, const double *V, const double *WH,
+                             const double *sumW, int n, int m, int r, double eps) {
+    ADD_KL_H_BODY(double)
+}
+void orc_add_kl_update_w_f64(double *W, const double *H, const double *V, const double *WH,
+                             const double *sumH, int n, int m, int r, double eps) {
+    ADD_KL_W_BODY(double)
+}
+void orc_add_kl_update_h_f32(float *H, const float *W, const float *V, const float *WH,
+                             const float *sumW, int n, int m, int r, float eps) {
+    ADD_KL_H_BODY(float)
+}
+void orc_add_kl_update_w_f32(float *W, const float *H, const float *V, const float *WH,
+                             const float *sumH, int n, int m, int r, float eps) {
+    ADD_KL_W_BODY(float)
+}
+
+/* One NMF_AdditiveEuclidian iteration (src/algorithms/nmf/nmf_additive_euclidian.cpp:31-56):   */
+/*   H half (:35-42): deltaH = W^T V ; aux = W^T W ; deltaH2 = aux H ; then the reference calls */
+/*     the multiplicative launcher with (nm = H, dm = deltaH, m = deltaH2) (:42), which         */
+/*     rewrites the scratch deltaH2 and leaves H unchanged.  fixH != 0 applies the additive     */
+/*     element rule to H instead: H = add(H, deltaH, deltaH2).                                  */
+/*   if (!updateW) return (:44)                                                                 */
+/*   W half (:47-55): deltaW = V H^T ; aux = H H^T ; deltaW2 = W aux ; W = add(W, deltaW, deltaW2) */
+#define ADD_EUC_ITER_BODY(T, GEMM, UPD)                                          \
+    T *aux = (T *)malloc(sizeof(T) * (size_t)r * r);                             \
+    if (fixH) {                                                                  \
+        T *dH = (T *)malloc(sizeof(T) * (size_t)r * m);                          \
+        T *dH2 = (T *)malloc(sizeof(T) * (size_t)r * m);                         \
+        GEMM('T', 'N', r, m, n, 1, W, n, V, n, 0, dH, r);                        \
+        GEMM('T', 'N', r, r, n, 1, W, n, W, n, 0, aux, r);                       \
+        GEMM('N', 'N', r, m, r, 1, aux, r, H, r, 0, dH2, r);                     \
+        UPD(dH, dH2, H, (long)r * m);                                            \
+        free(dH);                                                                \
+        free(dH2);                                                               \
+    }                                                                            \
+    if (updateW) {                                                               \
+        T *dW = (T *)malloc(sizeof(T) * (size_t)n * r);                          \
+        T *dW2 = (T *)malloc(sizeof(T) * (size_t)n * r);                         \
+        GEMM('N', 'T', n, r, m, 1, V, n, H, r, 0, dW, n);                        \
+        GEMM('N', 'T', r, r, m, 1, H, r, H, r, 0, aux, r);                       \
+        GEMM('N', 'N', n, r, r, 1, W, n, aux, r, 0, dW2, n);                     \
+        UPD(dW, dW2, W, (long)n * r);                                            \
+        free(dW);                                                                \
+        free(dW2);                                                               \
+    }                                                                            \
+    free(aux);
+
+void orc_add_euc_iter_f64(int n, int m, int r, const double *V, double *W, double *H, int updateW,
+                          int fixH) {
+    ADD_EUC_ITER_BODY(double, orc_gemm_f64, orc_add_update_f64)
+}
+void orc_add_euc_iter_f32(int n, int m, int r, const float *V, float *W, float *H, int updateW,
+                          int fixH) {
+    ADD_EUC_ITER_BODY(float, orc_gemm_f32, orc_add_update_f32)
+}
+
+/* One NMF_AdditiveDivergence iteration (src/algorithms/nmf/nmf_additive_divergence.cpp:30-51): */
+/*   WH = W H ; sumW = clamp(colsum W) ; additive H update ; if (!updateW) return ;            */
+/*   WH = W H (new H) ; sumH = clamp(rowsum H) ; additive W update                            */
+void orc_add_kl_iter_f64(int n, int m, int r, const double *V, double *W, double *H, double eps,
+                         int updateW) {
+    KL_ITER_BODY(double, orc_gemm_f64, orc_sum_w_f64, orc_sum_h_f64, orc_add_kl_update_h_f64,
+                 orc_add_kl_update_w_f64)
+}
+void orc_add_kl_iter_f32(int n, int m, int r, const float *V, float *W, float *H, float eps,
+                         int updateW) {
+    KL_ITER_BODY(float, orc_gemm_f32, orc_sum_w_f32, orc_sum_h_f32, orc_add_kl_update_h_f32,
+                 orc_add_kl_update_w_f32)
+}
+
+/* ------------------------------------------------------------------------ */
 /* Objectives, accumulated in double.                                         */
 /*  Frobenius  sum (V - WH)^2                                                 */
 /*  KL         sum V ln(V / WH) - V + WH, with 0 ln 0 = 0                     */
@@ -405,7 +527,17 @@ void orc_kl_iter_f32_omp(int n, int m, int r, const float *V, float *W, float *H
  * (nmf_multiplicative_divergence.cpp:29-50), with W H, the ratio and the
  * numerators fused per column of V and OpenMP over columns.  Summation inside
  * each dot product is sequential, as in the textbook loops. */
-void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double *H, double eps, int updateW) {
+/* add != 0: the additive KL iteration (orc_add_kl_iter_f64) with the same loop structure:
+ * q(i) = V / (WH + eps) - 1 and X = max(0, X + (X / sum) * num). */
+static double kl_apply(double x, double num, double sum, int add) {
+    if (!add) return x * (num / sum);
+    double v = x + (x / sum) * num;
+    return v < 0 ? 0 : v;
+}
+
+static void kl_iter_f64_omp_impl(int n, int m, int r, const double *V, double *W, double *H, double eps,
+                                 int updateW, int add) {
+    const double sub = add ? 1.0 : 0.0;
     double *sumW = (double *)malloc(sizeof(double) * (size_t)r);
     orc_sum_w_f64(n, r, W, eps, sumW);
     /* H half: per column u, q(i) = V(i,u) / ((W H)(i,u) + eps), H(a,u) *= sum_i W(i,a) q(i) / sumW[a] */
@@ -422,7 +554,7 @@ void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double
                 for (int i = 0; i < n; i++) q[i] += w[i] * ha;
             }
             const double *v = V + (size_t)u * n;
-            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps);
+            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps) - sub;
             double *hn = H + (size_t)u * r;
             double num[512];
             for (int a = 0; a < r; a++) {
@@ -431,7 +563,7 @@ void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double
                 for (int i = 0; i < n; i++) sacc += w[i] * q[i];
                 num[a] = sacc;
             }
-            for (int a = 0; a < r; a++) hn[a] *= (num[a] / sumW[a]);
+            for (int a = 0; a < r; a++) hn[a] = kl_apply(hn[a], num[a], sumW[a], add);
         }
         free(q);
     }
@@ -466,7 +598,7 @@ void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double
                 for (int i = 0; i < n; i++) q[i] += w[i] * ha;
             }
             const double *v = V + (size_t)u * n;
-            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps);
+            for (int i = 0; i < n; i++) q[i] = v[i] / (q[i] + eps) - sub;
             for (int a = 0; a < r; a++) {
                 const double ha = h[a];
                 double *dst = mine + (size_t)a * n;
@@ -479,10 +611,18 @@ void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double
         for (int i = 0; i < n; i++) {
             double num = 0.0;
             for (int t = 0; t < nt; t++) num += acc[(size_t)t * n * r + (size_t)a * n + i];
-            W[(size_t)a * n + i] *= (num / sumH[a]);
+            W[(size_t)a * n + i] = kl_apply(W[(size_t)a * n + i], num, sumH[a], add);
         }
     free(acc);
     free(sumH);
+}
+
+void orc_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double *H, double eps, int updateW) {
+    kl_iter_f64_omp_impl(n, m, r, V, W, H, eps, updateW, 0);
+}
+
+void orc_add_kl_iter_f64_omp(int n, int m, int r, const double *V, double *W, double *H, double eps, int updateW) {
+    kl_iter_f64_omp_impl(n, m, r, V, W, H, eps, updateW, 1);
 }
 
 int orc_omp_threads(void) {

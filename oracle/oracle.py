@@ -49,6 +49,17 @@ _SIGS = {
     "orc_euc_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
     "orc_kl_iter_f32_omp": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
     "orc_kl_iter_f64_omp": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
+    "orc_add_update_f64": (None, [_dp, _dp, _dp, _l]),
+    "orc_add_update_f32": (None, [_fp, _fp, _fp, _l]),
+    "orc_add_kl_update_h_f64": (None, [_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d]),
+    "orc_add_kl_update_w_f64": (None, [_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d]),
+    "orc_add_kl_update_h_f32": (None, [_fp, _fp, _fp, _fp, _fp, _i, _i, _i, _f]),
+    "orc_add_kl_update_w_f32": (None, [_fp, _fp, _fp, _fp, _fp, _i, _i, _i, _f]),
+    "orc_add_euc_iter_f64": (None, [_i, _i, _i, _dp, _dp, _dp, _i, _i]),
+    "orc_add_euc_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _i, _i]),
+    "orc_add_kl_iter_f64": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
+    "orc_add_kl_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
+    "orc_add_kl_iter_f64_omp": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
     "orc_frob_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_kl_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_quality_f64": (_d, [_l, _dp, _dp, _d]),
@@ -115,6 +126,27 @@ def kl_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
     return W, H
 
 
+def add_euc_iter(n, m, r, V, W, H, update_w=True, fix_h=False, iters=1):
+    """NMF_AdditiveEuclidian iterations (H is left unchanged unless fix_h)."""
+    L = lib()
+    fn, p = (L.orc_add_euc_iter_f64, _p64) if V.dtype == np.float64 else (L.orc_add_euc_iter_f32, _p32)
+    for _ in range(iters):
+        fn(n, m, r, p(V), p(W), p(H), 1 if update_w else 0, 1 if fix_h else 0)
+    return W, H
+
+
+def add_kl_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
+    """NMF_AdditiveDivergence iterations (omp: the fused OpenMP FP64 loops)."""
+    L = lib()
+    if V.dtype == np.float64:
+        fn, p, e = (L.orc_add_kl_iter_f64_omp if omp else L.orc_add_kl_iter_f64), _p64, float(eps)
+    else:
+        fn, p, e = L.orc_add_kl_iter_f32, _p32, float(np.float32(eps))
+    for _ in range(iters):
+        fn(n, m, r, p(V), p(W), p(H), e, 1 if update_w else 0)
+    return W, H
+
+
 def frob(n, m, r, V, W, H):
     return lib().orc_frob_f64(n, m, r, _p64(V), _p64(W), _p64(H))
 
@@ -153,6 +185,36 @@ def np_kl_iter(V, W, H, eps, update_w=True):
         WH = W @ H
         sumH = np.maximum(H.sum(axis=1), eps)
         W = W * (((V / (WH + eps)) @ H.T) / sumH[None, :])
+    return W, H
+
+
+def _np_add(X, d1, d2):
+    # nmf_kernels.cu:31-39: X + (X / d2) * (d1 - d2), negative results set to 0 (NaN kept)
+    v = X + (X / d2) * (d1 - d2)
+    return np.where(v < 0, 0.0, v)
+
+
+def np_add_euc_iter(V, W, H, update_w=True, fix_h=False):
+    """nmf_additive_euclidian.cpp:31-56 (H unchanged unless fix_h)."""
+    if fix_h:
+        H = _np_add(H, W.T @ V, (W.T @ W) @ H)
+    if update_w:
+        W = _np_add(W, V @ H.T, W @ (H @ H.T))
+    return W, H
+
+
+def np_add_kl_iter(V, W, H, eps, update_w=True):
+    """nmf_additive_divergence.cpp:30-51 with the kernels of
+    nmf_kernels_additive_divergence.cu:57-193 (sums of W * (ratio - 1))."""
+    WH = W @ H
+    sumW = np.maximum(W.sum(axis=0), eps)
+    v = H + (H / sumW[:, None]) * (W.T @ (V / (WH + eps) - 1.0))
+    H = np.where(v < 0, 0.0, v)
+    if update_w:
+        WH = W @ H
+        sumH = np.maximum(H.sum(axis=1), eps)
+        v = W + (W / sumH[None, :]) * ((V / (WH + eps) - 1.0) @ H.T)
+        W = np.where(v < 0, 0.0, v)
     return W, H
 
 

@@ -42,6 +42,27 @@ def test_workspace_bytes(native):
         assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
 
 
+def test_additive_rules_plan_fp32_only(native):
+    """The additive rules run the FP32 passes (same workspace as their multiplicative
+    counterparts); no narrow-precision store; the object API takes FP32 only."""
+    import ctypes
+
+    for add, mult in [(capi.ADDITIVE_EUCLIDEAN, capi.EUCLIDEAN), (capi.ADDITIVE_EUCLIDEAN_WITH_H, capi.EUCLIDEAN),
+                      (capi.ADDITIVE_KL, capi.KL)]:
+        a = native.gpumlib_nmf_workspace_bytes(add, capi.FP32, 1000, 700, 32)
+        assert a > 0 and a == native.gpumlib_nmf_workspace_bytes(mult, capi.FP32, 1000, 700, 32)
+        assert native.gpumlib_nmf_workspace_bytes(add, capi.BF16, 1000, 700, 32) == 0
+        assert native.gpumlib_nmf_workspace_bytes(add, capi.FP8, 1000, 700, 32) == 0
+    assert native.gpumlib_nmf_workspace_bytes(5, capi.FP32, 100, 100, 8) == 0
+    assert native.gpumlib_nmf_additive_update(None, None, None, -1, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_additive_kl_update_h(None, None, None, None, None, -1, 4, 4, 1e-8, None) \
+        == HIP_INVALID_VALUE
+    h = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: the precision check comes first
+    assert native.gpumlib_nmf_object_create(capi.ADDITIVE_KL, capi.BF16, 8, 8, 2, fake, fake, fake,
+                                            ctypes.byref(h)) == HIP_INVALID_VALUE
+
+
 def test_iterate_rejects_bad_arguments(native):
     f = native.gpumlib_nmf_iterate
     ws = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, 64, 64, 8)
