@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device to device three times (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
+    ap.add_argument("--no-additive", action="store_true",
+                    help="skip the additive-update legs (NMF_AdditiveEuclidian / NMF_AdditiveDivergence, FP32)")
     ap.add_argument("--gap-iterations", default="100,500,1000",
                     help="iteration counts at which the objective gap to the FP32 path is reported ('' = none)")
     return ap.parse_args()
@@ -191,6 +193,30 @@ def main():
                     legs[extra]["objective_rel_gap_vs_fp32_path"] = {
                         str(t): abs(legs[extra]["objective_at"][t] - legs["fp32"]["objective_at"][t])
                         / legs["fp32"]["objective_at"][t] for t in gap_iters}
+    # Additive-update variants on the same workload (FP32, the same pass kernels with the
+    # additive update): iterations per second and objectives, for the record.
+    def additive_leg(arule):
+        d = G.DeviceNMF(arule, V, W0, H0, n, m, r, eps=eps)
+        obj0 = d.objective(obj_kind)
+        d.launch(args.warmup)
+        d.stream.synchronize()
+        t0 = time.perf_counter()
+        d.launch(args.steps)
+        d.stream.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        flops = flops_w if arule == G.ADDITIVE_EUCLIDEAN else flops_iter  # that rule runs the W half only
+        return {"iters_per_s": 1e3 / ms, "ms_per_step": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                "objective_entry": obj0, "objective_after": d.objective(obj_kind),
+                "iterations_timed": args.steps, "precision": "fp32"}
+
+    additive = {}
+    if not args.no_additive:
+        names = ({"additive_euclidean": G.ADDITIVE_EUCLIDEAN, "additive_euclidean_with_h": G.ADDITIVE_EUCLIDEAN_WITH_H}
+                 if rule == G.EUCLIDEAN else {"additive_kl": G.ADDITIVE_KL})
+        for name, arule in names.items():
+            print(f"additive leg {name}", file=sys.stderr, flush=True)
+            additive[name] = additive_leg(arule)
+
     head = legs[args.precision]
     line = {
         "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",
@@ -220,6 +246,8 @@ def main():
                           "to the FP64 host oracle within 1e-4 by the parity tests",
         "gen_seconds": t_gen,
     }
+    if additive:
+        line["additive_legs"] = additive
 
     if args.serve_jobs > 0:
         from gpumlib_amd import serving

@@ -16,6 +16,7 @@ OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libgpumlib_nmf.so")
 EXAMPLES = os.path.join(HERE, "examples")
+NATIVE_TESTS = os.path.join(os.path.dirname(HERE), "tests", "native")
 BIN_DIR = os.path.join(HERE, "bin")
 
 ARCH = os.environ.get("GPUMLIB_ARCH", "gfx950")
@@ -62,14 +63,15 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    # example drivers over the C++ drop-in API
+    # example drivers over the C++ drop-in API, and the native checks of tests/native
     os.makedirs(BIN_DIR, exist_ok=True)
-    for f in sorted(os.listdir(EXAMPLES)):
-        if not f.endswith(".cpp"):
-            continue
-        exe = os.path.join(BIN_DIR, f[:-4])
+    sources = [os.path.join(EXAMPLES, f) for f in sorted(os.listdir(EXAMPLES)) if f.endswith(".cpp")]
+    if os.path.isdir(NATIVE_TESTS):
+        sources += [os.path.join(NATIVE_TESTS, f) for f in sorted(os.listdir(NATIVE_TESTS)) if f.endswith(".cpp")]
+    for src in sources:
+        exe = os.path.join(BIN_DIR, os.path.basename(src)[:-4])
         _run(["g++", *HOST_FLAGS, "-pthread", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-              os.path.join(EXAMPLES, f), "-o", exe + ".tmp", f"-L{OUT_DIR}", "-lgpumlib_nmf",
+              src, "-o", exe + ".tmp", f"-L{OUT_DIR}", "-lgpumlib_nmf",
               f"-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
         os.replace(exe + ".tmp", exe)
     return LIB

@@ -1,4 +1,5 @@
-"""ctypes binding of libgpumlib_nmf.so (the C-ABI in include/gpumlib_nmf.h).
+"""ctypes binding of libgpumlib_nmf.so (the C-ABI in include/gpumlib_nmf.h and
+include/gpumlib_reduction.h).
 
 The library is built in-tree by gpumlib_amd/build.py.  Loading fails loudly when
 it is missing: there is no fallback implementation.
@@ -91,6 +92,10 @@ SIGNATURES = {
     "gpumlib_nmf_server_wait": (_i, [_vp, _c.c_longlong, _vp, _vp, _c.POINTER(JobResult)]),
     "gpumlib_nmf_server_stats_get": (_i, [_vp, _c.POINTER(ServerStats)]),
     "gpumlib_nmf_server_destroy": (None, [_vp]),
+    "gpumlib_reduce_workspace_bytes": (_sz, [_l]),
+    "gpumlib_reduce_sum": (_i, [_vp, _l, _f, _vp, _vp, _sz, _vp]),
+    "gpumlib_reduce_min": (_i, [_vp, _l, _vp, _vp, _vp, _sz, _vp]),
+    "gpumlib_reduce_max": (_i, [_vp, _l, _vp, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_rng_u64": (_u64, [_u64, _u64, _u64]),
     "gpumlib_nmf_gen_uniform_f32": (None, [_u64, _u64, _sz, _vp]),
     "gpumlib_nmf_gen_euclidean_v": (None, [_u64, _i, _i, _i, _f, _vp]),

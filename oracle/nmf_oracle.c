@@ -21,6 +21,7 @@
  *   *_f32_omp  the same FP32 rules, cache-blocked and OpenMP-parallel: the
  *          host-cores CPU baseline.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -349,6 +350,36 @@ void orc_add_kl_iter_f32(int n, int m, int r, const float *V, float *W, float *H
                          int updateW) {
     KL_ITER_BODY(float, orc_gemm_f32, orc_sum_w_f32, orc_sum_h_f32, orc_add_kl_update_h_f32,
                  orc_add_kl_update_w_f32)
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reductions of the Reduction class (src/cuda/reduction/reduction.h:41-157). */
+/*  sum:      factor * sum_k x[k]  (Sum :60-71, Average :77-80), in double    */
+/*  min/max:  start from FLT_MAX / -FLT_MAX (MAX_CUDAFLOAT / MIN_CUDAFLOAT,   */
+/*            src/cuda/definitions.h:78, :91); an element replaces the        */
+/*            candidate only when strictly smaller / larger (the comparisons */
+/*            of reduction_kernels_min.cu:35-80 and _max.cu), scanned in index */
+/*            order, so the first of equal extremes is kept and NaN never     */
+/*            replaces; index -1 when nothing replaced the start value.       */
+/* ------------------------------------------------------------------------ */
+double orc_reduce_sum(const float *x, long count, double factor) {
+    double s = 0.0;
+    for (long k = 0; k < count; k++) s += (double)x[k];
+    return s * factor;
+}
+
+void orc_reduce_extreme(const float *x, long count, int is_max, float *value, int *index) {
+    float v = is_max ? -FLT_MAX : FLT_MAX;
+    int idx = -1;
+    for (long k = 0; k < count; k++) {
+        const int better = is_max ? (x[k] > v) : (x[k] < v);
+        if (better) {
+            v = x[k];
+            idx = (int)k;
+        }
+    }
+    *value = v;
+    *index = idx;
 }
 
 /* ------------------------------------------------------------------------ */

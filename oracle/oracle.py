@@ -60,6 +60,8 @@ _SIGS = {
     "orc_add_kl_iter_f64": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
     "orc_add_kl_iter_f32": (None, [_i, _i, _i, _fp, _fp, _fp, _f, _i]),
     "orc_add_kl_iter_f64_omp": (None, [_i, _i, _i, _dp, _dp, _dp, _d, _i]),
+    "orc_reduce_sum": (_d, [_fp, _l, _d]),
+    "orc_reduce_extreme": (None, [_fp, _l, _i, _fp, ctypes.POINTER(_i)]),
     "orc_frob_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_kl_f64": (_d, [_i, _i, _i, _dp, _dp, _dp]),
     "orc_quality_f64": (_d, [_l, _dp, _dp, _d]),
@@ -145,6 +147,18 @@ def add_kl_iter(n, m, r, V, W, H, eps, update_w=True, iters=1, omp=False):
     for _ in range(iters):
         fn(n, m, r, p(V), p(W), p(H), e, 1 if update_w else 0)
     return W, H
+
+
+def reduce_sum(x, factor=1.0):
+    """factor * sum of a float32 array, accumulated in double (Reduction::Sum / Average)."""
+    return lib().orc_reduce_sum(_p32(x), x.size, float(factor))
+
+
+def reduce_extreme(x, is_max):
+    """(value, index) of Reduction::Min/MinIndex (is_max False) or Max/MaxIndex."""
+    v, i = ctypes.c_float(), ctypes.c_int()
+    lib().orc_reduce_extreme(_p32(x), x.size, 1 if is_max else 0, ctypes.byref(v), ctypes.byref(i))
+    return v.value, i.value
 
 
 def frob(n, m, r, V, W, H):

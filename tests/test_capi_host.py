@@ -12,7 +12,8 @@ HIP_INVALID_VALUE = 1
 
 
 def header_functions():
-    text = open(os.path.join(ROOT, "include", "gpumlib_nmf.h")).read()
+    inc = os.path.join(ROOT, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b(gpumlib_\w+)\s*\(", text)
     return sorted(set(names))
@@ -96,6 +97,19 @@ def test_other_entry_points_reject_bad_arguments(native):
     assert native.gpumlib_nmf_object_create(0, 0, 0, 8, 2, None, None, None, ctypes.byref(h)) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_object_do_iteration(None, 1) == HIP_INVALID_VALUE
     native.gpumlib_nmf_object_destroy(None)  # no-op
+
+
+def test_reduction_rejects_bad_arguments(native):
+    ws = native.gpumlib_reduce_workspace_bytes(1 << 20)
+    assert ws > 0 and ws % 256 == 0 and ws <= 1024 * 8
+    assert native.gpumlib_reduce_workspace_bytes(0) > 0
+    fake = 0x1000  # never dereferenced: validation fails first
+    assert native.gpumlib_reduce_sum(None, 10, 1.0, fake, fake, ws, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_reduce_sum(fake, -1, 1.0, fake, fake, ws, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_reduce_sum(fake, 10, 1.0, None, fake, ws, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_reduce_sum(fake, 1 << 30, 1.0, fake, fake, 8, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_reduce_min(fake, 10, None, None, fake, ws, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_reduce_max(fake, 1 << 32, fake, fake, fake, 1 << 20, None) == HIP_INVALID_VALUE
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
