@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device to device three times (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
+    ap.add_argument("--no-reduction", action="store_true",
+                    help="skip the reduction-framework leg (sum / min / max over the FP32 V)")
     ap.add_argument("--no-additive", action="store_true",
                     help="skip the additive-update legs (NMF_AdditiveEuclidian / NMF_AdditiveDivergence, FP32)")
     ap.add_argument("--gap-iterations", default="100,500,1000",
@@ -217,6 +219,40 @@ def main():
             print(f"additive leg {name}", file=sys.stderr, flush=True)
             additive[name] = additive_leg(arule)
 
+    # Reduction framework (Reduction::Sum / MinIndex / MaxIndex) over the FP32 V of the
+    # workload: HBM-bound, 4 bytes read per element; HIP events on the launching stream.
+    def reduction_leg():
+        x = hip.Buffer.from_array(np.asarray(V, dtype=np.float32))
+        count = n * m
+        wsb = int(lib.gpumlib_reduce_workspace_bytes(count))
+        ws, out = hip.Buffer(wsb), hip.Buffer(16)
+        st = hip.Stream()
+        calls = {
+            "sum": lambda: lib.gpumlib_reduce_sum(x.ptr, count, 1.0, out.ptr, ws.ptr, wsb, st.handle),
+            "min_index": lambda: lib.gpumlib_reduce_min(x.ptr, count, out.ptr, out.ptr + 4, ws.ptr, wsb, st.handle),
+            "max_index": lambda: lib.gpumlib_reduce_max(x.ptr, count, out.ptr, out.ptr + 4, ws.ptr, wsb, st.handle),
+        }
+        res = {}
+        reps = max(5, args.kernel_reps)
+        for name, fn in calls.items():
+            for _ in range(3):
+                capi.check(fn(), name)
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(st)
+            for _ in range(reps):
+                capi.check(fn(), name)
+            e1.record(st)
+            e1.synchronize()
+            us = e0.elapsed_ms(e1) * 1e3 / reps
+            gbs = 4.0 * count / (us * 1e-6) / 1e9
+            res[name] = {"us_per_call": us, "gbs": gbs, "frac_of_hbm_peak": gbs / PEAK["hbm_gbs"]}
+        x.close()
+        return {"elements": count, "bytes": 4 * count, "calls": res,
+                "note": "two launches per call (partials, then one workgroup combining them); "
+                        "GB/s = 4 bytes per element / time per call"}
+
+    reduction = None if args.no_reduction else reduction_leg()
+
     head = legs[args.precision]
     line = {
         "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",
@@ -248,6 +284,8 @@ def main():
     }
     if additive:
         line["additive_legs"] = additive
+    if reduction:
+        line["reduction"] = reduction
 
     if args.serve_jobs > 0:
         from gpumlib_amd import serving

@@ -185,6 +185,30 @@ __device__ __forceinline__ void stage_lds(int total, Ld ld, St st) {
     }
 }
 
+// Two-phase staging of a small operand: load() issues this thread's U loads into
+// registers, store() writes them to LDS.  Loads issued between the two phases share
+// the same memory round trip.
+template <int U>
+struct RegStage {
+    float v[U];
+    template <typename Ld>
+    __device__ __forceinline__ void load(int total, Ld ld) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = threadIdx.x + u * NTHREADS;
+            v[u] = idx < total ? ld(idx) : 0.f;
+        }
+    }
+    template <typename St>
+    __device__ __forceinline__ void store(int total, St st) const {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = threadIdx.x + u * NTHREADS;
+            if (idx < total) st(idx, v[u]);
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Single-problem Euclidean updates, tiled so that every thread issues all of its
 // slice loads before it waits (the update is a short, latency-bound pass over
@@ -216,6 +240,42 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
     const int j0 = blockIdx.x * JT, nj = min(JT, count - j0);
     const int rq = round_up(r, 4) / 4;  // float4 groups of a row of P (ldr = 4*rq)
 
+    // C and the X rows, zero padded to RP.  Up to r = 64 their loads are issued first,
+    // into registers, so that they and the slice loads below take one round trip.
+    auto ld_g = [&](int idx) {  // C(b,a) in staging order: along b for gt (coalesced), along a otherwise
+        const int hi = idx / RP, lo = idx - hi * RP;
+        return (hi < r && lo < r) ? G[(size_t)hi * ldg + lo] : 0.f;
+    };
+    auto st_g = [&](int idx, float v) {
+        const int hi = idx / RP, lo = idx - hi * RP;
+        if (gt) sG[lo * RS + hi] = v; else sG[hi * RS + lo] = v;
+    };
+    auto ld_x = [&](int idx) {
+        if (xs_a == 1) {
+            const int jj = idx / RP, b = idx - jj * RP;
+            return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + b] : 0.f;
+        }
+        const int jj = idx % JT, b = idx / JT;
+        return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + (size_t)b * xs_a] : 0.f;
+    };
+    auto st_x = [&](int idx, float v) {
+        if (xs_a == 1) {
+            const int jj = idx / RP, b = idx - jj * RP;
+            sX[jj * RS + b] = v;
+        } else {
+            const int jj = idx % JT, b = idx / JT;
+            sX[jj * RS + b] = v;
+        }
+    };
+    constexpr int UG = (RP * RP + NTHREADS - 1) / NTHREADS, UX = (JT * RP + NTHREADS - 1) / NTHREADS;
+    constexpr bool EARLY = UG <= 16;
+    RegStage<EARLY ? UG : 1> stg_g;
+    RegStage<EARLY ? UX : 1> stg_x;
+    if constexpr (EARLY) {
+        stg_g.load(RP * RP, ld_g);
+        stg_x.load(JT * RP, ld_x);
+    }
+
     // 1. issue the slice loads of this thread's items
     float4 num[MAXI];
 #pragma unroll
@@ -239,28 +299,13 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
             num[k].x += x0.x, num[k].y += x0.y, num[k].z += x0.z, num[k].w += x0.w;
         }
     }
-    // 2. stage C and the X rows, zero padded to RP, loads batched
-    if (gt) {  // C(b,a) = G[a*ldg + b]: read along b (coalesced), write transposed
-        stage_lds<8>(RP * RP,
-                     [&](int idx) { const int a = idx / RP, b = idx - (idx / RP) * RP;
-                                    return (a < r && b < r) ? G[(size_t)a * ldg + b] : 0.f; },
-                     [&](int idx, float v) { const int a = idx / RP, b = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
+    // 2. C and the X rows into LDS
+    if constexpr (EARLY) {
+        stg_g.store(RP * RP, st_g);
+        stg_x.store(JT * RP, st_x);
     } else {
-        stage_lds<8>(RP * RP,
-                     [&](int idx) { const int b = idx / RP, a = idx - (idx / RP) * RP;
-                                    return (a < r && b < r) ? G[(size_t)b * ldg + a] : 0.f; },
-                     [&](int idx, float v) { const int b = idx / RP, a = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
-    }
-    if (xs_a == 1) {
-        stage_lds<8>(JT * RP,
-                     [&](int idx) { const int jj = idx / RP, b = idx - (idx / RP) * RP;
-                                    return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + b] : 0.f; },
-                     [&](int idx, float v) { const int jj = idx / RP, b = idx - (idx / RP) * RP; sX[jj * RS + b] = v; });
-    } else {
-        stage_lds<8>(JT * RP,
-                     [&](int idx) { const int jj = idx % JT, b = idx / JT;
-                                    return (jj < nj && b < r) ? X[(size_t)(j0 + jj) * xs_j + (size_t)b * xs_a] : 0.f; },
-                     [&](int idx, float v) { const int jj = idx % JT, b = idx / JT; sX[jj * RS + b] = v; });
+        stage_lds<8>(RP * RP, ld_g, st_g);
+        stage_lds<8>(JT * RP, ld_x, st_x);
     }
     __syncthreads();
     // 3. denominators and new values
@@ -358,6 +403,33 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restric
     const int rpad = round_up(r, 4);
     const int i0 = blockIdx.x * JT, nrow = min(JT, n - i0);
 
+    // HHt and the W rows, zero padded; up to r = 64 their loads go out before the slice
+    // loads (one memory round trip for all of them).
+    auto ld_g = [&](int idx) {
+        const int a = idx / RP, b = idx - a * RP;
+        return (a < r && b < r) ? G[(size_t)a * rpad + b] : 0.f;
+    };
+    auto st_g = [&](int idx, float v) {
+        const int a = idx / RP, b = idx - a * RP;
+        sG[b * RS + a] = v;
+    };
+    auto ld_w = [&](int idx) {
+        const int ii = idx % JT, b = idx / JT;
+        return (ii < nrow && b < r) ? W[(size_t)b * n + i0 + ii] : 0.f;
+    };
+    auto st_w = [&](int idx, float v) {
+        const int ii = idx % JT, b = idx / JT;
+        sWt[b * JS + ii] = v;
+    };
+    constexpr int UG = (RP * RP + NTHREADS - 1) / NTHREADS, UW = (RP * JT + NTHREADS - 1) / NTHREADS;
+    constexpr bool EARLY = UG <= 16;
+    RegStage<EARLY ? UG : 1> stg_g;
+    RegStage<EARLY ? UW : 1> stg_w;
+    if constexpr (EARLY) {
+        stg_g.load(RP * RP, ld_g);
+        stg_w.load(RP * JT, ld_w);
+    }
+
     float4 num[MAXI];
 #pragma unroll
     for (int k = 0; k < MAXI; k++) {
@@ -381,14 +453,13 @@ __global__ __launch_bounds__(NTHREADS) void k_update_w_euc_cols(float* __restric
             num[k].x += x0.x, num[k].y += x0.y, num[k].z += x0.z, num[k].w += x0.w;
         }
     }
-    stage_lds<8>(RP * RP,
-                 [&](int idx) { const int a = idx / RP, b = idx - (idx / RP) * RP;
-                                return (a < r && b < r) ? G[(size_t)a * rpad + b] : 0.f; },
-                 [&](int idx, float v) { const int a = idx / RP, b = idx - (idx / RP) * RP; sG[b * RS + a] = v; });
-    stage_lds<8>(RP * JT,
-                 [&](int idx) { const int ii = idx % JT, b = idx / JT;
-                                return (ii < nrow && b < r) ? W[(size_t)b * n + i0 + ii] : 0.f; },
-                 [&](int idx, float v) { const int ii = idx % JT, b = idx / JT; sWt[b * JS + ii] = v; });
+    if constexpr (EARLY) {
+        stg_g.store(RP * RP, st_g);
+        stg_w.store(RP * JT, st_w);
+    } else {
+        stage_lds<8>(RP * RP, ld_g, st_g);
+        stage_lds<8>(RP * JT, ld_w, st_w);
+    }
     __syncthreads();
     float4 out[MAXI];
 #pragma unroll

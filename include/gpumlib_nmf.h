@@ -1,5 +1,6 @@
 /*
- * gpumlib_nmf.h — C-ABI of the MI355X (gfx950) NMF multiplicative-update library.
+ * gpumlib_nmf.h — C-ABI of the MI355X (gfx950) NMF library (multiplicative and
+ * additive updates).
  *
  * Plain C types only: device pointers, sizes, a hipStream_t and hipError_t
  * return codes.  Matrices are column-major, as in GPUMLib
@@ -28,7 +29,7 @@ extern "C" {
 /*    Same names and arguments; cudaError_t becomes hipError_t.  The grid and */
 /*    block arguments are accepted for drop-in compatibility and ignored: the */
 /*    launch geometry is chosen for gfx950.  Work goes to the null stream, as */
-/*    in the reference.  The elementwise rule uses eps = 1e-8f, the value of  */
+/*    in the reference.  The ratios use eps = 1e-8f, the value of             */
 /*    SMALL_VALUE_TO_ADD_DENOMINATOR (nmf_kernels.h:34).                      */
 /* ------------------------------------------------------------------------ */
 
