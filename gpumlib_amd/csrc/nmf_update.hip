@@ -358,27 +358,37 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
     if constexpr (JT == 32) {
         // FP8 shadow: the workgroup's 32 rows are one scale block of every rank column a
         // (X(j,a) at Q[a*ldq + j], scale at Qsc[a*nblk + j/32]); rows past count are zeros
+        // Four consecutive lanes share a column a (r <= 64 here, so 256 threads cover every
+        // column once): each takes 8 of the 32 rows, the column's amax is combined across
+        // the four lanes, and each lane writes its 8 bytes.
         if (Q) {
-            for (int a = tid; a < r; a += NTHREADS) {
-                float amax = 0.f;
-                for (int jj = 0; jj < nj; jj++) amax = fmaxf(amax, fabsf(sN[jj * RS + a]));
-                const int sc = e8m0_for_amax(amax);
-                Qsc[(size_t)a * nblk + j0 / 32] = (uint8_t)sc;
-                uint32_t word[8];
+            const int a = tid >> 2, part = tid & 3;
+            float v8[8];
+            float amax = 0.f;
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    uint32_t v = 0;
+            for (int b = 0; b < 8; b++) {
+                const int jj = 8 * part + b;
+                v8[b] = (a < r && jj < nj) ? sN[jj * RS + a] : 0.f;
+                amax = fmaxf(amax, fabsf(v8[b]));
+            }
+            amax = fmaxf(amax, __shfl_xor(amax, 1, 64));
+            amax = fmaxf(amax, __shfl_xor(amax, 2, 64));
+            if (a < r) {
+                const int sc = e8m0_for_amax(amax);
+                if (part == 0) Qsc[(size_t)a * nblk + j0 / 32] = (uint8_t)sc;
+                uint32_t word[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    uint32_t w = 0;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const int jj = 4 * q + b;
-                        const uint32_t e = jj < nj ? f32_to_e4m3(e8m0_unscale(sN[jj * RS + a], sc)) : 0u;
-                        v |= e << (8 * b);
+                        const int jj = 8 * part + 4 * q + b;
+                        const uint32_t e = jj < nj ? f32_to_e4m3(e8m0_unscale(v8[4 * q + b], sc)) : 0u;
+                        w |= e << (8 * b);
                     }
-                    word[q] = v;
+                    word[q] = w;
                 }
-                uint4* dst = reinterpret_cast<uint4*>(Q + (size_t)a * ldq + j0);
-                dst[0] = make_uint4(word[0], word[1], word[2], word[3]);
-                dst[1] = make_uint4(word[4], word[5], word[6], word[7]);
+                *reinterpret_cast<uint2*>(Q + (size_t)a * ldq + j0 + 8 * part) = make_uint2(word[0], word[1]);
             }
         }
     }
