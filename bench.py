@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--serve-seq-jobs", type=int, default=512,
                     help="how many of those jobs are also run one at a time for the comparison")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the jobs (per s)")
+    ap.add_argument("--serve-autotune", type=int, default=2,
+                    help="timed iterations per candidate of the server's batch autotuner (0 = autotuner off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device to device three times (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
@@ -294,7 +296,7 @@ def main():
         jobs = serving.make_jobs(args.serve_jobs)
         data = [serving.job_data(j) for j in jobs]
         print("serving: batched run", file=sys.stderr, flush=True)
-        srv = serving.run_server(lib, jobs, data, rate=args.serve_rate)
+        srv = serving.run_server(lib, jobs, data, rate=args.serve_rate, autotune=args.serve_autotune)
         nseq = min(args.serve_seq_jobs, len(jobs))
         line["serving"] = {
             "workload": f"{len(jobs)} jobs, Euclidean/KL 50/50, m log-uniform [64,4096], n log-uniform [64,2048], "

@@ -26,7 +26,7 @@ class Dim3(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _c.c_size_t), ("max_active", _c.c_int), ("round_iterations", _c.c_int),
-                ("use_graph", _c.c_int), ("target_workgroups", _c.c_int)]
+                ("use_graph", _c.c_int), ("target_workgroups", _c.c_int), ("autotune", _c.c_int)]
 
 
 class JobResult(ctypes.Structure):
@@ -37,7 +37,14 @@ class JobResult(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", _c.c_longlong), ("completed", _c.c_longlong), ("rounds", _c.c_longlong),
                 ("iterations_batched", _c.c_longlong), ("peak_active", _c.c_int), ("pool_bytes", _c.c_size_t),
-                ("pool_high_water", _c.c_size_t)]
+                ("pool_high_water", _c.c_size_t), ("autotunes", _c.c_longlong), ("tuned_target", _c.c_int),
+                ("tuned_pass_grid", _c.c_int), ("tuned_update_grid", _c.c_int)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("target_workgroups", _c.c_int), ("pass_grid", _c.c_int), ("update_grid", _c.c_int),
+                ("ms_per_iteration", _c.c_double), ("ms_per_iteration_before", _c.c_double), ("trials", _c.c_int),
+                ("cached", _c.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -87,6 +94,7 @@ SIGNATURES = {
     "gpumlib_nmf_batch_graph_launch": (_i, [_vp, _i, _i, _vp]),
     "gpumlib_nmf_batch_objectives": (_i, [_vp, _c.POINTER(_d), _vp]),
     "gpumlib_nmf_batch_objective_device_ptr": (_vp, [_vp]),
+    "gpumlib_nmf_batch_autotune": (_i, [_vp, _i, _vp, _c.POINTER(BatchTuning)]),
     "gpumlib_nmf_server_create": (_i, [_c.POINTER(ServerConfig), _c.POINTER(_vp)]),
     "gpumlib_nmf_server_submit": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _f, _d, _i, _c.POINTER(_c.c_longlong)]),
     "gpumlib_nmf_server_wait": (_i, [_vp, _c.c_longlong, _vp, _vp, _c.POINTER(JobResult)]),

@@ -7,9 +7,17 @@
 // the per-problem arguments live in a device table that the host refreshes when
 // the set of problems changes.  Because the launches read everything from that
 // table, a hipGraph captured once stays valid across changes of the set.
+//
+// Autotuner (gpumlib_nmf_batch_autotune): the tile shape along K (how many K slices,
+// hence how deep each pass work item is, set through the per-problem workgroup
+// target) and the persistent grid sizes of the grouped launches are chosen by timing
+// trial iterations of the current mix on the device.  W and H are saved before and
+// restored after every trial, so tuning changes no problem's state; the winner is
+// cached per mix signature.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <new>
 #include <utility>
@@ -25,22 +33,43 @@ using namespace gpumlib_nmf;
 namespace {
 
 enum { T_PH_EUC = 0, T_PH_KL, T_PW_EUC, T_PW_KL, T_UH, T_UW, T_RED, T_COUNT };
-constexpr int kPassGrid = 1024;  // persistent workgroups of the grouped passes
+constexpr int kPassGrid = 1024;  // persistent workgroups of the grouped passes (before tuning)
 constexpr int kUpdGrid = 512;
 constexpr int kRedGrid = 256;
 constexpr int kBatchTargetWorkgroups = 32;  // per problem and pass, before tuning
+constexpr int kBatchSizingTarget = 128;     // workspaces are sized for targets up to this
+
+// Candidates of the autotuner.  Targets above the batch's sizing target are skipped
+// (a problem's workspace would be too small for that many K slices).
+constexpr int kTuneTargets[] = {4, 8, 16, 32, 64, 128};
+constexpr int kTunePassGrids[] = {512, 1024, 2048};
+constexpr int kTuneUpdGrids[] = {256, 512, 1024};
 
 struct Slot {
     bool active = false;
     int rule = 0, n = 0, m = 0, r = 0;
+    const float* V = nullptr;
+    float *W = nullptr, *H = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    float eps = 0.f;
     JobArgs args;
+};
+
+struct TuneConfig {
+    int target, pass_grid, upd_grid;
+    double ms_per_iteration, ms_default;
+    int trials;
 };
 
 }  // namespace
 
 struct gpumlib_nmf_batch {
     int maxj = 0;
-    int target = kBatchTargetWorkgroups;
+    int target = kBatchTargetWorkgroups;   // K-split target the problems are planned with
+    int size_target = kBatchSizingTarget;  // target their workspaces are sized for
+    int pass_grid = kPassGrid, upd_grid = kUpdGrid;
+    std::map<unsigned long long, TuneConfig> tuned;  // winners by mix signature
     std::vector<Slot> slots;
     JobArgs* d_jobs = nullptr;
     int* d_tables = nullptr;
@@ -110,20 +139,89 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
 static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, hipStream_t st) {
     const int stride = table_stride();
     const int* T = b->d_tables;
+    const int pg = b->pass_grid, ug = b->upd_grid;
     for (int it = 0; it < iterations; it++) {
         const bool obj = it < objectives;
         hipError_t e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC * stride, 0, obj ? OBJ_FROB : OBJ_NONE, kPassGrid, st)))
+        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC * stride, 0, obj ? OBJ_FROB : OBJ_NONE, pg, st)))
             return e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_KL * stride, 1, obj ? OBJ_KL : OBJ_NONE, kPassGrid, st)))
+        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_KL * stride, 1, obj ? OBJ_KL : OBJ_NONE, pg, st)))
             return e;
         if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
-        if ((e = launch_group_update_h(b->d_jobs, T + T_UH * stride, kUpdGrid, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC * stride, 0, kPassGrid, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_KL * stride, 1, kPassGrid, st))) return e;
-        if ((e = launch_group_update_w(b->d_jobs, T + T_UW * stride, kUpdGrid, st))) return e;
+        if ((e = launch_group_update_h(b->d_jobs, T + T_UH * stride, ug, st))) return e;
+        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC * stride, 0, pg, st))) return e;
+        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_KL * stride, 1, pg, st))) return e;
+        if ((e = launch_group_update_w(b->d_jobs, T + T_UW * stride, ug, st))) return e;
     }
     return hipSuccess;
+}
+
+// Plans slot sl for K-split target `target` in its own workspace and zeroes the
+// arrival counters at the plan's offset (stream-ordered, outside graph capture).
+static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStream_t st) {
+    Plan pl;
+    if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target) || pl.bytes > sl.ws_bytes)
+        return hipErrorInvalidValue;
+    JobArgs a;
+    memset(&a, 0, sizeof(a));
+    pass_args(pl, sl.V, sl.W, sl.H, sl.ws, sl.eps, 1, &a.h);
+    pass_args(pl, sl.V, sl.W, sl.H, sl.ws, sl.eps, 3, &a.w);
+    a.W = sl.W;
+    a.H = sl.H;
+    a.obj = sl.args.obj;
+    a.rule = sl.rule;
+    a.nobj = pl.tiles1 * pl.S1;
+    sl.args = a;
+    b->dirty = true;
+    return hipMemsetAsync(a.h.gcount, 0, 2 * sizeof(int), st);
+}
+
+static hipError_t replan_all(gpumlib_nmf_batch* b, int target, hipStream_t st) {
+    for (Slot& sl : b->slots) {
+        if (!sl.active) continue;
+        const hipError_t e = plan_slot(b, sl, target, st);
+        if (e != hipSuccess) return e;
+    }
+    b->target = target;
+    return hipSuccess;
+}
+
+static void drop_graphs(gpumlib_nmf_batch* b) {
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
+}
+
+// Order-independent signature of the active mix (FNV-1a over the sorted shapes).
+static unsigned long long mix_signature(const gpumlib_nmf_batch* b) {
+    std::vector<unsigned long long> keys;
+    for (const Slot& sl : b->slots)
+        if (sl.active)
+            keys.push_back(((unsigned long long)sl.rule << 62) ^ ((unsigned long long)sl.n << 40) ^
+                           ((unsigned long long)sl.m << 16) ^ (unsigned long long)sl.r);
+    std::sort(keys.begin(), keys.end());
+    unsigned long long h = 1469598103934665603ull;
+    for (unsigned long long k : keys)
+        for (int i = 0; i < 8; i++) {
+            h ^= (k >> (8 * i)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    return h ^ ((unsigned long long)b->size_target << 1);
+}
+
+// Milliseconds per iteration of the current configuration: one untimed iteration,
+// then `iterations` timed with events on the stream.
+static hipError_t time_config(gpumlib_nmf_batch* b, int iterations, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                              double* ms) {
+    hipError_t e = upload(b, st);
+    if (e == hipSuccess) e = enqueue(b, 1, 0, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = enqueue(b, iterations, 0, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    *ms = (double)t / iterations;
+    return e;
 }
 
 extern "C" {
@@ -154,7 +252,7 @@ hipError_t gpumlib_nmf_batch_create(int max_jobs, gpumlib_nmf_batch_handle* out)
 void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b) {
     if (!b) return;
     wait_staging(b);
-    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    drop_graphs(b);
     if (b->staged) (void)hipEventDestroy(b->staged);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_tables) (void)hipFree(b->d_tables);
@@ -167,13 +265,14 @@ void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b) {
 hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgroups_per_pass) {
     if (!b || workgroups_per_pass < 1) return hipErrorInvalidValue;
     b->target = workgroups_per_pass;
+    b->size_target = workgroups_per_pass;
     return hipSuccess;
 }
 
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r) {
     Plan pl;
     if (!b || r > 64 || (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) ||
-        !make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->target))
+        !make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, std::max(b->target, b->size_target)))
         return 0;
     return pl.bytes;
 }
@@ -190,26 +289,22 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     if (s == b->maxj) return hipErrorOutOfMemory;
     wait_staging(b);
     Slot& sl = b->slots[s];
-    sl.active = true;
     sl.rule = rule;
     sl.n = n;
     sl.m = m;
     sl.r = r;
-    memset(&sl.args, 0, sizeof(sl.args));
-    pass_args(pl, V, W, H, workspace, eps, 1, &sl.args.h);
-    pass_args(pl, V, W, H, workspace, eps, 3, &sl.args.w);
-    // gram arrival counters start at zero (outside any captured graph)
-    const hipError_t ze = hipMemset(sl.args.h.gcount, 0, 2 * sizeof(int));
-    if (ze != hipSuccess) {
-        sl.active = false;
-        return ze;
-    }
-    sl.args.W = W;
-    sl.args.H = H;
+    sl.V = V;
+    sl.W = W;
+    sl.H = H;
+    sl.ws = workspace;
+    sl.ws_bytes = workspace_bytes;
+    sl.eps = eps;
     sl.args.obj = b->d_obj + 2 * (size_t)s;
-    sl.args.rule = rule;
-    sl.args.nobj = pl.tiles1 * pl.S1;
-    b->dirty = true;
+    // gram arrival counters start at zero (null stream, outside any captured graph)
+    hipError_t e = plan_slot(b, sl, b->target, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return e;
+    sl.active = true;
     *slot = s;
     return hipSuccess;
 }
@@ -270,3 +365,117 @@ hipError_t gpumlib_nmf_batch_objectives(gpumlib_nmf_batch_handle b, double* host
 const double* gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b) { return b ? b->d_obj : nullptr; }
 
 }  // extern "C"
+
+extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int trial_iterations, hipStream_t stream,
+                                                 gpumlib_nmf_batch_tuning* out) {
+    if (!b || !stream || trial_iterations < 1) return hipErrorInvalidValue;
+    if (gpumlib_nmf_batch_active(b) == 0) return hipErrorInvalidValue;
+    auto report = [&](const TuneConfig& c, int cached) {
+        if (!out) return;
+        out->target_workgroups = c.target;
+        out->pass_grid = c.pass_grid;
+        out->update_grid = c.upd_grid;
+        out->ms_per_iteration = c.ms_per_iteration;
+        out->ms_per_iteration_before = c.ms_default;
+        out->trials = cached ? 0 : c.trials;
+        out->cached = cached;
+    };
+    const int t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid;
+    const unsigned long long sig = mix_signature(b);
+    auto hit = b->tuned.find(sig);
+    if (hit != b->tuned.end()) {  // this mix was tuned before: apply its winner
+        const TuneConfig& c = hit->second;
+        hipError_t e = replan_all(b, c.target, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return e;
+        b->pass_grid = c.pass_grid;
+        b->upd_grid = c.upd_grid;
+        if (b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+        report(c, 1);
+        return hipSuccess;
+    }
+    // every problem's workspace must hold the plan of a candidate target
+    auto fits = [&](int target) {
+        for (const Slot& sl : b->slots) {
+            if (!sl.active) continue;
+            Plan pl;
+            if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target) || pl.bytes > sl.ws_bytes)
+                return false;
+        }
+        return true;
+    };
+    // W and H of every active problem are saved, and restored after each trial
+    size_t floats = 0;
+    for (const Slot& sl : b->slots)
+        if (sl.active) floats += (size_t)sl.n * sl.r + (size_t)sl.r * sl.m;
+    float* save = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&save, floats * sizeof(float));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto copy_wh = [&](bool to_save) {
+        size_t off = 0;
+        hipError_t ce = hipSuccess;
+        for (const Slot& sl : b->slots) {
+            if (!sl.active || ce != hipSuccess) continue;
+            const size_t nw = (size_t)sl.n * sl.r, nh = (size_t)sl.r * sl.m;
+            float* sw = save + off;
+            float* sh = save + off + nw;
+            if (to_save) {
+                ce = hipMemcpyAsync(sw, sl.W, nw * sizeof(float), hipMemcpyDeviceToDevice, stream);
+                if (ce == hipSuccess) ce = hipMemcpyAsync(sh, sl.H, nh * sizeof(float), hipMemcpyDeviceToDevice, stream);
+            } else {
+                ce = hipMemcpyAsync(sl.W, sw, nw * sizeof(float), hipMemcpyDeviceToDevice, stream);
+                if (ce == hipSuccess) ce = hipMemcpyAsync(sl.H, sh, nh * sizeof(float), hipMemcpyDeviceToDevice, stream);
+            }
+            off += nw + nh;
+        }
+        return ce;
+    };
+    TuneConfig best = {t0, pg0, ug0, 1e30, 0.0, 0};
+    auto trial = [&](int target, int pg, int ug) -> hipError_t {
+        hipError_t te = replan_all(b, target, stream);
+        if (te != hipSuccess) return te;
+        b->pass_grid = pg;
+        b->upd_grid = ug;
+        double ms = 0.0;
+        te = time_config(b, trial_iterations, stream, e0, e1, &ms);
+        if (te == hipSuccess) te = copy_wh(false);
+        if (te != hipSuccess) return te;
+        if (++best.trials == 1) best.ms_default = ms;
+        if (ms < best.ms_per_iteration) {
+            best.target = target;
+            best.pass_grid = pg;
+            best.upd_grid = ug;
+            best.ms_per_iteration = ms;
+        }
+        return hipSuccess;
+    };
+    // the configuration in effect first, then the K-split targets with the grids in
+    // effect, then the pass grids and the update grids around the best so far
+    if (e == hipSuccess) e = copy_wh(true);
+    if (e == hipSuccess) e = trial(t0, pg0, ug0);
+    for (int t : kTuneTargets)
+        if (e == hipSuccess && t != t0 && fits(t)) e = trial(t, pg0, ug0);
+    const int tb = best.target;
+    for (int pg : kTunePassGrids)
+        if (e == hipSuccess && pg != pg0) e = trial(tb, pg, ug0);
+    const int pgb = best.pass_grid;
+    for (int ug : kTuneUpdGrids)
+        if (e == hipSuccess && ug != ug0) e = trial(tb, pgb, ug);
+    // apply the winner, or the configuration in effect before if a trial failed
+    const bool ok = (e == hipSuccess);
+    hipError_t ae = replan_all(b, ok ? best.target : t0, stream);
+    b->pass_grid = ok ? best.pass_grid : pg0;
+    b->upd_grid = ok ? best.upd_grid : ug0;
+    if (b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+    if (ae == hipSuccess) ae = hipStreamSynchronize(stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (save) (void)hipFree(save);
+    if (!ok) return e;
+    if (ae != hipSuccess) return ae;
+    b->tuned[sig] = best;
+    report(best, 0);
+    return hipSuccess;
+}

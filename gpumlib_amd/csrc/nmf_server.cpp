@@ -15,6 +15,10 @@
 //     the next round.
 // Problems admitted while others are in flight join the next round, so work of
 // different sizes and ages overlaps on the device.
+//   * autotuning (config.autotune > 0): when the number of resident problems has
+//     doubled or halved since the last tuning (and is at least 4), the batch
+//     autotuner picks the K-split depth and grid sizes for the current mix before
+//     the round (trials restore W and H, so no problem's state changes).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -148,6 +152,7 @@ struct gpumlib_nmf_server {
     bool abandon = false;  // process exit: leave after the round in flight, do not finish queued work
     hipError_t fatal = hipSuccess;
     gpumlib_nmf_server_stats stats;
+    int tuned_at = 0;  // resident problems at the last autotuning (0: never tuned)
     std::thread engine;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -230,6 +235,21 @@ void gpumlib_nmf_server::run() {
         if (e == hipSuccess) e = hipStreamSynchronize(stream);  // host input buffers may be released after this
         for (auto& jp : admitted) {
             std::vector<float>().swap(jp->V);
+        }
+        if (e == hipSuccess && cfg.autotune > 0) {
+            const int resident = gpumlib_nmf_batch_active(batch);
+            if (resident >= 4 && (tuned_at == 0 || resident >= 2 * tuned_at || 2 * resident <= tuned_at)) {
+                gpumlib_nmf_batch_tuning t;
+                e = gpumlib_nmf_batch_autotune(batch, cfg.autotune, stream, &t);
+                if (e == hipSuccess) {
+                    tuned_at = resident;
+                    std::lock_guard<std::mutex> lk(mu);
+                    stats.autotunes++;
+                    stats.tuned_target = t.target_workgroups;
+                    stats.tuned_pass_grid = t.pass_grid;
+                    stats.tuned_update_grid = t.update_grid;
+                }
+            }
         }
         // one round
         if (e == hipSuccess) {

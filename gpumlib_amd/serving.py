@@ -48,12 +48,14 @@ def _percentile(xs, q):
 
 
 def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, max_active=1024, round_iters=10,
-               use_graph=1):
+               use_graph=1, autotune=0):
     """Submits the jobs with exponential inter-arrival times (mean 1/rate) and waits for all.
 
+    autotune > 0 lets the server run the batch autotuner (that many timed iterations per
+    candidate) whenever the number of resident jobs doubles or halves.
     Returns throughput (jobs/s over first submit .. last finish), latency percentiles
     (finish - submit, seconds) and the server's counters."""
-    cfg = capi.ServerConfig(budget, max_active, round_iters, use_graph, 0)
+    cfg = capi.ServerConfig(budget, max_active, round_iters, use_graph, 0, autotune)
     h = ctypes.c_void_p()
     capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gpumlib_nmf_server_create")
     rng = np.random.default_rng(12345)
@@ -91,7 +93,10 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
             "latency_p50_s": _percentile(lat, 50), "latency_p99_s": _percentile(lat, 99),
             "iterations_mean": float(np.mean(iters)), "converged": conv,
             "rounds": st.rounds, "peak_active": st.peak_active, "pool_high_water_bytes": st.pool_high_water,
-            "arrival_rate_per_s": rate, "round_iterations": round_iters, "graph": bool(use_graph)}
+            "arrival_rate_per_s": rate, "round_iterations": round_iters, "graph": bool(use_graph),
+            "autotune_trial_iterations": autotune, "autotunes": st.autotunes,
+            "tuned": {"target_workgroups": st.tuned_target, "pass_grid": st.tuned_pass_grid,
+                      "update_grid": st.tuned_update_grid} if st.autotunes else None}
 
 
 def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=0.0, progress=None):

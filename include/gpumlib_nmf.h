@@ -218,7 +218,9 @@ typedef struct gpumlib_nmf_batch *gpumlib_nmf_batch_handle;
 
 hipError_t gpumlib_nmf_batch_create(int max_jobs, gpumlib_nmf_batch_handle *out); /* max_jobs <= 1024 */
 void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b);
-/* Work items per problem and pass that the K split aims for (default 32); applies to problems added later. */
+/* Work items per problem and pass that the K split aims for (default 32); applies to problems added later.
+ * Workspaces (gpumlib_nmf_batch_workspace_bytes) are sized for max(target, 128) by default -- room for
+ * every candidate of the autotuner -- and for exactly this target after the call. */
 hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgroups_per_pass);
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r);
 hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r, const float *V, float *W,
@@ -234,6 +236,26 @@ hipError_t gpumlib_nmf_batch_graph_launch(gpumlib_nmf_batch_handle b, int iterat
 /* Copies the 2 * max_jobs objective slots to the host (synchronises `stream`). */
 hipError_t gpumlib_nmf_batch_objectives(gpumlib_nmf_batch_handle b, double *host_out, hipStream_t stream);
 const double *gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b);
+
+/* Autotuner: times `trial_iterations` iterations of the active mix (after one untimed iteration) for
+ * each candidate configuration -- the K-split target per problem, i.e. the depth along K of a pass
+ * work item (4, 8, 16, 32, 64 or 128 workgroups per problem and pass, those that fit every problem's
+ * workspace), then the persistent grid sizes of the grouped passes (512, 1024, 2048) and updates
+ * (256, 512, 1024) -- and keeps the fastest.  W and H of every problem are saved before and restored
+ * after each trial (a temporary device buffer of their size is allocated for that), so no problem's
+ * state changes; a re-planned problem sums its K slices in the new grouping afterwards (results within
+ * the FP32 tolerance of the untuned ones, not bit-identical to them).  The winner is cached per mix
+ * signature (the multiset of rule, n, m, r): a later call for the same mix applies it without trials.
+ * Synchronises the stream; not under graph capture.  No reference counterpart. */
+typedef struct {
+    int target_workgroups, pass_grid, update_grid;
+    double ms_per_iteration;        /* of the chosen configuration */
+    double ms_per_iteration_before; /* of the configuration in effect before the call */
+    int trials;                     /* configurations timed (0 when a cached choice was applied) */
+    int cached;
+} gpumlib_nmf_batch_tuning;
+hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int trial_iterations, hipStream_t stream,
+                                      gpumlib_nmf_batch_tuning *out);
 
 /* ------------------------------------------------------------------------ */
 /* 6. Continuous-batching server (no reference counterpart).  Any host thread */
@@ -253,6 +275,8 @@ typedef struct {
     int round_iterations;    /* iterations per scheduling round */
     int use_graph;           /* 1: replay each round from a captured hipGraph */
     int target_workgroups;   /* K-split target per problem and pass, 0 = default */
+    int autotune;            /* > 0: autotune the batch (this many timed iterations per candidate) whenever
+                                the number of resident problems has doubled or halved; 0: off */
 } gpumlib_nmf_server_config;
 typedef struct {
     int iterations;
@@ -264,6 +288,8 @@ typedef struct {
     long long submitted, completed, rounds, iterations_batched;
     int peak_active;
     size_t pool_bytes, pool_high_water;
+    long long autotunes;                                    /* autotuner calls made */
+    int tuned_target, tuned_pass_grid, tuned_update_grid;  /* configuration chosen last */
 } gpumlib_nmf_server_stats;
 
 hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config *cfg, gpumlib_nmf_server_handle *out);

@@ -148,11 +148,57 @@ def test_batch_objectives_match_single(native, orc, cuda):
     b.close()
 
 
+# --------------------------------------------------------------- autotuner --
+TUNE_TARGETS, TUNE_PASS, TUNE_UPD = {4, 8, 16, 32, 64, 128}, {512, 1024, 2048}, {256, 512, 1024}
+
+
+def test_autotune_keeps_state_and_parity(native, orc, cuda):
+    """The autotuner times candidate K-split depths and grid sizes on the live mix; W and H
+    are restored after every trial (bitwise unchanged by the call), the choice is cached
+    per mix, and the tuned configuration still matches the FP64 oracle."""
+    b = Batch(cuda, native)
+    outs = []
+    for k, (rule, n, m, r) in enumerate(JOBS):
+        V, W, H = make_job(k, rule, n, m, r)
+        slot, dW, dH = b.add(rule, n, m, r, V, W, H)
+        outs.append((rule, n, m, r, V, W, H, dW, dH))
+    b.iterate(3)
+    before = [(dW.download(np.float32), dH.download(np.float32)) for *_, dW, dH in outs]
+    t = capi.BatchTuning()
+    capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t)))
+    print(f"tuned target={t.target_workgroups} pass_grid={t.pass_grid} update_grid={t.update_grid} "
+          f"ms {t.ms_per_iteration_before:.3f} -> {t.ms_per_iteration:.3f} trials={t.trials}", flush=True)
+    assert t.cached == 0 and t.trials >= 3
+    assert t.target_workgroups in TUNE_TARGETS and t.pass_grid in TUNE_PASS and t.update_grid in TUNE_UPD
+    assert 0 < t.ms_per_iteration <= t.ms_per_iteration_before
+    for (w0, h0), (*_, dW, dH) in zip(before, outs):
+        assert np.array_equal(dW.download(np.float32), w0) and np.array_equal(dH.download(np.float32), h0)
+    t2 = capi.BatchTuning()
+    capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t2)))
+    assert t2.cached == 1 and t2.trials == 0
+    assert (t2.target_workgroups, t2.pass_grid, t2.update_grid) == (t.target_workgroups, t.pass_grid, t.update_grid)
+    b.iterate(10)
+    b.iterate(12, graph=True)  # graphs captured after tuning use the tuned grids
+    for rule, n, m, r, V, W, H, dW, dH in outs:
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=25)
+        assert rel_fro(dW.download(np.float32), w) < 1e-4
+        assert rel_fro(dH.download(np.float32), h) < 1e-4
+    b.close()
+
+
+def test_autotune_rejects_empty_batch(native, cuda):
+    b = Batch(cuda, native)
+    t = capi.BatchTuning()
+    assert native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t)) != 0
+    b.close()
+
+
 # ------------------------------------------------------------------ server --
 class Server:
-    def __init__(self, lib, budget, max_active=64, round_iters=10, graph=1):
+    def __init__(self, lib, budget, max_active=64, round_iters=10, graph=1, autotune=0):
         self.lib = lib
-        cfg = capi.ServerConfig(budget, max_active, round_iters, graph, 0)
+        cfg = capi.ServerConfig(budget, max_active, round_iters, graph, 0, autotune)
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -198,6 +244,27 @@ def test_server_matches_oracle_and_retires(native, orc, cuda):
         assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
     st = s.stats()
     assert st.completed == len(JOBS)
+    s.close()
+
+
+def test_server_with_autotuning_matches_oracle(native, orc, cuda):
+    """Autotuning inside the server (when the resident count doubles) keeps every job
+    within the FP32 tolerance of the oracle."""
+    s = Server(native, budget=512 << 20, autotune=2)
+    ids = []
+    for k in range(3):
+        for j, job in enumerate(JOBS):
+            V, W, H = make_job(10 * k + j, *job)
+            ids.append((s.submit(*job, V, W, H, tol=0.0, max_iters=30), job, V, W, H))
+    for jid, (rule, n, m, r), V, W, H in ids:
+        gw, gh, res = s.wait(jid, n, m, r)
+        assert res.iterations == 30
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=30)
+        assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
+    st = s.stats()
+    print(f"autotunes={st.autotunes} target={st.tuned_target} grids={st.tuned_pass_grid}/{st.tuned_update_grid}")
+    assert st.autotunes >= 1 and st.tuned_target in TUNE_TARGETS
     s.close()
 
 
@@ -249,7 +316,7 @@ sys.path.insert(0, {root!r})
 from gpumlib_amd import capi
 from gpumlib_amd import nmf as G
 lib = capi.load()
-cfg = capi.ServerConfig(64 << 20, 16, 10, 1, 0)
+cfg = capi.ServerConfig(64 << 20, 16, 10, 1, 0, 0)
 h = ctypes.c_void_p()
 capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)))
 ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
