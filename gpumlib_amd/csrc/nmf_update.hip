@@ -594,16 +594,17 @@ hipError_t launch_zero_i32(int* p, int n, hipStream_t st) {
 
 // Deterministic reduction of count doubles (fixed strided order, then a fixed tree).
 __global__ __launch_bounds__(NTHREADS) void k_reduce_f64(const double* __restrict__ in, int count, double* out) {
-    __shared__ double red[NTHREADS];
+    __shared__ double red[NTHREADS / 64];
     double v = 0.0;
     for (int i = threadIdx.x; i < count; i += NTHREADS) v += in[i];
-    red[threadIdx.x] = v;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);  // wave64 shuffle tree
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int off = NTHREADS / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NTHREADS / 64; w++) t += red[w];
+        out[0] = t;
     }
-    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // ---------------------------------------------------------------------------
