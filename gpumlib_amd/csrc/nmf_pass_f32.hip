@@ -17,8 +17,8 @@
 // Every workgroup has 4 waves; a wave owns 32 columns (H pass) or 32 rows
 // (W pass) of the tile and all r (padded to MB*32) rank rows, i.e. MB 32x32
 // MFMA accumulators.  K is staged through double-buffered LDS in steps of KC
-// with one barrier per step; the next step's global loads are in flight while
-// the current step's MFMAs run.
+// with one barrier per step; the next one or two steps' global loads are in
+// flight (in registers) while the current step's MFMAs run.
 #include "nmf_common.hpp"
 #include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
@@ -119,8 +119,12 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         }
     }
 
-    float4 ra[MB], rb[4];
-    auto gload = [&](int k0) __attribute__((always_inline)) {
+    // Register staging of the next chunks' global loads.  The plain Euclidean pass keeps
+    // two chunks in flight (two register sets; the LDS is double-buffered); the KL and
+    // objective variants, whose registers also hold H and the ratio, keep one.
+    constexpr bool DEEP = !KL && OBJ == OBJ_NONE && GEMM2 && sizeof(TV) == 4;
+    float4 ra0[MB], rb0[4], ra1[DEEP ? MB : 1], rb1[DEEP ? 4 : 1];
+    auto gload = [&](float4 (&rb)[4], float4 (&ra)[MB], int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
@@ -151,7 +155,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             ra[j] = load4(p.W + (size_t)a * p.n + k0 + kq, a < p.r, k0 + kq, kend, vec);
         }
     };
-    auto sstore = [&](int buf) __attribute__((always_inline)) {
+    auto sstore = [&](const float4 (&rb)[4], const float4 (&ra)[MB], int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
@@ -177,14 +181,10 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     // The gram-carrying and the plain workgroups run separate copies of the loop, so
     // that no MFMA accumulator sits under a data-dependent branch.
     const int nchunks = (kend - kbeg + KC - 1) / KC;
-    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
-    constexpr bool GR = decltype(gram_tag)::value;
-    gload(kbeg);
-    sstore(0);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ch++) {
+    // One chunk out of LDS buffer ch & 1.
+    auto chunk = [&](auto gram_tag, int ch) __attribute__((always_inline)) {
+        constexpr bool GR = decltype(gram_tag)::value;
         const int buf = ch & 1;
-        if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
 
         if (do_sums) {
 #pragma unroll
@@ -248,9 +248,40 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             gacc = mfma32x2(av.z, bv.z, gacc);
             gacc = mfma32x2(av.w, bv.w, gacc);
         }
-        if (ch + 1 < nchunks) sstore(buf ^ 1);
-        __syncthreads();
-    }
+    };
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+        if constexpr (DEEP) {
+            // chunk ch is in LDS, the register set passed holds chunk ch + 1 (in flight), the
+            // other set chunk ch + 2; after the multiply chunk ch + 1 goes to the other LDS
+            // buffer and its register set takes the loads of chunk ch + 3
+            auto step = [&](int ch, float4 (&rbn)[4], float4 (&ran)[MB]) __attribute__((always_inline)) {
+                chunk(gram_tag, ch);
+                if (ch + 1 < nchunks) {
+                    sstore(rbn, ran, (ch + 1) & 1);
+                    if (ch + 3 < nchunks) gload(rbn, ran, kbeg + (ch + 3) * KC);
+                }
+                __syncthreads();
+            };
+            gload(rb0, ra0, kbeg);
+            if (nchunks > 1) gload(rb1, ra1, kbeg + KC);
+            sstore(rb0, ra0, 0);
+            __syncthreads();
+            if (nchunks > 2) gload(rb0, ra0, kbeg + 2 * KC);
+            for (int ch = 0; ch < nchunks; ch += 2) {
+                step(ch, rb1, ra1);
+                if (ch + 1 < nchunks) step(ch + 1, rb0, ra0);
+            }
+        } else {
+            gload(rb0, ra0, kbeg);
+            sstore(rb0, ra0, 0);
+            __syncthreads();
+            for (int ch = 0; ch < nchunks; ch++) {
+                if (ch + 1 < nchunks) gload(rb0, ra0, kbeg + (ch + 1) * KC);
+                chunk(gram_tag, ch);
+                if (ch + 1 < nchunks) sstore(rb0, ra0, (ch + 1) & 1);
+                __syncthreads();
+            }
+        }
     };
     if (gcar)
         mainloop(std::true_type{});
@@ -327,8 +358,11 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     constexpr int BQ = RP / 4;                // float4 per H column chunk
     constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
 
-    float4 ra[4], rb[BLOADS];
-    auto gload = [&](int k0) __attribute__((always_inline)) {
+    // two chunks of global loads in flight (two register sets, double-buffered LDS) up to
+    // r = 96; one at r = 128, where a second set would not fit two workgroups per CU
+    constexpr bool DEEP = MB <= 3;
+    float4 ra0[4], rb0[BLOADS], ra1[DEEP ? 4 : 1], rb1[DEEP ? BLOADS : 1];
+    auto gload = [&](float4 (&ra)[4], float4 (&rb)[BLOADS], int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -340,7 +374,7 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
         }
     };
-    auto sstore = [&](int buf) __attribute__((always_inline)) {
+    auto sstore = [&](const float4 (&ra)[4], const float4 (&rb)[BLOADS], int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
@@ -362,14 +396,10 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
 
     // separate loop copies for gram-carrying and plain workgroups (no MFMA under a branch)
     const int nchunks = (kend - kbeg + KC - 1) / KC;
-    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+    auto chunk = [&](auto gram_tag, int ch) __attribute__((always_inline)) {
         constexpr bool GR = decltype(gram_tag)::value;
-        gload(kbeg);
-        sstore(0);
-        __syncthreads();
-        for (int ch = 0; ch < nchunks; ch++) {
-            const int buf = ch & 1;
-            if (ch + 1 < nchunks) gload(kbeg + (ch + 1) * KC);
+        const int buf = ch & 1;
+        {
 #pragma unroll
             for (int kk = 0; kk < KC / 2; kk++) {
                 const float av = sA[buf][2 * kk + h][wave * 32 + c];
@@ -384,8 +414,37 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
                     gacc = mfma32x2(sB[buf][2 * kk + h][gmb * 32 + c], sB[buf][2 * kk + h][gnb * 32 + c], gacc);
                 }
             }
-            if (ch + 1 < nchunks) sstore(buf ^ 1);
+        }
+    };
+    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
+        if constexpr (DEEP) {  // the schedule of the H pass
+            auto step = [&](int ch, float4 (&ran)[4], float4 (&rbn)[BLOADS]) __attribute__((always_inline)) {
+                chunk(gram_tag, ch);
+                if (ch + 1 < nchunks) {
+                    sstore(ran, rbn, (ch + 1) & 1);
+                    if (ch + 3 < nchunks) gload(ran, rbn, kbeg + (ch + 3) * KC);
+                }
+                __syncthreads();
+            };
+            gload(ra0, rb0, kbeg);
+            if (nchunks > 1) gload(ra1, rb1, kbeg + KC);
+            sstore(ra0, rb0, 0);
             __syncthreads();
+            if (nchunks > 2) gload(ra0, rb0, kbeg + 2 * KC);
+            for (int ch = 0; ch < nchunks; ch += 2) {
+                step(ch, ra1, rb1);
+                if (ch + 1 < nchunks) step(ch + 1, ra0, rb0);
+            }
+        } else {
+            gload(ra0, rb0, kbeg);
+            sstore(ra0, rb0, 0);
+            __syncthreads();
+            for (int ch = 0; ch < nchunks; ch++) {
+                if (ch + 1 < nchunks) gload(ra0, rb0, kbeg + (ch + 1) * KC);
+                chunk(gram_tag, ch);
+                if (ch + 1 < nchunks) sstore(ra0, rb0, (ch + 1) & 1);
+                __syncthreads();
+            }
         }
     };
     if (gcar)
