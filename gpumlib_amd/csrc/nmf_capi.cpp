@@ -222,6 +222,10 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
     return launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st);
 }
 
+// Slice grams the low-precision updates sum themselves (0: the pass reduces them into G
+// at its tail).  Register-staged grams take up to 4 slices of r <= 64.
+static int update_gram_slices(int S, int r) { return (S <= 4 && r <= 64) ? S : 0; }
+
 // BF16 storage: V store = [V (ld ldv8) | V^T (ld ldt8)] in BF16.  W and H stay FP32
 // (the updates and their ratios are FP32); BF16 shadows feed the MFMA passes.
 hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
@@ -266,14 +270,19 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     aw.S = pl.S3;
     aw.tiles = pl.tiles3;
     aw.pstride = pl.pstride3;
+    const int gs1 = update_gram_slices(pl.S1, pl.r), gs3 = update_gram_slices(pl.S3, pl.r);
+    ah.gkeep = gs1 > 0;
+    aw.gkeep = gs3 > 0;
+    const float* G1 = gs1 ? P + (size_t)pl.m * pl.ldr : ah.G;  // slice 0's gram columns, or the reduced gram
+    const float* G3 = gs3 ? P + (size_t)pl.n * pl.ldr : aw.G;
     for (int it = 0; it < iterations; it++) {
         if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
             CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
         CK(launch_pass_tn_bf16(ah, st));
-        CK(launch_update_h_euc(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hb, pl.ldt8, st));
+        CK(launch_update_h_euc(H, P, G1, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hb, pl.ldt8, st, 0, gs1));
         if (!update_w) continue;
         CK(launch_pass_tn_bf16(aw, st));
-        CK(launch_update_w_euc_t(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st));
+        CK(launch_update_w_euc_t(W, P, G3, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st, 0, gs3));
     }
     return hipSuccess;
 }
@@ -320,14 +329,21 @@ hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, floa
     CK(launch_shadow_fp8(W, pl.n, pl.m, pl.r, 0, Wq, pl.ld16n, Wsc, st));
     CK(launch_shadow_fp8(H, pl.n, pl.m, pl.r, 1, Hq, pl.ld16m, Hsc, st));
     CK(launch_zero_i32(ah.gcount, 2, st));
+    const int gs1 = update_gram_slices(pl.S1, pl.r), gs3 = update_gram_slices(pl.S3, pl.r);
+    ah.gkeep = gs1 > 0;
+    aw.gkeep = gs3 > 0;
+    const float* G1 = gs1 ? P + (size_t)pl.m * pl.ldr : ah.G;  // slice 0's gram columns, or the reduced gram
+    const float* G3 = gs3 ? P + (size_t)pl.n * pl.ldr : aw.G;
     for (int it = 0; it < iterations; it++) {
         if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
             CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
         CK(launch_pass_tn_fp8(ah, st));
-        CK(launch_update_h_euc_q8(H, P, ah.G, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hq, pl.ld16m, Hsc, st));
+        CK(launch_update_h_euc_q8(H, P, G1, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hq, pl.ld16m, Hsc, st,
+                                  gs1));
         if (!update_w) continue;
         CK(launch_pass_tn_fp8(aw, st));
-        CK(launch_update_w_euc_t_q8(W, P, aw.G, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wq, pl.ld16n, Wsc, st));
+        CK(launch_update_w_euc_t_q8(W, P, G3, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wq, pl.ld16n, Wsc, st,
+                                    gs3));
     }
     return hipSuccess;
 }

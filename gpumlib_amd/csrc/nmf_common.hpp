@@ -117,8 +117,12 @@ __device__ void gram_finish(const float* P, long pstride, int S, int arrivals, i
 
 // Same for a gram stored contiguously in every slice: P points at slice 0's
 // gram, count floats (a multiple of 4, 16-byte aligned) per slice.  Each thread
-// sums two float4 groups, 4 slices per step (8 loads of 16 bytes in flight),
-// adding the slices in order.
+// sums up to QJ float4 groups, 4 slices per step (4 QJ loads of 16 bytes in
+// flight), adding the slices in order.  The last-arriving workgroup runs this
+// while its slot could start another tile, so its round trips lengthen the pass:
+// QJ = 8 takes an r = 64 gram over 8 slices in two round trips, for 128 VGPRs,
+// which only kernels already held to 2 waves per SIMD can spend.
+template <int QJ = 2>
 __device__ inline void gram_finish4(const float* P, long pstride, int S, int arrivals, int count, float* G,
                                     int* counter) {
     __shared__ int last;
@@ -131,30 +135,36 @@ __device__ inline void gram_finish4(const float* P, long pstride, int S, int arr
     const int nq = count >> 2, stride = blockDim.x;
     const float4* P4 = reinterpret_cast<const float4*>(P);
     const long ps4 = pstride >> 2;
-    for (int q0 = threadIdx.x; q0 < nq; q0 += 2 * stride) {
-        const int q1 = (q0 + stride < nq) ? q0 + stride : q0;
-        float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
+    for (int q0 = threadIdx.x; q0 < nq; q0 += QJ * stride) {
+        int q[QJ];
+        float4 g[QJ];
+#pragma unroll
+        for (int j = 0; j < QJ; j++) {
+            q[j] = (q0 + j * stride < nq) ? q0 + j * stride : q0;
+            g[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         int s = 0;
         for (; s + 4 <= S; s += 4) {
-            float4 x[4], y[4];
+            float4 x[4][QJ];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                x[j] = P4[(size_t)(s + j) * ps4 + q0];
-                y[j] = P4[(size_t)(s + j) * ps4 + q1];
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < QJ; j++) x[i][j] = P4[(size_t)(s + i) * ps4 + q[j]];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < QJ; j++)
+                    g[j].x += x[i][j].x, g[j].y += x[i][j].y, g[j].z += x[i][j].z, g[j].w += x[i][j].w;
+        }
+        for (; s < S; s++)
+#pragma unroll
+            for (int j = 0; j < QJ; j++) {
+                const float4 x = P4[(size_t)s * ps4 + q[j]];
+                g[j].x += x.x, g[j].y += x.y, g[j].z += x.z, g[j].w += x.w;
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                g0.x += x[j].x, g0.y += x[j].y, g0.z += x[j].z, g0.w += x[j].w;
-                g1.x += y[j].x, g1.y += y[j].y, g1.z += y[j].z, g1.w += y[j].w;
-            }
-        }
-        for (; s < S; s++) {
-            const float4 x = P4[(size_t)s * ps4 + q0], y = P4[(size_t)s * ps4 + q1];
-            g0.x += x.x, g0.y += x.y, g0.z += x.z, g0.w += x.w;
-            g1.x += y.x, g1.y += y.y, g1.z += y.z, g1.w += y.w;
-        }
-        reinterpret_cast<float4*>(G)[q0] = g0;
-        if (q0 + stride < nq) reinterpret_cast<float4*>(G)[q0 + stride] = g1;
+        for (int j = 0; j < QJ; j++)
+            if (q0 + j * stride < nq) reinterpret_cast<float4*>(G)[q0 + j * stride] = g[j];
     }
     if (threadIdx.x == 0) atomicExch(counter, 0);
 }
@@ -201,6 +211,7 @@ struct Fp8PassArgs {
     int bcol_shift, bs_col, bs_k;
     float* G;
     int* gcount;
+    int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
 };
 
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
@@ -216,6 +227,7 @@ struct LpPassArgs {
     long pstride;
     float* G;
     int* gcount;
+    int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
 };
 
 // One problem of a grouped (batched) launch: the arguments of its two passes and

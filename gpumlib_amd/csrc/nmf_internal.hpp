@@ -28,11 +28,13 @@ hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
 // Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
-// G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a].
+// G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a];  with
+// gsl in [1, 4] (r <= 64), the first of the gsl slice grams of the pass (stride pstride),
+// which the update sums in slice order.
 // add != 0 (every Euclidean / KL update below): the additive-update rule of
 // NMF_AdditiveEuclidian / NMF_AdditiveDivergence in place of the multiplicative one.
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add = 0);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add = 0, int gsl = 0);
 // Euclidean W:  W(i,a) *= VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)
 // G: the reduced gram H H^T written by the W pass, HHt(a,b) = G[b*rpad + a].
 hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
@@ -40,13 +42,15 @@ hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, 
 // Euclidean W from partials laid out P[s][i*ldr + a] (gram rows at i >= n), as the
 // BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add = 0);
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add = 0, int gsl = 0);
 // FP8 storage path: the same updates, also writing the E4M3 shadow of the new factor
 // (one E8M0 scale per 32 rows along the contraction).
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
-                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st);
+                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st,
+                                  int gsl = 0);
 hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, int n, int r, int ldr, int S,
-                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st);
+                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st,
+                                    int gsl = 0);
 // Zeroes n ints (arrival counters) with a kernel, so that it is safe inside captured graphs.
 hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 // KL H:  H(a,u) *= num(a,u) / max(sum_s Psum[s][a], eps)

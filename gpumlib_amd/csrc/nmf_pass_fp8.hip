@@ -216,7 +216,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                         make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
             }
         }
-        gram_finish4(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G, p.gcount);
+        if (!p.gkeep)
+            gram_finish4<(MB >= 2 ? 8 : 2)>(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G,
+                                            p.gcount);
     }
 }
 

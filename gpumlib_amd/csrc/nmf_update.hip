@@ -209,6 +209,52 @@ struct RegStage {
     }
 };
 
+// Register staging in 16-byte groups: group g covers elements 4g .. 4g+3.  ptr(g) is
+// the group's 16-byte-aligned address (nullptr: zeros); mask(g, j) says whether
+// element j of the group exists (the others are staged as zeros).
+template <int U>
+struct RegStage4 {
+    float4 v[U];
+    // v = sum over ns <= 4 slices (stride ps floats) of the group at ptr(g), in slice
+    // order; all 4 U loads are issued before the first add.
+    template <typename Ptr, typename Mask>
+    __device__ __forceinline__ void load_sum(int groups, int ns, long ps, Ptr ptr, Mask mask) {
+        float4 x[U][4];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int g = threadIdx.x + u * NTHREADS;
+            const float* q = g < groups ? ptr(g) : nullptr;
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++)
+                x[u][sl] = (q && sl < ns) ? *reinterpret_cast<const float4*>(q + (size_t)sl * ps)
+                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            float4 t = x[u][0];
+#pragma unroll
+            for (int sl = 1; sl < 4; sl++)
+                if (sl < ns) t.x += x[u][sl].x, t.y += x[u][sl].y, t.z += x[u][sl].z, t.w += x[u][sl].w;
+            const int g = threadIdx.x + u * NTHREADS;
+            v[u] = make_float4(mask(g, 0) ? t.x : 0.f, mask(g, 1) ? t.y : 0.f, mask(g, 2) ? t.z : 0.f,
+                               mask(g, 3) ? t.w : 0.f);
+        }
+    }
+    template <typename St>
+    __device__ __forceinline__ void store(int groups, St st) const {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int g = threadIdx.x + u * NTHREADS;
+            if (g < groups) {
+                st(4 * g, v[u].x);
+                st(4 * g + 1, v[u].y);
+                st(4 * g + 2, v[u].z);
+                st(4 * g + 3, v[u].w);
+            }
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Single-problem Euclidean updates, tiled so that every thread issues all of its
 // slice loads before it waits (the update is a short, latency-bound pass over
@@ -230,7 +276,8 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
                                                               int ldg, int gt, int count, int r, int ldr, int S,
                                                               long pstride, float eps, uint16_t* __restrict__ Sh,
                                                               int ldsh, uint8_t* __restrict__ Q, int ldq,
-                                                              uint8_t* __restrict__ Qsc, int nblk, int add) {
+                                                              uint8_t* __restrict__ Qsc, int nblk, int add,
+                                                              long gps, int gsl) {
     extern __shared__ __attribute__((aligned(16))) float smr[];
     constexpr int RS = RP + 4, NQ = RP / 4, ITEMS = JT * NQ, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
     float* sG = smr;            // sG[b*RS + a] = C(b,a), zero for a or b >= r
@@ -242,9 +289,20 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
 
     // C and the X rows, zero padded to RP.  Up to r = 64 their loads are issued first,
     // into registers, so that they and the slice loads below take one round trip.
+    // gsl = 0: G is the reduced gram; gsl in [1, 4] (r <= 64 only): G holds the grams of
+    // the pass's gsl K slices at stride gps, summed here in slice order.
     auto ld_g = [&](int idx) {  // C(b,a) in staging order: along b for gt (coalesced), along a otherwise
         const int hi = idx / RP, lo = idx - hi * RP;
         return (hi < r && lo < r) ? G[(size_t)hi * ldg + lo] : 0.f;
+    };
+    // 16-byte groups of C in the same order (RP and ldg are multiples of 4, G 16-byte aligned)
+    auto ptr_g4 = [&](int g) -> const float* {
+        const int hi = (4 * g) / RP, lo = 4 * g - hi * RP;
+        return (hi < r && lo < r) ? G + (size_t)hi * ldg + lo : nullptr;
+    };
+    auto mask_g4 = [&](int g, int j) {
+        const int hi = (4 * g) / RP, lo = 4 * g - hi * RP + j;
+        return hi < r && lo < r;
     };
     auto st_g = [&](int idx, float v) {
         const int hi = idx / RP, lo = idx - hi * RP;
@@ -268,11 +326,12 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         }
     };
     constexpr int UG = (RP * RP + NTHREADS - 1) / NTHREADS, UX = (JT * RP + NTHREADS - 1) / NTHREADS;
+    constexpr int UG4 = (RP * RP / 4 + NTHREADS - 1) / NTHREADS;
     constexpr bool EARLY = UG <= 16;
-    RegStage<EARLY ? UG : 1> stg_g;
+    RegStage4<EARLY ? UG4 : 1> stg_g;
     RegStage<EARLY ? UX : 1> stg_x;
     if constexpr (EARLY) {
-        stg_g.load(RP * RP, ld_g);
+        stg_g.load_sum(RP * RP / 4, gsl > 0 ? gsl : 1, gps, ptr_g4, mask_g4);
         stg_x.load(JT * RP, ld_x);
     }
 
@@ -301,7 +360,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
     }
     // 2. C and the X rows into LDS
     if constexpr (EARLY) {
-        stg_g.store(RP * RP, st_g);
+        stg_g.store(RP * RP / 4, st_g);
         stg_x.store(JT * RP, st_x);
     } else {
         stage_lds<8>(RP * RP, ld_g, st_g);
@@ -830,24 +889,28 @@ static hipError_t set_smem(K kernel, size_t sm) {
 template <int JT, int RP>
 static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                                 int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                                uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, hipStream_t st) {
+                                uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, long gps, int gsl,
+                                hipStream_t st) {
     const size_t sm = (size_t)(RP + 2 * JT) * (RP + 4) * sizeof(float);
     hipError_t e = set_smem(k_update_rows_euc<JT, RP>, sm);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_update_rows_euc<JT, RP>), dim3((count + JT - 1) / JT), dim3(NTHREADS), sm, st, X, xs_j, xs_a,
-                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk, add);
+                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk, add, gps, gsl);
     return hipGetLastError();
 }
 
 // Q (FP8 shadow) needs 32 rows per workgroup (one scale block); it is given for r <= 64.
+// gsl > 0 (gram slices summed by the kernel) needs the register-staged gram: r <= 64.
 static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                               int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
-                              uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, hipStream_t st) {
+                              uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, long gps, int gsl,
+                              hipStream_t st) {
     if (r < 1 || r > 128 || (Q && (r > 64 || ldq % 32 || ldq < count))) return hipErrorInvalidValue;
+    if (gsl < 0 || gsl > 4 || (gsl > 0 && r > 64)) return hipErrorInvalidValue;
     const bool wide = count >= 16384 || Q != nullptr;
 #define LR(JT, RP)                                                                                                \
     return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, \
-                                 Qsc, nblk, add, st)
+                                 Qsc, nblk, add, gps, gsl, st)
     if (r <= 32) { if (wide) LR(32, 32); LR(16, 32); }
     if (r <= 64) { if (wide) LR(32, 64); LR(16, 64); }
     LR(16, 128);
@@ -855,23 +918,29 @@ static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, co
 }
 
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add) {
-    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, add, st);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add, int gsl) {
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, add,
+                       pstride, gsl, st);
 }
 
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add) {
-    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, add, st);
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add, int gsl) {
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, add,
+                       pstride, gsl, st);
 }
 
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,
-                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st) {
-    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, nullptr, 0, Hq, ldq, Hsc, ldq / 32, 0, st);
+                                  long pstride, float eps, uint8_t* Hq, int ldq, uint8_t* Hsc, hipStream_t st,
+                                  int gsl) {
+    return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, nullptr, 0, Hq, ldq, Hsc, ldq / 32, 0,
+                       pstride, gsl, st);
 }
 
 hipError_t launch_update_w_euc_t_q8(float* W, const float* P, const float* G, int n, int r, int ldr, int S,
-                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st) {
-    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, nullptr, 0, Wq, ldq, Wsc, ldq / 32, 0, st);
+                                    long pstride, float eps, uint8_t* Wq, int ldq, uint8_t* Wsc, hipStream_t st,
+                                    int gsl) {
+    return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, nullptr, 0, Wq, ldq, Wsc, ldq / 32, 0,
+                       pstride, gsl, st);
 }
 
 template <int JT, int RP>

@@ -251,16 +251,16 @@ def test_server_with_autotuning_matches_oracle(native, orc, cuda):
     """Autotuning inside the server (when the resident count doubles) keeps every job
     within the FP32 tolerance of the oracle."""
     s = Server(native, budget=512 << 20, autotune=2)
-    ids = []
-    for k in range(3):
-        for j, job in enumerate(JOBS):
-            V, W, H = make_job(10 * k + j, *job)
-            ids.append((s.submit(*job, V, W, H, tol=0.0, max_iters=30), job, V, W, H))
+    # all inputs are made before the first submission and every job runs 8 rounds, so
+    # that at least 4 are resident at some round boundary however fast a round is
+    iters = 80
+    data = [(job, *make_job(10 * k + j, *job)) for k in range(3) for j, job in enumerate(JOBS)]
+    ids = [(s.submit(*job, V, W, H, tol=0.0, max_iters=iters), job, V, W, H) for job, V, W, H in data]
     for jid, (rule, n, m, r), V, W, H in ids:
         gw, gh, res = s.wait(jid, n, m, r)
-        assert res.iterations == 30
+        assert res.iterations == iters
         v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
-        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=30)
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS, iters=iters)
         assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
     st = s.stats()
     print(f"autotunes={st.autotunes} target={st.tuned_target} grids={st.tuned_pass_grid}/{st.tuned_update_grid}")
