@@ -34,7 +34,7 @@ static int choose_slices(int tiles, int K, int kc, int target, int* kslice) {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-size_t vstore_bf16_offset_t(int n, int m) { return align256((size_t)m * bf16_ld(n) * 2); }
+size_t vstore_bf16_offset_t(int n, int m) { return align256(bf16_tiled_elems(n, m) * 2); }
 
 Fp8Store fp8_store_layout(int n, int m) {
     Fp8Store l;
@@ -47,7 +47,7 @@ Fp8Store fp8_store_layout(int n, int m) {
 size_t vstore_bytes(int precision, int n, int m) {
     if (n < 1 || m < 1) return 0;
     if (precision == GPUMLIB_NMF_FP32) return (size_t)n * m * 4;
-    if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256((size_t)n * bf16_ld(m) * 2);
+    if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256(bf16_tiled_elems(m, n) * 2);
     if (precision == GPUMLIB_NMF_FP8) return fp8_store_layout(n, m).bytes;
     return 0;
 }
@@ -149,6 +149,7 @@ void pass_args(const Plan& pl, const float* V, const float* W, const float* H, v
     a->eps = eps;
     a->vec = (pl.n % 4 == 0) && (pl.r % 4 == 0) && aligned16(V) && aligned16(W) && aligned16(H);
     a->ldv = (pl.precision == GPUMLIB_NMF_FP32) ? pl.n : (pl.precision == GPUMLIB_NMF_BF16 ? pl.ldv8 : pl.ld16n);
+    a->vtiles = lp_tiles(pl.m);
     if (pl.precision == GPUMLIB_NMF_FP8) {
         a->vsc = (const uint8_t*)V + fp8_store_layout(pl.n, pl.m).off_sc;
         a->nbi = pl.nbi;
@@ -226,7 +227,7 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
 // at its tail).  Register-staged grams take up to 4 slices of r <= 64.
 static int update_gram_slices(int S, int r) { return (S <= 4 && r <= 64) ? S : 0; }
 
-// BF16 storage: V store = [V (ld ldv8) | V^T (ld ldt8)] in BF16.  W and H stay FP32
+// BF16 storage: V store = [V | V^T] in BF16, each tiled for its pass (bf16_tiled_index).  W and H stay FP32
 // (the updates and their ratios are FP32); BF16 shadows feed the MFMA passes.
 hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
                                int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
@@ -251,7 +252,6 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     ah.N = pl.m;
     ah.r = pl.r;
     ah.lda = pl.ldv8;
-    ah.ldb = pl.ldv8;
     ah.ldr = pl.ldr;
     ah.kslice = pl.ks1;
     ah.S = pl.S1;
@@ -265,7 +265,6 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     aw.K = pl.m;
     aw.N = pl.n;
     aw.lda = pl.ldt8;
-    aw.ldb = pl.ldt8;
     aw.kslice = pl.ks3;
     aw.S = pl.S3;
     aw.tiles = pl.tiles3;
@@ -534,11 +533,11 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         a.G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)r * pl.rpad);
         a.gcount = (int*)(base + pl.off_cnt) + which;
         if (which == 0) {
-            a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = a.ldb = pl.ldv8;
+            a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = pl.ldv8;
             a.kslice = pl.ks1, a.S = pl.S1, a.tiles = pl.tiles1, a.pstride = pl.pstride1;
         } else {
             a.A = Hb, a.B = (const uint16_t*)((const char*)V + vstore_bf16_offset_t(n, m)), a.K = m, a.N = n;
-            a.lda = a.ldb = pl.ldt8;
+            a.lda = pl.ldt8;
             a.kslice = pl.ks3, a.S = pl.S3, a.tiles = pl.tiles3, a.pstride = pl.pstride3;
         }
         return launch_pass_tn_bf16(a, stream);

@@ -33,6 +33,13 @@ constexpr int UPD_ROWS = 32;
 __host__ __device__ inline int round_up(int x, int q) { return (x + q - 1) / q * q; }
 inline int xcd_grid(int work) { return (work + 7) / 8 * 8; }
 
+// Element (k, col) of a tiled BF16 V store orientation with ntiles column tiles: blocks of
+// 64 columns x 256 contraction elements, block (k / 256, col / 64) at k / 256 * ntiles +
+// col / 64, column-major inside the block.
+__host__ __device__ inline size_t bf16_tiled_index(int k, int col, int ntiles) {
+    return (((size_t)(k >> 8) * ntiles + (col >> 6)) * 64 + (col & 63)) * 256 + (k & 255);
+}
+
 #if defined(__HIPCC__)
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -189,6 +196,7 @@ struct PassArgs {
     float eps;
     int vec;          // V, W, H rows/columns are 16-byte aligned (n % 4 == 0 and r % 4 == 0)
     int ldv;          // leading dimension of V (n for FP32 V; a multiple of 8 for BF16 storage)
+    int vtiles;       // BF16 storage: column tiles of the tiled V (ceil(m / 64))
     const uint8_t* vsc;  // FP8 storage: E8M0 scale of V's 32 x 32 tile (ub * nbi + ib)
     int nbi;             // FP8 storage: 32-row blocks of V
     float* G;            // reduced gram of this pass's factor: gram(a,b) at G[b*rpad + a]
@@ -215,14 +223,15 @@ struct Fp8PassArgs {
 };
 
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
-// A: column a at A + a*lda; B: column u at B + u*ldb for u < N; the extra gram tile
-// (u in [N, N + r)) takes its columns from A.  P[s][u*ldr + a].
+// A: column a at A + a*lda; B: the tiled BF16 store of one V orientation (element (k, u)
+// at B[bf16_tiled_index(k, u, tiles)]); the slice's gram (columns N .. N + r - 1 of P)
+// takes both operands from A.  P[s][u*ldr + a].
 struct LpPassArgs {
     const uint16_t* A;
     const uint16_t* B;
     float* P;
     int K, N, r;
-    int lda, ldb, ldr;
+    int lda, ldr;
     int kslice, S, tiles;
     long pstride;
     float* G;

@@ -74,11 +74,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
 
     u32x4 rb[BL], ra[AL];
     auto gload = [&](int k0) __attribute__((always_inline)) {
+        // the chunk's B block: 32 KB contiguous, element (kq, col) at col*KB + kq, so load q
+        // (16 bytes) lands at sB[q / LPC][(q % LPC) * 8]
+        const uint16_t* bblk = p.B + ((size_t)(k0 / KB) * p.tiles + t) * (TILE_LP * KB);
 #pragma unroll
         for (int j = 0; j < BL; j++) {
-            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 8;
-            const int cc = col < ncols ? col : ncols - 1;
-            rb[j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
+            const int q = tid + NTHREADS * j;
+            rb[j] = *reinterpret_cast<const u32x4*>(bblk + (size_t)q * 8);
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
@@ -202,25 +204,26 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
 }
 
 // ---------------------------------------------------------------------------
-// V (n x m, FP32 column-major) -> Vb (column-major, ld ldv) and Vt (V^T column-major,
-// ld ldt), both BF16 (round to nearest even).  64 x 64 tiles through LDS.
-// The grid covers the padded stores (ldv and ldt multiples of 256): padding gets zeros.
+// V (n x m, FP32 column-major) -> Vb (contraction i, columns u) and Vt (contraction u,
+// columns i), both BF16 (round to nearest even) in the tiled layout of bf16_tiled_index.
+// 64 x 64 tiles through LDS; the grid covers i < ldv and u < ldt (the padded contraction
+// lengths, multiples of 256, which also cover the padded column counts): padding gets zeros.
 __global__ __launch_bounds__(NTHREADS) void k_pack_v_bf16(const float* __restrict__ V, int n, int m,
-                                                          uint16_t* __restrict__ Vb, int ldv,
-                                                          uint16_t* __restrict__ Vt, int ldt) {
+                                                          uint16_t* __restrict__ Vb, int ntm,
+                                                          uint16_t* __restrict__ Vt, int ntn) {
     __shared__ uint16_t tile[64][66];
     const int i0 = blockIdx.x * 64, u0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     for (int uu = ty; uu < 64; uu += 4) {
         const int i = i0 + tx, u = u0 + uu;
         const uint16_t b = (i < n && u < m) ? __builtin_bit_cast(uint16_t, (__bf16)V[(size_t)u * n + i]) : 0;
-        if (u < m) Vb[(size_t)u * ldv + i] = b;
+        if (u < 64 * ntm) Vb[bf16_tiled_index(i, u, ntm)] = b;
         tile[uu][tx] = b;
     }
     __syncthreads();
     for (int ii = ty; ii < 64; ii += 4) {
         const int u = u0 + tx, i = i0 + ii;
-        if (i < n) Vt[(size_t)i * ldt + u] = tile[tx][ii];
+        if (i < 64 * ntn) Vt[bf16_tiled_index(u, i, ntn)] = tile[tx][ii];
     }
 }
 
@@ -262,9 +265,11 @@ hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st) {
 
 hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ldv, uint16_t* Vt, int ldt,
                               hipStream_t st) {
-    if (ldv % 64 || ldt % 64 || ldv < n || ldt < m) return hipErrorInvalidValue;
+    // ldv, ldt: the padded contraction lengths, whole 256-element chunks (covering the
+    // padded column counts 64 ceil(n/64), 64 ceil(m/64) as well)
+    if (ldv % 256 || ldt % 256 || ldv < n || ldt < m) return hipErrorInvalidValue;
     dim3 grid(ldv / 64, ldt / 64);
-    hipLaunchKernelGGL(k_pack_v_bf16, grid, dim3(NTHREADS), 0, st, V, n, m, Vb, ldv, Vt, ldt);
+    hipLaunchKernelGGL(k_pack_v_bf16, grid, dim3(NTHREADS), 0, st, V, n, m, Vb, (m + 63) / 64, Vt, (n + 63) / 64);
     return hipGetLastError();
 }
 

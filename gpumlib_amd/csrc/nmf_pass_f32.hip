@@ -47,7 +47,7 @@ __device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, 
     if (!ok) return v;
     uint16_t e[4];
     if (k + 3 < kend) {
-        const uint2 x = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned: leading dimension % 8 == 0
+        const uint2 x = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned: k % 4 == 0 in a tiled column
         e[0] = x.x & 0xffff, e[1] = x.x >> 16, e[2] = x.y & 0xffff, e[3] = x.y >> 16;
     } else {
         for (int j = 0; j < 4; j++) e[j] = (k + j < kend) ? p[j] : (uint16_t)0;
@@ -142,7 +142,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 }
                 rb[j] = v;
             } else if constexpr (sizeof(TV) == 2) {
-                rb[j] = load4_bf16(reinterpret_cast<const uint16_t*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,
+                rb[j] = load4_bf16(reinterpret_cast<const uint16_t*>(Vt) + bf16_tiled_index(k0 + kq, col0 + col, p.vtiles),
                                    col < ncols, k0 + kq, kend);
             } else {
                 rb[j] = load4(reinterpret_cast<const float*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,

@@ -46,7 +46,15 @@ struct Fp8Store {
 };
 Fp8Store fp8_store_layout(int n, int m);
 
-// Byte layout of the BF16 V store: V (column-major, ld bf16_ld(n)) then V^T (ld bf16_ld(m)).
+// BF16 V store: V (contraction over n, m columns), then V^T (contraction over m, n columns)
+// from a 256-byte boundary.  Each orientation is tiled for the pass that streams it: blocks
+// of 64 columns x 256 contraction elements (32 KB, column-major inside the block: 512
+// contiguous bytes per column), block (chunk c = k / 256, column tile t = col / 64) at block
+// index c * ntiles + t, ntiles = ceil(columns / 64); padding (k up to bf16_ld(K), columns
+// up to 64 ntiles) zero.  One pass chunk of a column tile is one contiguous 32 KB read.
+// (bf16_tiled_index, the element offset, is in nmf_common.hpp for the kernels.)
+inline int lp_tiles(int cols) { return (cols + 63) / 64; }
+inline size_t bf16_tiled_elems(int K, int cols) { return (size_t)bf16_ld(K) * 64 * lp_tiles(cols); }
 size_t vstore_bf16_offset_t(int n, int m);
 size_t vstore_bytes(int precision, int n, int m);
 

@@ -139,10 +139,12 @@ enum {
 };
 
 /* V store: the device form of V that gpumlib_nmf_iterate reads for a precision (no reference
- * counterpart).  GPUMLIB_NMF_FP32: V itself (float, n x m column-major).  GPUMLIB_NMF_BF16: V and V^T in
- * BF16 (round to nearest even), each column-major with its leading dimension rounded up to a multiple of
- * 256 (ldv = 256*ceil(n/256) for V, ldt = 256*ceil(m/256) for V^T), padding zero, V^T starting at a 256-byte
- * boundary; the W pass then reads V^T with unit stride.  GPUMLIB_NMF_FP8: V and V^T in OCP E4M3 with
+ * counterpart).  GPUMLIB_NMF_FP32: V itself (float, n x m column-major).  GPUMLIB_NMF_BF16: V (contraction
+ * over i, m columns u) then V^T (contraction over u, n columns i, from a 256-byte boundary) in BF16 (round to
+ * nearest even), each tiled for the pass that streams it: blocks of 64 columns x 256 contraction elements
+ * (32 KB, column-major inside the block), element (k, col) at ((k/256 * T + col/64) * 64 + col%64) * 256 +
+ * k%256 with T = ceil(columns/64), the contraction padded to a multiple of 256 and the columns to 64 T with
+ * zeros; every chunk a pass workgroup reads is one contiguous 32 KB block.  GPUMLIB_NMF_FP8: V and V^T in OCP E4M3 with
  * leading dimensions rounded up to multiples of 512 bytes (ldv = 512*ceil(n/512), ldt = 512*ceil(m/512)),
  * padding zero, 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V
  * (scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding tiles hold 127 = 2^0), shared by

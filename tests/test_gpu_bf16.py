@@ -27,6 +27,15 @@ def BF16Problem(V, W, H, n, m, r, eps=G.REFERENCE_EPS):
     return G.DeviceNMF(G.EUCLIDEAN, V, W, H, n, m, r, eps=eps, precision=G.BF16)
 
 
+def untile(raw, K, cols):
+    """One orientation of the BF16 V store -> its padded matrix M[k, col]: blocks of 64
+    columns x 256 contraction elements, block (k // 256, col // 64) at index
+    k // 256 * ntiles + col // 64, column-major inside the block."""
+    ldk, nt = (K + 255) // 256 * 256, (cols + 63) // 64
+    blocks = raw[:ldk * 64 * nt].reshape(ldk // 256, nt, 64, 256)  # [chunk, tile, column, k in chunk]
+    return blocks.transpose(0, 3, 1, 2).reshape(ldk, nt * 64)
+
+
 @pytest.mark.parametrize("nm", [(64, 48), (97, 61), (1000, 333)])
 def test_pack_bit_exact(native, cuda, nm):
     n, m = nm
@@ -34,15 +43,15 @@ def test_pack_bit_exact(native, cuda, nm):
     W, H = G.gen_init(n, m, 4, 1)
     p = BF16Problem(V, W, H, n, m, 4)
     raw = p.V.download(np.uint16)
-    # leading dimensions are whole 256-element chunks; the padding is zero
-    ldv, ldt = (n + 255) // 256 * 256, (m + 255) // 256 * 256
-    off_t = ((m * ldv * 2 + 255) // 256 * 256) // 2
+    ldv, ldt, tm, tn = (n + 255) // 256 * 256, (m + 255) // 256 * 256, (m + 63) // 64, (n + 63) // 64
+    off_t = ((ldv * 64 * tm * 2 + 255) // 256 * 256) // 2
+    assert raw.size * 2 == off_t * 2 + (ldt * 64 * tn * 2 + 255) // 256 * 256
     Vm = G.as_matrix(V, n, m)
-    vb = raw[:m * ldv].reshape(m, ldv)                 # column u of V
-    vt = raw[off_t:off_t + n * ldt].reshape(n, ldt)    # column i of V^T
-    assert np.array_equal(vb[:, :n], bf16_bits(Vm.T))
-    assert np.array_equal(vt[:, :m], bf16_bits(Vm))
-    assert not vb[:, n:].any() and not vt[:, m:].any()
+    vb = untile(raw, n, m)           # vb[i, u] = V(i, u)
+    vt = untile(raw[off_t:], m, n)   # vt[u, i] = V(i, u)
+    assert np.array_equal(vb[:n, :m], bf16_bits(Vm))
+    assert np.array_equal(vt[:m, :n], bf16_bits(Vm.T))
+    assert not vb[n:].any() and not vb[:, m:].any() and not vt[m:].any() and not vt[:, n:].any()
 
 
 def _gaps(n, m, r, iters, seed):
