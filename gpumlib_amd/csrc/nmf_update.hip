@@ -44,8 +44,7 @@ __device__ __forceinline__ float additive_kl_rule(float x, float sum, float t) {
 // WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = G[b*ldr + a] (reduced by the pass).
 __device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const float* __restrict__ P,
                                                   const float* __restrict__ G, int m, int r, int ldr, int S,
-                                                  long pstride, float eps, uint16_t* __restrict__ Hs, int ldhs,
-                                                  int blk, float* sm) {
+                                                  long pstride, float eps, int blk, float* sm) {
     const int rs = r + 1;
     float* sG = sm;           // sG[a*rs + b] = WtW(a,b)
     float* sH = sm + r * rs;  // sH[uu*rs + b] = H(b, u0 + uu)
@@ -68,60 +67,6 @@ __device__ __forceinline__ void update_h_euc_body(float* __restrict__ H, const f
         float d = 0.f;
         for (int b = 0; b < r; b++) d += sG[a * rs + b] * sH[uu * rs + b];
         H[(size_t)u * r + a] = sH[uu * rs + a] * (wtv / (d + eps));
-    }
-    if (Hs) {
-        __syncthreads();  // this workgroup's H columns are written
-        for (int idx = tid; idx < ncol * r; idx += NTHREADS) {
-            const int a = idx / ncol, uu = idx - a * ncol;
-            Hs[(size_t)a * ldhs + u0 + uu] = __builtin_bit_cast(uint16_t, (__bf16)H[(size_t)(u0 + uu) * r + a]);
-        }
-    }
-}
-
-__global__ __launch_bounds__(NTHREADS) void k_update_h_euc(float* __restrict__ H, const float* __restrict__ P,
-                                                           const float* __restrict__ G, int m, int r, int ldr, int S,
-                                                           long pstride, float eps, uint16_t* __restrict__ Hs,
-                                                           int ldhs) {
-    extern __shared__ float sm[];
-    update_h_euc_body(H, P, G, m, r, ldr, S, pstride, eps, Hs, ldhs, blockIdx.x, sm);
-}
-
-// Euclidean W update from partials in the layout of the BF16/FP8 pass over V^T:
-// VHt(i,a) = sum_s P[s][i*ldr + a];  HHt(b,a) = G[a*ldr + b].
-__global__ __launch_bounds__(NTHREADS) void k_update_w_euc_t(float* __restrict__ W, const float* __restrict__ P,
-                                                             const float* __restrict__ G, int n, int r, int ldr, int S,
-                                                             long pstride, float eps, uint16_t* __restrict__ Wb,
-                                                             int ldwb) {
-    extern __shared__ float sm[];
-    const int rs = r + 1;
-    float* sG = sm;                      // sG[b*rs + a] = HHt(b,a)
-    float* sW = sG + r * rs;             // sW[ii*rs + b] = W(i0 + ii, b)
-    float* sP = sW + UPD_ROWS * rs;      // sP[ii*rs + a] = VHt(i0 + ii, a)
-    const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * UPD_ROWS;
-    const int nrow = min(UPD_ROWS, n - i0);
-    for (int idx = tid; idx < r * r; idx += NTHREADS) {
-        const int a = idx / r, b = idx - a * r;
-        sG[b * rs + a] = G[(size_t)a * ldr + b];
-    }
-    for (int idx = tid; idx < nrow * r; idx += NTHREADS) {
-        const int ii = idx / r, a = idx - ii * r;
-        sP[ii * rs + a] = sum_slices(P + (size_t)(i0 + ii) * ldr + a, S, pstride);
-    }
-    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
-        const int ii = idx % UPD_ROWS, b = idx / UPD_ROWS;
-        if (ii < nrow) sW[ii * rs + b] = W[(size_t)b * n + i0 + ii];
-    }
-    __syncthreads();
-    for (int idx = tid; idx < UPD_ROWS * r; idx += NTHREADS) {
-        const int ii = idx % UPD_ROWS, a = idx / UPD_ROWS;
-        if (ii >= nrow) continue;
-        const int i = i0 + ii;
-        float d = 0.f;
-        for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
-        const float w = sW[ii * rs + a] * (sP[ii * rs + a] / (d + eps));
-        W[(size_t)a * n + i] = w;
-        if (Wb) Wb[(size_t)a * ldwb + i] = __builtin_bit_cast(uint16_t, (__bf16)w);
     }
 }
 
@@ -155,13 +100,6 @@ __device__ __forceinline__ void update_w_euc_body(float* __restrict__ W, const f
         for (int b = 0; b < r; b++) d += sW[ii * rs + b] * sG[b * rs + a];
         W[(size_t)a * n + i] = sW[ii * rs + a] * (vht / (d + eps));
     }
-}
-
-__global__ __launch_bounds__(NTHREADS) void k_update_w_euc(float* __restrict__ W, const float* __restrict__ P,
-                                                           const float* __restrict__ G, int n, int r, int ldp, int S,
-                                                           long pstride, float eps) {
-    extern __shared__ float sm[];
-    update_w_euc_body(W, P, G, n, r, ldp, S, pstride, eps, blockIdx.x, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,7 +207,7 @@ struct RegStage4 {
 // Sh (optional) receives the BF16 shadow of the new X: X(j,a) at Sh[a*ldsh + j].
 // Each thread owns (row jj, 4 consecutive a); JT rows per workgroup.  RP is r rounded
 // up to 32: the staged C and X are zero beyond r, so the inner product runs over RP
-// terms, fully unrolled (the added terms are exact zeros).
+// terms (the added terms are exact zeros), on the matrix cores.
 template <int JT, int RP>
 __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict__ X, long xs_j, long xs_a,
                                                               const float* __restrict__ P, const float* __restrict__ G,
@@ -367,19 +305,39 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         stage_lds<8>(JT * RP, ld_x, st_x);
     }
     __syncthreads();
-    // 3. denominators and new values
+    // 3. denominators D = X C on the matrix cores: 32 x 32 blocks (row block jb of the
+    // tile, column block ab of a), one wave per block, the whole contraction in order
+    // (32x32x2 FP32 MFMAs: exact products, FP32 accumulation); rows past JT are zeros.
+    // D goes to sN, where each item then replaces its own d by the new value.
+    {
+        constexpr int NJB = (JT + 31) / 32, NAB = RP / 32;
+        const int lane = tid & 63, c = lane & 31, h = lane >> 5;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        for (int blk = wave; blk < NJB * NAB; blk += NTHREADS / 64) {
+            const int jb = blk / NAB, ab = blk - jb * NAB;
+            const int row = jb * 32 + c;
+            const float* xa = sX + row * RS + h;
+            const float* gb = sG + h * RS + ab * 32 + c;
+            f32x16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[v] = 0.f;
+#pragma unroll 8
+            for (int k = 0; k < RP; k += 2) acc = mfma32x2(row < JT ? xa[k] : 0.f, gb[k * RS], acc);
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = jb * 32 + acc_row(v, h);
+                if (i < JT) sN[i * RS + ab * 32 + c] = acc[v];
+            }
+        }
+    }
+    __syncthreads();
+    // 4. new values
 #pragma unroll
     for (int k = 0; k < MAXI; k++) {
         const int item = tid + k * NTHREADS, jj = item / NQ, q = item - jj * NQ;
         if (item >= ITEMS || jj >= nj) continue;
-        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
         const float* xr = sX + jj * RS;
-#pragma unroll 16
-        for (int b = 0; b < RP; b++) {
-            const float4 g = *reinterpret_cast<const float4*>(sG + b * RS + 4 * q);
-            const float xb = xr[b];
-            d.x += xb * g.x, d.y += xb * g.y, d.z += xb * g.z, d.w += xb * g.w;
-        }
+        const float4 d = *reinterpret_cast<const float4*>(sN + jj * RS + 4 * q);
         const float4 xo = *reinterpret_cast<const float4*>(xr + 4 * q);
         float4 o;
         if (add) {
@@ -396,7 +354,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         *reinterpret_cast<float4*>(sN + jj * RS + 4 * q) = o;
     }
     __syncthreads();
-    // 4. coalesced writes of X and of its shadow
+    // 5. coalesced writes of X and of its shadow
     if (xs_a == 1) {
         for (int idx = tid; idx < nj * r; idx += NTHREADS) {
             const int jj = idx / r, a = idx - jj * r;
@@ -816,7 +774,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_update_h(const JobArgs* __re
         const PassArgs& p = ja.h;
         __syncthreads();
         if (ja.rule == 0)
-            update_h_euc_body(ja.H, p.P, p.G, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, nullptr, 0, item - first[j], sm);
+            update_h_euc_body(ja.H, p.P, p.G, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm);
         else
             update_h_kl_body(ja.H, p.P, p.Psum, p.m, p.r, p.ldr, p.S, p.pstride, p.eps, item - first[j], sm, 0);
     }
