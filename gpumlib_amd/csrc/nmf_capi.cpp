@@ -38,8 +38,8 @@ size_t vstore_bf16_offset_t(int n, int m) { return align256(bf16_tiled_elems(n, 
 
 Fp8Store fp8_store_layout(int n, int m) {
     Fp8Store l;
-    l.off_t = align256((size_t)m * fp8_ld(n));
-    l.off_sc = l.off_t + align256((size_t)n * fp8_ld(m));
+    l.off_t = align256((size_t)fp8_ld(n) * 64 * lp_tiles(m));
+    l.off_sc = l.off_t + align256((size_t)fp8_ld(m) * 64 * lp_tiles(n));
     l.bytes = l.off_sc + align256((size_t)(fp8_ld(n) / 32) * (fp8_ld(m) / 32));
     return l;
 }
@@ -301,14 +301,14 @@ static void fp8_args(const Plan& pl, const void* Vstore, void* ws, int which, Fp
         a->A = (const uint8_t*)(base + pl.off_Wb);
         a->Asc = (const uint8_t*)(base + pl.off_Wsc);
         a->B = (const uint8_t*)Vstore;
-        a->K = pl.n, a->N = pl.m, a->lda = a->ldb = pl.ld16n, a->nbk = pl.nbi;
+        a->K = pl.n, a->N = pl.m, a->lda = pl.ld16n, a->nbk = pl.nbi;
         a->kslice = pl.ks1, a->S = pl.S1, a->tiles = pl.tiles1, a->pstride = pl.pstride1;
         a->bs_col = pl.nbi, a->bs_k = 1;  // tile scale Vsc[ub * nbi + ib], column u -> ub
     } else {  // W pass: A = H^T (blocks along u), B = V^T (columns i, K = u)
         a->A = (const uint8_t*)(base + pl.off_Hb);
         a->Asc = (const uint8_t*)(base + pl.off_Hsc);
         a->B = (const uint8_t*)Vstore + L.off_t;
-        a->K = pl.m, a->N = pl.n, a->lda = a->ldb = pl.ld16m, a->nbk = pl.nbu;
+        a->K = pl.m, a->N = pl.n, a->lda = pl.ld16m, a->nbk = pl.nbu;
         a->kslice = pl.ks3, a->S = pl.S3, a->tiles = pl.tiles3, a->pstride = pl.pstride3;
         a->bs_col = 1, a->bs_k = pl.nbi;  // column i -> ib, K block -> ub
     }

@@ -40,6 +40,11 @@ __host__ __device__ inline size_t bf16_tiled_index(int k, int col, int ntiles) {
     return (((size_t)(k >> 8) * ntiles + (col >> 6)) * 64 + (col & 63)) * 256 + (k & 255);
 }
 
+// The same for the FP8 V store: blocks of 64 columns x 512 contraction elements (32 KB).
+__host__ __device__ inline size_t fp8_tiled_index(int k, int col, int ntiles) {
+    return (((size_t)(k >> 9) * ntiles + (col >> 6)) * 64 + (col & 63)) * 512 + (k & 511);
+}
+
 #if defined(__HIPCC__)
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -212,7 +217,7 @@ struct Fp8PassArgs {
     const uint8_t* Bsc;
     float* P;
     int K, N, r;
-    int lda, ldb, ldr;
+    int lda, ldr;     // A: column a at A + a*lda;  B: tiled (fp8_tiled_index, `tiles` column tiles)
     int kslice, S, tiles;
     long pstride;
     int nbk;

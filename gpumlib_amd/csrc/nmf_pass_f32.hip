@@ -133,7 +133,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 const int kk = k0 + kq, u = col0 + col;
                 float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (col < ncols && kk < kend) {
-                    const uint8_t* q8 = reinterpret_cast<const uint8_t*>(Vt) + (size_t)u * p.ldv + kk;
+                    const uint8_t* q8 = reinterpret_cast<const uint8_t*>(Vt) + fp8_tiled_index(kk, u, p.vtiles);
                     const float sc = e8m0_value(p.vsc[(size_t)(u >> 5) * p.nbi + (kk >> 5)]);
                     v.x = e4m3_to_f32(q8[0]) * sc;
                     if (kk + 1 < kend) v.y = e4m3_to_f32(q8[1]) * sc;

@@ -81,11 +81,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     uint32_t rsa = 0;   // 4 A scales (row tid/4, blocks 4*(tid%4) .. +3)
     uint8_t rsb = 127;  // one B scale (column block tid/16, block tid%16), threads < 32
     auto gload = [&](int k0) __attribute__((always_inline)) {
+        // the chunk's B block: 32 KB contiguous, element (kq, col) at col*K8 + kq, so load q
+        // (16 bytes) lands at sB[q / LPC][(q % LPC) * 16]
+        const uint8_t* bblk = p.B + ((size_t)(k0 / K8) * p.tiles + t) * (TILE_LP * K8);
 #pragma unroll
         for (int j = 0; j < BL; j++) {
-            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 16;
-            const int cc = col < ncols ? col : ncols - 1;
-            rb[j] = *reinterpret_cast<const u32x4*>(p.B + (size_t)(col0 + cc) * p.ldb + k0 + kq);
+            const int q = tid + NTHREADS * j;
+            rb[j] = *reinterpret_cast<const u32x4*>(bblk + (size_t)q * 16);
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
@@ -227,8 +229,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
 // E8M0 scale per 32 x 32 tile: Vsc[ub * nbi + ib] (nbi = ldv / 32).  One workgroup per
 // tile of the padded stores; tiles past V's edges get zeros and the scale 127 (2^0).
 __global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict__ V, int n, int m,
-                                                         uint8_t* __restrict__ Vq, int ldv, uint8_t* __restrict__ Vtq,
-                                                         int ldt, uint8_t* __restrict__ Vsc, int nbi) {
+                                                         uint8_t* __restrict__ Vq, int ntm, uint8_t* __restrict__ Vtq,
+                                                         int ntn, uint8_t* __restrict__ Vsc, int nbi) {
     __shared__ float tile[32][33];
     __shared__ float red[NTHREADS / 64];
     __shared__ int sscale;
@@ -257,11 +259,11 @@ __global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict
     // the grid covers the padded stores (ldv x ldt): padding gets zeros (tile values 0)
     for (int uu = ty; uu < 32; uu += 8) {
         const int i = i0 + tx, u = u0 + uu;
-        if (u < m) Vq[(size_t)u * ldv + i] = f32_to_e4m3(e8m0_unscale(tile[uu][tx], sc));
+        if (u < ntm * 64) Vq[fp8_tiled_index(i, u, ntm)] = f32_to_e4m3(e8m0_unscale(tile[uu][tx], sc));
     }
     for (int ii = ty; ii < 32; ii += 8) {
         const int u = u0 + tx, i = i0 + ii;
-        if (i < n) Vtq[(size_t)i * ldt + u] = f32_to_e4m3(e8m0_unscale(tile[tx][ii], sc));
+        if (i < ntn * 64) Vtq[fp8_tiled_index(u, i, ntn)] = f32_to_e4m3(e8m0_unscale(tile[tx][ii], sc));
     }
 }
 
@@ -305,9 +307,12 @@ hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st) {
 
 hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv, uint8_t* Vtq, int ldt, uint8_t* Vsc,
                              hipStream_t st) {
-    if (ldv % 32 || ldt % 32 || ldv < n || ldt < m) return hipErrorInvalidValue;
+    // ldv, ldt: the padded contraction lengths, whole 512-byte chunks (covering the padded
+    // column counts 64 ceil(n/64), 64 ceil(m/64) as well)
+    if (ldv % 512 || ldt % 512 || ldv < n || ldt < m) return hipErrorInvalidValue;
     const int nbi = ldv / 32, nbu = ldt / 32;
-    hipLaunchKernelGGL(k_pack_v_fp8, dim3(nbi, nbu), dim3(NTHREADS), 0, st, V, n, m, Vq, ldv, Vtq, ldt, Vsc, nbi);
+    hipLaunchKernelGGL(k_pack_v_fp8, dim3(nbi, nbu), dim3(NTHREADS), 0, st, V, n, m, Vq, (m + 63) / 64, Vtq,
+                       (n + 63) / 64, Vsc, nbi);
     return hipGetLastError();
 }
 

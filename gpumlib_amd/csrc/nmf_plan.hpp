@@ -39,8 +39,10 @@ struct Plan {
 inline int bf16_ld(int k) { return (k + 255) / 256 * 256; }
 inline int fp8_ld(int k) { return (k + 511) / 512 * 512; }
 
-// Byte layout of the FP8 V store: Vq (m columns, ld fp8_ld(n)) | Vtq (n columns, ld fp8_ld(m)) |
-// tile scales (fp8_ld(m)/32 x fp8_ld(n)/32, padding tiles 127 = 2^0).
+// Byte layout of the FP8 V store: Vq (contraction i, m columns) | Vtq (contraction u, n columns),
+// each tiled in blocks of 64 columns x 512 contraction bytes (fp8_tiled_index; contraction padded
+// to fp8_ld, columns to 64 ceil(cols/64), zeros) | tile scales (fp8_ld(m)/32 x fp8_ld(n)/32,
+// padding tiles 127 = 2^0).
 struct Fp8Store {
     size_t off_t, off_sc, bytes;
 };

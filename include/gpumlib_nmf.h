@@ -144,11 +144,12 @@ enum {
  * nearest even), each tiled for the pass that streams it: blocks of 64 columns x 256 contraction elements
  * (32 KB, column-major inside the block), element (k, col) at ((k/256 * T + col/64) * 64 + col%64) * 256 +
  * k%256 with T = ceil(columns/64), the contraction padded to a multiple of 256 and the columns to 64 T with
- * zeros; every chunk a pass workgroup reads is one contiguous 32 KB block.  GPUMLIB_NMF_FP8: V and V^T in OCP E4M3 with
- * leading dimensions rounded up to multiples of 512 bytes (ldv = 512*ceil(n/512), ldt = 512*ceil(m/512)),
- * padding zero, 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V
- * (scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding tiles hold 127 = 2^0), shared by
- * both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run
+ * zeros; every chunk a pass workgroup reads is one contiguous 32 KB block.  GPUMLIB_NMF_FP8: V and V^T in OCP
+ * E4M3, tiled the same way in blocks of 64 columns x 512 contraction bytes (element (k, col) at ((k/512 * T +
+ * col/64) * 64 + col%64) * 512 + k%512, the contraction padded to a multiple of 512 and the columns to 64 T with
+ * zeros), 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V (ldv = 512 *
+ * ceil(n/512), ldt = 512 * ceil(m/512); scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding
+ * tiles hold 127 = 2^0), shared by both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run
  * the Euclidean rule; FP8 supports r <= 64. */
 size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,

@@ -39,6 +39,15 @@ def host_store(native, V, n, m):
     return q, sc, deq
 
 
+def untile(raw, K, cols):
+    """One orientation of the FP8 V store -> its padded matrix M[k, col]: blocks of 64
+    columns x 512 contraction bytes, block (k // 512, col // 64) at index
+    k // 512 * ntiles + col // 64, column-major inside the block."""
+    ldk, nt = (K + 511) // 512 * 512, (cols + 63) // 64
+    blocks = raw[:ldk * 64 * nt].reshape(ldk // 512, nt, 64, 512)  # [chunk, tile, column, k in chunk]
+    return blocks.transpose(0, 3, 1, 2).reshape(ldk, nt * 64)
+
+
 @pytest.mark.parametrize("nm", [(64, 64), (97, 61), (300, 170)])
 def test_fp8_pack_bit_exact(native, cuda, nm):
     n, m = nm
@@ -46,17 +55,18 @@ def test_fp8_pack_bit_exact(native, cuda, nm):
     W, H = G.gen_init(n, m, 4, 1)
     p = G.DeviceNMF(G.EUCLIDEAN, V, W, H, n, m, 4, precision=G.FP8)
     raw = p.V.download(np.uint8)
-    # leading dimensions are whole 512-byte chunks: zero padding, padding tiles scaled 127 (2^0)
-    ldn, ldm = (n + 511) // 512 * 512, (m + 511) // 512 * 512
-    off_t = (m * ldn + 255) // 256 * 256
-    off_sc = off_t + (n * ldm + 255) // 256 * 256
+    # contraction padded to whole 512-byte chunks, columns to whole 64-column tiles: zero
+    # padding; padding scale tiles 127 (2^0)
+    ldn, ldm, tm, tn = (n + 511) // 512 * 512, (m + 511) // 512 * 512, (m + 63) // 64, (n + 63) // 64
+    off_t = (ldn * 64 * tm + 255) // 256 * 256
+    off_sc = off_t + (ldm * 64 * tn + 255) // 256 * 256
     nbi, nbu = ldn // 32, ldm // 32
     q, sc, _ = host_store(native, V, n, m)
-    vq = raw[:m * ldn].reshape(m, ldn)
-    vtq = raw[off_t:off_t + n * ldm].reshape(n, ldm)
+    vq = untile(raw, n, m)            # vq[i, u]
+    vtq = untile(raw[off_t:], m, n)   # vtq[u, i]
     tab = raw[off_sc:off_sc + nbi * nbu].reshape(nbu, nbi)
-    assert np.array_equal(vq[:, :n], q.T) and not vq[:, n:].any()
-    assert np.array_equal(vtq[:, :m], q) and not vtq[:, m:].any()
+    assert np.array_equal(vq[:n, :m], q) and not vq[n:].any() and not vq[:, m:].any()
+    assert np.array_equal(vtq[:m, :n], q.T) and not vtq[m:].any() and not vtq[:, n:].any()
     assert np.array_equal(tab[:sc.shape[0], :sc.shape[1]], sc)
     pad = np.ones_like(tab, dtype=bool)
     pad[:sc.shape[0], :sc.shape[1]] = False
