@@ -80,13 +80,19 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
     // the BF16 / FP8 passes fold the whole gram into the first column tile.
     const int nbk = (r + 31) / 32, gram_tiles = (precision == GPUMLIB_NMF_FP32 && euc) ? nbk * nbk : 0;
-    const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP;  // columns (rows) per workgroup
+    // The BF16 passes take two 64-column blocks per workgroup of 8 waves for r <= 64: each A
+    // chunk (the W or H^T shadow, re-read from L2 by every workgroup) then serves twice the
+    // columns of V, halving the A loads; one such workgroup fills a CU (101 KB of LDS, the
+    // same 8 waves as two 4-wave workgroups), so the pass aims at half the workgroups.
+    pl->lp_nt = (precision == GPUMLIB_NMF_BF16 && r <= 64) ? 2 : 1;
+    const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP * pl->lp_nt;  // columns (rows) per workgroup
+    const int tgt = target / pl->lp_nt;
     const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
     pl->tiles1 = vt1 > gram_tiles ? vt1 : gram_tiles;
-    pl->S1 = choose_slices(pl->tiles1, n, kc, target, &pl->ks1);
+    pl->S1 = choose_slices(pl->tiles1, n, kc, tgt, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
     pl->tiles3 = vt3 > gram_tiles ? vt3 : gram_tiles;
-    pl->S3 = choose_slices(pl->tiles3, m, kc, target, &pl->ks3);
+    pl->S3 = choose_slices(pl->tiles3, m, kc, tgt, &pl->ks3);
     // FP32 W pass writes P[s][a*ldp + i]; the BF16 pass over V^T writes P[s][i*ldr + a].
     pl->pstride3 = (precision == GPUMLIB_NMF_FP32) ? (long)r * pl->ldp : (long)(n + r) * pl->ldr;
     const size_t p1 = (size_t)pl->S1 * pl->pstride1, p3 = (size_t)pl->S3 * pl->pstride3;
@@ -256,6 +262,7 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     ah.kslice = pl.ks1;
     ah.S = pl.S1;
     ah.tiles = pl.tiles1;
+    ah.nt = pl.lp_nt;
     ah.pstride = pl.pstride1;
     aw = ah;
     aw.G = G + (size_t)pl.r * pl.rpad;
@@ -514,10 +521,14 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         return hipErrorInvalidValue;
     const int kl = rule_is_kl(rule);
     // The arrival counters were zeroed by gpumlib_nmf_iterate and every pass leaves them at
-    // zero, so the pass runs alone: its launch is the one being timed or profiled.
+    // zero, so the pass runs alone: its launch is the one being timed or profiled, configured
+    // as in the iteration (the narrow passes leave the slice grams to the update when the
+    // iteration does).
+    const int gslices = update_gram_slices(which == 0 ? pl.S1 : pl.S3, r);
     if (precision == GPUMLIB_NMF_FP8) {
         Fp8PassArgs a;
         fp8_args(pl, V, workspace, which, &a);
+        a.gkeep = gslices > 0;
         return launch_pass_tn_fp8(a, stream);
     }
     if (precision == GPUMLIB_NMF_BF16) {
@@ -532,6 +543,8 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         a.ldr = pl.ldr;
         a.G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)r * pl.rpad);
         a.gcount = (int*)(base + pl.off_cnt) + which;
+        a.nt = pl.lp_nt;
+        a.gkeep = gslices > 0;
         if (which == 0) {
             a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = pl.ldv8;
             a.kslice = pl.ks1, a.S = pl.S1, a.tiles = pl.tiles1, a.pstride = pl.pstride1;

@@ -238,6 +238,7 @@ struct LpPassArgs {
     int K, N, r;
     int lda, ldr;
     int kslice, S, tiles;
+    int nt;       // 64-column store blocks per workgroup tile (1, or 2 for r <= 64); tiles counts those tiles
     long pstride;
     float* G;
     int* gcount;

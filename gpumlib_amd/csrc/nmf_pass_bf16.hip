@@ -47,17 +47,23 @@ __device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
 // bounds check; lanes past the last column or rank row re-read the last valid one:
 // those products land only in output columns that are not stored and in rank rows
 // >= r of P and of the gram, which no update reads.
-template <int MB>
-__global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
+template <int MB, int NT>
+__global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassArgs p) {
+    constexpr int THR = NTHREADS * NT;                // threads: 4 waves per 64 columns
+    constexpr int NW = THR / 64;                      // waves
     constexpr int RP = MB * 32;
-    constexpr int NBLK = 2 * MB;                      // output blocks of 32 x 32
-    constexpr int BPW = NBLK >= 4 ? NBLK / 4 : 1;     // blocks per wave
-    constexpr int GB = (MB * MB + 3) / 4;             // gram blocks per wave
-    constexpr int BL = TILE_LP * KB / 8 / NTHREADS;   // 16-byte B loads per thread per chunk
-    constexpr int AL = RP * KB / 8 / NTHREADS;        // 16-byte A loads per thread per chunk
+    constexpr int TW = NT * TILE_LP;                  // columns per workgroup
+    constexpr int NCB = TW / 32;                      // column blocks of 32
+    constexpr int NBLK = MB * NCB;                    // output blocks of 32 x 32
+    constexpr int BPW = NBLK >= NW ? NBLK / NW : 1;   // blocks per wave
+    constexpr int GB = (MB * MB + NW - 1) / NW;       // gram blocks per wave
+    constexpr int BLB = TILE_LP * KB / 8 / THR;       // 16-byte loads per thread per 64-column store block
+    constexpr int BL = NT * BLB;                      // 16-byte B loads per thread per chunk
+    constexpr int AL = RP * KB / 8 / THR;             // 16-byte A loads per thread per chunk
     constexpr int LPC = KB / 8;                       // lanes per column (or row) in a load
+    static_assert(AL >= 1 && BLB >= 1, "load split");
     __shared__ __attribute__((aligned(16))) uint16_t sA[RP][LDB];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[TILE_LP][LDB];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[TW][LDB];
 
     const int total = p.tiles * p.S;
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
@@ -67,24 +73,29 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const bool gram = (t == 0) && p.G != nullptr;
-    const int col0 = t * TILE_LP;
-    const int ncols = min(TILE_LP, p.N - col0);
+    const int col0 = t * TW;
+    const int ncols = min(TW, p.N - col0);
     const int kbeg = s * p.kslice;
     const int kend = min(round_up(p.K, KB), kbeg + p.kslice);
 
     u32x4 rb[BL], ra[AL];
     auto gload = [&](int k0) __attribute__((always_inline)) {
-        // the chunk's B block: 32 KB contiguous, element (kq, col) at col*KB + kq, so load q
-        // (16 bytes) lands at sB[q / LPC][(q % LPC) * 8]
-        const uint16_t* bblk = p.B + ((size_t)(k0 / KB) * p.tiles + t) * (TILE_LP * KB);
+        // the chunk's NT store blocks of 64 columns (32 KB each, adjacent in the store: one
+        // contiguous read): element (kq, col) of block b at b*TILE_LP*KB + col*KB + kq, so load q
+        // (16 bytes) lands at sB[q / LPC][(q % LPC) * 8].  A block past the store's last (the
+        // second half of a last, odd tile) re-reads the last one: those columns are not stored.
+        const int nt64 = (p.N + TILE_LP - 1) / TILE_LP;
+        const uint16_t* crow = p.B + (size_t)(k0 / KB) * nt64 * (TILE_LP * KB);
 #pragma unroll
         for (int j = 0; j < BL; j++) {
-            const int q = tid + NTHREADS * j;
-            rb[j] = *reinterpret_cast<const u32x4*>(bblk + (size_t)q * 8);
+            const int q = tid + THR * j, tb = j / BLB;  // store block of this load
+            const int b = min(t * NT + tb, nt64 - 1);
+            rb[j] = *reinterpret_cast<const u32x4*>(crow + (size_t)b * (TILE_LP * KB) +
+                                                    (size_t)(q - tb * BLB * THR) * 8);
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
-            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 8;
+            const int q = tid + THR * j, a = q / LPC, kq = (q % LPC) * 8;
             const int aa = a < p.r ? a : p.r - 1;
             ra[j] = *reinterpret_cast<const u32x4*>(p.A + (size_t)aa * p.lda + k0 + kq);
         }
@@ -93,12 +104,12 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     auto sstore = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < BL; j++) {
-            const int q = tid + NTHREADS * j, col = q / LPC, kq = (q % LPC) * 8;
+            const int q = tid + THR * j, col = q / LPC, kq = (q % LPC) * 8;
             *reinterpret_cast<u32x4*>(&sB[col][kq]) = rb[j];
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
-            const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 8;
+            const int q = tid + THR * j, a = q / LPC, kq = (q % LPC) * 8;
             *reinterpret_cast<u32x4*>(&sA[a][kq]) = ra[j];
         }
     };
@@ -120,8 +131,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
 #pragma unroll
             for (int q = 0; q < BPW; q++) {
                 const int blk = wave * BPW + q;
-                if (NBLK % 4 == 0 || blk < NBLK) {
-                    const int mb = blk >> 1, nb = blk & 1;
+                if (NBLK % NW == 0 || blk < NBLK) {
+                    const int mb = blk / NCB, nb = blk % NCB;
                     const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
                     const u32x4 bv = *reinterpret_cast<const u32x4*>(&sB[nb * 32 + c][g * 16 + h * 8]);
                     acc[q] = mfma_bf16(av, bv, acc[q]);
@@ -130,8 +141,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
             if constexpr (GR) {
 #pragma unroll
                 for (int q = 0; q < GB; q++) {
-                    const int blk = wave + 4 * q;
-                    if ((MB * MB) % 4 == 0 || blk < MB * MB) {
+                    const int blk = wave + NW * q;
+                    if ((MB * MB) % NW == 0 || blk < MB * MB) {
                         const int mb = blk / MB, nb = blk - (blk / MB) * MB;
                         const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
                         const u32x4 bv = *reinterpret_cast<const u32x4*>(&sA[nb * 32 + c][g * 16 + h * 8]);
@@ -169,7 +180,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
     for (int q = 0; q < BPW; q++) {
         const int blk = wave * BPW + q;
         if (blk >= NBLK) continue;
-        const int mb = blk >> 1, ul = (blk & 1) * 32 + c;
+        const int mb = blk / NCB, ul = (blk % NCB) * 32 + c;
         if (ul >= ncols) continue;
         float* dst = slice + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
@@ -184,7 +195,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_bf16(LpPassArgs p) {
         // gacc[q][v] = gram(a = mb*32 + acc_row(v,h), b = nb*32 + c), stored at column N + b
 #pragma unroll
         for (int q = 0; q < GB; q++) {
-            const int blk = wave + 4 * q;
+            const int blk = wave + NW * q;
             if (blk >= MB * MB) continue;
             const int mb = blk / MB, b = (blk - (blk / MB) * MB) * 32 + c;
             if (b >= p.r) continue;
@@ -245,19 +256,19 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_bf16(const float* __restric
 }
 
 // ---------------------------------------------------------------------------
-template <int MB>
+template <int MB, int NT>
 static void launch_tn(const LpPassArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_pass_tn_bf16<MB>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS), 0, st, a);
+    hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
 }
 
 hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st) {
-    const int rp = round_up(a.r, 32);
+    const int rp = round_up(a.r, 32), nt = a.nt == 2 ? 2 : 1;
     if (rp <= 32)
-        launch_tn<1>(a, st);
+        nt == 2 ? launch_tn<1, 2>(a, st) : launch_tn<1, 1>(a, st);
     else if (rp <= 64)
-        launch_tn<2>(a, st);
-    else if (rp <= 128)
-        launch_tn<4>(a, st);
+        nt == 2 ? launch_tn<2, 2>(a, st) : launch_tn<2, 1>(a, st);
+    else if (rp <= 128 && nt == 1)
+        launch_tn<4, 1>(a, st);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

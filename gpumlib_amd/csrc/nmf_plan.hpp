@@ -24,6 +24,7 @@ struct Plan {
     int rule, precision, n, m, r;
     int npad, rpad, ldr, ldp;
     int ldv8, ldt8;       // BF16 storage: leading dimensions of V and V^T (bf16_ld of n and of m)
+    int lp_nt;            // BF16 passes: 64-column blocks per workgroup (1 or 2)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;

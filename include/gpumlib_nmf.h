@@ -175,7 +175,7 @@ hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, c
 /* Runs one contraction pass of the fused iteration alone (which = 0: the pass over the rows of V that
  * feeds the H update; which = 1: the pass over the columns of V that feeds the W update), writing only
  * workspace partials; W and H are unchanged.  For per-kernel timing and profiling: exactly one kernel is
- * launched.  The workspace must have been used by a gpumlib_nmf_iterate call for the same problem (it
+ * launched, configured as gpumlib_nmf_iterate launches it.  The workspace must have been used by a gpumlib_nmf_iterate call for the same problem (it
  * zeroes the arrival counters of the passes, which every pass leaves at zero; with BF16 / FP8 storage the
  * pass also reads the narrow copies of W and H that call left in `workspace`). */
 hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, int r, const void *V, const float *W,
