@@ -200,8 +200,8 @@ struct PassArgs {
     long pstride;     // floats per slice of P
     float eps;
     int vec;          // V, W, H rows/columns are 16-byte aligned (n % 4 == 0 and r % 4 == 0)
-    int ldv;          // leading dimension of V (n for FP32 V; a multiple of 8 for BF16 storage)
-    int vtiles;       // BF16 storage: column tiles of the tiled V (ceil(m / 64))
+    int ldv;          // leading dimension of V (FP32 V: n)
+    int vtiles;       // BF16 / FP8 storage: column tiles of the tiled V (ceil(m / 64))
     const uint8_t* vsc;  // FP8 storage: E8M0 scale of V's 32 x 32 tile (ub * nbi + ib)
     int nbi;             // FP8 storage: 32-row blocks of V
     float* G;            // reduced gram of this pass's factor: gram(a,b) at G[b*rpad + a]

@@ -1,6 +1,6 @@
-// BF16 storage path: V held in BF16 in both orientations (V and V^T, each
-// column-major, leading dimensions whole 256-element chunks, zero padded), BF16 shadows of W and
-// H as MFMA operands, FP32 accumulation and FP32 updates.
+// BF16 storage path: V held in BF16 in both orientations (V and V^T, each tiled in
+// blocks of 64 columns x 256 contraction elements, zero padded: bf16_tiled_index), BF16
+// shadows of W and H as MFMA operands, FP32 accumulation and FP32 updates.
 //
 // With V^T stored, the W half of an iteration is the H half of the transposed
 // problem (V^T, H^T, W^T): U_W(V, W, H) = U_H(V^T, H^T, W^T)^T.  So both
@@ -29,15 +29,18 @@ __device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
                                                    0, 0, 0);
 }
 
-// One workgroup per (slice s, column tile t) of TILE_LP = 64 columns.  Every chunk is
-// 256 elements deep, so each column is read in 512-byte runs (long runs keep HBM pages
-// open; 128-byte runs over 128 columns measured markedly slower).  The chunk's B
-// columns and A rows are staged through LDS; the next chunk's loads are in flight in
-// registers while the current one is multiplied.  The 2*MB output blocks of 32 x 32
-// (rank block mb, column block nb) are spread over the 4 waves.
+// One workgroup per (slice s, column tile t) of NT * 64 columns, 4 NT waves.  Every
+// chunk is 256 elements deep; a tile's chunk is NT adjacent blocks of the tiled store,
+// one contiguous read.  NT = 2 (r <= 64): every A chunk (the W or H^T shadow, re-read
+// from L2 by each workgroup) serves 128 columns of V instead of 64, halving the A loads
+// that otherwise take half of each CU's load traffic; one such workgroup holds the same
+// 8 waves per CU as two of 4.  The chunk's B columns and A rows are staged through LDS;
+// the next chunk's loads are in flight in registers while the current one is
+// multiplied.  The MB * 2 NT output blocks of 32 x 32 (rank block mb, column block nb)
+// are spread over the waves.
 //
 // The first column tile of every slice also accumulates the slice's gram A^T A (gram
-// block (mb, nb) on wave mb*MB + nb, modulo 4); the last of the S gram workgroups
+// block (mb, nb) on wave mb*MB + nb, modulo the wave count); the last of the S gram workgroups
 // reduces the partial grams into G, unless gkeep is set: then the update kernel sums
 // the S partials as it stages the gram.  That reduction (a fence, an atomic and a
 // round trip on one workgroup) otherwise runs after every other workgroup of a

@@ -225,9 +225,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
 }
 
 // ---------------------------------------------------------------------------
-// V (n x m, FP32) -> Vq (column-major, ld ldv), Vtq (V^T column-major, ld ldt) and one
-// E8M0 scale per 32 x 32 tile: Vsc[ub * nbi + ib] (nbi = ldv / 32).  One workgroup per
-// tile of the padded stores; tiles past V's edges get zeros and the scale 127 (2^0).
+// V (n x m, FP32) -> Vq (contraction i, m columns) and Vtq (contraction u, n columns), both
+// tiled (fp8_tiled_index with ntm / ntn column tiles), and one E8M0 scale per 32 x 32 tile:
+// Vsc[ub * nbi + ib] (nbi = ldv / 32).  One workgroup per tile of the padded stores (ldv x
+// ldt, the padded contraction lengths); tiles past V's edges get zeros and the scale 127 (2^0).
 __global__ __launch_bounds__(NTHREADS) void k_pack_v_fp8(const float* __restrict__ V, int n, int m,
                                                          uint8_t* __restrict__ Vq, int ntm, uint8_t* __restrict__ Vtq,
                                                          int ntn, uint8_t* __restrict__ Vsc, int nbi) {

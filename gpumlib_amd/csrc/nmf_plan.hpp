@@ -23,7 +23,8 @@ inline bool rule_is_additive(int rule) { return rule >= 2; }
 struct Plan {
     int rule, precision, n, m, r;
     int npad, rpad, ldr, ldp;
-    int ldv8, ldt8;       // BF16 storage: leading dimensions of V and V^T (bf16_ld of n and of m)
+    int ldv8, ldt8;       // BF16 storage: padded contraction lengths (bf16_ld of n and of m): the K extent
+                          // of the tiled V and V^T and the leading dimensions of the W / H^T shadows
     int lp_nt;            // BF16 passes: 64-column blocks per workgroup (1 or 2)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
@@ -34,9 +35,9 @@ struct Plan {
     size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
 };
 
-// Leading dimensions of the narrow V stores and operand shadows: whole pass chunks
-// (256 BF16 or 512 FP8 elements along the contraction), the padding zero-filled,
-// so the streaming passes load every chunk without bounds checks.
+// Padded contraction lengths of the narrow V stores and leading dimensions of the operand
+// shadows: whole pass chunks (256 BF16 or 512 FP8 elements along the contraction), the
+// padding zero-filled, so the streaming passes load every chunk without bounds checks.
 inline int bf16_ld(int k) { return (k + 255) / 256 * 256; }
 inline int fp8_ld(int k) { return (k + 511) / 512 * 512; }
 
