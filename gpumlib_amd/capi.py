@@ -66,6 +66,8 @@ SIGNATURES = {
     "gpumlib_nmf_quality_workspace_bytes": (_sz, [_l]),
     "gpumlib_nmf_quality": (_i, [_vp, _vp, _l, _f, _vp, _sz, _vp, _vp]),
     "gpumlib_nmf_sgemm": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
+    "gpumlib_nmf_gemm_bf16": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
+    "gpumlib_nmf_gemm_fp8": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _i, _vp, _i, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "gpumlib_nmf_vstore_bytes": (_sz, [_i, _i, _i]),
     "gpumlib_nmf_vstore_pack": (_i, [_i, _i, _i, _vp, _vp, _sz, _vp]),

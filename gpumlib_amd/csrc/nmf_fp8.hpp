@@ -53,10 +53,13 @@ NMF_HD inline uint8_t f32_to_e4m3(float x) {
     return (uint8_t)q;  // q == 8 encodes 2^-6, the smallest normal (E=1, m=0)
 }
 
+// Value of an OCP E4M3 code (the NaN codes 0x7f / 0xff are not decoded; the NMF stores hold
+// non-negative codes only, the GEMM entry point takes either sign).
 NMF_HD inline float e4m3_to_f32(uint8_t v) {
     const int E = (v >> 3) & 0xf, q = v & 7;
-    if (E == 0) return (float)q * (1.0f / 512.0f);
-    return bits_f32((uint32_t)(E - 7 + 127) << 23 | ((uint32_t)q << 20));
+    const float mag =
+        (E == 0) ? (float)q * (1.0f / 512.0f) : bits_f32((uint32_t)(E - 7 + 127) << 23 | ((uint32_t)q << 20));
+    return (v & 0x80) ? -mag : mag;
 }
 
 // Smallest power-of-two scale 2^(s-127) with amax / scale <= 448 (s clamped to [0, 254]).

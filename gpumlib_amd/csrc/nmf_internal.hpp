@@ -86,5 +86,9 @@ hipError_t launch_kl_update_w_wh(float* W, const float* H, const float* V, const
 // General FP32 GEMM, column-major, cuBLAS-style operation flags (nmf_gemm.hip).
 hipError_t launch_sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,
                         const float* B, int ldb, float beta, float* C, int ldc, hipStream_t st);
+// The same with operands in format fmt (0 FP32, 1 BF16 bit patterns, 2 OCP E4M3 times the
+// operand's scale sa / sb; sa, sb are ignored for 0 and 1).
+hipError_t launch_gemm(int fmt, char ta, char tb, int M, int N, int K, float alpha, const void* A, int lda, float sa,
+                       const void* B, int ldb, float sb, float beta, float* C, int ldc, hipStream_t st);
 
 }  // namespace gpumlib_nmf

@@ -85,6 +85,14 @@ def test_iterate_rejects_bad_arguments(native):
 def test_other_entry_points_reject_bad_arguments(native):
     assert native.gpumlib_nmf_sgemm(b"Q", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_sgemm(b"N", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 2, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_gemm_bf16(b"N", b"Z", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4,
+                                        None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_gemm_fp8(b"N", b"N", 4, 4, 4, 1.0, None, 4, 255, None, 4, 127, 0.0, None, 4,
+                                       None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_gemm_fp8(b"N", b"N", 4, 4, 4, 1.0, None, 4, 127, None, 4, -1, 0.0, None, 4,
+                                       None) == HIP_INVALID_VALUE
+    # empty products launch nothing
+    assert native.gpumlib_nmf_gemm_bf16(b"N", b"T", 0, 4, 4, 1.0, None, 1, None, 4, 0.0, None, 1, None) == 0
     assert native.gpumlib_nmf_quality(None, None, 10, 1e-8, None, 0, None, None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_sum_h(None, 300, 10, 1e-8, None, None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_mu_update(None, None, None, -1, 1e-8, None) == HIP_INVALID_VALUE

@@ -40,8 +40,11 @@ def quant(native, x, scale):
 
 
 def test_decode_table(native):
-    for q in range(0x7F):
-        assert native.gpumlib_nmf_fp8_decode(q, 127) == np.float32(VALS[q])
+    for q in range(256):
+        if q & 0x7F == 0x7F:
+            continue  # NaN codes
+        got = native.gpumlib_nmf_fp8_decode(q, 127)
+        assert got == np.float32(VALS[q]) and np.signbit(got) == bool(q & 0x80)
     assert native.gpumlib_nmf_fp8_decode(0x7E, 127) == 448.0
     assert native.gpumlib_nmf_fp8_decode(0x38, 130) == 8.0  # 1.0 * 2^3
 

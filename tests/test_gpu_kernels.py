@@ -67,6 +67,67 @@ def test_sgemm_rejects_bad_flags(native, cuda):
     assert native.gpumlib_nmf_sgemm(b"X", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) != 0
 
 
+def bf16_value(bits):
+    """BF16 bit patterns -> their values (float64)."""
+    return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def e4m3_value(q):
+    """OCP E4M3 bytes -> their values (float64), from the format's definition: sign, 4-bit
+    exponent (bias 7), 3-bit mantissa, subnormals at exponent 0 (the NaN codes are not used)."""
+    q = q.astype(np.int64)
+    sign = np.where(q & 0x80, -1.0, 1.0)
+    e, mt = (q >> 3) & 0xF, q & 7
+    mag = np.where(e == 0, mt / 8.0 * 2.0 ** -6, (1.0 + mt / 8.0) * np.ldexp(1.0, e - 7))
+    return sign * mag
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (130, 257, 95), (1000, 3, 700)])
+def test_gemm_bf16(native, orc, cuda, ta, tb, mnk):
+    """BF16 operands, FP32 accumulation: the FP64 GEMM of the stored values within FP32
+    accumulation error (the products are exact in FP32)."""
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 5 + N * 11 + K)
+    Ar, Ac = (M, K) if ta == "N" else (K, M)
+    Br, Bc = (K, N) if tb == "N" else (N, K)
+    A = (rng.standard_normal(Ar * Ac).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    B = (rng.standard_normal(Br * Bc).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    C = rng.random(M * N, dtype=np.float32)
+    alpha, beta = 0.75, -0.5
+    dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
+    capi.check(native.gpumlib_nmf_gemm_bf16(ta.encode(), tb.encode(), M, N, K, alpha, dA.ptr, Ar, dB.ptr, Br, beta,
+                                            dC.ptr, M, stream(cuda)))
+    ref = C.astype(np.float64)
+    orc.gemm(ta, tb, M, N, K, alpha, bf16_value(A), Ar, bf16_value(B), Br, beta, ref, M)
+    assert rel_fro(get(dC), ref) < 1e-5
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("mnk", [(33, 17, 5), (130, 257, 95), (64, 64, 512)])
+def test_gemm_fp8(native, orc, cuda, ta, tb, mnk):
+    """E4M3 operands with one E8M0 scale each: the FP64 GEMM of the scaled stored values."""
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 13 + N + K * 3)
+    Ar, Ac = (M, K) if ta == "N" else (K, M)
+    Br, Bc = (K, N) if tb == "N" else (N, K)
+    codes = np.array([c for c in range(256) if (c & 0x7F) != 0x7F], np.uint8)  # no NaN codes
+    A = rng.choice(codes, Ar * Ac)
+    B = rng.choice(codes, Br * Bc)
+    sa, sb = 121, 130  # 2^-6 and 2^3
+    C = rng.random(M * N, dtype=np.float32)
+    dA, dB, dC = dev(cuda, A), dev(cuda, B), dev(cuda, C)
+    capi.check(native.gpumlib_nmf_gemm_fp8(ta.encode(), tb.encode(), M, N, K, 1.25, dA.ptr, Ar, sa, dB.ptr, Br, sb,
+                                           0.5, dC.ptr, M, stream(cuda)))
+    ref = C.astype(np.float64)
+    orc.gemm(ta, tb, M, N, K, 1.25, e4m3_value(A) * 2.0 ** (sa - 127), Ar, e4m3_value(B) * 2.0 ** (sb - 127), Br,
+             0.5, ref, M)
+    assert rel_fro(get(dC), ref) < 1e-5
+    # the decoding is the library's own host decoder's
+    assert all(native.gpumlib_nmf_fp8_decode(int(c), 127) == e4m3_value(np.array([c]))[0] for c in codes)
+
+
 def test_mu_update_bit_exact(native, orc, cuda):
     rng = np.random.default_rng(3)
     E = 100_003
