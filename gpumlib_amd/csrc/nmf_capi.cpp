@@ -231,8 +231,8 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
 }
 
 // Slice grams the low-precision updates sum themselves (0: the pass reduces them into G
-// at its tail).  Register-staged grams take up to 4 slices of r <= 64.
-static int update_gram_slices(int S, int r) { return (S <= 4 && r <= 64) ? S : 0; }
+// at its tail).  Register-staged grams take up to kMaxGramSlices slices of r <= 64.
+static int update_gram_slices(int S, int r) { return (S <= kMaxGramSlices && r <= 64) ? S : 0; }
 
 // BF16 storage: V store = [V | V^T] in BF16, each tiled for its pass (bf16_tiled_index).  W and H stay FP32
 // (the updates and their ratios are FP32); BF16 shadows feed the MFMA passes.

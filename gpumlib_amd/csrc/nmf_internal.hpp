@@ -5,6 +5,9 @@
 
 namespace gpumlib_nmf {
 
+// Slice grams a row-form update sums itself while staging the gram (register-staged: r <= 64).
+constexpr int kMaxGramSlices = 8;
+
 // Pass kernels (nmf_pass_f32.hip).
 hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st);
 hipError_t launch_pass_w_f32(const PassArgs& a, int kl, hipStream_t st);
@@ -29,7 +32,7 @@ hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
 // Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
 // G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a];  with
-// gsl in [1, 4] (r <= 64), the first of the gsl slice grams of the pass (stride pstride),
+// gsl in [1, kMaxGramSlices] (r <= 64), the first of the gsl slice grams of the pass (stride pstride),
 // which the update sums in slice order.
 // add != 0 (every Euclidean / KL update below): the additive-update rule of
 // NMF_AdditiveEuclidian / NMF_AdditiveDivergence in place of the multiplicative one.

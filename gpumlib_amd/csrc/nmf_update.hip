@@ -153,17 +153,18 @@ struct RegStage {
 template <int U>
 struct RegStage4 {
     float4 v[U];
-    // v = sum over ns <= 4 slices (stride ps floats) of the group at ptr(g), in slice
-    // order; all 4 U loads are issued before the first add.
+    // v = sum over ns <= kMaxGramSlices slices (stride ps floats) of the group at ptr(g),
+    // in slice order; all ns U loads are issued before the first add.
     template <typename Ptr, typename Mask>
     __device__ __forceinline__ void load_sum(int groups, int ns, long ps, Ptr ptr, Mask mask) {
-        float4 x[U][4];
+        constexpr int MS = kMaxGramSlices;
+        float4 x[U][MS];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int g = threadIdx.x + u * NTHREADS;
             const float* q = g < groups ? ptr(g) : nullptr;
 #pragma unroll
-            for (int sl = 0; sl < 4; sl++)
+            for (int sl = 0; sl < MS; sl++)
                 x[u][sl] = (q && sl < ns) ? *reinterpret_cast<const float4*>(q + (size_t)sl * ps)
                                           : make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -171,7 +172,7 @@ struct RegStage4 {
         for (int u = 0; u < U; u++) {
             float4 t = x[u][0];
 #pragma unroll
-            for (int sl = 1; sl < 4; sl++)
+            for (int sl = 1; sl < MS; sl++)
                 if (sl < ns) t.x += x[u][sl].x, t.y += x[u][sl].y, t.z += x[u][sl].z, t.w += x[u][sl].w;
             const int g = threadIdx.x + u * NTHREADS;
             v[u] = make_float4(mask(g, 0) ? t.x : 0.f, mask(g, 1) ? t.y : 0.f, mask(g, 2) ? t.z : 0.f,
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
 
     // C and the X rows, zero padded to RP.  Up to r = 64 their loads are issued first,
     // into registers, so that they and the slice loads below take one round trip.
-    // gsl = 0: G is the reduced gram; gsl in [1, 4] (r <= 64 only): G holds the grams of
+    // gsl = 0: G is the reduced gram; gsl in [1, kMaxGramSlices] (r <= 64 only): G holds the grams of
     // the pass's gsl K slices at stride gps, summed here in slice order.
     auto ld_g = [&](int idx) {  // C(b,a) in staging order: along b for gt (coalesced), along a otherwise
         const int hi = idx / RP, lo = idx - hi * RP;
@@ -864,7 +865,7 @@ static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, co
                               uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, long gps, int gsl,
                               hipStream_t st) {
     if (r < 1 || r > 128 || (Q && (r > 64 || ldq % 32 || ldq < count))) return hipErrorInvalidValue;
-    if (gsl < 0 || gsl > 4 || (gsl > 0 && r > 64)) return hipErrorInvalidValue;
+    if (gsl < 0 || gsl > kMaxGramSlices || (gsl > 0 && r > 64)) return hipErrorInvalidValue;
     const bool wide = count >= 16384 || Q != nullptr;
 #define LR(JT, RP)                                                                                                \
     return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, \
