@@ -21,10 +21,22 @@ PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops"
 
 WORKLOADS = {
     # name: (rule, n, m, r, r0, seed)
+    "euclid-256x128-r16": ("euclidean", 256, 128, 16, 16, 0x5EED + 1),
     "euclid-16384x8192-r64": ("euclidean", 16384, 8192, 64, 64, 0x5EED + 4),
     "euclid-8192x4096-r64": ("euclidean", 8192, 4096, 64, 64, 0x5EED + 2),
     "kl-4096x4096-r32": ("kl", 4096, 4096, 32, 32, 0x5EED + 3),
 }
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def load_traffic(workload):
@@ -53,6 +65,10 @@ def parse():
                     help="further precisions measured and reported beside the headline (comma list, '' = none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-single-seconds", type=float, default=0.0,
+                    help="also time the reference-faithful single-thread restatement (naive loops, the reference's "
+                         "operation order) for about this long, at least one iteration (0 = off; a C4 iteration "
+                         "takes tens of seconds)")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--serve-jobs", type=int, default=512,
                     help="jobs of the batched-serving mix run through the server (0 = skip serving)")
@@ -326,8 +342,27 @@ def main():
             t += time.perf_counter() - s0
             k += 1
         line["cpu_baseline"] = {"value": k / t, "unit": "iter/s", "cores": orc.omp_threads(), "kind": "port",
-                                "sample": f"{k} iterations of the same workload ({args.workload}, FP32 update rules, "
+                                "cpu": host_cpu_model(), "sample": f"{k} iterations of the same workload ({args.workload}, FP32 update rules, "
                                           f"OpenMP host-cores restatement) after 1 warm-up"}
+    if args.cpu_single_seconds > 0:
+        from oracle import oracle as orc
+
+        Vh = np.ascontiguousarray(V)
+        Wh, Hh = W0.copy(), H0.copy()
+        fn = orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter
+        k, t = 0, 0.0
+        while t < args.cpu_single_seconds or k < 1:
+            s0 = time.perf_counter()
+            fn(n, m, r, Vh, Wh, Hh, eps, iters=1, omp=False)
+            dt = time.perf_counter() - s0
+            t += dt
+            k += 1
+            if dt > 1.0 or k % 100 == 0:
+                print(f"cpu single thread: {k} iterations, {t:.1f} s", file=sys.stderr, flush=True)
+        line["cpu_reference_faithful"] = {
+            "value": k / t, "unit": "iter/s", "cores": 1, "kind": "port", "cpu": host_cpu_model(),
+            "sample": f"{k} iterations of the same workload ({args.workload}), single thread, naive loops in the "
+                      f"reference's operation order (no warm-up: each iteration is long)"}
     print(json.dumps(line), flush=True)
 
 
