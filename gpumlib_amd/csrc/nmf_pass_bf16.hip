@@ -265,12 +265,13 @@ static void launch_tn(const LpPassArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st) {
+    // 128-column tiles for r <= 64 (as the plan chooses them), 64-column tiles above
     const int rp = round_up(a.r, 32), nt = a.nt == 2 ? 2 : 1;
-    if (rp <= 32)
-        nt == 2 ? launch_tn<1, 2>(a, st) : launch_tn<1, 1>(a, st);
-    else if (rp <= 64)
-        nt == 2 ? launch_tn<2, 2>(a, st) : launch_tn<2, 1>(a, st);
-    else if (rp <= 128 && nt == 1)
+    if (rp <= 32 && nt == 2)
+        launch_tn<1, 2>(a, st);
+    else if (rp <= 64 && nt == 2)
+        launch_tn<2, 2>(a, st);
+    else if (rp > 64 && rp <= 128 && nt == 1)
         launch_tn<4, 1>(a, st);
     else
         return hipErrorInvalidValue;
