@@ -132,7 +132,9 @@ def main():
 
     def leg(prec_name):
         prec = {"fp32": G.FP32, "bf16": G.BF16, "fp8": G.FP8}[prec_name]
+        t_setup = time.perf_counter()
         d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
+        t_setup = time.perf_counter() - t_setup
         vbytes_el = {G.FP32: 4, G.BF16: 2, G.FP8: 1}[prec]
         obj0 = d.objective(obj_kind)
         d.launch(args.warmup)
@@ -142,7 +144,18 @@ def main():
         d.stream.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1e3 / args.steps
+        t_d2h = time.perf_counter()
+        d.get_w()
+        d.get_h()
+        t_d2h = time.perf_counter() - t_d2h
         obj1 = d.objective(obj_kind)
+        # The same K iterations with the host hand-over around them: V, W, H from pageable host
+        # memory, the V store built, W and H copied back (never the headline value).
+        e2e = t_setup + (t1 - t0) + t_d2h
+        end_to_end = {"setup_s": t_setup, "upload_s": d.setup_s["upload"], "pack_s": d.setup_s["pack"],
+                      "upload_gbs": 4.0 * (n * m + (n + m) * r) / d.setup_s["upload"] / 1e9,
+                      "iterations": args.steps, "iterate_s": t1 - t0, "download_s": t_d2h,
+                      "iters_per_s_incl_transfers": args.steps / e2e}
 
         # Per-pass durations: HIP events on the stream the kernels are launched on, with
         # the H and W passes alternating as they do in an iteration (each streams its own
@@ -180,7 +193,7 @@ def main():
             "iters_per_s": 1e3 / ms, "ms_per_step": ms,
             "mfma_peak_frac": flops_iter / (ms * 1e-3) / 1e12 / PEAK[f"{prec_name}_mfma_tflops"],
             "v_stream_gbs": 2.0 * vbytes_el * n * m / (ms * 1e-3) / 1e9,
-            "objective_entry": obj0, "objective_after": obj1, "objective_at": gaps,
+            "objective_entry": obj0, "objective_after": obj1, "objective_at": gaps, "end_to_end": end_to_end,
             "kernels": {"pass_h_ms": ms_h, "pass_w_ms": ms_w,
                         "pass_h_tflops": flops_h / (ms_h * 1e-3) / 1e12,
                         "pass_w_tflops": flops_w / (ms_w * 1e-3) / 1e12},

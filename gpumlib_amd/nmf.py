@@ -61,18 +61,24 @@ def gen_init(n, m, r, seed):
 class DeviceNMF:
     """One NMF problem resident on the GPU, driven through gpumlib_nmf_iterate.
 
-    precision BF16/FP8: V is packed into the V store of that precision on the device."""
+    precision BF16/FP8: V is packed into the V store of that precision on the device.
+    setup_s: wall-clock seconds of the set-up phases, "upload" (device buffers allocated and
+    V, W, H copied from pageable host memory) and "pack" (the V store built on the device)."""
 
     def __init__(self, rule, V, W, H, n, m, r, eps=REFERENCE_EPS, precision=FP32, stream=None):
+        import time
+
         from . import hip
 
         self.hip = hip
         self.lib = capi.load()
         self.rule, self.precision, self.n, self.m, self.r, self.eps = rule, precision, n, m, r, float(eps)
         self.stream = stream or hip.Stream()
+        t0 = time.perf_counter()
         self.Vf = hip.Buffer.from_array(np.asarray(V, dtype=np.float32))
         self.W = hip.Buffer.from_array(np.asarray(W, dtype=np.float32))
         self.H = hip.Buffer.from_array(np.asarray(H, dtype=np.float32))
+        t1 = time.perf_counter()
         if precision == FP32:
             self.V = self.Vf
         else:
@@ -83,6 +89,7 @@ class DeviceNMF:
             capi.check(self.lib.gpumlib_nmf_vstore_pack(precision, n, m, self.Vf.ptr, self.V.ptr, vsb,
                                                         self.stream.handle), "gpumlib_nmf_vstore_pack")
             self.stream.synchronize()
+        self.setup_s = {"upload": t1 - t0, "pack": time.perf_counter() - t1}
         self.ws_bytes = int(self.lib.gpumlib_nmf_workspace_bytes(rule, precision, n, m, r))
         if self.ws_bytes == 0:
             raise ValueError("unsupported problem shape / precision")

@@ -43,6 +43,22 @@ def job_data(job):
     return V, W, H
 
 
+def iteration_work(rule, n, m, r):
+    """Algorithmic work of one iteration of a job: FLOPs (Euclidean 4nmr + 4r^2(n+m), KL
+    8nmr) and bytes (the FP32 V streamed twice, W and H each read and written once)."""
+    flops = 4.0 * n * m * r + 4.0 * r * r * (n + m) if rule == G.EUCLIDEAN else 8.0 * n * m * r
+    return flops, 8.0 * n * m + 8.0 * (n + m) * r
+
+
+def _aggregate(jobs, iters, span):
+    """Aggregate rates over the span from the jobs' logged iteration counts (the
+    stopping rule's objective evaluations are not counted as work)."""
+    flops = sum(iteration_work(j[0], j[1], j[2], j[3])[0] * it for j, it in zip(jobs, iters))
+    nbytes = sum(iteration_work(j[0], j[1], j[2], j[3])[1] * it for j, it in zip(jobs, iters))
+    return {"iterations_total": int(sum(iters)), "aggregate_tflops": flops / span / 1e12,
+            "aggregate_gbs": nbytes / span / 1e9}
+
+
 def _percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
 
@@ -91,7 +107,7 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
     span = last - first
     return {"jobs": len(ids), "jobs_per_s": len(ids) / span, "span_s": span,
             "latency_p50_s": _percentile(lat, 50), "latency_p99_s": _percentile(lat, 99),
-            "iterations_mean": float(np.mean(iters)), "converged": conv,
+            "iterations_mean": float(np.mean(iters)), **_aggregate(jobs, iters, span), "converged": conv,
             "rounds": st.rounds, "peak_active": st.peak_active, "pool_high_water_bytes": st.pool_high_water,
             "arrival_rate_per_s": rate, "round_iterations": round_iters, "graph": bool(use_graph),
             "autotune_trial_iterations": autotune, "autotunes": st.autotunes,
@@ -138,4 +154,4 @@ def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=
     return {"jobs": len(jobs), "jobs_per_s": len(jobs) / span, "span_s": span,
             "service_p50_s": _percentile(lat, 50), "service_p99_s": _percentile(lat, 99),
             "latency_p50_s": _percentile(queued, 50), "latency_p99_s": _percentile(queued, 99),
-            "iterations_mean": float(np.mean(iters)), "arrival_rate_per_s": rate}
+            "iterations_mean": float(np.mean(iters)), **_aggregate(jobs, iters, span), "arrival_rate_per_s": rate}

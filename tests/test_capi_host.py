@@ -39,8 +39,13 @@ def test_workspace_bytes(native):
     assert ok > 0 and ok < 64 << 20  # partials only: far below V itself (512 MiB)
     assert native.gpumlib_nmf_workspace_bytes(capi.KL, capi.FP32, 1, 1, 1) > 0
     for bad in [(capi.EUCLIDEAN, capi.FP32, 10, 10, 0), (capi.EUCLIDEAN, capi.FP32, 10, 10, 129),
-                (7, capi.FP32, 10, 10, 4), (capi.EUCLIDEAN, 9, 10, 10, 4), (capi.KL, capi.FP32, 0, 10, 4)]:
+                (7, capi.FP32, 10, 10, 4), (capi.EUCLIDEAN, 9, 10, 10, 4), (capi.KL, capi.FP32, 0, 10, 4),
+                (capi.EUCLIDEAN, capi.BF16, 10, 10, 129), (capi.EUCLIDEAN, capi.FP8, 10, 10, 65)]:
         assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
+    # the largest rank of each precision plans: 128 (FP32, BF16), 64 (FP8)
+    for ok in [(capi.EUCLIDEAN, capi.BF16, 10, 10, 128), (capi.EUCLIDEAN, capi.BF16, 10, 10, 65),
+               (capi.EUCLIDEAN, capi.FP8, 10, 10, 64), (capi.KL, capi.FP32, 10, 10, 128)]:
+        assert native.gpumlib_nmf_workspace_bytes(*ok) > 0
 
 
 def test_additive_rules_plan_fp32_only(native):

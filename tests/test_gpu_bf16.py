@@ -102,8 +102,9 @@ def test_bf16_objective_gap(native, orc, cuda):
     assert g["emu_w"] < 1e-3 and g["emu_h"] < 1e-3
 
 
+# r = 65 is the first rank on 64-column tiles, r = 128 the largest rank of the path
 @pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 100), (1, 9, 1), (16500, 90, 64),
-                                   (70, 16390, 20)])
+                                   (70, 16390, 20), (129, 300, 65), (128, 384, 128)])
 def test_bf16_shapes(native, orc, cuda, shape):
     n, m, r = shape
     g = _gaps(n, m, r, 20, seed=n + m)
