@@ -348,15 +348,18 @@ def main():
         Wh, Hh = W0.copy(), H0.copy()
         fn = orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter
         fn(n, m, r, Vh, Wh, Hh, eps, iters=1, omp=True)  # warm-up
-        k, t = 0, 0.0
-        while t < args.cpu_seconds or k < 2:
+        k, t, each = 0, 0.0, []
+        while t < args.cpu_seconds or k < 5:
             s0 = time.perf_counter()
             fn(n, m, r, Vh, Wh, Hh, eps, iters=1, omp=True)
-            t += time.perf_counter() - s0
+            each.append(time.perf_counter() - s0)
+            t += each[-1]
             k += 1
-        line["cpu_baseline"] = {"value": k / t, "unit": "iter/s", "cores": orc.omp_threads(), "kind": "port",
+        line["cpu_baseline"] = {"value": k / t, "unit": "iter/s", "median_iter_per_s": 1.0 / float(np.median(each)),
+                                "cores": orc.omp_threads(), "kind": "port",
                                 "cpu": host_cpu_model(), "sample": f"{k} iterations of the same workload ({args.workload}, FP32 update rules, "
-                                          f"OpenMP host-cores restatement) after 1 warm-up"}
+                                          f"OpenMP host-cores restatement) after 1 warm-up; value over "
+                                          f"their total time, median_iter_per_s from the median one"}
     if args.cpu_single_seconds > 0:
         from oracle import oracle as orc
 
