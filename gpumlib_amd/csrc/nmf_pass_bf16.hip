@@ -39,10 +39,14 @@ __device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
 // multiplied.  The MB * 2 NT output blocks of 32 x 32 (rank block mb, column block nb)
 // are spread over the waves.
 //
-// The first column tile of every slice also accumulates the slice's gram A^T A (gram
-// block (mb, nb) on wave mb*MB + nb, modulo the wave count); the last of the S gram workgroups
-// reduces the partial grams into G, unless gkeep is set: then the update kernel sums
-// the S partials as it stages the gram.  That reduction (a fence, an atomic and a
+// The slice's gram A^T A is accumulated beside the product.  With at least MB * MB column
+// tiles, gram block t = (mb, nb) rides on column tile t and its 16 k steps per chunk are
+// split over the waves (step g on wave g % NW), so a gram tile does 1/NW more MFMA work
+// than a plain one and no workgroup holds the one-round grid back; the waves' partial
+// blocks are then added in wave order through LDS.  With fewer tiles every block rides on
+// tile 0 (block mb*MB + nb on wave (mb*MB + nb) % NW, all k steps).  The last of the gram
+// workgroups reduces the S slice grams into G, unless gkeep is set: then the update kernel
+// sums the S partials as it stages the gram.  That reduction (a fence, an atomic and a
 // round trip on one workgroup) otherwise runs after every other workgroup of a
 // one-round grid has finished, some 3-4 us of pure tail per pass at 16384 x 8192.
 //
@@ -75,7 +79,8 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
-    const bool gram = (t == 0) && p.G != nullptr;
+    const bool spread = p.tiles >= MB * MB;  // grid-uniform
+    const bool gram = p.G != nullptr && (spread ? t < MB * MB : t == 0);
     const int col0 = t * TW;
     const int ncols = min(TW, p.N - col0);
     const int kbeg = s * p.kslice;
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
     // lane (c, h) supplies k = g*16 + h*8 + j (j < 8) for both operands; the gram-carrying
     // and the plain workgroups run separate copies of the loop (no branch between MFMAs)
     auto compute = [&](auto gram_tag) __attribute__((always_inline)) {
-        constexpr bool GR = decltype(gram_tag)::value;
+        constexpr int GR = decltype(gram_tag)::value;  // 0 no gram, 1 on tile 0 by wave, 2 spread
 #pragma unroll
         for (int g = 0; g < KB / 16; g++) {
 #pragma unroll
@@ -141,17 +146,30 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
                     acc[q] = mfma_bf16(av, bv, acc[q]);
                 }
             }
-            if constexpr (GR) {
+            if constexpr (GR == 1) {
+                {
 #pragma unroll
-                for (int q = 0; q < GB; q++) {
-                    const int blk = wave + NW * q;
-                    if ((MB * MB) % NW == 0 || blk < MB * MB) {
-                        const int mb = blk / MB, nb = blk - (blk / MB) * MB;
-                        const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
-                        const u32x4 bv = *reinterpret_cast<const u32x4*>(&sA[nb * 32 + c][g * 16 + h * 8]);
-                        gacc[q] = mfma_bf16(av, bv, gacc[q]);
+                    for (int q = 0; q < GB; q++) {
+                        const int blk = wave + NW * q;
+                        if ((MB * MB) % NW == 0 || blk < MB * MB) {
+                            const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+                            const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
+                            const u32x4 bv = *reinterpret_cast<const u32x4*>(&sA[nb * 32 + c][g * 16 + h * 8]);
+                            gacc[q] = mfma_bf16(av, bv, gacc[q]);
+                        }
                     }
                 }
+            }
+        }
+        if constexpr (GR == 2) {
+            // this wave's k steps g = wave + NW i of gram block t
+            const int mb = t / MB, nb = t - (t / MB) * MB;
+#pragma unroll
+            for (int i = 0; i < KB / 16 / NW; i++) {
+                const int g = wave + NW * i;
+                const u32x4 av = *reinterpret_cast<const u32x4*>(&sA[mb * 32 + c][g * 16 + h * 8]);
+                const u32x4 bv = *reinterpret_cast<const u32x4*>(&sA[nb * 32 + c][g * 16 + h * 8]);
+                gacc[0] = mfma_bf16(av, bv, gacc[0]);
             }
         }
     };
@@ -172,10 +190,12 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
             }
         }
     };
-    if (gram)
-        mainloop(std::true_type{});
+    if (gram && spread)
+        mainloop(std::integral_constant<int, 2>{});
+    else if (gram)
+        mainloop(std::integral_constant<int, 1>{});
     else
-        mainloop(std::false_type{});
+        mainloop(std::integral_constant<int, 0>{});
 
     // acc[q][v] = P(a = mb*32 + acc_row(v,h), u = nb*32 + c) for block wave*BPW + q
     float* slice = p.P + (size_t)s * p.pstride;
@@ -194,7 +214,26 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
                     make_float4(acc[q][g * 4], acc[q][g * 4 + 1], acc[q][g * 4 + 2], acc[q][g * 4 + 3]);
         }
     }
-    if (gram) {
+    if (gram && spread) {
+        // gacc[0][v] of wave w = its k steps' part of gram(a = mb*32 + acc_row(v,h), b = nb*32 + c)
+        // for block t = (mb, nb).  The LDS of the chunks is free (the main loop ends with a
+        // barrier after the last multiply): the NW parts meet there and are added in wave order.
+        static_assert(NW * 16 * 64 * sizeof(float) <= sizeof(sB), "gram partials fit in sB");
+        float* red = reinterpret_cast<float*>(&sB[0][0]);
+#pragma unroll
+        for (int v = 0; v < 16; v++) red[(wave * 16 + v) * 64 + lane] = gacc[0][v];
+        __syncthreads();
+        const int mb = t / MB, nb = t - (t / MB) * MB;
+        float* gdst = slice + (size_t)p.N * p.ldr;
+        for (int e = tid; e < 16 * 64; e += THR) {
+            const int v = e >> 6, ln = e & 63;
+            float sum = red[v * 64 + ln];
+#pragma unroll
+            for (int w2 = 1; w2 < NW; w2++) sum += red[(w2 * 16 + v) * 64 + ln];
+            const int a = mb * 32 + acc_row(v, ln >> 5), b = nb * 32 + (ln & 31);
+            if (b < p.r && a < p.ldr) gdst[(size_t)b * p.ldr + a] = sum;
+        }
+    } else if (gram) {
         // gacc[q][v] = gram(a = mb*32 + acc_row(v,h), b = nb*32 + c), stored at column N + b
 #pragma unroll
         for (int q = 0; q < GB; q++) {
@@ -211,10 +250,10 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
                         make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
             }
         }
-        if (!p.gkeep)
-            gram_finish4<(MB >= 2 ? 8 : 2)>(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G,
-                                            p.gcount);
     }
+    if (gram && !p.gkeep)  // S slices of MB * MB (spread) or one gram workgroup(s) each
+        gram_finish4<(MB >= 2 ? 8 : 2)>(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, spread ? p.S * MB * MB : p.S,
+                                        p.r * p.ldr, p.G, p.gcount);
 }
 
 // ---------------------------------------------------------------------------

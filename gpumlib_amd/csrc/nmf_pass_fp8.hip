@@ -35,7 +35,9 @@ __device__ __forceinline__ f32x16 mfma_fp8(u32x4 a0, u32x4 a1, u32x4 b0, u32x4 b
 // columns, chunks 512 elements (bytes) deep so every column is read in 512-byte runs,
 // the chunk's B columns and A rows (and their block scales) staged through LDS, the next
 // chunk's loads in flight in registers while the current one is multiplied, the 2*MB
-// output blocks spread over the 4 waves, the slice's gram on the first column tile.
+// output blocks spread over the 4 waves, the slice's gram spread as in the BF16 pass
+// (block t on column tile t, its 8 k steps per chunk split over the waves, the waves'
+// parts added in wave order; on tile 0, by wave, when there are fewer than MB * MB tiles).
 //
 // B scales come from the 32 x 32 tile scales of V: one scale per (32-column block, K
 // block), so the 64-column tile needs two rows of 16 scales per chunk; A scales are per
@@ -71,7 +73,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
-    const bool gram = (t == 0) && p.G != nullptr;
+    const bool spread = p.tiles >= MB * MB;  // grid-uniform
+    const bool gram = p.G != nullptr && (spread ? t < MB * MB : t == 0);
     const int col0 = t * TILE_LP;
     const int ncols = min(TILE_LP, p.N - col0);
     const int kbeg = s * p.kslice;
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
         for (int v = 0; v < 16; v++) gacc[q][v] = 0.f;
 
     auto compute = [&](auto gram_tag) __attribute__((always_inline)) {
-        constexpr bool GR = decltype(gram_tag)::value;
+        constexpr int GR = decltype(gram_tag)::value;  // 0 no gram, 1 on tile 0 by wave, 2 spread
 #pragma unroll
         for (int g = 0; g < K8 / 64; g++) {
 #pragma unroll
@@ -146,19 +149,36 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                                       sAs[mb * 32 + c][2 * g + h], sBs[nb][2 * g + h], acc[q]);
                 }
             }
-            if constexpr (GR) {
+            if constexpr (GR == 1) {
+                {
 #pragma unroll
-                for (int q = 0; q < GB; q++) {
-                    const int blk = wave + 4 * q;
-                    if ((MB * MB) % 4 == 0 || blk < MB * MB) {
-                        const int mb = blk / MB, nb = blk - (blk / MB) * MB;
-                        const uint8_t* pa = &sA[mb * 32 + c][g * 64 + h * 16];
-                        const uint8_t* pb = &sA[nb * 32 + c][g * 64 + h * 16];
-                        gacc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
-                                           *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
-                                           sAs[mb * 32 + c][2 * g + h], sAs[nb * 32 + c][2 * g + h], gacc[q]);
+                    for (int q = 0; q < GB; q++) {
+                        const int blk = wave + 4 * q;
+                        if ((MB * MB) % 4 == 0 || blk < MB * MB) {
+                            const int mb = blk / MB, nb = blk - (blk / MB) * MB;
+                            const uint8_t* pa = &sA[mb * 32 + c][g * 64 + h * 16];
+                            const uint8_t* pb = &sA[nb * 32 + c][g * 64 + h * 16];
+                            gacc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa),
+                                               *reinterpret_cast<const u32x4*>(pa + 32),
+                                               *reinterpret_cast<const u32x4*>(pb),
+                                               *reinterpret_cast<const u32x4*>(pb + 32), sAs[mb * 32 + c][2 * g + h],
+                                               sAs[nb * 32 + c][2 * g + h], gacc[q]);
+                        }
                     }
                 }
+            }
+        }
+        if constexpr (GR == 2) {
+            // this wave's k steps g = wave + 4 i of gram block t
+            const int mb = t / MB, nb = t - (t / MB) * MB;
+#pragma unroll
+            for (int i = 0; i < K8 / 64 / 4; i++) {
+                const int g = wave + 4 * i;
+                const uint8_t* pa = &sA[mb * 32 + c][g * 64 + h * 16];
+                const uint8_t* pb = &sA[nb * 32 + c][g * 64 + h * 16];
+                gacc[0] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
+                                   *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
+                                   sAs[mb * 32 + c][2 * g + h], sAs[nb * 32 + c][2 * g + h], gacc[0]);
             }
         }
     };
@@ -179,10 +199,12 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
             }
         }
     };
-    if (gram)
-        mainloop(std::true_type{});
+    if (gram && spread)
+        mainloop(std::integral_constant<int, 2>{});
+    else if (gram)
+        mainloop(std::integral_constant<int, 1>{});
     else
-        mainloop(std::false_type{});
+        mainloop(std::integral_constant<int, 0>{});
 
     // acc[q][v] = P(a = mb*32 + acc_row(v,h), u = nb*32 + c) for block wave*BPW + q
     float* slice = p.P + (size_t)s * p.pstride;
@@ -201,7 +223,25 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                     make_float4(acc[q][g * 4], acc[q][g * 4 + 1], acc[q][g * 4 + 2], acc[q][g * 4 + 3]);
         }
     }
-    if (gram) {
+    if (gram && spread) {
+        // the 4 waves' parts of gram block t = (mb, nb) meet in the chunks' LDS (free: the
+        // main loop ends with a barrier after the last multiply) and are added in wave order
+        static_assert(4 * 16 * 64 * sizeof(float) <= sizeof(sB), "gram partials fit in sB");
+        float* red = reinterpret_cast<float*>(&sB[0][0]);
+#pragma unroll
+        for (int v = 0; v < 16; v++) red[(wave * 16 + v) * 64 + lane] = gacc[0][v];
+        __syncthreads();
+        const int mb = t / MB, nb = t - (t / MB) * MB;
+        float* gdst = slice + (size_t)p.N * p.ldr;
+        for (int e = tid; e < 16 * 64; e += NTHREADS) {
+            const int v = e >> 6, ln = e & 63;
+            float sum = red[v * 64 + ln];
+#pragma unroll
+            for (int w2 = 1; w2 < 4; w2++) sum += red[(w2 * 16 + v) * 64 + ln];
+            const int a = mb * 32 + acc_row(v, ln >> 5), b = nb * 32 + (ln & 31);
+            if (b < p.r && a < p.ldr) gdst[(size_t)b * p.ldr + a] = sum;
+        }
+    } else if (gram) {
         // gacc[q][v] = gram(a = mb*32 + acc_row(v,h), b = nb*32 + c), stored at column N + b
 #pragma unroll
         for (int q = 0; q < GB; q++) {
@@ -218,10 +258,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                         make_float4(gacc[q][g * 4], gacc[q][g * 4 + 1], gacc[q][g * 4 + 2], gacc[q][g * 4 + 3]);
             }
         }
-        if (!p.gkeep)
-            gram_finish4<(MB >= 2 ? 8 : 2)>(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, p.S, p.r * p.ldr, p.G,
-                                            p.gcount);
     }
+    if (gram && !p.gkeep)  // S slices of MB * MB (spread) or one gram workgroup(s) each
+        gram_finish4<(MB >= 2 ? 8 : 2)>(p.P + (size_t)p.N * p.ldr, p.pstride, p.S, spread ? p.S * MB * MB : p.S,
+                                        p.r * p.ldr, p.G, p.gcount);
 }
 
 // ---------------------------------------------------------------------------

@@ -102,9 +102,12 @@ def test_bf16_objective_gap(native, orc, cuda):
     assert g["emu_w"] < 1e-3 and g["emu_h"] < 1e-3
 
 
-# r = 65 is the first rank on 64-column tiles, r = 128 the largest rank of the path
+# r = 65 is the first rank on 64-column tiles, r = 128 the largest rank of the path;
+# (16500, 600, 48) and (300, 1100, 100) carry the gram spread over column tiles into the
+# in-pass reduction of the slice grams (more than 8 slices, or r > 64)
 @pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 100), (1, 9, 1), (16500, 90, 64),
-                                   (70, 16390, 20), (129, 300, 65), (128, 384, 128)])
+                                   (70, 16390, 20), (129, 300, 65), (128, 384, 128), (16500, 600, 48),
+                                   (300, 1100, 100)])
 def test_bf16_shapes(native, orc, cuda, shape):
     n, m, r = shape
     g = _gaps(n, m, r, 20, seed=n + m)

@@ -126,7 +126,8 @@ def test_fp8_deterministic(native, cuda):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 64), (1, 40, 1)])
+# (20000, 300, 64): gram spread over column tiles and more than 8 slices (in-pass reduction)
+@pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 64), (1, 40, 1), (20000, 300, 64)])
 def test_fp8_shapes(native, orc, cuda, shape):
     """Ragged shapes against the restatement of the FP8 arithmetic (bounds as above)."""
     n, m, r = shape
