@@ -79,7 +79,8 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
     // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
     // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
-    // the BF16 / FP8 passes fold the whole gram into the first column tile.
+    // the BF16 / FP8 passes put block t on column tile t when there are enough column
+    // tiles (each block's k steps split over the waves), else every block on tile 0.
     const int nbk = (r + 31) / 32, gram_tiles = (precision == GPUMLIB_NMF_FP32 && euc) ? nbk * nbk : 0;
     // The BF16 passes take two 64-column blocks per workgroup of 8 waves for r <= 64: each A
     // chunk (the W or H^T shadow, re-read from L2 by every workgroup) then serves twice the

@@ -205,7 +205,7 @@ struct PassArgs {
     const uint8_t* vsc;  // FP8 storage: E8M0 scale of V's 32 x 32 tile (ub * nbi + ib)
     int nbi;             // FP8 storage: 32-row blocks of V
     float* G;            // reduced gram of this pass's factor: gram(a,b) at G[b*rpad + a]
-    int* gcount;         // arrival counter of the gram tile's S workgroups (zero between passes)
+    int* gcount;         // arrival counter of the gram-carrying workgroups (zero between passes)
 };
 
 // FP8 K-contiguous pass: as LpPassArgs, with E8M0 scales.  A's scale for column a
@@ -230,7 +230,8 @@ struct Fp8PassArgs {
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
 // A: column a at A + a*lda; B: the tiled BF16 store of one V orientation (element (k, u)
 // at B[bf16_tiled_index(k, u, tiles)]); the slice's gram (columns N .. N + r - 1 of P)
-// takes both operands from A.  P[s][u*ldr + a].
+// takes both operands from A (block t on column tile t when tiles >= its block count, else
+// all on tile 0).  P[s][u*ldr + a].
 struct LpPassArgs {
     const uint16_t* A;
     const uint16_t* B;
