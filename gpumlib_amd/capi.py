@@ -72,6 +72,7 @@ SIGNATURES = {
     "gpumlib_nmf_vstore_bytes": (_sz, [_i, _i, _i]),
     "gpumlib_nmf_vstore_pack": (_i, [_i, _i, _i, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_iterate": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "gpumlib_nmf_iterate_noalloc": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _i, _i, _vp, _vp]),
     "gpumlib_nmf_objective": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
     "gpumlib_nmf_pass": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _vp]),
     "gpumlib_nmf_object_create": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _c.POINTER(_vp)]),

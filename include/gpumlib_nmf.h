@@ -178,6 +178,19 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
                                float eps, int iterations, int update_w, int objective, double *d_objective,
                                void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  gpumlib_nmf_iterate with the
+ * workspace taken from a static device array of GPUMLIB_NMF_NOALLOC_SCRATCH_BYTES (32 MiB) that the
+ * library holds on every GPU it runs on until the process ends; the call allocates nothing.  Same
+ * arguments, results and errors as gpumlib_nmf_iterate, plus hipErrorOutOfMemory when the problem's
+ * workspace (gpumlib_nmf_workspace_bytes) exceeds the array: e.g. FP32 16384 x 8192 needs 16.2 MiB
+ * at r = 64 and 32.6 MiB at r = 128 (rejected).  Every call on a GPU shares that GPU's array and
+ * leaves its contents behind: calls that overlap on one GPU (other streams, other threads) race. */
+#define GPUMLIB_NMF_NOALLOC_SCRATCH_BYTES (32u << 20)
+hipError_t gpumlib_nmf_iterate_noalloc(int rule, int precision, int n, int m, int r, const void *V, float *W,
+                                       float *H, float eps, int iterations, int update_w, int objective,
+                                       double *d_objective, hipStream_t stream);
+
 /* Objective of the current state, FP64, written to d_objective[0] (device). */
 hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, const void *V, const float *W,
                                  const float *H, float eps, int objective, double *d_objective, void *workspace,
