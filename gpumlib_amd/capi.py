@@ -65,6 +65,7 @@ SIGNATURES = {
     "gpumlib_nmf_kl_update_w": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "gpumlib_nmf_quality_workspace_bytes": (_sz, [_l]),
     "gpumlib_nmf_quality": (_i, [_vp, _vp, _l, _f, _vp, _sz, _vp, _vp]),
+    "gpumlib_nmf_quality_f32": (_i, [_vp, _vp, _l, _f, _vp, _sz, _vp, _vp]),
     "gpumlib_nmf_sgemm": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_gemm_bf16": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_gemm_fp8": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _i, _vp, _i, _i, _f, _vp, _i, _vp]),

@@ -53,11 +53,44 @@ size_t vstore_bytes(int precision, int n, int m) {
     return 0;
 }
 
+// Ranks above kMaxFusedRank (FP32 storage): the unfused iteration's workspace.
+static bool make_unfused_plan(int rule, int n, int m, int r, Plan* pl) {
+    memset(pl, 0, sizeof(*pl));
+    pl->rule = rule;
+    pl->precision = GPUMLIB_NMF_FP32;
+    pl->n = n;
+    pl->m = m;
+    pl->r = r;
+    pl->unfused = 1;
+    pl->rpad = round_up(r, 4);
+    const size_t nm = (size_t)n * m, rm = (size_t)r * m, nr = (size_t)n * r;
+    const size_t big = rm > nr ? rm : nr;
+    size_t off = 0;
+    pl->off_uWH = off;
+    off += align256(sizeof(float) * nm);
+    pl->off_uA = off;
+    off += align256(sizeof(float) * big);
+    pl->off_uB = off;
+    off += align256(sizeof(float) * big);
+    pl->off_uG = off;
+    off += align256(sizeof(float) * (size_t)r * r);
+    pl->off_usum = off;
+    off += align256(sizeof(float) * (size_t)r);
+    pl->off_uobj = off;
+    off += align256(sizeof(double) * (size_t)objective_wh_partials((long)nm));
+    pl->bytes = off;
+    return true;
+}
+
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
     if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
     if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8) return false;
     if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
-    if (n < 1 || m < 1 || r < 1 || r > kMaxFusedRank) return false;
+    if (n < 1 || m < 1 || r < 1) return false;
+    if (r > kMaxFusedRank) {
+        if (precision != GPUMLIB_NMF_FP32) return false;
+        return make_unfused_plan(rule, n, m, r, pl);
+    }
     if (precision == GPUMLIB_NMF_FP8 && r > 64) return false;
     memset(pl, 0, sizeof(*pl));
     pl->rule = rule;
@@ -229,6 +262,70 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
         CK(launch_pass_h_f32(a1, 0, objective, 0, st));
     }
     return launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st);
+}
+
+// Ranks above kMaxFusedRank: the reference's own GEMM sequence over materialised operands
+// (nmf_multiplicative_euclidian.cpp:29-67, nmf_multiplicative_divergence.cpp:29-50,
+// nmf_additive_euclidian.cpp:31-56, nmf_additive_divergence.cpp:30-51), the GEMMs on the
+// library's MFMA GEMM, the element-wise rules as in the fused path.
+hipError_t run_objective_unfused(const Plan& pl, const float* V, const float* W, const float* H, float eps,
+                                 int objective, double* d_objective, void* ws, hipStream_t st) {
+    char* base = (char*)ws;
+    float* WH = (float*)(base + pl.off_uWH);
+    const int n = pl.n, m = pl.m, r = pl.r;
+    CK(launch_sgemm('N', 'N', n, m, r, 1.f, W, n, H, r, 0.f, WH, n, st));
+    return launch_objective_wh(V, WH, (long)n * m, eps, objective, (double*)(base + pl.off_uobj), d_objective, st);
+}
+
+hipError_t run_iterations_unfused(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations,
+                                  int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
+    char* base = (char*)ws;
+    float* WH = (float*)(base + pl.off_uWH);
+    float* A = (float*)(base + pl.off_uA);
+    float* B = (float*)(base + pl.off_uB);
+    float* G = (float*)(base + pl.off_uG);
+    float* sum = (float*)(base + pl.off_usum);
+    const int n = pl.n, m = pl.m, r = pl.r;
+    const long nm = (long)n * m, rm = (long)r * m, nr = (long)n * r;
+    const int add = rule_is_additive(pl.rule);
+    for (int it = 0; it < iterations; it++) {
+        if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
+            CK(run_objective_unfused(pl, V, W, H, eps, objective, d_objective, ws, st));
+        if (rule_is_kl(pl.rule)) {
+            // H from the old W: WH = W H, sumW, Q = V / (WH + eps), H = rule(H, W^T Q, sumW)
+            CK(launch_sgemm('N', 'N', n, m, r, 1.f, W, n, H, r, 0.f, WH, n, st));
+            CK(launch_sum_w(W, n, r, eps, sum, st));
+            CK(launch_kl_ratio(V, WH, nm, eps, add, st));
+            CK(launch_sgemm('T', 'N', r, m, n, 1.f, W, n, WH, n, 0.f, A, r, st));
+            CK(launch_kl_apply(H, A, sum, rm, r, 1, add, st));
+            if (!update_w) continue;
+            // W from the new H, WH recomputed
+            CK(launch_sgemm('N', 'N', n, m, r, 1.f, W, n, H, r, 0.f, WH, n, st));
+            CK(launch_sum_h(H, r, m, eps, sum, st));
+            CK(launch_kl_ratio(V, WH, nm, eps, add, st));
+            CK(launch_sgemm('N', 'T', n, r, m, 1.f, WH, n, H, r, 0.f, A, n, st));
+            CK(launch_kl_apply(W, A, sum, nr, n, 0, add, st));
+        } else {
+            if (pl.rule != GPUMLIB_NMF_ADDITIVE_EUCLIDEAN) {  // that rule leaves H unchanged (fused path: same)
+                CK(launch_sgemm('T', 'N', r, m, n, 1.f, W, n, V, n, 0.f, A, r, st));  // W^T V
+                CK(launch_sgemm('T', 'N', r, r, n, 1.f, W, n, W, n, 0.f, G, r, st));  // W^T W
+                CK(launch_sgemm('N', 'N', r, m, r, 1.f, G, r, H, r, 0.f, B, r, st));  // W^T W H
+                if (add)
+                    CK(launch_add_update(H, A, B, rm, st));
+                else
+                    CK(launch_mu_update(A, B, H, rm, eps, st));
+            }
+            if (!update_w) continue;
+            CK(launch_sgemm('N', 'T', n, r, m, 1.f, V, n, H, r, 0.f, A, n, st));  // V H^T
+            CK(launch_sgemm('N', 'T', r, r, m, 1.f, H, r, H, r, 0.f, G, r, st));  // H H^T
+            CK(launch_sgemm('N', 'N', n, r, r, 1.f, W, n, G, r, 0.f, B, n, st));  // W H H^T
+            if (add)
+                CK(launch_add_update(W, A, B, nr, st));
+            else
+                CK(launch_mu_update(A, B, W, nr, eps, st));
+        }
+    }
+    return hipSuccess;
 }
 
 // Slice grams the low-precision updates sum themselves (0: the pass reduces them into G
@@ -421,7 +518,7 @@ hipError_t gpumlib_nmf_sum_w(const float* W, int n, int r, float eps, float* sum
 }
 
 hipError_t gpumlib_nmf_sum_h(const float* H, int r, int m, float eps, float* sumH, hipStream_t stream) {
-    if (m < 0 || r < 0 || r > 256) return hipErrorInvalidValue;
+    if (m < 0 || r < 0) return hipErrorInvalidValue;
     return launch_sum_h(H, r, m, eps, sumH, stream);
 }
 
@@ -454,6 +551,14 @@ hipError_t gpumlib_nmf_quality(const float* V, const float* WH, long count, floa
     const int np = quality_partials(count);
     if (!workspace || workspace_bytes < sizeof(double) * (size_t)np) return hipErrorInvalidValue;
     return launch_quality(V, WH, count, eps, (double*)workspace, np, d_quality, stream);
+}
+
+hipError_t gpumlib_nmf_quality_f32(const float* V, const float* WH, long count, float eps, void* workspace,
+                                   size_t workspace_bytes, float* d_quality, hipStream_t stream) {
+    if (count < 0 || !d_quality) return hipErrorInvalidValue;
+    const int np = quality_partials(count);
+    if (!workspace || workspace_bytes < sizeof(double) * (size_t)np) return hipErrorInvalidValue;
+    return launch_quality(V, WH, count, eps, (double*)workspace, np, nullptr, stream, d_quality);
 }
 
 hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, float alpha, const float* A, int lda,
@@ -490,8 +595,13 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
     if (objective != GPUMLIB_NMF_OBJ_NONE && !d_objective) return hipErrorInvalidValue;
     if (iterations == 0) {
         if (objective == GPUMLIB_NMF_OBJ_NONE) return hipSuccess;
+        if (pl.unfused)
+            return run_objective_unfused(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
         return run_objective(pl, V, W, H, eps, objective, d_objective, workspace, stream);
     }
+    if (pl.unfused)
+        return run_iterations_unfused(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective,
+                                      workspace, stream);
     if (precision == GPUMLIB_NMF_BF16)
         return run_iterations_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     if (precision == GPUMLIB_NMF_FP8)
@@ -507,6 +617,8 @@ hipError_t gpumlib_nmf_objective(int rule, int precision, int n, int m, int r, c
     if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
     if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || !d_objective) return hipErrorInvalidValue;
     if (objective < GPUMLIB_NMF_OBJ_FROBENIUS || objective > GPUMLIB_NMF_OBJ_QUALITY) return hipErrorInvalidValue;
+    if (pl.unfused)
+        return run_objective_unfused(pl, (const float*)V, W, H, eps, objective, d_objective, workspace, stream);
     return run_objective(pl, V, W, H, eps, objective, d_objective, workspace, stream);
 }
 
@@ -532,7 +644,7 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
                             const float* H, float eps, void* workspace, size_t workspace_bytes, hipStream_t stream) {
     Plan pl;
     if (!make_plan(rule, precision, n, m, r, &pl)) return hipErrorInvalidValue;
-    if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1))
+    if (!V || !W || !H || !workspace || workspace_bytes < pl.bytes || (which != 0 && which != 1) || pl.unfused)
         return hipErrorInvalidValue;
     const int kl = rule_is_kl(rule);
     // The arrival counters were zeroed by gpumlib_nmf_iterate and every pass leaves them at

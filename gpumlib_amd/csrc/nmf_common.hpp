@@ -45,6 +45,9 @@ __host__ __device__ inline size_t fp8_tiled_index(int k, int col, int ntiles) {
     return (((size_t)(k >> 9) * ntiles + (col >> 6)) * 64 + (col & 63)) * 512 + (k & 511);
 }
 
+// Objective selector of the objective-carrying passes and kernels.
+enum ObjKind { OBJ_NONE = 0, OBJ_FROB = 1, OBJ_KL = 2, OBJ_QUALITY = 3 };
+
 #if defined(__HIPCC__)
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -53,6 +56,39 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 //   A[row c][k h], B[k h][col c];  result register v holds D[row 8*(v/4) + 4*h + v%4][col c].
 __device__ __forceinline__ f32x16 mfma32x2(float a, float b, f32x16 acc) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+}
+
+// Additive Euclidean element rule (nmf_kernels.cu:31-39): x + (x / d2) * (d1 - d2),
+// set to 0 when negative (a NaN is kept: `v < 0` is false for it).  No eps.
+__device__ __forceinline__ float additive_rule(float x, float d1, float d2) {
+    const float v = x + (x / d2) * (d1 - d2);
+    return v < 0.f ? 0.f : v;
+}
+
+// Additive KL rule (nmf_kernels_additive_divergence.cu:109-121, 178-190):
+// x + (x / sum) * t, t = sum_k F(k) (ratio(k) - 1), set to 0 when negative.
+__device__ __forceinline__ float additive_kl_rule(float x, float sum, float t) {
+    const float v = x + (x / sum) * t;
+    return v < 0.f ? 0.f : v;
+}
+
+// Per-element objective terms in FP64 (ObjKind): Frobenius (v - wh)^2; KL
+// v ln(v / wh) - v + wh with 0 ln 0 = 0; the reference quality v log10(wh + eps) - wh.
+template <int OBJ>
+__device__ __forceinline__ double obj_term(float v, float wh, float eps) {
+    if (OBJ == OBJ_FROB) {
+        const double d = (double)v - (double)wh;
+        return d * d;
+    } else if (OBJ == OBJ_KL) {
+        const double dv = v, dwh = wh;
+        const double t = (v > 0.f) ? dv * log(dv / dwh) : 0.0;
+        return t - dv + dwh;
+    } else if (OBJ == OBJ_QUALITY) {
+        // reference: V[k] * log10(WH[k] + eps) - WH[k]  (nmf_kernels_quality.cu:35)
+        const float whe = wh + eps;
+        return (double)v * log10((double)whe) - (double)wh;
+    }
+    return 0.0;
 }
 
 // Row of the 32x32 MFMA result held in accumulator register v by half-wave h.
@@ -265,7 +301,5 @@ struct JobArgs {
 constexpr int kGroupMaxJobs = 1024;
 inline int group_table_ints(int maxj) { return 3 + 2 * maxj; }
 
-// Objective selector for the fused H pass.
-enum ObjKind { OBJ_NONE = 0, OBJ_FROB = 1, OBJ_KL = 2, OBJ_QUALITY = 3 };
 
 }  // namespace gpumlib_nmf

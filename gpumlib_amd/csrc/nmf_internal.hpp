@@ -79,12 +79,26 @@ hipError_t launch_mu_update(const float* nm, const float* dm, float* mm, long el
 hipError_t launch_add_update(float* X, const float* d1, const float* d2, long elements, hipStream_t st);
 hipError_t launch_sum_w(const float* W, int n, int r, float eps, float* out, hipStream_t st);
 hipError_t launch_sum_h(const float* H, int r, int m, float eps, float* out, hipStream_t st);
+// out_f32 non-null: the total is written there as a float instead of to out.
 hipError_t launch_quality(const float* V, const float* WH, long count, float eps, double* partials,
-                          int npartials, double* out, hipStream_t st);
+                          int npartials, double* out, hipStream_t st, float* out_f32 = nullptr);
 hipError_t launch_kl_update_h_wh(float* H, const float* W, const float* V, const float* WH,
                                  const float* sumW, int n, int m, int r, float eps, hipStream_t st, int add = 0);
 hipError_t launch_kl_update_w_wh(float* W, const float* H, const float* V, const float* WH,
                                  const float* sumH, int n, int m, int r, float eps, hipStream_t st, int add = 0);
+
+// Unfused iteration (ranks above kMaxFusedRank; nmf_unfused.hip).
+// Q = V / (WH + eps) (minus 1 when add != 0), in place over WH.
+hipError_t launch_kl_ratio(const float* V, float* WH, long count, float eps, int add, hipStream_t st);
+// X[k] = X[k] num[k] / sum[a(k)] (add: the additive KL rule), a(k) = k % rows when rank_is_row
+// (H, r x m), else k / rows (W, n x r).
+hipError_t launch_kl_apply(float* X, const float* num, const float* sum, long count, int rows, int rank_is_row,
+                           int add, hipStream_t st);
+// FP64 objective (OBJ_FROB / OBJ_KL / OBJ_QUALITY) of V against a materialised WH into out[0];
+// partials holds objective_wh_partials(count) doubles.
+int objective_wh_partials(long count);
+hipError_t launch_objective_wh(const float* V, const float* WH, long count, float eps, int obj, double* partials,
+                               double* out, hipStream_t st);
 
 // General FP32 GEMM, column-major, cuBLAS-style operation flags (nmf_gemm.hip).
 hipError_t launch_sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,

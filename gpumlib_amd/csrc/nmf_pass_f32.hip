@@ -59,23 +59,6 @@ __device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, 
     return v;
 }
 
-template <int OBJ>
-__device__ __forceinline__ double obj_term(float v, float wh, float eps) {
-    if (OBJ == OBJ_FROB) {
-        const double d = (double)v - (double)wh;
-        return d * d;
-    } else if (OBJ == OBJ_KL) {
-        const double dv = v, dwh = wh;
-        const double t = (v > 0.f) ? dv * log(dv / dwh) : 0.0;
-        return t - dv + dwh;
-    } else if (OBJ == OBJ_QUALITY) {
-        // reference: V[k] * log10(WH[k] + eps) - WH[k]  (nmf_kernels_quality.cu:35)
-        const float whe = wh + eps;
-        return (double)v * log10((double)whe) - (double)wh;
-    }
-    return 0.0;
-}
-
 // ---------------------------------------------------------------------------
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
 // entry; GEMM2 false: objective only (no partials written).

@@ -10,7 +10,8 @@ namespace gpumlib_nmf {
 // eps of the reference: SMALL_VALUE_TO_ADD_DENOMINATOR = CUDA_VALUE(10e-9) = 1e-8f
 // (src/cuda/nmf/nmf_kernels.h:34).
 constexpr float kReferenceEpsilon = 10e-9f;
-// Fused kernels hold up to 4 MFMA row blocks of 32 rank rows.
+// Fused kernels hold up to 4 MFMA row blocks of 32 rank rows.  Larger ranks (FP32 storage)
+// run the unfused iteration: the reference's GEMM sequence over materialised operands.
 constexpr int kMaxFusedRank = 128;
 // Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
 constexpr int kTargetWorkgroups = 512;
@@ -22,6 +23,7 @@ inline bool rule_is_additive(int rule) { return rule >= 2; }
 
 struct Plan {
     int rule, precision, n, m, r;
+    int unfused;          // 1: r > kMaxFusedRank, the unfused iteration (offsets off_u*)
     int npad, rpad, ldr, ldp;
     int ldv8, ldt8;       // BF16 storage: padded contraction lengths (bf16_ld of n and of m): the K extent
                           // of the tiled V and V^T and the leading dimensions of the W / H^T shadows
@@ -33,6 +35,9 @@ struct Plan {
     int nobjp;
     int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (fp8_ld of n, m) and their 32-blocks
     size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
+    // unfused iteration: WH (n x m; the KL ratio in place), numerator and denominator products
+    // (max(r m, n r) floats each), the r x r gram, the clamped sums (r), FP64 objective partials
+    size_t off_uWH, off_uA, off_uB, off_uG, off_usum, off_uobj;
 };
 
 // Padded contraction lengths of the narrow V stores and leading dimensions of the operand
@@ -70,6 +75,12 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
                           int objective, double* d_objective, void* ws, hipStream_t st);
 hipError_t run_objective(const Plan& pl, const void* V, const float* W, const float* H, float eps, int objective,
                          double* d_objective, void* ws, hipStream_t st);
+hipError_t run_iterations_unfused(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations,
+                                  int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
+hipError_t run_objective_unfused(const Plan& pl, const float* V, const float* W, const float* H, float eps,
+                                 int objective, double* d_objective, void* ws, hipStream_t st);
+hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                              int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
                                int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 

@@ -25,20 +25,6 @@ __device__ __forceinline__ float sum_slices(const float* src, int S, long pstrid
     return v;
 }
 
-// Additive Euclidean element rule (nmf_kernels.cu:31-39): x + (x / d2) * (d1 - d2),
-// set to 0 when negative (a NaN is kept: `v < 0` is false for it).  No eps.
-__device__ __forceinline__ float additive_rule(float x, float d1, float d2) {
-    const float v = x + (x / d2) * (d1 - d2);
-    return v < 0.f ? 0.f : v;
-}
-
-// Additive KL rule (nmf_kernels_additive_divergence.cu:109-121, 178-190):
-// x + (x / sum) * t, t = sum_k F(k) (ratio(k) - 1), set to 0 when negative.
-__device__ __forceinline__ float additive_kl_rule(float x, float sum, float t) {
-    const float v = x + (x / sum) * t;
-    return v < 0.f ? 0.f : v;
-}
-
 // Euclidean H update, reference nmf_multiplicative_euclidian.cpp:36-45 and
 // nmf_kernels.cu:46:  H(a,u) *= WtV(a,u) / (WtWH(a,u) + eps), WtWH = WtW H.
 // WtV(a,u) = sum_s P[s][u*ldr + a];  WtW(a,b) = G[b*ldr + a] (reduced by the pass).
@@ -625,6 +611,21 @@ __global__ __launch_bounds__(NTHREADS) void k_reduce_f64(const double* __restric
     }
 }
 
+// The same reduction, the total written as a float (the reference quality's type).
+__global__ __launch_bounds__(NTHREADS) void k_reduce_f64_to_f32(const double* __restrict__ in, int count, float* out) {
+    __shared__ double red[NTHREADS / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += NTHREADS) v += in[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NTHREADS / 64; w++) t += red[w];
+        out[0] = (float)t;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Reference-shaped kernels over materialised operands.
 
@@ -658,22 +659,28 @@ __global__ __launch_bounds__(NTHREADS) void k_sum_w(const float* __restrict__ W,
 }
 
 // Row sums of H (r x m, column-major), clamped at eps: nmf_kernels.cu:149-195.
-// A workgroup reads whole columns of H (contiguous) and keeps r partial sums.
-__global__ __launch_bounds__(NTHREADS) void k_sum_h(const float* __restrict__ H, int r, int m, float eps,
-                                                    float* out) {
-    __shared__ float red[NTHREADS];
-    // thread t owns rank row a = t % r of column group t / r
-    const int per = NTHREADS / r;  // columns processed per sweep (r <= 256)
-    const int a = threadIdx.x % r, g = threadIdx.x / r;
+// One workgroup per block of up to 64 rank rows, any r: thread t sums row a0 + t % rb
+// over the columns u = g, g + groups, ... (g = t / rb), so a wave reads rb consecutive
+// floats of a column; the column groups are then added in group order.
+constexpr int kSumHThreads = 1024;
+__global__ __launch_bounds__(kSumHThreads) void k_sum_h(const float* __restrict__ H, int r, int m, float eps,
+                                                        float* out) {
+    __shared__ float red[kSumHThreads];
+    const int a0 = blockIdx.x * 64;
+    const int rb = min(64, r - a0);
+    const int groups = kSumHThreads / rb;
+    const int al = threadIdx.x % rb, g = threadIdx.x / rb;
     float v = 0.f;
-    if (g < per)
-        for (int u = g; u < m; u += per) v += H[(size_t)u * r + a];
-    red[threadIdx.x] = (g < per) ? v : 0.f;
+    if (g < groups) {
+#pragma unroll 8
+        for (int u = g; u < m; u += groups) v += H[(size_t)u * r + a0 + al];
+    }
+    red[threadIdx.x] = v;
     __syncthreads();
-    if (threadIdx.x < r) {
+    if (threadIdx.x < rb) {
         float s = 0.f;
-        for (int j = 0; j < per; j++) s += red[j * r + threadIdx.x];
-        out[threadIdx.x] = (s < eps) ? eps : s;
+        for (int j = 0; j < groups; j++) s += red[j * rb + threadIdx.x];
+        out[a0 + threadIdx.x] = (s < eps) ? eps : s;
     }
 }
 
@@ -969,16 +976,19 @@ hipError_t launch_sum_w(const float* W, int n, int r, float eps, float* out, hip
 
 hipError_t launch_sum_h(const float* H, int r, int m, float eps, float* out, hipStream_t st) {
     if (r <= 0) return hipSuccess;
-    if (r > NTHREADS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sum_h, dim3(1), dim3(NTHREADS), 0, st, H, r, m, eps, out);
+    hipLaunchKernelGGL(k_sum_h, dim3((r + 63) / 64), dim3(kSumHThreads), 0, st, H, r, m, eps, out);
     return hipGetLastError();
 }
 
 hipError_t launch_quality(const float* V, const float* WH, long count, float eps, double* partials, int npartials,
-                          double* out, hipStream_t st) {
+                          double* out, hipStream_t st, float* out_f32) {
     hipLaunchKernelGGL(k_quality, dim3(npartials), dim3(NTHREADS), 0, st, V, WH, count, eps, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (out_f32) {
+        hipLaunchKernelGGL(k_reduce_f64_to_f32, dim3(1), dim3(NTHREADS), 0, st, partials, npartials, out_f32);
+        return hipGetLastError();
+    }
     return launch_reduce_f64(partials, npartials, out, st);
 }
 

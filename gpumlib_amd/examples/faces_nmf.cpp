@@ -3,7 +3,8 @@
 // drop-in C++ API (HostMatrix + NMF_Multiplicative*).
 //
 //   * images: 19 x 19 binary PGM (P5, maxval 255) files face00001.pgm ... as the
-//     columns of V, scaled by 1/255 or histogram-equalised (LoadImage, :409-472);
+//     columns of V, scaled by 1/255 or histogram-equalised, raster read as the reference
+//     reads it (LoadImage, :409-472; see load_pgm), --dump-v writes V as loaded;
 //     or a seeded synthetic V (--synthetic n m r0 seed);
 //   * training: DoIteration(true) until the quality improvement changes and drops
 //     below tol, or the iteration cap (Train, :658-689);
@@ -43,16 +44,24 @@ struct Options {
     int max_iterations = 1000;
     double tol = 0.01;
     bool equalize = false;
+    bool pgm_standard = false;
+    bool load_only = false;
+    std::string dump_v;
     int group = 0;
 };
 
-bool load_pgm(const std::string& path, int column, bool equalize, HostMatrix<cudafloat>& V) {
+// The reference reads the raster right after the maxval token (`imageFile >> maxGrayValue`
+// then `read`, FacesNMFDlg.cpp:433-441), so the whitespace byte that ends the header becomes
+// pixel 0 and the last raster byte is never read: V is shifted by one byte against the image.
+// That is the default here, so V is the reference's V; standard = true skips that one byte as
+// the Netpbm format defines (no reference counterpart).
+bool load_pgm(const std::string& path, int column, bool equalize, bool standard, HostMatrix<cudafloat>& V) {
     std::ifstream f(path, std::ios::binary);
     std::string magic;
     int w = 0, h = 0, maxv = 0;
     f >> magic >> w >> h >> maxv;
     if (!f || magic != "P5" || w != 19 || h != 19 || maxv != 255) return false;
-    f.get();  // the single whitespace byte before the raster
+    if (standard) f.get();  // the single whitespace byte before the raster
     const int pixels = w * h;
     std::vector<unsigned char> buf(pixels);
     f.read(reinterpret_cast<char*>(buf.data()), pixels);
@@ -120,9 +129,9 @@ HostMatrix<cudafloat> columns(const HostMatrix<cudafloat>& V, size_t c0, size_t 
 
 int usage() {
     fprintf(stderr,
-            "usage: faces_nmf (--pgm DIR [--images N] [--equalize] | --synthetic n m r0 seed)\n"
+            "usage: faces_nmf (--pgm DIR [--images N] [--equalize] [--pgm-standard] | --synthetic n m r0 seed)\n"
             "                 [--rank R] [--method euclidean|kl] [--max-iterations N] [--tol T]\n"
-            "                 [--groups G] [--test-images T] [--seed S] [--out DIR]\n");
+            "                 [--groups G] [--test-images T] [--seed S] [--out DIR] [--dump-v FILE] [--load-only]\n");
     return 2;
 }
 
@@ -137,6 +146,9 @@ int main(int argc, char** argv) {
         if (a == "--pgm" && (v = next())) o.pgm_dir = v;
         else if (a == "--images" && (v = next())) o.images = atoi(v);
         else if (a == "--equalize") o.equalize = true;
+        else if (a == "--pgm-standard") o.pgm_standard = true;
+        else if (a == "--dump-v" && (v = next())) o.dump_v = v;
+        else if (a == "--load-only") o.load_only = true;
         else if (a == "--synthetic" && i + 4 < argc) {
             o.syn_n = atoi(argv[++i]);
             o.syn_m = atoi(argv[++i]);
@@ -160,7 +172,7 @@ int main(int argc, char** argv) {
         for (int k = 0; k < o.images; k++) {
             std::ostringstream p;
             p << o.pgm_dir << "/face" << std::setfill('0') << std::setw(5) << (k + 1) << ".pgm";
-            if (!load_pgm(p.str(), k, o.equalize, Vall)) {
+            if (!load_pgm(p.str(), k, o.equalize, o.pgm_standard, Vall)) {
                 fprintf(stderr, "cannot load %s\n", p.str().c_str());
                 return 1;
             }
@@ -175,6 +187,15 @@ int main(int argc, char** argv) {
         return usage();
     }
     const size_t n = Vall.Rows(), mall = Vall.Columns();
+    if (!o.dump_v.empty()) {  // V as loaded: n x columns float32, column-major
+        FILE* fv = fopen(o.dump_v.c_str(), "wb");
+        if (!fv || fwrite(Vall.Pointer(), sizeof(cudafloat), Vall.Elements(), fv) != Vall.Elements()) return 1;
+        fclose(fv);
+    }
+    if (o.load_only) {  // loading and --dump-v only: nothing touches the GPU
+        printf("{\"n\": %zu, \"columns\": %zu}\n", n, mall);
+        return 0;
+    }
     const size_t mtest = (size_t)o.test_images, mtrain = mall - mtest;
     const int r = o.rank;
     HostMatrix<cudafloat> V = columns(Vall, 0, mtrain);

@@ -72,8 +72,19 @@ class NMF {
         if (hipEventCreateWithFlags(&qualityReady, hipEventDisableTiming) != hipSuccess) qualityReady = nullptr;
         const size_t bytes =
             gpumlib_nmf_workspace_bytes(rule, precision, (int)V.Rows(), (int)V.Columns(), (int)W.Columns());
+        // any rank plans (ranks above 128 run the unfused GEMM sequence); 0 means an empty or
+        // out-of-range shape, which the reference rejects by assert as well
+        assert(bytes > 0);
+        if (bytes == 0) {
+            RecordError(hipErrorInvalidValue);
+            return;
+        }
         workspace.ResizeWithoutPreservingData(bytes);
-        if (workspace.Length() != bytes) lastError = hipErrorOutOfMemory;
+        if (workspace.Length() != bytes) RecordError(hipErrorOutOfMemory);
+    }
+
+    void RecordError(hipError_t e) {
+        if (e != hipSuccess && lastError == hipSuccess) lastError = e;
     }
 
     // Reads the quality computed at the previous quality point (0 before the first).
@@ -123,6 +134,7 @@ class NMF {
 
     // One iteration of the object's update rule (H, then W unless updateW is false).
     void Iterate(bool updateW) {
+        if (workspace.Length() == 0) return;  // construction failed: LastError() says why
         const bool qualityPoint = qualityInterval > 0 && (iterationsDone % qualityInterval) == 0;
         if (qualityPoint) ConsumeQuality();
         hipError_t e = gpumlib_nmf_iterate(rule, precision, (int)V.Rows(), (int)V.Columns(), (int)W.Columns(),
@@ -134,7 +146,7 @@ class NMF {
             vquality.UpdateValueAsync(d_quality.Pointer(), stream);
             if (qualityReady && hipEventRecord(qualityReady, stream) == hipSuccess) qualityPending = true;
         }
-        if (e != hipSuccess) lastError = e;
+        RecordError(e);
         iterationsDone++;
     }
 
@@ -159,9 +171,14 @@ class NMF {
     //! Gets the approximation W H of V (blocking read-back)
     HostMatrix<cudafloat> GetWH() {
         (void)hipStreamSynchronize(stream);
-        if (WH.Rows() != V.Rows() || WH.Columns() != V.Columns())
-            WH.ResizeWithoutPreservingData(V.Rows(), V.Columns());
-        DeviceMatrix<cudafloat>::Multiply(W, H, WH);  // makes WH column-major
+        if (WH.Rows() != V.Rows() || WH.Columns() != V.Columns() || WH.IsRowMajor())
+            WH = DeviceMatrix<cudafloat>(V.Rows(), V.Columns(), ColumnMajor);
+        // DeviceMatrix<cudafloat>::Multiply(W, H, WH) with W, H and WH column-major, on the
+        // object's stream and with its status recorded
+        RecordError(gpumlib_nmf_sgemm('N', 'N', (int)WH.Rows(), (int)WH.Columns(), (int)W.Columns(), CUDA_VALUE(1.0),
+                                      W.Pointer(), (int)W.Rows(), H.Pointer(), (int)H.Rows(), CUDA_VALUE(0.0),
+                                      WH.Pointer(), (int)WH.Rows(), stream));
+        RecordError(hipStreamSynchronize(stream));
         return HostMatrix<cudafloat>(WH);
     }
 
@@ -186,7 +203,7 @@ class NMF {
                                              d_quality.Pointer(), workspace.Pointer(), workspace.Length(), stream);
         if (e == hipSuccess) e = hipMemcpyAsync(&value, d_quality.Pointer(), sizeof(double), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) lastError = e;
+        RecordError(e);
         return value;
     }
     //! First error reported by the library for this object (hipSuccess if none).

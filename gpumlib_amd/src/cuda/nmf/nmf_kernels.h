@@ -16,39 +16,34 @@ namespace GPUMLib {
 // reference value (nmf_kernels.h:34).
 #define SMALL_VALUE_TO_ADD_DENOMINATOR (CUDA_VALUE(10e-9))
 
-// sumW[a] = max(sum_i W(i,a), eps); blockSize is accepted and ignored.
-inline void KernelSumW(int /*blockSize*/, cudafloat* W, int n, int r, cudafloat* sumW) {
-    (void)gpumlib_nmf_sum_w(W, n, r, SMALL_VALUE_TO_ADD_DENOMINATOR, sumW, nullptr);
+// sumW[a] = max(sum_i W(i,a), eps), any r; blockSize is accepted and ignored.  Null stream, as
+// the reference; a launch error is returned (the reference's helper returned nothing).
+inline hipError_t KernelSumW(int /*blockSize*/, cudafloat* W, int n, int r, cudafloat* sumW) {
+    return gpumlib_nmf_sum_w(W, n, r, SMALL_VALUE_TO_ADD_DENOMINATOR, sumW, nullptr);
 }
 
-// sumH[a] = max(sum_u H(a,u), eps); blockSize is accepted and ignored.
-inline void KernelSumH(int /*blockSize*/, cudafloat* H, int r, int m, cudafloat* sumH) {
-    (void)gpumlib_nmf_sum_h(H, r, m, SMALL_VALUE_TO_ADD_DENOMINATOR, sumH, nullptr);
+// sumH[a] = max(sum_u H(a,u), eps), any r; blockSize is accepted and ignored.
+inline hipError_t KernelSumH(int /*blockSize*/, cudafloat* H, int r, int m, cudafloat* sumH) {
+    return gpumlib_nmf_sum_h(H, r, m, SMALL_VALUE_TO_ADD_DENOMINATOR, sumH, nullptr);
 }
 
-// quality[0] = sum_k V[k] log10(WH[k] + eps) - WH[k] over n elements, written as a
-// cudafloat like the reference.  Runs on the null stream; the FP64 partial sums
-// live in a small device buffer owned by this helper for the life of the process.
-inline void KernelNMFquality(int /*blockSize*/, cudafloat* V, cudafloat* WH, int n, cudafloat* quality) {
-    static double* scratch = nullptr;
+// quality[0] = sum_k V[k] log10(WH[k] + eps) - WH[k] over n elements, written as a cudafloat
+// like the reference, stream-ordered on the null stream (nothing is read back to the host).
+// The FP64 partial sums live in a device buffer this helper allocates on the first call (and
+// again only when a larger n arrives), kept for the life of the process.
+inline hipError_t KernelNMFquality(int /*blockSize*/, cudafloat* V, cudafloat* WH, int n, cudafloat* quality) {
+    static void* scratch = nullptr;
     static size_t scratchBytes = 0;
-    const size_t need = gpumlib_nmf_quality_workspace_bytes(n) + sizeof(double);
+    const size_t need = gpumlib_nmf_quality_workspace_bytes(n);
     if (scratchBytes < need) {
         if (scratch) (void)hipFree(scratch);
         scratch = nullptr;
         scratchBytes = 0;
-        if (hipMalloc((void**)&scratch, need) != hipSuccess) return;
+        const hipError_t e = hipMalloc(&scratch, need);
+        if (e != hipSuccess) return e;
         scratchBytes = need;
     }
-    double* result = scratch;
-    if (gpumlib_nmf_quality(V, WH, n, SMALL_VALUE_TO_ADD_DENOMINATOR, scratch + 1, scratchBytes - sizeof(double),
-                            result, nullptr) != hipSuccess)
-        return;
-    double q = 0.0;
-    if (hipMemcpy(&q, result, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
-        const cudafloat qf = (cudafloat)q;
-        (void)hipMemcpy(quality, &qf, sizeof(cudafloat), hipMemcpyHostToDevice);
-    }
+    return gpumlib_nmf_quality_f32(V, WH, n, SMALL_VALUE_TO_ADD_DENOMINATOR, scratch, scratchBytes, quality, nullptr);
 }
 
 }  // namespace GPUMLib

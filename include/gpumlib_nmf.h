@@ -69,7 +69,7 @@ hipError_t gpumlib_nmf_mu_update(const float *nm, const float *dm, float *m, lon
 /* sumW[a] = max(sum_i W(i,a), eps).  Replaces KernelSumW (nmf_kernels.h:64; nmf_kernels.cu:63-147). */
 hipError_t gpumlib_nmf_sum_w(const float *W, int n, int r, float eps, float *sumW, hipStream_t stream);
 
-/* sumH[a] = max(sum_u H(a,u), eps), r <= 256.  Replaces KernelSumH (nmf_kernels.h:73; nmf_kernels.cu:149-233). */
+/* sumH[a] = max(sum_u H(a,u), eps).  Replaces KernelSumH (nmf_kernels.h:73; nmf_kernels.cu:149-233). */
 hipError_t gpumlib_nmf_sum_h(const float *H, int r, int m, float eps, float *sumH, hipStream_t stream);
 
 /* KL updates over a materialised WH = W H (the reference kernel contract):
@@ -98,6 +98,10 @@ hipError_t gpumlib_nmf_additive_kl_update_w(float *W, const float *H, const floa
 size_t gpumlib_nmf_quality_workspace_bytes(long count);
 hipError_t gpumlib_nmf_quality(const float *V, const float *WH, long count, float eps, void *workspace,
                                size_t workspace_bytes, double *d_quality, hipStream_t stream);
+/* The same quality, written as a float to d_quality[0] (the type KernelNMFquality writes, nmf_kernels.h:44);
+ * stream-ordered, nothing read back to the host. */
+hipError_t gpumlib_nmf_quality_f32(const float *V, const float *WH, long count, float eps, void *workspace,
+                                   size_t workspace_bytes, float *d_quality, hipStream_t stream);
 
 /* C = alpha op(A) op(B) + beta C, column-major, op 'N' or 'T'.  Replaces the cublasSgemm calls of
  * DeviceMatrix<float>::Multiply / MultiplyBySelfTranspose (src/memory/DeviceMatrix.h:195-227). */
@@ -173,7 +177,10 @@ size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r)
  * the V store of `precision` (GPUMLIB_NMF_FP32: the float n x m matrix itself).  When objective != OBJ_NONE the
  * objective of the state on entry is written, in FP64, to d_objective[0] (device memory); it is
  * fused into the first iteration's pass over V.  update_w = 0 updates H only (DoIteration(false)).
- * Ranks up to 128 are supported. */
+ * Ranks up to 128 (up to 64 with FP8 storage) run the fused passes.  Larger ranks run, with FP32 storage
+ * only, the reference's own sequence of GEMMs over materialised operands (W^T V, W^T W, W^T W H, V H^T,
+ * H H^T, W H H^T; KL: W H and the ratio V / (W H + eps) in an n x m buffer) on the library's MFMA GEMM;
+ * their workspace then holds that n x m buffer. */
 hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, const void *V, float *W, float *H,
                                float eps, int iterations, int update_w, int objective, double *d_objective,
                                void *workspace, size_t workspace_bytes, hipStream_t stream);
@@ -293,6 +300,9 @@ hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int trial_iter
 /*    grouped launches (optionally replayed from a hipGraph), and retires a   */
 /*    problem when its relative objective change over one iteration, measured */
 /*    each round, is below its tolerance or it reached max_iterations.        */
+/*    Problems advance in whole rounds, so a cap that is not a multiple of    */
+/*    round_iterations is reached at the end of the round that crosses it     */
+/*    (cap 15, rounds of 10: 20 iterations); the result reports the count run. */
 /*    Times are seconds since the server was created.                          */
 /* ------------------------------------------------------------------------ */
 typedef struct gpumlib_nmf_server *gpumlib_nmf_server_handle;

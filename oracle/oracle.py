@@ -317,3 +317,51 @@ def np_quality(V, W, H, eps):
     # the reference adds eps in float before log10 (nmf_kernels_quality.cu:34-35)
     whe = (WH.astype(np.float32) + np.float32(eps)).astype(np.float64)
     return float(np.sum(V * np.log10(whe) - WH))
+
+
+# ---------------------------------------------------------------------------
+# FacesNMF image loading (src/examples/FacesNMF/FacesNMFDlg.cpp:409-472).
+_WS = b" \t\n\v\f\r"
+
+
+def pgm_column(raw, equalize, standard=False, width=19, height=19):
+    """The V column LoadImage builds from the bytes of one P5 file (float32, width*height).
+
+    The header is read as `imageFile >> s >> width >> height >> maxGrayValue` reads it
+    (each token after skipping whitespace, the last one stopping before the whitespace
+    that follows it); the raster is read from there (:441), so the whitespace byte that
+    ends the header is pixel 0.  standard=True skips that byte first (Netpbm).  Scaling
+    (:464-469): p / 255 in float.  Histogram equalisation (:443-462): float counts,
+    running sums, the first non-zero running sum as cdfmin, then (h - cdfmin) /
+    (pixels - cdfmin) in float, looked up per pixel.  Returns None where LoadImage fails."""
+    pos = 0
+
+    def token():
+        nonlocal pos
+        while pos < len(raw) and raw[pos] in _WS:
+            pos += 1
+        start = pos
+        while pos < len(raw) and raw[pos] not in _WS:
+            pos += 1
+        return raw[start:pos]
+
+    try:
+        magic, w, h, maxv = token(), int(token()), int(token()), int(token())
+    except ValueError:
+        return None
+    if magic != b"P5" or w != width or h != height or maxv != 255:
+        return None
+    if standard:
+        pos += 1
+    pixels = width * height
+    buf = np.frombuffer(raw[pos:pos + pixels], np.uint8)
+    if buf.size < pixels:
+        return None
+    if not equalize:
+        return buf.astype(np.float32) / np.float32(255.0)
+    hist = np.cumsum(np.bincount(buf, minlength=256)).astype(np.float32)  # exact: counts <= pixels
+    i = int(np.argmax(hist != 0))
+    cdfmin = hist[i]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hist[i:] = (hist[i:] - cdfmin) / (np.float32(pixels) - cdfmin)
+    return hist[buf]

@@ -38,10 +38,16 @@ def test_workspace_bytes(native):
     ok = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, 16384, 8192, 64)
     assert ok > 0 and ok < 64 << 20  # partials only: far below V itself (512 MiB)
     assert native.gpumlib_nmf_workspace_bytes(capi.KL, capi.FP32, 1, 1, 1) > 0
-    for bad in [(capi.EUCLIDEAN, capi.FP32, 10, 10, 0), (capi.EUCLIDEAN, capi.FP32, 10, 10, 129),
-                (7, capi.FP32, 10, 10, 4), (capi.EUCLIDEAN, 9, 10, 10, 4), (capi.KL, capi.FP32, 0, 10, 4),
-                (capi.EUCLIDEAN, capi.BF16, 10, 10, 129), (capi.EUCLIDEAN, capi.FP8, 10, 10, 65)]:
+    for bad in [(capi.EUCLIDEAN, capi.FP32, 10, 10, 0), (7, capi.FP32, 10, 10, 4), (capi.EUCLIDEAN, 9, 10, 10, 4),
+                (capi.KL, capi.FP32, 0, 10, 4), (capi.EUCLIDEAN, capi.BF16, 10, 10, 129),
+                (capi.EUCLIDEAN, capi.FP8, 10, 10, 65)]:
         assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
+    # ranks above 128 plan the unfused iteration (FP32 storage only): its workspace holds an
+    # n x m buffer (W H, or the KL ratio) beside the products
+    for rule in (capi.EUCLIDEAN, capi.KL, capi.ADDITIVE_KL, capi.ADDITIVE_EUCLIDEAN):
+        big = native.gpumlib_nmf_workspace_bytes(rule, capi.FP32, 1000, 700, 129)
+        assert big >= 4 * 1000 * 700 + 2 * 4 * 129 * 1000
+    assert native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, 10, 10, 300) > 0
     # the largest rank of each precision plans: 128 (FP32, BF16), 64 (FP8)
     for ok in [(capi.EUCLIDEAN, capi.BF16, 10, 10, 128), (capi.EUCLIDEAN, capi.BF16, 10, 10, 65),
                (capi.EUCLIDEAN, capi.FP8, 10, 10, 64), (capi.KL, capi.FP32, 10, 10, 128)]:
@@ -79,8 +85,12 @@ def test_iterate_rejects_bad_arguments(native):
     assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, ws - 1, None) == HIP_INVALID_VALUE
     # objective requested without an output
     assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, 1, 1, 1, None, fake, ws, None) == HIP_INVALID_VALUE
-    # unsupported rank
-    assert f(0, 0, 64, 64, 200, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, 1 << 30, None) == HIP_INVALID_VALUE
+    # unsupported rank: above 128 with narrow storage (FP32 storage plans any rank)
+    assert f(0, 1, 64, 64, 200, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, 1 << 30, None) == HIP_INVALID_VALUE
+    assert f(0, 0, 64, 64, 0, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, 1 << 30, None) == HIP_INVALID_VALUE
+    # an unfused-rank workspace one byte short
+    wsu = native.gpumlib_nmf_workspace_bytes(capi.KL, capi.FP32, 64, 64, 200)
+    assert f(1, 0, 64, 64, 200, fake, fake, fake, 1e-8, 1, 1, 0, None, fake, wsu - 1, None) == HIP_INVALID_VALUE
     # negative iteration count
     assert f(0, 0, 64, 64, 8, fake, fake, fake, 1e-8, -1, 1, 0, None, fake, ws, None) == HIP_INVALID_VALUE
     # zero iterations, no objective: nothing to do
@@ -99,11 +109,15 @@ def test_other_entry_points_reject_bad_arguments(native):
     # empty products launch nothing
     assert native.gpumlib_nmf_gemm_bf16(b"N", b"T", 0, 4, 4, 1.0, None, 1, None, 4, 0.0, None, 1, None) == 0
     assert native.gpumlib_nmf_quality(None, None, 10, 1e-8, None, 0, None, None) == HIP_INVALID_VALUE
-    assert native.gpumlib_nmf_sum_h(None, 300, 10, 1e-8, None, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_sum_h(None, -1, 10, 1e-8, None, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_sum_h(None, 300, -1, 1e-8, None, None) == HIP_INVALID_VALUE
+    assert native.gpumlib_nmf_quality_f32(None, None, 10, 1e-8, None, 0, None, None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_mu_update(None, None, None, -1, 1e-8, None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_objective(0, 0, 8, 8, 2, 0x10, 0x10, 0x10, 1e-8, 0, 0x10, 0x10, 1 << 20,
                                         None) == HIP_INVALID_VALUE
     assert native.gpumlib_nmf_pass(0, 0, 2, 8, 8, 2, 0x10, 0x10, 0x10, 1e-8, 0x10, 1 << 20, None) == HIP_INVALID_VALUE
+    # the single-pass entry runs the fused passes only (r <= 128)
+    assert native.gpumlib_nmf_pass(0, 0, 0, 8, 8, 130, 0x10, 0x10, 0x10, 1e-8, 0x10, 1 << 20, None) == HIP_INVALID_VALUE
     import ctypes
 
     h = ctypes.c_void_p()

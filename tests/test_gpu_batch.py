@@ -345,3 +345,17 @@ def test_server_left_running_at_process_exit(native, cuda):
                        timeout=120)
     assert p.returncode == 0, p.stderr[-2000:]
     assert "submitted" in p.stdout
+
+
+def test_server_cap_is_reached_at_a_round_end(native, orc, cuda):
+    """Jobs advance in whole rounds: a cap of 15 with rounds of 10 stops after the second
+    round, 20 iterations, and the result says so (and matches the oracle at 20)."""
+    s = Server(native, budget=64 << 20)
+    job = (G.EUCLIDEAN, 300, 200, 16)
+    V, W, H = make_job(91, *job)
+    gw, gh, res = s.wait(s.submit(*job, V, W, H, tol=0.0, max_iters=15), 300, 200, 16)
+    s.close()
+    assert res.iterations == 20 and not res.converged
+    v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+    orc.euc_iter(300, 200, 16, v, w, h, G.REFERENCE_EPS, iters=20)
+    assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4

@@ -92,3 +92,35 @@ def test_quality_stopping(native, cuda, tmp_path):
     res = run(["--synthetic", "361", "400", "10", "3", "--rank", "10", "--max-iterations", "5000", "--tol", "1e-3"],
               tmp_path)
     assert 3 <= res["iterations"] < 5000
+
+
+@pytest.mark.parametrize("method,equalize", [("euclidean", False), ("kl", True)])
+def test_cbcl_faces_train_and_project(native, orc, cuda, tmp_path, method, equalize):
+    """The demo's own data: the 2429 CBCL faces (V 361 x 2429 as LoadImage builds it), rank
+    49 (Definitions.h:29), 2000 training and 429 test images, against the FP64 oracle."""
+    from test_faces_loader import cbcl_blobs, write_files
+
+    blobs = cbcl_blobs()
+    write_files(tmp_path / "img", blobs)
+    r, iters, mtest, seed = 49, 60, 429, 0x5EED
+    args = ["--pgm", str(tmp_path / "img"), "--images", str(len(blobs)), "--rank", str(r), "--method", method,
+            "--max-iterations", str(iters), "--tol", "-1e30", "--test-images", str(mtest), "--seed", str(seed)]
+    res = run(args + (["--equalize"] if equalize else []), tmp_path)
+    assert res["n"] == 361 and res["m_train"] == 2000 and res["iterations"] == iters
+    Vm = np.stack([orc.pgm_column(b, equalize) for b in blobs], axis=1).astype(np.float64)
+    n, mt = 361, 2000
+    step = orc.np_euc_iter if method == "euclidean" else orc.np_kl_iter
+    w = G.as_matrix(uniform(seed, 4, n * r), n, r).astype(np.float64)
+    h = G.as_matrix(uniform(seed, 5, r * mt), r, mt).astype(np.float64)
+    obj = orc.np_frob if method == "euclidean" else orc.np_kl
+    f0 = obj(Vm[:, :mt], w, h)
+    for _ in range(iters):
+        w, h = step(Vm[:, :mt], w, h, G.REFERENCE_EPS)
+    assert obj(Vm[:, :mt], w, h) < 0.5 * f0  # it learns
+    assert rel_fro(read_csv(tmp_path / "W.csv", n), w) < 1e-4
+    assert rel_fro(read_csv(tmp_path / "Htrain.csv", r), h) < 1e-4
+    wt = read_csv(tmp_path / "W.csv", n)
+    ht = G.as_matrix(uniform(seed, 6, r * mtest), r, mtest).astype(np.float64)
+    for _ in range(iters):
+        _, ht = step(Vm[:, mt:], wt, ht, G.REFERENCE_EPS, update_w=False)
+    assert rel_fro(read_csv(tmp_path / "Htest.csv", r), ht) < 1e-4
