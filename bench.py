@@ -70,11 +70,14 @@ def parse():
                          "operation order) for about this long, at least one iteration (0 = off; a C4 iteration "
                          "takes tens of seconds)")
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--serve-jobs", type=int, default=512,
+    ap.add_argument("--serve-jobs", type=int, default=4096,
                     help="jobs of the batched-serving mix run through the server (0 = skip serving)")
-    ap.add_argument("--serve-seq-jobs", type=int, default=512,
+    ap.add_argument("--serve-seq-jobs", type=int, default=4096,
                     help="how many of those jobs are also run one at a time for the comparison")
-    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the jobs (per s)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0,
+                    help="Poisson arrival rate of the jobs (per s; the default saturates the server)")
+    ap.add_argument("--serve-low-rate", type=float, default=100.0,
+                    help="a second, sub-saturating arrival rate for the batched server (0 = skip)")
     ap.add_argument("--serve-autotune", type=int, default=2,
                     help="timed iterations per candidate of the server's batch autotuner (0 = autotuner off)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -323,23 +326,40 @@ def main():
 
         print(f"serving: generating {args.serve_jobs} jobs", file=sys.stderr, flush=True)
         jobs = serving.make_jobs(args.serve_jobs)
-        data = [serving.job_data(j) for j in jobs]
-        print("serving: batched run", file=sys.stderr, flush=True)
+        data = serving.make_data(jobs)
+
+        def public(res):
+            return {k: v for k, v in res.items() if not k.startswith("_")}
+
+        print(f"serving: batched run at {args.serve_rate:g}/s", file=sys.stderr, flush=True)
         srv = serving.run_server(lib, jobs, data, rate=args.serve_rate, autotune=args.serve_autotune)
-        nseq = min(args.serve_seq_jobs, len(jobs))
         line["serving"] = {
-            "workload": f"{len(jobs)} jobs, Euclidean/KL 50/50, m log-uniform [64,4096], n log-uniform [64,2048], "
-                        f"r in {{8,16,32,64}}, to relative objective change < 1e-4 or 500 iterations, "
-                        f"Poisson arrivals at {args.serve_rate:g}/s",
-            "batched": srv,
+            "workload": f"{len(jobs)} jobs, Euclidean/KL 50/50, V n x m in GPUMLib's naming (n rows log-uniform "
+                        f"[64,2048], m columns log-uniform [64,4096]), r in {{8,16,32,64}}, to relative objective "
+                        f"change < 1e-4 or 500 iterations (checked every 10-iteration round), Poisson arrivals",
+            "batched": public(srv),
         }
+        if args.serve_low_rate > 0:
+            print(f"serving: batched run at {args.serve_low_rate:g}/s", file=sys.stderr, flush=True)
+            line["serving"]["batched_low_rate"] = public(
+                serving.run_server(lib, jobs, data, rate=args.serve_low_rate, autotune=args.serve_autotune))
+        nseq = min(args.serve_seq_jobs, len(jobs))
         if nseq > 0:
+            # the unbatched comparison point: the same jobs through the same server, one at a
+            # time, each through the single-problem fused iteration (pooled device memory, no
+            # host work per round beyond reading the two objectives)
             print(f"serving: one-at-a-time run of {nseq} jobs", file=sys.stderr, flush=True)
-            seq = serving.run_one_at_a_time(
-                jobs[:nseq], data[:nseq], rate=args.serve_rate,
-                progress=lambda k: print(f"serving: {k} jobs done one at a time", file=sys.stderr, flush=True))
-            line["serving"]["one_at_a_time"] = seq
+            seq = serving.run_server(lib, jobs[:nseq], data[:nseq], rate=args.serve_rate, sequential=1)
+            one = public(seq)
+            if args.serve_low_rate > 0:
+                one["at_low_rate"] = {"arrival_rate_per_s": args.serve_low_rate,
+                                      **serving.queued_latency(seq["_service_s"], args.serve_low_rate),
+                                      "note": "one FIFO server fed the low-rate Poisson stream, from the service "
+                                              "times measured in this run"}
+            line["serving"]["one_at_a_time"] = one
             line["serving"]["jobs_per_s_speedup"] = srv["jobs_per_s"] / seq["jobs_per_s"]
+            if nseq < len(jobs):
+                line["serving"]["note"] = f"one-at-a-time over the first {nseq} of the {len(jobs)} jobs"
 
     if not args.no_cpu_baseline:
         from oracle import oracle as orc

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libgpumlib_nmf.so")
 EUCLIDEAN, KL = 0, 1
 ADDITIVE_EUCLIDEAN, ADDITIVE_KL, ADDITIVE_EUCLIDEAN_WITH_H = 2, 3, 4
 FP32, BF16, FP8 = 0, 1, 2
+BATCH_SPLIT_K, BATCH_FUSED = 0, 1
 OBJ_NONE, OBJ_FROBENIUS, OBJ_KL, OBJ_QUALITY = 0, 1, 2, 3
 REFERENCE_EPS = 1e-8  # float(10e-9f) == float(1e-8f)
 
@@ -26,7 +27,8 @@ class Dim3(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _c.c_size_t), ("max_active", _c.c_int), ("round_iterations", _c.c_int),
-                ("use_graph", _c.c_int), ("target_workgroups", _c.c_int), ("autotune", _c.c_int)]
+                ("use_graph", _c.c_int), ("target_workgroups", _c.c_int), ("autotune", _c.c_int),
+                ("split_k", _c.c_int), ("sequential", _c.c_int)]
 
 
 class JobResult(ctypes.Structure):
@@ -38,11 +40,11 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", _c.c_longlong), ("completed", _c.c_longlong), ("rounds", _c.c_longlong),
                 ("iterations_batched", _c.c_longlong), ("peak_active", _c.c_int), ("pool_bytes", _c.c_size_t),
                 ("pool_high_water", _c.c_size_t), ("autotunes", _c.c_longlong), ("tuned_target", _c.c_int),
-                ("tuned_pass_grid", _c.c_int), ("tuned_update_grid", _c.c_int)]
+                ("tuned_pass_grid", _c.c_int), ("tuned_update_grid", _c.c_int), ("tuned_engine", _c.c_int)]
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("target_workgroups", _c.c_int), ("pass_grid", _c.c_int), ("update_grid", _c.c_int),
+    _fields_ = [("engine", _c.c_int), ("target_workgroups", _c.c_int), ("pass_grid", _c.c_int), ("update_grid", _c.c_int),
                 ("ms_per_iteration", _c.c_double), ("ms_per_iteration_before", _c.c_double), ("trials", _c.c_int),
                 ("cached", _c.c_int)]
 
@@ -90,6 +92,8 @@ SIGNATURES = {
     "gpumlib_nmf_batch_create": (_i, [_i, _c.POINTER(_vp)]),
     "gpumlib_nmf_batch_destroy": (None, [_vp]),
     "gpumlib_nmf_batch_set_target": (_i, [_vp, _i]),
+    "gpumlib_nmf_batch_set_engine": (_i, [_vp, _i]),
+    "gpumlib_nmf_batch_engine": (_i, [_vp]),
     "gpumlib_nmf_batch_workspace_bytes": (_sz, [_vp, _i, _i, _i, _i]),
     "gpumlib_nmf_batch_add": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _c.POINTER(_i)]),
     "gpumlib_nmf_batch_remove": (_i, [_vp, _i]),

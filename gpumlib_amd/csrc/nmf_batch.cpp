@@ -32,7 +32,14 @@ using namespace gpumlib_nmf;
 
 namespace {
 
-enum { T_PH_EUC = 0, T_PH_KL, T_PW_EUC, T_PW_KL, T_UH, T_UW, T_RED, T_COUNT };
+// Launch tables.  Split-K engine: the passes (per rule) and the updates.  Fused engine:
+// the grams of W and H (Euclidean problems) and the fused passes (per rule).  Both: the
+// per-problem objective reduction.
+enum {
+    T_PH_EUC = 0, T_PH_KL, T_PW_EUC, T_PW_KL, T_UH, T_UW, T_RED,
+    T_GW, T_GH, T_FH_EUC, T_FH_KL, T_FW_EUC, T_FW_KL, T_COUNT
+};
+enum { ENGINE_SPLIT = 0, ENGINE_FUSED = 1 };
 constexpr int kPassGrid = 1024;  // persistent workgroups of the grouped passes (before tuning)
 constexpr int kUpdGrid = 512;
 constexpr int kRedGrid = 256;
@@ -57,15 +64,25 @@ struct Slot {
 };
 
 struct TuneConfig {
-    int target, pass_grid, upd_grid;
+    int engine, target, pass_grid, upd_grid;
     double ms_per_iteration, ms_default;
     int trials;
 };
+
+// Fused engine: one slice per pass, the whole contraction per work item.  Its workspace
+// use: the grams of W and H (r x rpad floats each) from offset 0, then one objective
+// partial per column tile of the H pass (inside the split-K plan's workspace: the fused
+// engine never reads the split-K partials and the split-K passes rewrite them).
+size_t fused_objp_offset(int r) { return (2 * (size_t)r * round_up(r, 4) * sizeof(float) + 255) / 256 * 256; }
+size_t fused_bytes(int m, int r) {
+    return fused_objp_offset(r) + ((size_t)((m + TILE - 1) / TILE) * sizeof(double) + 255) / 256 * 256;
+}
 
 }  // namespace
 
 struct gpumlib_nmf_batch {
     int maxj = 0;
+    int engine = ENGINE_FUSED;             // how the problems are planned and launched
     int target = kBatchTargetWorkgroups;   // K-split target the problems are planned with
     int size_target = kBatchSizingTarget;  // target their workspaces are sized for
     int pass_grid = kPassGrid, upd_grid = kUpdGrid;
@@ -97,27 +114,54 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
     wait_staging(b);
     const int stride = table_stride();
     memset(b->h_tables, 0, sizeof(int) * (size_t)stride * T_COUNT);
+    const bool fused = b->engine == ENGINE_FUSED;
+    std::vector<std::pair<long, int>> order;  // (cost, slot)
     for (int k = 0; k < T_COUNT; k++) {
         int* t = b->h_tables + (size_t)k * stride;
         int J = 0, T = 0;
         int* first = t + 2;
         int* idx = t + 3 + kGroupMaxJobs;
+        order.clear();
         for (int s = 0; s < b->maxj; s++) {
             const Slot& sl = b->slots[s];
             if (!sl.active) continue;
+            const bool euc = sl.rule == GPUMLIB_NMF_EUCLIDEAN;
             int items = 0;
+            long cost = 0;  // fused tables run longest items first (the grid strides over them in order)
             switch (k) {
-                case T_PH_EUC: items = sl.rule == GPUMLIB_NMF_EUCLIDEAN ? sl.args.h.tiles * sl.args.h.S : 0; break;
-                case T_PH_KL: items = sl.rule == GPUMLIB_NMF_KL ? sl.args.h.tiles * sl.args.h.S : 0; break;
-                case T_PW_EUC: items = sl.rule == GPUMLIB_NMF_EUCLIDEAN ? sl.args.w.tiles * sl.args.w.S : 0; break;
-                case T_PW_KL: items = sl.rule == GPUMLIB_NMF_KL ? sl.args.w.tiles * sl.args.w.S : 0; break;
-                case T_UH: items = (sl.m + UPD_COLS - 1) / UPD_COLS; break;
-                case T_UW: items = (sl.n + UPD_ROWS - 1) / UPD_ROWS; break;
+                case T_PH_EUC: items = !fused && euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
+                case T_PH_KL: items = !fused && !euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
+                case T_PW_EUC: items = !fused && euc ? sl.args.w.tiles * sl.args.w.S : 0; break;
+                case T_PW_KL: items = !fused && !euc ? sl.args.w.tiles * sl.args.w.S : 0; break;
+                case T_UH: items = !fused ? (sl.m + UPD_COLS - 1) / UPD_COLS : 0; break;
+                case T_UW: items = !fused ? (sl.n + UPD_ROWS - 1) / UPD_ROWS : 0; break;
                 case T_RED: items = 1; break;
+                case T_GW: items = fused && euc ? 1 : 0, cost = (long)sl.n * sl.r; break;
+                case T_GH: items = fused && euc ? 1 : 0, cost = (long)sl.m * sl.r; break;
+                case T_FH_EUC: items = fused && euc ? sl.args.h.tiles : 0, cost = sl.n; break;
+                case T_FH_KL: items = fused && !euc ? sl.args.h.tiles : 0, cost = sl.n; break;
+                case T_FW_EUC: items = fused && euc ? sl.args.w.tiles : 0, cost = sl.m; break;
+                case T_FW_KL: items = fused && !euc ? sl.args.w.tiles : 0, cost = sl.m; break;
             }
             if (items == 0) continue;
+            order.emplace_back(-cost, s);
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first < y.first; });
+        for (const auto& o : order) {
+            const Slot& sl = b->slots[o.second];
+            int items = 0;
+            switch (k) {
+                case T_PH_EUC: case T_PH_KL: items = sl.args.h.tiles * sl.args.h.S; break;
+                case T_PW_EUC: case T_PW_KL: items = sl.args.w.tiles * sl.args.w.S; break;
+                case T_UH: items = (sl.m + UPD_COLS - 1) / UPD_COLS; break;
+                case T_UW: items = (sl.n + UPD_ROWS - 1) / UPD_ROWS; break;
+                case T_RED: case T_GW: case T_GH: items = 1; break;
+                case T_FH_EUC: case T_FH_KL: items = sl.args.h.tiles; break;
+                case T_FW_EUC: case T_FW_KL: items = sl.args.w.tiles; break;
+            }
             first[J] = T;
-            idx[J] = s;
+            idx[J] = o.second;
             T += items;
             J++;
         }
@@ -140,6 +184,24 @@ static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, 
     const int stride = table_stride();
     const int* T = b->d_tables;
     const int pg = b->pass_grid, ug = b->upd_grid;
+    if (b->engine == ENGINE_FUSED) {
+        // per iteration: gram of W; fused H passes (objective of the entry state fused in);
+        // gram of the new H; fused W passes
+        for (int it = 0; it < iterations; it++) {
+            const bool obj = it < objectives;
+            hipError_t e;
+            if ((e = launch_group_gram(b->d_jobs, T + T_GW * stride, 0, ug, st))) return e;
+            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC * stride, 0, obj ? OBJ_FROB : OBJ_NONE, pg, st)))
+                return e;
+            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_KL * stride, 1, obj ? OBJ_KL : OBJ_NONE, pg, st)))
+                return e;
+            if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
+            if ((e = launch_group_gram(b->d_jobs, T + T_GH * stride, 1, ug, st))) return e;
+            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC * stride, 0, pg, st))) return e;
+            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_KL * stride, 1, pg, st))) return e;
+        }
+        return hipSuccess;
+    }
     for (int it = 0; it < iterations; it++) {
         const bool obj = it < objectives;
         hipError_t e;
@@ -171,18 +233,43 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStrea
     a.obj = sl.args.obj;
     a.rule = sl.rule;
     a.nobj = pl.tiles1 * pl.S1;
+    if (b->engine == ENGINE_FUSED) {
+        if (fused_bytes(sl.m, sl.r) > sl.ws_bytes) return hipErrorInvalidValue;
+        char* base = (char*)sl.ws;
+        const size_t gsz = (size_t)sl.r * round_up(sl.r, 4);
+        for (PassArgs* p : {&a.h, &a.w}) {
+            p->S = 1;
+            p->P = nullptr;
+            p->Psum = nullptr;
+            p->gcount = nullptr;
+        }
+        a.h.tiles = (sl.m + TILE - 1) / TILE;
+        a.h.kslice = round_up(sl.n, KC);
+        a.h.G = (float*)base;  // W^T W
+        a.h.objp = (double*)(base + fused_objp_offset(sl.r));
+        a.w.tiles = (sl.n + TILE - 1) / TILE;
+        a.w.kslice = round_up(sl.m, KC);
+        a.w.G = (float*)base + gsz;  // H H^T
+        a.w.objp = nullptr;
+        a.nobj = a.h.tiles;
+        sl.args = a;
+        b->dirty = true;
+        return hipSuccess;
+    }
     sl.args = a;
     b->dirty = true;
     return hipMemsetAsync(a.h.gcount, 0, 2 * sizeof(int), st);
 }
 
-static hipError_t replan_all(gpumlib_nmf_batch* b, int target, hipStream_t st) {
+static hipError_t replan_all(gpumlib_nmf_batch* b, int engine, int target, hipStream_t st) {
+    b->engine = engine;
     for (Slot& sl : b->slots) {
         if (!sl.active) continue;
         const hipError_t e = plan_slot(b, sl, target, st);
         if (e != hipSuccess) return e;
     }
     b->target = target;
+    b->dirty = true;
     return hipSuccess;
 }
 
@@ -207,6 +294,8 @@ static unsigned long long mix_signature(const gpumlib_nmf_batch* b) {
         }
     return h ^ ((unsigned long long)b->size_target << 1);
 }
+
+// A launch sequence captured for one engine stays valid for that engine only.
 
 // Milliseconds per iteration of the current configuration: one untimed iteration,
 // then `iterations` timed with events on the stream.
@@ -269,6 +358,18 @@ hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgrou
     return hipSuccess;
 }
 
+hipError_t gpumlib_nmf_batch_set_engine(gpumlib_nmf_batch_handle b, int engine) {
+    if (!b || (engine != GPUMLIB_NMF_BATCH_SPLIT_K && engine != GPUMLIB_NMF_BATCH_FUSED)) return hipErrorInvalidValue;
+    if (engine == b->engine) return hipSuccess;
+    wait_staging(b);
+    drop_graphs(b);
+    hipError_t e = replan_all(b, engine, b->target, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
+}
+
+int gpumlib_nmf_batch_engine(gpumlib_nmf_batch_handle b) { return b ? b->engine : -1; }
+
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r) {
     Plan pl;
     if (!b || r > 64 || (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) ||
@@ -300,9 +401,9 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     sl.ws_bytes = workspace_bytes;
     sl.eps = eps;
     sl.args.obj = b->d_obj + 2 * (size_t)s;
-    // gram arrival counters start at zero (null stream, outside any captured graph)
+    // split-K: the gram arrival counters start at zero (null stream, outside any captured graph)
     hipError_t e = plan_slot(b, sl, b->target, nullptr);
-    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess && b->engine == ENGINE_SPLIT) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return e;
     sl.active = true;
     *slot = s;
@@ -372,6 +473,7 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
     if (gpumlib_nmf_batch_active(b) == 0) return hipErrorInvalidValue;
     auto report = [&](const TuneConfig& c, int cached) {
         if (!out) return;
+        out->engine = c.engine;
         out->target_workgroups = c.target;
         out->pass_grid = c.pass_grid;
         out->update_grid = c.upd_grid;
@@ -380,17 +482,17 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         out->trials = cached ? 0 : c.trials;
         out->cached = cached;
     };
-    const int t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid;
+    const int en0 = b->engine, t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid;
     const unsigned long long sig = mix_signature(b);
     auto hit = b->tuned.find(sig);
     if (hit != b->tuned.end()) {  // this mix was tuned before: apply its winner
         const TuneConfig& c = hit->second;
-        hipError_t e = replan_all(b, c.target, stream);
+        hipError_t e = replan_all(b, c.engine, c.target, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return e;
         b->pass_grid = c.pass_grid;
         b->upd_grid = c.upd_grid;
-        if (b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+        if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
         report(c, 1);
         return hipSuccess;
     }
@@ -432,9 +534,9 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         }
         return ce;
     };
-    TuneConfig best = {t0, pg0, ug0, 1e30, 0.0, 0};
-    auto trial = [&](int target, int pg, int ug) -> hipError_t {
-        hipError_t te = replan_all(b, target, stream);
+    TuneConfig best = {en0, t0, pg0, ug0, 1e30, 0.0, 0};
+    auto trial = [&](int engine, int target, int pg, int ug) -> hipError_t {
+        hipError_t te = replan_all(b, engine, target, stream);
         if (te != hipSuccess) return te;
         b->pass_grid = pg;
         b->upd_grid = ug;
@@ -444,6 +546,7 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         if (te != hipSuccess) return te;
         if (++best.trials == 1) best.ms_default = ms;
         if (ms < best.ms_per_iteration) {
+            best.engine = engine;
             best.target = target;
             best.pass_grid = pg;
             best.upd_grid = ug;
@@ -451,24 +554,26 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         }
         return hipSuccess;
     };
-    // the configuration in effect first, then the K-split targets with the grids in
-    // effect, then the pass grids and the update grids around the best so far
+    // the configuration in effect first; then the tile shape along K: the fused engine
+    // (one slice, the whole contraction per tile) and the split-K targets, with the grids in
+    // effect; then the pass grids and the update (fused: gram) grids around the best so far
     if (e == hipSuccess) e = copy_wh(true);
-    if (e == hipSuccess) e = trial(t0, pg0, ug0);
+    if (e == hipSuccess) e = trial(en0, t0, pg0, ug0);
+    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, t0, pg0, ug0);
     for (int t : kTuneTargets)
-        if (e == hipSuccess && t != t0 && fits(t)) e = trial(t, pg0, ug0);
-    const int tb = best.target;
+        if (e == hipSuccess && (en0 != ENGINE_SPLIT || t != t0) && fits(t)) e = trial(ENGINE_SPLIT, t, pg0, ug0);
+    const int enb = best.engine, tb = best.target;
     for (int pg : kTunePassGrids)
-        if (e == hipSuccess && pg != pg0) e = trial(tb, pg, ug0);
+        if (e == hipSuccess && pg != pg0) e = trial(enb, tb, pg, ug0);
     const int pgb = best.pass_grid;
     for (int ug : kTuneUpdGrids)
-        if (e == hipSuccess && ug != ug0) e = trial(tb, pgb, ug);
+        if (e == hipSuccess && ug != ug0) e = trial(enb, tb, pgb, ug);
     // apply the winner, or the configuration in effect before if a trial failed
     const bool ok = (e == hipSuccess);
-    hipError_t ae = replan_all(b, ok ? best.target : t0, stream);
+    hipError_t ae = replan_all(b, ok ? best.engine : en0, ok ? best.target : t0, stream);
     b->pass_grid = ok ? best.pass_grid : pg0;
     b->upd_grid = ok ? best.upd_grid : ug0;
-    if (b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+    if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
     if (ae == hipSuccess) ae = hipStreamSynchronize(stream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -69,7 +69,12 @@ __device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, 
 // carry a gram block only), and inside a tile the 4 waves each take a quarter of
 // every chunk's k range, so no workgroup carries more than an eighth extra MFMA
 // work and the grid stays one shape.
-template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
+// FUSE (grouped full-K launches, one slice): the item's partial is the whole product, so the
+// workgroup applies the H update to its columns itself instead of writing partials.
+// Euclidean: H(a,u) *= WtV(a,u) / (sum_b G(a,b) H(b,u) + eps) with G = W^T W from p.G (computed by
+// the gram launch before this one, G(a,b) at p.G[b*rpad + a]); KL: H(a,u) *= num(a,u) /
+// max(colsum_a, eps), the column sums of W taken over the whole of K in this item.
+template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float, bool FUSE = false>
 __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr bool NEED_WH = KL || (OBJ != OBJ_NONE);
@@ -86,7 +91,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int col0 = t * TILE;
     const int ncols = (t < vt) ? min(TILE, p.m - col0) : 0;
     const int nblk = nb * nb;
-    const bool gcar = (!KL) && GEMM2 && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
+    const bool gcar = (!FUSE) && (!KL) && GEMM2 && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
     const int kend = min(p.n, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -159,7 +164,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int gmb = gcar ? t / nb : 0, gnb = gcar ? t - (t / nb) * nb : 0;  // carried gram block
     double objacc = 0.0;
     float colsum = 0.f;  // KL: sum over this slice of W(i, tid) (tile 0 only)
-    const bool do_sums = KL && GEMM2 && (t == 0) && (tid < RP);
+    const bool do_sums = KL && GEMM2 && (FUSE || t == 0) && (tid < RP);
 
     // The gram-carrying and the plain workgroups run separate copies of the loop, so
     // that no MFMA accumulator sits under a data-dependent branch.
@@ -271,7 +276,61 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     else
         mainloop(std::false_type{});
 
-    if (GEMM2) {
+    if constexpr (FUSE) {
+        // acc[mb][v] = WtV or the KL numerator at (a = mb*32 + acc_row(v,h), u = col0 + wave*32 + c)
+        constexpr int GS = RP + 1;
+        float* sG = &sA[0][0][0];  // LDS is free after the loop's last barrier
+        const int ul = wave * 32 + c, u = col0 + ul;
+        const bool ucol = ul < ncols;
+        if (KL) {
+            if (do_sums) sG[tid] = (colsum < p.eps) ? p.eps : colsum;
+        } else {
+            const int rpad = round_up(p.r, 4);
+            for (int e = tid; e < RP * RP; e += NTHREADS) {
+                const int a = e / RP, b = e - a * RP;
+                sG[a * GS + b] = (a < p.r && b < p.r) ? p.G[(size_t)b * rpad + a] : 0.f;
+            }
+        }
+        __syncthreads();
+        float hb[KL ? 1 : MB * 16];  // B operand of G H: H(b = 2kk + h, u)
+        if constexpr (!KL) {
+#pragma unroll
+            for (int kk = 0; kk < MB * 16; kk++) {
+                const int b = 2 * kk + h;
+                if constexpr (NEED_WH)
+                    hb[kk] = hreg[kk];
+                else
+                    hb[kk] = (ucol && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+            if (mb >= nb) continue;
+            f32x16 d;
+            for (int v = 0; v < 16; v++) d[v] = 0.f;
+            if constexpr (!KL) {
+#pragma unroll
+                for (int kk = 0; kk < MB * 16; kk++)
+                    if (kk < nb * 16) d = mfma32x2(sG[(mb * 32 + c) * GS + 2 * kk + h], hb[kk], d);
+            }
+            if (!ucol) continue;
+            float* hcol = const_cast<float*>(p.H) + (size_t)u * p.r;  // the problem's own H (JobArgs::H)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int a0 = mb * 32 + g * 8 + h * 4;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int a = a0 + j;
+                    if (a >= p.r) continue;
+                    const float num = acc[mb][g * 4 + j];
+                    if (KL)
+                        hcol[a] *= (num / sG[a]);
+                    else
+                        hcol[a] *= (num / (d[g * 4 + j] + p.eps));
+                }
+            }
+        }
+    } else if (GEMM2) {
         // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
         const int ul = wave * 32 + c;
         if (ul < ncols) {
@@ -316,7 +375,9 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 // ---------------------------------------------------------------------------
 // W pass, Euclidean rule: P[s](i,a) = sum_u V(i,u) H(a,u), and the gram H H^T spread
 // over the first tiles of every slice as in the H pass (k quarters per wave).
-template <int MB>
+// FUSE: as in the H pass, the workgroup applies W(i,a) *= VHt(i,a) / (sum_b W(i,b) G(b,a) + eps)
+// to its rows, G = H H^T from p.G (the gram launch before this one).
+template <int MB, bool FUSE = false>
 __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr int PA = TILE + 32;  // rows k and k+1 land 32 banks apart
@@ -334,7 +395,7 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     const int row0 = t * TILE;
     const int nrows = (t < vt) ? min(TILE, p.n - row0) : 0;
     const int nblk = nb * nb;
-    const bool gcar = p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
+    const bool gcar = !FUSE && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
     const int kend = min(p.m, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -435,6 +496,45 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     else
         mainloop(std::false_type{});
 
+    if constexpr (FUSE) {
+        // acc[mb][v] = VHt(i = row0 + wave*32 + acc_row(v,h), a = mb*32 + c)
+        constexpr int GS = RP + 1;
+        float* sG = &sA[0][0][0];  // free after the loop's last barrier; sG[b*GS + a] = G(b, a)
+        const int rpad = round_up(p.r, 4);
+        for (int e = tid; e < RP * RP; e += NTHREADS) {
+            const int b = e / RP, a = e - b * RP;
+            sG[b * GS + a] = (a < p.r && b < p.r) ? p.G[(size_t)a * rpad + b] : 0.f;
+        }
+        __syncthreads();
+        const int il = wave * 32 + c;  // A operand row of W G: W(row0 + il, b = 2kk + h)
+        float wa[MB * 16];
+#pragma unroll
+        for (int kk = 0; kk < MB * 16; kk++) {
+            const int b = 2 * kk + h;
+            wa[kk] = (il < nrows && b < p.r) ? p.W[(size_t)b * p.n + row0 + il] : 0.f;
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+            if (mb >= nb) continue;
+            f32x16 d;
+            for (int v = 0; v < 16; v++) d[v] = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < MB * 16; kk++)
+                if (kk < nb * 16) d = mfma32x2(wa[kk], sG[(2 * kk + h) * GS + mb * 32 + c], d);
+            const int a = mb * 32 + c;
+            if (a >= p.r) continue;
+            float* wcol = const_cast<float*>(p.W) + (size_t)a * p.n + row0;  // the problem's own W
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int i = wave * 32 + g * 8 + h * 4 + j;
+                    if (i < nrows) wcol[i] *= (acc[mb][g * 4 + j] / (d[g * 4 + j] + p.eps));
+                }
+            }
+        }
+        return;
+    }
     // acc[mb][v] = P(i = wave*32 + acc_row(v,h), a = mb*32 + c)
     const int orow0 = row0;
     const int rlim = p.npad - row0;
@@ -477,7 +577,9 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
 // W pass, KL rule: P[s](i,a) = sum_u V(i,u) H(a,u) / ((W H)(i,u) + eps).
 // (W H)^T is formed per step as H^T W^T so that the ratio lands in the A-operand
 // layout of the second product without a shuffle.
-template <int MB>
+// FUSE: the workgroup applies W(i,a) *= num(i,a) / max(rowsum_a, eps) to its rows, the row sums
+// of H taken over the whole of K in this item.
+template <int MB, bool FUSE = false>
 __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr int PV = TILE + 8;  // rows u and u+4 land 32 banks apart
@@ -542,8 +644,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
 #pragma unroll
     for (int mb = 0; mb < MB; mb++)
         for (int v = 0; v < 16; v++) acc[mb][v] = 0.f;
-    float rowsum = 0.f;  // sum over this slice of H(tid, u) (tile 0 only)
-    const bool do_sums = (t == 0) && (tid < RP);
+    float rowsum = 0.f;  // sum over this slice of H(tid, u) (tile 0 only, or every tile when fused)
+    const bool do_sums = (FUSE || t == 0) && (tid < RP);
 
     const int nchunks = (kend - kbeg + KC - 1) / KC;
     gload(kbeg);
@@ -587,6 +689,26 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         __syncthreads();
     }
 
+    if constexpr (FUSE) {
+        float* sS = &sH[0][0][0];  // free after the loop's last barrier
+        if (do_sums) sS[tid] = (rowsum < p.eps) ? p.eps : rowsum;
+        __syncthreads();
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+            const int a = mb * 32 + c;
+            if (mb >= nb || a >= p.r) continue;
+            float* wcol = const_cast<float*>(p.W) + (size_t)a * p.n + row0;  // the problem's own W
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int i = wave * 32 + g * 8 + h * 4 + j;
+                    if (i < nrows) wcol[i] *= (acc[mb][g * 4 + j] / sS[a]);
+                }
+            }
+        }
+        return;
+    }
     const int rlim = p.npad - row0;
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) {
@@ -690,6 +812,182 @@ hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, in
         hipLaunchKernelGGL((k_group_pass_w<true>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     else
         hipLaunchKernelGGL((k_group_pass_w<false>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused full-K grouped launches (one slice per problem and pass, r <= 64): every
+// work item is one column tile (H) or row tile (W) of one problem over the whole
+// contraction, and applies that tile's update itself (pass bodies with FUSE).  The
+// Euclidean updates take their gram from a gram launch that precedes the pass.
+
+// G = X^T X over the whole contraction of one factor, written G(a,b) at G[b*rpad + a]:
+// WHICH 0: X = W (n x r, column-major, k = i); WHICH 1: X = H^T (H r x m, k = u).
+// Stages of KG contraction elements through LDS (loads of the next stage in flight);
+// the 4 waves take interleaved k steps of every stage, and their partial blocks are
+// added in wave order.
+template <int WHICH>
+__device__ __forceinline__ void gram_body(const float* __restrict__ X, int K, int r, int ld, float* __restrict__ G,
+                                          bool vec) {
+    constexpr int RP = 64, KG = 128, XP = RP + 4;
+    constexpr int LOADS = RP * KG / 4 / NTHREADS;  // float4 per thread per stage
+    __shared__ __attribute__((aligned(16))) float sX[KG][XP];  // sX[k][a] = X(k, a)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 31, h = lane >> 5;
+    const int nb = (r + 31) >> 5;
+    float4 rx[LOADS];
+    auto gload = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < LOADS; j++) {
+            const int q = tid + NTHREADS * j;
+            if (WHICH == 0) {  // W(k, a) = W[a*ld + k]: 4 consecutive k of row a
+                const int a = q / (KG / 4), kq = (q % (KG / 4)) * 4;
+                rx[j] = load4(X + (size_t)a * ld + k0 + kq, a < r, k0 + kq, K, vec);
+            } else {  // H(a, k) = H[k*ld + a]: 4 consecutive a of column k
+                const int k = q / (RP / 4), aq = (q % (RP / 4)) * 4;
+                rx[j] = load4(X + (size_t)(k0 + k) * ld + aq, k0 + k < K, aq, r, vec);
+            }
+        }
+    };
+    auto sstore = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < LOADS; j++) {
+            const int q = tid + NTHREADS * j;
+            if (WHICH == 0) {
+                const int a = q / (KG / 4), kq = (q % (KG / 4)) * 4;
+                sX[kq][a] = rx[j].x;
+                sX[kq + 1][a] = rx[j].y;
+                sX[kq + 2][a] = rx[j].z;
+                sX[kq + 3][a] = rx[j].w;
+            } else {
+                const int k = q / (RP / 4), aq = (q % (RP / 4)) * 4;
+                *reinterpret_cast<float4*>(&sX[k][aq]) = rx[j];
+            }
+        }
+    };
+    f32x16 gacc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+            for (int v = 0; v < 16; v++) gacc[x][y][v] = 0.f;
+    const int stages = (K + KG - 1) / KG;
+    if (stages > 0) gload(0);
+    for (int st = 0; st < stages; st++) {
+        sstore();
+        __syncthreads();
+        if (st + 1 < stages) gload((st + 1) * KG);
+#pragma unroll
+        for (int j = 0; j < KG / 8; j++) {
+            const int k = 2 * (wave + 4 * j) + h;
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+                if (x >= nb) continue;
+                const float av = sX[k][x * 32 + c];
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    if (y < nb) gacc[x][y] = mfma32x2(av, sX[k][y * 32 + c], gacc[x][y]);
+            }
+        }
+        __syncthreads();
+    }
+    // gacc[x][y][v] = partial G(a = x*32 + acc_row(v,h), b = y*32 + c) of this wave
+    float* red = &sX[0][0];  // 4 waves x 64 lanes x 16 (free after the last barrier)
+    const int rpad = round_up(r, 4);
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            if (x >= nb || y >= nb) continue;
+#pragma unroll
+            for (int v = 0; v < 16; v++) red[(wave * 64 + lane) * 16 + v] = gacc[x][y][v];
+            __syncthreads();
+            for (int e = tid; e < 64 * 16; e += NTHREADS) {
+                const int le = e >> 4, v = e & 15;
+                const float g = ((red[le * 16 + v] + red[(64 + le) * 16 + v]) + red[(128 + le) * 16 + v]) +
+                                red[(192 + le) * 16 + v];
+                const int a = x * 32 + acc_row(v, le >> 5), b = y * 32 + (le & 31);
+                if (a < r && b < r) G[(size_t)b * rpad + a] = g;
+            }
+            __syncthreads();
+        }
+}
+
+template <int WHICH>
+__global__ __launch_bounds__(NTHREADS) void k_group_gram(const JobArgs* __restrict__ jobs, const int* table) {
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find(first, J, item);
+        const JobArgs& ja = jobs[idx[j]];
+        const PassArgs& p = WHICH == 0 ? ja.h : ja.w;
+        __syncthreads();
+        if (WHICH == 0)
+            gram_body<0>(p.W, p.n, p.r, p.n, p.G, p.vec != 0);
+        else
+            gram_body<1>(p.H, p.m, p.r, p.r, p.G, p.vec != 0);
+    }
+}
+
+template <bool KL, int OBJ>
+__global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __restrict__ jobs, const int* table) {
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find(first, J, item);
+        const PassArgs p = jobs[idx[j]].h;
+        __syncthreads();
+        pass_h_body<2, KL, OBJ, true, float, true>(p, item - first[j], (p.r + 31) >> 5);
+    }
+}
+
+template <bool KL>
+__global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_w(const JobArgs* __restrict__ jobs, const int* table) {
+    const int J = table[0], T = table[1];
+    const int* first = table + 2;
+    const int* idx = table + 3 + kGroupMaxJobs;
+    for (int item = blockIdx.x; item < T; item += gridDim.x) {
+        const int j = group_find(first, J, item);
+        const PassArgs p = jobs[idx[j]].w;
+        __syncthreads();
+        if (KL)
+            pass_w_kl_body<2, true>(p, item - first[j], (p.r + 31) >> 5);
+        else
+            pass_w_euc_body<2, true>(p, item - first[j], (p.r + 31) >> 5);
+    }
+}
+
+hipError_t launch_group_gram(const JobArgs* jobs, const int* table, int which, int grid, hipStream_t st) {
+    if (which == 0)
+        hipLaunchKernelGGL(k_group_gram<0>, dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else
+        hipLaunchKernelGGL(k_group_gram<1>, dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int grid, hipStream_t st) {
+#define LF(K, O)                                                                                  \
+    hipLaunchKernelGGL((k_group_fused_h<K, O>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+    return hipGetLastError();
+    if (kl) {
+        if (obj == OBJ_NONE) { LF(true, OBJ_NONE) }
+        if (obj == OBJ_KL) { LF(true, OBJ_KL) }
+    } else {
+        if (obj == OBJ_NONE) { LF(false, OBJ_NONE) }
+        if (obj == OBJ_FROB) { LF(false, OBJ_FROB) }
+    }
+#undef LF
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
+    if (kl)
+        hipLaunchKernelGGL((k_group_fused_w<true>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else
+        hipLaunchKernelGGL((k_group_fused_w<false>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     return hipGetLastError();
 }
 

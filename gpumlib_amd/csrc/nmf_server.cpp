@@ -155,13 +155,18 @@ struct gpumlib_nmf_server {
     int tuned_at = 0;  // resident problems at the last autotuning (0: never tuned)
     std::thread engine;
 
+    double* d_obj = nullptr;  // one-at-a-time mode: the round's two objectives
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
+    hipError_t run_one(Job& j);
     bool admit_locked(std::vector<std::shared_ptr<Job>>& admitted);
 };
 
-static size_t job_footprint(gpumlib_nmf_batch_handle b, Job& j) {
-    j.ws_bytes = gpumlib_nmf_batch_workspace_bytes(b, j.rule, j.n, j.m, j.r);
+// Device bytes of a problem: V, W, H and its workspace (the batch's plan, or in the
+// one-at-a-time mode the single-problem iteration's).
+static size_t job_footprint(gpumlib_nmf_batch_handle b, bool sequential, Job& j) {
+    j.ws_bytes = sequential ? gpumlib_nmf_workspace_bytes(j.rule, GPUMLIB_NMF_FP32, j.n, j.m, j.r)
+                            : gpumlib_nmf_batch_workspace_bytes(b, j.rule, j.n, j.m, j.r);
     if (j.ws_bytes == 0) return 0;
     j.offV = 0;
     j.offW = align256(sizeof(float) * (size_t)j.n * j.m);
@@ -189,9 +194,97 @@ bool gpumlib_nmf_server::admit_locked(std::vector<std::shared_ptr<Job>>& admitte
     return any;
 }
 
+// One-at-a-time mode: the admitted problem runs alone, round after round, through the
+// single-problem fused iteration (gpumlib_nmf_iterate) on its pool memory, with the
+// stopping rule of the batched rounds (objectives on entry to the round's first two
+// iterations, read back once per round).
+hipError_t gpumlib_nmf_server::run_one(Job& j) {
+    char* base = d_pool + j.off;
+    float* dV = (float*)(base + j.offV);
+    float* dW = (float*)(base + j.offW);
+    float* dH = (float*)(base + j.offH);
+    void* ws = base + j.offWs;
+    const int R = cfg.round_iterations;
+    const int kind = j.rule == GPUMLIB_NMF_KL ? GPUMLIB_NMF_OBJ_KL : GPUMLIB_NMF_OBJ_FROBENIUS;
+    hipError_t e = hipMemcpyAsync(dV, j.V.data(), sizeof(float) * j.V.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dW, j.W.data(), sizeof(float) * j.W.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dH, j.H.data(), sizeof(float) * j.H.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    j.t_admit = now();
+    std::vector<float>().swap(j.V);
+    while (e == hipSuccess) {
+        e = gpumlib_nmf_iterate(j.rule, GPUMLIB_NMF_FP32, j.n, j.m, j.r, dV, dW, dH, j.eps, 1, 1, kind, d_obj, ws,
+                                j.ws_bytes, stream);
+        if (e == hipSuccess && R >= 2)
+            e = gpumlib_nmf_iterate(j.rule, GPUMLIB_NMF_FP32, j.n, j.m, j.r, dV, dW, dH, j.eps, 1, 1, kind, d_obj + 1,
+                                    ws, j.ws_bytes, stream);
+        if (e == hipSuccess && R > 2)
+            e = gpumlib_nmf_iterate(j.rule, GPUMLIB_NMF_FP32, j.n, j.m, j.r, dV, dW, dH, j.eps, R - 2, 1,
+                                    GPUMLIB_NMF_OBJ_NONE, nullptr, ws, j.ws_bytes, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_obj, d_obj, 2 * sizeof(double), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) break;
+        j.iters += R;
+        const double f0 = h_obj[0], f1 = R >= 2 ? h_obj[1] : f0;
+        j.objective = f1;
+        const double rel = (f0 != 0) ? std::fabs(f0 - f1) / std::fabs(f0) : 0.0;
+        if (R >= 2 && rel < j.tol) j.converged = 1;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stats.rounds++;
+            stats.iterations_batched += R;
+            stats.peak_active = std::max(stats.peak_active, 1);
+        }
+        if (j.converged || j.iters >= j.max_iters) break;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(j.W.data(), dW, sizeof(float) * j.W.size(), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(j.H.data(), dH, sizeof(float) * j.H.size(), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
 void gpumlib_nmf_server::run() {
     (void)hipSetDevice(device);
     const int rounds_iters = cfg.round_iterations;
+    if (cfg.sequential) {
+        for (;;) {
+            std::shared_ptr<Job> jp;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (;;) {
+                    if (abandon) return;
+                    std::vector<std::shared_ptr<Job>> admitted;
+                    admit_locked(admitted);  // max_active is 1 in this mode
+                    if (!admitted.empty()) {
+                        jp = admitted[0];
+                        break;
+                    }
+                    if (stop) return;
+                    cv_work.wait(lk);
+                }
+            }
+            const hipError_t e = run_one(*jp);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                pool.release(jp->off, jp->bytes);
+                jp->status = e;
+                jp->t_finish = now();
+                jp->done = true;
+                stats.completed++;
+                stats.pool_high_water = pool.high_water();
+                if (e != hipSuccess) {
+                    fatal = e;
+                    for (auto& q : pending) {
+                        q->status = e;
+                        q->done = true;
+                    }
+                    pending.clear();
+                }
+            }
+            cv_done.notify_all();
+            if (e != hipSuccess) return;
+        }
+    }
     for (;;) {
         std::vector<std::shared_ptr<Job>> admitted;
         {
@@ -248,6 +341,7 @@ void gpumlib_nmf_server::run() {
                     stats.tuned_target = t.target_workgroups;
                     stats.tuned_pass_grid = t.pass_grid;
                     stats.tuned_update_grid = t.update_grid;
+                    stats.tuned_engine = t.engine;
                 }
             }
         }
@@ -332,21 +426,26 @@ extern "C" {
 
 hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpumlib_nmf_server_handle* out) {
     if (!cfg || !out || cfg->hbm_budget_bytes == 0 || cfg->max_active < 1 || cfg->max_active > 1024 ||
-        cfg->round_iterations < 1)
+        cfg->round_iterations < 1 || cfg->split_k < 0 || cfg->split_k > 1 || cfg->sequential < 0 ||
+        cfg->sequential > 1)
         return hipErrorInvalidValue;
     *out = nullptr;
     gpumlib_nmf_server* s = new (std::nothrow) gpumlib_nmf_server;
     if (!s) return hipErrorOutOfMemory;
     s->cfg = *cfg;
+    if (s->cfg.sequential) s->cfg.max_active = 1;
     s->t0 = Clock::now();
     memset(&s->stats, 0, sizeof(s->stats));
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_pool, cfg->hbm_budget_bytes);
     if (e == hipSuccess) e = gpumlib_nmf_batch_create(cfg->max_active, &s->batch);
     if (e == hipSuccess && cfg->target_workgroups > 0) e = gpumlib_nmf_batch_set_target(s->batch, cfg->target_workgroups);
+    if (e == hipSuccess && cfg->split_k) e = gpumlib_nmf_batch_set_engine(s->batch, GPUMLIB_NMF_BATCH_SPLIT_K);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_obj, sizeof(double) * 2 * cfg->max_active, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_obj, 2 * sizeof(double));
     if (e != hipSuccess) {
+        if (s->d_obj) (void)hipFree(s->d_obj);
         if (s->h_obj) (void)hipHostFree(s->h_obj);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         if (s->batch) gpumlib_nmf_batch_destroy(s->batch);
@@ -380,7 +479,7 @@ hipError_t gpumlib_nmf_server_submit(gpumlib_nmf_server_handle s, int rule, int 
     j->eps = eps;
     j->tol = tol;
     j->max_iters = max_iterations;
-    j->bytes = job_footprint(s->batch, *j);
+    j->bytes = job_footprint(s->batch, s->cfg.sequential != 0, *j);
     if (j->bytes == 0) return hipErrorInvalidValue;
     if (j->bytes > s->pool.total()) return hipErrorOutOfMemory;  // can never be admitted
     j->V.assign(V, V + (size_t)n * m);
@@ -449,6 +548,7 @@ void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s) {
     if (s->engine.joinable()) s->engine.join();  // finishes every admitted and queued problem first
     (void)hipStreamSynchronize(s->stream);
     (void)hipHostFree(s->h_obj);
+    (void)hipFree(s->d_obj);
     (void)hipStreamDestroy(s->stream);
     gpumlib_nmf_batch_destroy(s->batch);
     (void)hipFree(s->d_pool);

@@ -1,9 +1,10 @@
 """Serving workload: a Poisson stream of NMF jobs of uneven sizes, run through the
-continuous-batching server and, for comparison, one job at a time through the
-single-problem path with the same stopping rule.
+continuous-batching server and, for comparison, through the same server's one-at-a-time
+mode (each job alone through the single-problem iteration) with the same stopping rule.
 
-Job mix (the batched-serving configuration): Euclidean and KL jobs 50/50; m
-log-uniform in [64, 4096], n log-uniform in [64, 2048], r from {8, 16, 32, 64};
+Job mix (the batched-serving configuration): Euclidean and KL jobs 50/50; in GPUMLib's
+naming (V is n x m: n rows, m columns, nmf_multiplicative_euclidian.h:47) m log-uniform in
+[64, 4096] columns and n log-uniform in [64, 2048] rows, r from {8, 16, 32, 64};
 V = W0 H0 + U[0, 0.01) noise (Euclidean) or Poisson counts with rate W0 H0
 (KL), from the seeded generators; each job runs until the relative objective
 change over one iteration drops below 1e-4, or 500 iterations.
@@ -43,6 +44,15 @@ def job_data(job):
     return V, W, H
 
 
+def make_data(jobs, threads=16):
+    """job_data of every job, generated on a thread pool (the generators are native and
+    release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(job_data, jobs))
+
+
 def iteration_work(rule, n, m, r):
     """Algorithmic work of one iteration of a job: FLOPs (Euclidean 4nmr + 4r^2(n+m), KL
     8nmr) and bytes (the FP32 V streamed twice, W and H each read and written once)."""
@@ -64,21 +74,24 @@ def _percentile(xs, q):
 
 
 def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, max_active=1024, round_iters=10,
-               use_graph=1, autotune=0):
+               use_graph=1, autotune=0, split_k=0, sequential=0):
     """Submits the jobs with exponential inter-arrival times (mean 1/rate) and waits for all.
 
     autotune > 0 lets the server run the batch autotuner (that many timed iterations per
-    candidate) whenever the number of resident jobs doubles or halves.
+    candidate) whenever the number of resident jobs doubles or halves.  sequential = 1 runs
+    the server's one-at-a-time mode (each job alone through the single-problem iteration,
+    same rounds and stopping rule).
     Returns throughput (jobs/s over first submit .. last finish), latency percentiles
-    (finish - submit, seconds) and the server's counters."""
-    cfg = capi.ServerConfig(budget, max_active, round_iters, use_graph, 0, autotune)
+    (finish - submit, seconds), per-job service times (finish - admit) and the server's
+    counters."""
+    cfg = capi.ServerConfig(budget, max_active, round_iters, use_graph, 0, autotune, split_k, sequential)
     h = ctypes.c_void_p()
     capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gpumlib_nmf_server_create")
     rng = np.random.default_rng(12345)
     gaps = rng.exponential(1.0 / rate, size=len(jobs)) if rate > 0 else np.zeros(len(jobs))
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     ids = []
-    lat, iters, conv, first, last = [], [], 0, None, None
+    lat, svc, iters, conv, first, last = [], [], [], 0, None, None
     st = capi.ServerStats()
     try:
         due = time.perf_counter()
@@ -97,6 +110,7 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
             res = capi.JobResult()
             capi.check(lib.gpumlib_nmf_server_wait(h, jid, None, None, ctypes.byref(res)), "gpumlib_nmf_server_wait")
             lat.append(res.finish_s - res.submit_s)
+            svc.append(res.finish_s - res.admit_s)
             iters.append(res.iterations)
             conv += res.converged
             first = res.submit_s if first is None else min(first, res.submit_s)
@@ -107,51 +121,28 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
     span = last - first
     return {"jobs": len(ids), "jobs_per_s": len(ids) / span, "span_s": span,
             "latency_p50_s": _percentile(lat, 50), "latency_p99_s": _percentile(lat, 99),
+            "service_p50_s": _percentile(svc, 50), "service_p99_s": _percentile(svc, 99),
+            "_service_s": svc,
             "iterations_mean": float(np.mean(iters)), **_aggregate(jobs, iters, span), "converged": conv,
             "rounds": st.rounds, "peak_active": st.peak_active, "pool_high_water_bytes": st.pool_high_water,
             "arrival_rate_per_s": rate, "round_iterations": round_iters, "graph": bool(use_graph),
+            "max_active": 1 if sequential else max_active,
+            "engine": "one_at_a_time" if sequential else ("split_k" if split_k else "fused"),
             "autotune_trial_iterations": autotune, "autotunes": st.autotunes,
-            "tuned": {"target_workgroups": st.tuned_target, "pass_grid": st.tuned_pass_grid,
+            "tuned": {"engine": "fused" if st.tuned_engine == capi.BATCH_FUSED else "split_k",
+                      "target_workgroups": st.tuned_target, "pass_grid": st.tuned_pass_grid,
                       "update_grid": st.tuned_update_grid} if st.autotunes else None}
 
 
-def run_one_at_a_time(jobs, data, tol=1e-4, max_iters=500, round_iters=10, rate=0.0, progress=None):
-    """The same jobs launched one after another through gpumlib_nmf_iterate (one
-    problem, its own buffers), with the server's stopping rule and cadence.
-
-    Latency is reported for the same Poisson arrival stream as the batched run
-    (arrival times from the same seed): a job starts when it has arrived and the
-    previous one has finished, and takes its measured service time."""
-    lat, iters = [], []
-    t0 = time.perf_counter()
-    for job, (V, W, H) in zip(jobs, data):
-        s0 = time.perf_counter()
-        rule, n, m, r, _ = job
-        eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
-        kind = G.OBJ_FROBENIUS if rule == G.EUCLIDEAN else G.OBJ_KL
-        d = G.DeviceNMF(rule, V, W, H, n, m, r, eps=eps)
-        done = 0
-        while done < max_iters:
-            f0 = d.iterate(1, objective=kind)
-            f1 = d.iterate(1, objective=kind)
-            d.iterate(round_iters - 2)
-            done += round_iters
-            if f0 != 0 and abs(f0 - f1) < tol * abs(f0):
-                break
-        d.get_w()
-        d.get_h()
-        lat.append(time.perf_counter() - s0)
-        iters.append(done)
-        if progress and len(lat) % 256 == 0:
-            progress(len(lat))
-    span = time.perf_counter() - t0
-    rng = np.random.default_rng(12345)
-    arrive = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs))) if rate > 0 else np.zeros(len(jobs))
-    finish, queued = 0.0, []
-    for a, svc in zip(arrive, lat):
-        finish = max(finish, a) + svc
-        queued.append(finish - a)
-    return {"jobs": len(jobs), "jobs_per_s": len(jobs) / span, "span_s": span,
-            "service_p50_s": _percentile(lat, 50), "service_p99_s": _percentile(lat, 99),
-            "latency_p50_s": _percentile(queued, 50), "latency_p99_s": _percentile(queued, 99),
-            "iterations_mean": float(np.mean(iters)), **_aggregate(jobs, iters, span), "arrival_rate_per_s": rate}
+def queued_latency(service_s, rate, seed=12345):
+    """Latency percentiles of one FIFO server with the given per-job service times (seconds,
+    measured) under the Poisson arrival stream of run_server (same seed): job k starts when it
+    has arrived and job k-1 has finished."""
+    rng = np.random.default_rng(seed)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=len(service_s))) if rate > 0 else np.zeros(len(service_s))
+    finish, lat = 0.0, []
+    for a, s in zip(arrive, service_s):
+        finish = max(finish, a) + s
+        lat.append(finish - a)
+    return {"latency_p50_s": _percentile(lat, 50), "latency_p99_s": _percentile(lat, 99),
+            "span_s": finish - float(arrive[0]) if len(arrive) else 0.0}

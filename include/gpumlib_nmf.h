@@ -256,6 +256,14 @@ void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b);
  * Workspaces (gpumlib_nmf_batch_workspace_bytes) are sized for max(target, 128) by default -- room for
  * every candidate of the autotuner -- and for exactly this target after the call. */
 hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgroups_per_pass);
+/* Engines: GPUMLIB_NMF_BATCH_FUSED (default): per problem and pass one work item per 128-column (H) or
+ * 128-row (W) tile over the whole contraction, which applies its tile's update itself (the Euclidean
+ * updates take W^T W and H H^T from a gram launch before each pass); four grouped launches per
+ * iteration plus one per rule.  GPUMLIB_NMF_BATCH_SPLIT_K: the contraction split into K slices (the
+ * target above), partials summed by separate update launches.  A switch re-plans every problem. */
+enum { GPUMLIB_NMF_BATCH_SPLIT_K = 0, GPUMLIB_NMF_BATCH_FUSED = 1 };
+hipError_t gpumlib_nmf_batch_set_engine(gpumlib_nmf_batch_handle b, int engine);
+int gpumlib_nmf_batch_engine(gpumlib_nmf_batch_handle b);
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r);
 hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r, const float *V, float *W,
                                  float *H, float eps, void *workspace, size_t workspace_bytes, int *slot);
@@ -272,9 +280,10 @@ hipError_t gpumlib_nmf_batch_objectives(gpumlib_nmf_batch_handle b, double *host
 const double *gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b);
 
 /* Autotuner: times `trial_iterations` iterations of the active mix (after one untimed iteration) for
- * each candidate configuration -- the K-split target per problem, i.e. the depth along K of a pass
- * work item (4, 8, 16, 32, 64 or 128 workgroups per problem and pass, those that fit every problem's
- * workspace), then the persistent grid sizes of the grouped passes (512, 1024, 2048) and updates
+ * each candidate configuration -- the tile shape along K: the fused engine (a tile's whole contraction
+ * in one work item) or the split-K engine with a K-split target per problem (4, 8, 16, 32, 64 or 128
+ * workgroups per problem and pass, those that fit every problem's workspace), then the persistent grid
+ * sizes of the grouped passes (512, 1024, 2048) and of the updates or, fused, the gram launches
  * (256, 512, 1024) -- and keeps the fastest.  W and H of every problem are saved before and restored
  * after each trial (a temporary device buffer of their size is allocated for that), so no problem's
  * state changes; a re-planned problem sums its K slices in the new grouping afterwards (results within
@@ -282,6 +291,7 @@ const double *gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b)
  * signature (the multiset of rule, n, m, r): a later call for the same mix applies it without trials.
  * Synchronises the stream; not under graph capture.  No reference counterpart. */
 typedef struct {
+    int engine;                     /* GPUMLIB_NMF_BATCH_FUSED or GPUMLIB_NMF_BATCH_SPLIT_K */
     int target_workgroups, pass_grid, update_grid;
     double ms_per_iteration;        /* of the chosen configuration */
     double ms_per_iteration_before; /* of the configuration in effect before the call */
@@ -314,6 +324,11 @@ typedef struct {
     int target_workgroups;   /* K-split target per problem and pass, 0 = default */
     int autotune;            /* > 0: autotune the batch (this many timed iterations per candidate) whenever
                                 the number of resident problems has doubled or halved; 0: off */
+    int split_k;             /* 0: the fused batch engine (default); 1: the split-K engine (both as
+                                gpumlib_nmf_batch_set_engine describes; the autotuner may switch) */
+    int sequential;          /* 1: one problem at a time (max_active is taken as 1), each run to its end
+                                through gpumlib_nmf_iterate on its pool memory with the same rounds and
+                                stopping rule -- the unbatched comparison point; 0: continuous batching */
 } gpumlib_nmf_server_config;
 typedef struct {
     int iterations;
@@ -327,6 +342,7 @@ typedef struct {
     size_t pool_bytes, pool_high_water;
     long long autotunes;                                    /* autotuner calls made */
     int tuned_target, tuned_pass_grid, tuned_update_grid;  /* configuration chosen last */
+    int tuned_engine;                                       /* GPUMLIB_NMF_BATCH_FUSED or _SPLIT_K */
 } gpumlib_nmf_server_stats;
 
 hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config *cfg, gpumlib_nmf_server_handle *out);

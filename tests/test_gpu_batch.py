@@ -34,12 +34,27 @@ def make_job(k, rule, n, m, r):
     return V, W, H
 
 
+ENGINES = [capi.BATCH_FUSED, capi.BATCH_SPLIT_K]
+
+EDGE_JOBS = [  # odd shapes for the fused engine's tiles: partial tiles, unaligned rows, long contractions
+    (G.EUCLIDEAN, 1, 7, 1),
+    (G.KL, 130, 257, 33),
+    (G.EUCLIDEAN, 2048, 70, 64),
+    (G.KL, 50, 4096, 17),
+    (G.EUCLIDEAN, 333, 129, 63),
+    (G.KL, 129, 1000, 64),
+]
+
+
 class Batch:
-    def __init__(self, hip, lib, max_jobs=16):
+    def __init__(self, hip, lib, max_jobs=16, engine=None):
         self.hip, self.lib = hip, lib
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_batch_create(max_jobs, ctypes.byref(h)))
         self.h = h
+        if engine is not None:
+            capi.check(lib.gpumlib_nmf_batch_set_engine(h, engine))
+            assert lib.gpumlib_nmf_batch_engine(h) == engine
         self.keep = []
         self.stream = hip.Stream()
 
@@ -68,10 +83,12 @@ class Batch:
         self.lib.gpumlib_nmf_batch_destroy(self.h)
 
 
-def test_batch_matches_oracle(native, orc, cuda):
-    b = Batch(cuda, native)
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("jobs", ["mix", "edge"])
+def test_batch_matches_oracle(native, orc, cuda, engine, jobs):
+    b = Batch(cuda, native, engine=engine)
     outs = []
-    for k, (rule, n, m, r) in enumerate(JOBS):
+    for k, (rule, n, m, r) in enumerate(JOBS if jobs == "mix" else EDGE_JOBS):
         V, W, H = make_job(k, rule, n, m, r)
         slot, dW, dH = b.add(rule, n, m, r, V, W, H)
         outs.append((rule, n, m, r, V, W, H, dW, dH))
@@ -85,13 +102,15 @@ def test_batch_matches_oracle(native, orc, cuda):
     b.close()
 
 
-def test_alone_equals_mix(native, cuda):
-    k, (rule, n, m, r) = 3, JOBS[3]
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("k", [3, 4])
+def test_alone_equals_mix(native, cuda, engine, k):
+    rule, n, m, r = JOBS[k]
     V, W, H = make_job(k, rule, n, m, r)
-    b1 = Batch(cuda, native)
+    b1 = Batch(cuda, native, engine=engine)
     _, w1, h1 = b1.add(rule, n, m, r, V, W, H)
     b1.iterate(15)
-    b2 = Batch(cuda, native)
+    b2 = Batch(cuda, native, engine=engine)
     for kk, job in enumerate(JOBS):
         if kk == k:
             _, w2, h2 = b2.add(rule, n, m, r, V, W, H)
@@ -104,8 +123,9 @@ def test_alone_equals_mix(native, cuda):
     b2.close()
 
 
-def test_graph_replays_equal_eager(native, cuda):
-    eager, graphed = Batch(cuda, native), Batch(cuda, native)
+@pytest.mark.parametrize("engine", ENGINES)
+def test_graph_replays_equal_eager(native, cuda, engine):
+    eager, graphed = Batch(cuda, native, engine=engine), Batch(cuda, native, engine=engine)
     pairs = []
     for k, job in enumerate(JOBS[:4]):
         V, W, H = make_job(k, *job)
@@ -129,8 +149,9 @@ def test_graph_replays_equal_eager(native, cuda):
     graphed.close()
 
 
-def test_batch_objectives_match_single(native, orc, cuda):
-    b = Batch(cuda, native)
+@pytest.mark.parametrize("engine", ENGINES)
+def test_batch_objectives_match_single(native, orc, cuda, engine):
+    b = Batch(cuda, native, engine=engine)
     slots = []
     for k, job in enumerate(JOBS[:3]):
         V, W, H = make_job(k, *job)
@@ -166,17 +187,20 @@ def test_autotune_keeps_state_and_parity(native, orc, cuda):
     before = [(dW.download(np.float32), dH.download(np.float32)) for *_, dW, dH in outs]
     t = capi.BatchTuning()
     capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t)))
-    print(f"tuned target={t.target_workgroups} pass_grid={t.pass_grid} update_grid={t.update_grid} "
-          f"ms {t.ms_per_iteration_before:.3f} -> {t.ms_per_iteration:.3f} trials={t.trials}", flush=True)
+    print(f"tuned engine={t.engine} target={t.target_workgroups} pass_grid={t.pass_grid} "
+          f"update_grid={t.update_grid} ms {t.ms_per_iteration_before:.3f} -> {t.ms_per_iteration:.3f} "
+          f"trials={t.trials}", flush=True)
     assert t.cached == 0 and t.trials >= 3
-    assert t.target_workgroups in TUNE_TARGETS and t.pass_grid in TUNE_PASS and t.update_grid in TUNE_UPD
+    assert t.engine in (capi.BATCH_FUSED, capi.BATCH_SPLIT_K) and native.gpumlib_nmf_batch_engine(b.h) == t.engine
+    assert t.target_workgroups in TUNE_TARGETS | {32} and t.pass_grid in TUNE_PASS and t.update_grid in TUNE_UPD
     assert 0 < t.ms_per_iteration <= t.ms_per_iteration_before
     for (w0, h0), (*_, dW, dH) in zip(before, outs):
         assert np.array_equal(dW.download(np.float32), w0) and np.array_equal(dH.download(np.float32), h0)
     t2 = capi.BatchTuning()
     capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t2)))
     assert t2.cached == 1 and t2.trials == 0
-    assert (t2.target_workgroups, t2.pass_grid, t2.update_grid) == (t.target_workgroups, t.pass_grid, t.update_grid)
+    assert (t2.engine, t2.target_workgroups, t2.pass_grid, t2.update_grid) == \
+        (t.engine, t.target_workgroups, t.pass_grid, t.update_grid)
     b.iterate(10)
     b.iterate(12, graph=True)  # graphs captured after tuning use the tuned grids
     for rule, n, m, r, V, W, H, dW, dH in outs:
@@ -196,9 +220,9 @@ def test_autotune_rejects_empty_batch(native, cuda):
 
 # ------------------------------------------------------------------ server --
 class Server:
-    def __init__(self, lib, budget, max_active=64, round_iters=10, graph=1, autotune=0):
+    def __init__(self, lib, budget, max_active=64, round_iters=10, graph=1, autotune=0, sequential=0):
         self.lib = lib
-        cfg = capi.ServerConfig(budget, max_active, round_iters, graph, 0, autotune)
+        cfg = capi.ServerConfig(budget, max_active, round_iters, graph, 0, autotune, 0, sequential)
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -359,3 +383,25 @@ def test_server_cap_is_reached_at_a_round_end(native, orc, cuda):
     v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
     orc.euc_iter(300, 200, 16, v, w, h, G.REFERENCE_EPS, iters=20)
     assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
+
+
+def test_server_one_at_a_time_matches_oracle(native, orc, cuda):
+    """The server's one-at-a-time mode (the unbatched comparison point of the serving
+    benchmark): each job alone through the single-problem iteration, the same rounds and
+    stopping rule, results within the FP32 tolerance of the oracle, one job resident at a time."""
+    s = Server(native, budget=256 << 20, sequential=1)
+    ids = []
+    for k, job in enumerate(JOBS):
+        V, W, H = make_job(k, *job)
+        ids.append((s.submit(*job, V, W, H, tol=1e-4, max_iters=60), job, V, W, H))
+    for jid, (rule, n, m, r), V, W, H in ids:
+        gw, gh, res = s.wait(jid, n, m, r)
+        assert res.iterations % 10 == 0 and 10 <= res.iterations <= 60
+        assert res.finish_s >= res.admit_s >= res.submit_s
+        v, w, h = (np.ascontiguousarray(x, dtype=np.float64) for x in (V, W, H))
+        (orc.euc_iter if rule == G.EUCLIDEAN else orc.kl_iter)(n, m, r, v, w, h, G.REFERENCE_EPS,
+                                                               iters=res.iterations)
+        assert rel_fro(gw, w) < 1e-4 and rel_fro(gh, h) < 1e-4
+    st = s.stats()
+    assert st.completed == len(JOBS) and st.peak_active == 1
+    s.close()

@@ -35,3 +35,24 @@ def test_make_jobs_ranges_and_seed():
         assert 64 <= n <= 2048 and 64 <= m <= 4096 and r in (8, 16, 32, 64)
     kinds = [j[0] for j in jobs]
     assert 0.35 < kinds.count(G.EUCLIDEAN) / len(kinds) < 0.65
+
+
+def test_queued_latency_fifo():
+    """One FIFO server: with every arrival after the previous finish the latency is the service
+    time; with all arrivals at once it is the running sum of service times."""
+    import numpy as np
+
+    svc = [0.5, 0.25, 1.0, 0.125]
+    burst = serving.queued_latency(svc, rate=0.0)  # all arrive at t = 0
+    assert burst["latency_p50_s"] == pytest.approx(np.percentile(np.cumsum(svc), 50))
+    assert burst["latency_p99_s"] == pytest.approx(np.percentile(np.cumsum(svc), 99))
+    slow = serving.queued_latency([1e-6] * 50, rate=10.0)  # service far below the mean gap
+    assert slow["latency_p99_s"] < 1e-3
+    # the same arrival stream as run_server: exponential gaps from seed 12345
+    rng = np.random.default_rng(12345)
+    arrive = np.cumsum(rng.exponential(1.0 / 3.0, size=4))
+    fin, lat = 0.0, []
+    for a, s in zip(arrive, svc):
+        fin = max(fin, a) + s
+        lat.append(fin - a)
+    assert serving.queued_latency(svc, rate=3.0)["latency_p50_s"] == pytest.approx(np.percentile(lat, 50))
