@@ -931,19 +931,172 @@ hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, 
     return launch_cols_t<16, 128>(W, P, G, n, r, ldp, S, pstride, eps, add, st);
 }
 
+// ---------------------------------------------------------------------------
+// Single-problem KL updates (nmf_kernels.cu:289-291, 337-339; additive form
+// nmf_kernels_additive_divergence.cu:109-121, 178-190) over the split-K partials.  Each
+// thread owns 16-byte groups of partials and issues the loads of KLB slices of all its
+// groups (and of the clamped sums) before it adds any of them, so a workgroup pays about
+// S / KLB memory round trips instead of one per element; slices are added in slice order,
+// as the per-element loop did (bit-identical results).
+constexpr int KLB = 8;
+
+// Threads t < rpad / 4 also sum the S slices of Psum (stride rpad) for rank rows 4t .. 4t+3:
+// the clamped sums max(sum, eps) and the raw sums (additive form) go to LDS.
+// H update, row form: thread -> (column u = u0 + t / rq, rank rows 4 (t % rq) .. +3),
+// partials P[s][u*ldr + a]; JT = NTHREADS / rq columns per workgroup.
+__global__ __launch_bounds__(NTHREADS) void k_update_h_kl_v(float* __restrict__ H, const float* __restrict__ P,
+                                                            const float* __restrict__ Psum, int m, int r, int ldr,
+                                                            int S, long pstride, float eps, int add) {
+    __shared__ float ssum[2 * 128];
+    const int tid = threadIdx.x, rq = ldr >> 2, jt = NTHREADS / rq;
+    const int rpad = round_up(r, 4);
+    const int u = blockIdx.x * jt + tid / rq, q = tid % rq;
+    const bool item = (tid / rq) < jt && u < m && 4 * q < r;
+    const bool sumt = tid < rq;
+    const float4* P4 = reinterpret_cast<const float4*>(P) + (size_t)u * rq + q;
+    const float4* S4 = reinterpret_cast<const float4*>(Psum) + tid;
+    const long ps4 = pstride >> 2, ss4 = rpad >> 2;
+    float4 num = make_float4(0.f, 0.f, 0.f, 0.f), sm = num;
+    for (int s0 = 0; s0 < S; s0 += KLB) {
+        float4 x[KLB], y[KLB];
+#pragma unroll
+        for (int b = 0; b < KLB; b++) {
+            const bool ok = s0 + b < S;
+            x[b] = (item && ok) ? P4[(size_t)(s0 + b) * ps4] : make_float4(0.f, 0.f, 0.f, 0.f);
+            y[b] = (sumt && ok) ? S4[(size_t)(s0 + b) * ss4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int b = 0; b < KLB; b++) {
+            if (s0 + b >= S) break;
+            num.x += x[b].x, num.y += x[b].y, num.z += x[b].z, num.w += x[b].w;
+            sm.x += y[b].x, sm.y += y[b].y, sm.z += y[b].z, sm.w += y[b].w;
+        }
+    }
+    if (sumt) {
+        const float v[4] = {sm.x, sm.y, sm.z, sm.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int a = 4 * tid + j;
+            if (a < r) {
+                ssum[a] = (v[j] < eps) ? eps : v[j];
+                ssum[128 + a] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    if (!item) return;
+    const float nv[4] = {num.x, num.y, num.z, num.w};
+    float* h = H + (size_t)u * r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int a = 4 * q + j;
+        if (a >= r) break;
+        if (add)
+            h[a] = additive_kl_rule(h[a], ssum[a], nv[j] - ssum[128 + a]);
+        else
+            h[a] *= (nv[j] / ssum[a]);
+    }
+}
+
+// W update, column form: partials P[s][a*ldp + i] (i contiguous); thread item -> (rank
+// column a, rows 4 g .. 4 g + 3 of the workgroup's JT rows); up to 2 items per thread.
+__global__ __launch_bounds__(NTHREADS) void k_update_w_kl_v(float* __restrict__ W, const float* __restrict__ P,
+                                                            const float* __restrict__ Psum, int n, int r, int ldp,
+                                                            int S, long pstride, float eps, int add, int jt,
+                                                            int vec) {
+    __shared__ float ssum[2 * 128];
+    const int tid = threadIdx.x, rpad = round_up(r, 4), rq = rpad >> 2;
+    const int gq = jt >> 2;  // row quads per workgroup
+    const int i0 = blockIdx.x * jt;
+    const bool sumt = tid < rq;
+    const long ss4 = rpad >> 2;
+    const float4* S4 = reinterpret_cast<const float4*>(Psum) + tid;
+    int ia[2], aa[2];
+    bool it[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int idx = tid + k * NTHREADS;
+        aa[k] = idx / gq;
+        ia[k] = i0 + 4 * (idx % gq);
+        it[k] = aa[k] < r && ia[k] < n;
+    }
+    float4 num[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+    float4 sm = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s0 = 0; s0 < S; s0 += KLB) {
+        float4 x[2][KLB], y[KLB];
+#pragma unroll
+        for (int b = 0; b < KLB; b++) {
+            const bool ok = s0 + b < S;
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                x[k][b] = (it[k] && ok) ? *reinterpret_cast<const float4*>(P + (size_t)(s0 + b) * pstride +
+                                                                            (size_t)aa[k] * ldp + ia[k])
+                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+            y[b] = (sumt && ok) ? S4[(size_t)(s0 + b) * ss4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int b = 0; b < KLB; b++) {
+            if (s0 + b >= S) break;
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                num[k].x += x[k][b].x, num[k].y += x[k][b].y, num[k].z += x[k][b].z, num[k].w += x[k][b].w;
+            sm.x += y[b].x, sm.y += y[b].y, sm.z += y[b].z, sm.w += y[b].w;
+        }
+    }
+    if (sumt) {
+        const float v[4] = {sm.x, sm.y, sm.z, sm.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int a = 4 * tid + j;
+            if (a < r) {
+                ssum[a] = (v[j] < eps) ? eps : v[j];
+                ssum[128 + a] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (!it[k]) continue;
+        const int a = aa[k];
+        float* w = W + (size_t)a * n + ia[k];
+        const float nv[4] = {num[k].x, num[k].y, num[k].z, num[k].w};
+        const float cs = ssum[a], raw = ssum[128 + a];
+        if (vec) {
+            float4 o = *reinterpret_cast<float4*>(w);
+            float ov[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) ov[j] = add ? additive_kl_rule(ov[j], cs, nv[j] - raw) : ov[j] * (nv[j] / cs);
+            *reinterpret_cast<float4*>(w) = make_float4(ov[0], ov[1], ov[2], ov[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (ia[k] + j >= n) break;
+                w[j] = add ? additive_kl_rule(w[j], cs, nv[j] - raw) : w[j] * (nv[j] / cs);
+            }
+        }
+    }
+}
+
 hipError_t launch_update_h_kl(float* H, const float* P, const float* Psum, int m, int r, int ldr, int S, long pstride,
                               float eps, hipStream_t st, int add) {
-    if (r > 512) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_update_h_kl, dim3((m + UPD_COLS - 1) / UPD_COLS), dim3(NTHREADS), 0, st, H, P, Psum, m, r,
-                       ldr, S, pstride, eps, add);
+    if (r > 128 || ldr != round_up(r, 4)) return hipErrorInvalidValue;
+    const int jt = NTHREADS / (ldr >> 2);
+    hipLaunchKernelGGL(k_update_h_kl_v, dim3((m + jt - 1) / jt), dim3(NTHREADS), 0, st, H, P, Psum, m, r, ldr, S,
+                       pstride, eps, add);
     return hipGetLastError();
 }
 
 hipError_t launch_update_w_kl(float* W, const float* P, const float* Psum, int n, int r, int ldp, int S, long pstride,
                               float eps, hipStream_t st, int add) {
-    if (r > 512) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_update_w_kl, dim3((n + UPD_ROWS - 1) / UPD_ROWS), dim3(NTHREADS), 0, st, W, P, Psum, n, r,
-                       ldp, S, pstride, eps, add);
+    if (r > 128) return hipErrorInvalidValue;
+    // rows per workgroup: (jt / 4) row quads x r rank columns <= 2 items per thread
+    const int rpad = round_up(r, 4);
+    int jt = 4 * ((2 * NTHREADS) / rpad);
+    if (jt > 256) jt = 256;
+    const int vec = (n % 4 == 0) && (((uintptr_t)W & 15) == 0);
+    hipLaunchKernelGGL(k_update_w_kl_v, dim3((n + jt - 1) / jt), dim3(NTHREADS), 0, st, W, P, Psum, n, r, ldp, S,
+                       pstride, eps, add, jt, vec);
     return hipGetLastError();
 }
 
