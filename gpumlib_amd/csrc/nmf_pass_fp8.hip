@@ -63,7 +63,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     constexpr int LPC = K8 / 16;                      // lanes per column (or row) in a load
     __shared__ __attribute__((aligned(16))) uint8_t sA[RP][LD8];
     __shared__ __attribute__((aligned(16))) uint8_t sB[TILE_LP][LD8];
-    __shared__ __attribute__((aligned(16))) uint8_t sAs[RP][NKB];  // scale of (rank row, K block)
+    // scale of (rank row a, K block kb) at sAs[kb][a]: the 32 lanes of a half-wave read 32
+    // consecutive bytes (8 dwords, no bank conflict) where a row-major [a][kb] layout put
+    // lanes c and c + 8 on one bank
+    __shared__ __attribute__((aligned(16))) uint8_t sAs[NKB][RP];
     __shared__ __attribute__((aligned(16))) uint8_t sBs[2][NKB];   // scale of (32-column block, K block)
 
     const int total = p.tiles * p.S;
@@ -121,7 +124,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
             const int q = tid + NTHREADS * j, a = q / LPC, kq = (q % LPC) * 16;
             *reinterpret_cast<u32x4*>(&sA[a][kq]) = ra[j];
         }
-        if (tid < RP * 4) *reinterpret_cast<uint32_t*>(&sAs[tid >> 2][4 * (tid & 3)]) = rsa;
+        if (tid < RP * 4) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) sAs[4 * (tid & 3) + j][tid >> 2] = (uint8_t)(rsa >> (8 * j));
+        }
         if (tid < 2 * NKB) sBs[tid / NKB][tid % NKB] = rsb;
     };
 
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                     const uint8_t* pb = &sB[nb * 32 + c][g * 64 + h * 16];
                     acc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
                                       *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
-                                      sAs[mb * 32 + c][2 * g + h], sBs[nb][2 * g + h], acc[q]);
+                                      sAs[2 * g + h][mb * 32 + c], sBs[nb][2 * g + h], acc[q]);
                 }
             }
             if constexpr (GR == 1) {
@@ -161,8 +167,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                             gacc[q] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa),
                                                *reinterpret_cast<const u32x4*>(pa + 32),
                                                *reinterpret_cast<const u32x4*>(pb),
-                                               *reinterpret_cast<const u32x4*>(pb + 32), sAs[mb * 32 + c][2 * g + h],
-                                               sAs[nb * 32 + c][2 * g + h], gacc[q]);
+                                               *reinterpret_cast<const u32x4*>(pb + 32), sAs[2 * g + h][mb * 32 + c],
+                                               sAs[2 * g + h][nb * 32 + c], gacc[q]);
                         }
                     }
                 }
@@ -178,7 +184,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
                 const uint8_t* pb = &sA[nb * 32 + c][g * 64 + h * 16];
                 gacc[0] = mfma_fp8(*reinterpret_cast<const u32x4*>(pa), *reinterpret_cast<const u32x4*>(pa + 32),
                                    *reinterpret_cast<const u32x4*>(pb), *reinterpret_cast<const u32x4*>(pb + 32),
-                                   sAs[mb * 32 + c][2 * g + h], sAs[nb * 32 + c][2 * g + h], gacc[0]);
+                                   sAs[2 * g + h][mb * 32 + c], sAs[2 * g + h][nb * 32 + c], gacc[0]);
             }
         }
     };
