@@ -85,6 +85,8 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
     ap.add_argument("--no-reduction", action="store_true",
                     help="skip the reduction-framework leg (sum / min / max over the FP32 V)")
+    ap.add_argument("--no-class", action="store_true",
+                    help="skip the drop-in class legs (DoIteration through the object handle, FP32)")
     ap.add_argument("--no-additive", action="store_true",
                     help="skip the additive-update legs (NMF_AdditiveEuclidian / NMF_AdditiveDivergence, FP32)")
     ap.add_argument("--gap-iterations", default="100,500,1000",
@@ -245,6 +247,42 @@ def main():
                 "objective_entry": obj0, "objective_after": d.objective(obj_kind),
                 "iterations_timed": args.steps, "precision": "fp32"}
 
+    # The drop-in C++ classes (NMF_MultiplicativeEuclidianDistance / NMF_MultiplicativeDivergence through
+    # the object handle, FP32): what a caller of the reference API sees per DoIteration(), with the
+    # default quality cadence (the reference quality, nmf_kernels_quality.cu:35, fused into every
+    # iteration's pass over V, as nmf_base.h computes it whenever it can) and with it off.
+    def class_leg(quality_interval):
+        h = ctypes.c_void_p()
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        capi.check(lib.gpumlib_nmf_object_create(rule, G.FP32, n, m, r, ptr(V), ptr(W0), ptr(H0), ctypes.byref(h)),
+                   "gpumlib_nmf_object_create")
+        try:
+            capi.check(lib.gpumlib_nmf_object_set_epsilon(h, eps))
+            capi.check(lib.gpumlib_nmf_object_set_quality_interval(h, quality_interval))
+            wout = np.empty(n * r, np.float32)
+            capi.check(lib.gpumlib_nmf_object_do_iterations(h, args.warmup, 1))
+            capi.check(lib.gpumlib_nmf_object_get_w(h, ptr(wout)))  # blocking read-back: the stream is idle
+            t0 = time.perf_counter()
+            capi.check(lib.gpumlib_nmf_object_get_w(h, ptr(wout)))
+            t_get = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            capi.check(lib.gpumlib_nmf_object_do_iterations(h, args.steps, 1))
+            capi.check(lib.gpumlib_nmf_object_get_w(h, ptr(wout)))
+            t = time.perf_counter() - t0 - t_get
+            return {"iters_per_s": args.steps / t, "ms_per_step": t * 1e3 / args.steps,
+                    "quality_interval": quality_interval,
+                    "quality_improvement": float(lib.gpumlib_nmf_object_quality_improvement(h))}
+        finally:
+            lib.gpumlib_nmf_object_destroy(h)
+
+    class_api = {}
+    if not args.no_class:
+        print("class-API legs", file=sys.stderr, flush=True)
+        class_api = {"default_quality_cadence": class_leg(1), "quality_off": class_leg(0),
+                     "note": "DoIteration() of the drop-in class through gpumlib_nmf_object_do_iterations, FP32; "
+                             "timed from the launch of the iterations to a blocking GetW(), minus that read-back's "
+                             "own time"}
+
     additive = {}
     if not args.no_additive:
         names = ({"additive_euclidean": G.ADDITIVE_EUCLIDEAN, "additive_euclidean_with_h": G.ADDITIVE_EUCLIDEAN_WITH_H}
@@ -316,6 +354,8 @@ def main():
                           "to the FP64 host oracle within 1e-4 by the parity tests",
         "gen_seconds": t_gen,
     }
+    if class_api:
+        line["class_api"] = class_api
     if additive:
         line["additive_legs"] = additive
     if reduction:

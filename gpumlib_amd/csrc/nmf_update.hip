@@ -292,29 +292,27 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
         stage_lds<8>(JT * RP, ld_x, st_x);
     }
     __syncthreads();
-    // 3. denominators D = X C on the matrix cores: 32 x 32 blocks (row block jb of the
-    // tile, column block ab of a), one wave per block, the whole contraction in order
-    // (32x32x2 FP32 MFMAs: exact products, FP32 accumulation); rows past JT are zeros.
+    // 3. denominators D = X C on the matrix cores: 16 x 16 blocks (row block jb of the
+    // tile, column block ab of a) dealt to the 4 waves in turn, the whole contraction in
+    // order per block (16x16x4 FP32 MFMAs: exact products, FP32 accumulation).  Small
+    // blocks keep every wave busy and each dependent chain short (RP / 4 MFMAs).
     // D goes to sN, where each item then replaces its own d by the new value.
     {
-        constexpr int NJB = (JT + 31) / 32, NAB = RP / 32;
-        const int lane = tid & 63, c = lane & 31, h = lane >> 5;
+        constexpr int NJB = JT / 16, NAB = RP / 16;
+        const int lane = tid & 63, c = lane & 15, g = lane >> 4;
         const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         for (int blk = wave; blk < NJB * NAB; blk += NTHREADS / 64) {
             const int jb = blk / NAB, ab = blk - jb * NAB;
-            const int row = jb * 32 + c;
-            const float* xa = sX + row * RS + h;
-            const float* gb = sG + h * RS + ab * 32 + c;
-            f32x16 acc;
-#pragma unroll
-            for (int v = 0; v < 16; v++) acc[v] = 0.f;
+            // lane (c, g): A = X(row jb*16 + c, k + g), B = C(k + g, col ab*16 + c);
+            // result register v = D(row jb*16 + 4g + v, col ab*16 + c)
+            const float* xa = sX + (jb * 16 + c) * RS + g;
+            const float* gb = sG + g * RS + ab * 16 + c;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 8
-            for (int k = 0; k < RP; k += 2) acc = mfma32x2(row < JT ? xa[k] : 0.f, gb[k * RS], acc);
+            for (int k = 0; k < RP; k += 4)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[k], gb[k * RS], acc, 0, 0, 0);
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int i = jb * 32 + acc_row(v, h);
-                if (i < JT) sN[i * RS + ab * 32 + c] = acc[v];
-            }
+            for (int v = 0; v < 4; v++) sN[(jb * 16 + 4 * g + v) * RS + ab * 16 + c] = acc[v];
         }
     }
     __syncthreads();
