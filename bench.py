@@ -251,7 +251,7 @@ def main():
     # the object handle, FP32): what a caller of the reference API sees per DoIteration(), with the
     # default quality cadence (the reference quality, nmf_kernels_quality.cu:35, fused into every
     # iteration's pass over V, as nmf_base.h computes it whenever it can) and with it off.
-    def class_leg(quality_interval):
+    def class_leg(quality_interval, read_each=False):
         h = ctypes.c_void_p()
         ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         capi.check(lib.gpumlib_nmf_object_create(rule, G.FP32, n, m, r, ptr(V), ptr(W0), ptr(H0), ctypes.byref(h)),
@@ -266,11 +266,16 @@ def main():
             capi.check(lib.gpumlib_nmf_object_get_w(h, ptr(wout)))
             t_get = time.perf_counter() - t0
             t0 = time.perf_counter()
-            capi.check(lib.gpumlib_nmf_object_do_iterations(h, args.steps, 1))
+            if read_each:  # the usual caller's loop: DoIteration(), then QualityImprovement()
+                for _ in range(args.steps):
+                    capi.check(lib.gpumlib_nmf_object_do_iteration(h, 1))
+                    lib.gpumlib_nmf_object_quality_improvement(h)
+            else:
+                capi.check(lib.gpumlib_nmf_object_do_iterations(h, args.steps, 1))
             capi.check(lib.gpumlib_nmf_object_get_w(h, ptr(wout)))
             t = time.perf_counter() - t0 - t_get
             return {"iters_per_s": args.steps / t, "ms_per_step": t * 1e3 / args.steps,
-                    "quality_interval": quality_interval,
+                    "quality_interval": quality_interval, "quality_read_each_iteration": read_each,
                     "quality_improvement": float(lib.gpumlib_nmf_object_quality_improvement(h))}
         finally:
             lib.gpumlib_nmf_object_destroy(h)
@@ -278,10 +283,12 @@ def main():
     class_api = {}
     if not args.no_class:
         print("class-API legs", file=sys.stderr, flush=True)
-        class_api = {"default_quality_cadence": class_leg(1), "quality_off": class_leg(0),
-                     "note": "DoIteration() of the drop-in class through gpumlib_nmf_object_do_iterations, FP32; "
-                             "timed from the launch of the iterations to a blocking GetW(), minus that read-back's "
-                             "own time"}
+        class_api = {"default_quality_cadence": class_leg(1),
+                     "default_quality_cadence_read_each_iteration": class_leg(1, read_each=True),
+                     "quality_off": class_leg(0),
+                     "note": "DoIteration() of the drop-in class through gpumlib_nmf_object_do_iterations (or one "
+                             "DoIteration() and one QualityImprovement() call at a time), FP32; timed from the "
+                             "launch of the iterations to a blocking GetW(), minus that read-back's own time"}
 
     additive = {}
     if not args.no_additive:

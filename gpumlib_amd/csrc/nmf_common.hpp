@@ -72,8 +72,10 @@ __device__ __forceinline__ float additive_kl_rule(float x, float sum, float t) {
     return v < 0.f ? 0.f : v;
 }
 
-// Per-element objective terms in FP64 (ObjKind): Frobenius (v - wh)^2; KL
-// v ln(v / wh) - v + wh with 0 ln 0 = 0; the reference quality v log10(wh + eps) - wh.
+// Per-element objective terms, returned in FP64 for an FP64 sum (ObjKind): Frobenius
+// (v - wh)^2 and KL v ln(v / wh) - v + wh (0 ln 0 = 0), both formed in FP64; the reference
+// quality v log10(wh + eps) - wh formed in FP32 as the reference forms it (its value is
+// reported as a float).
 template <int OBJ>
 __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
     if (OBJ == OBJ_FROB) {
@@ -85,8 +87,7 @@ __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
         return t - dv + dwh;
     } else if (OBJ == OBJ_QUALITY) {
         // reference: V[k] * log10(WH[k] + eps) - WH[k]  (nmf_kernels_quality.cu:35)
-        const float whe = wh + eps;
-        return (double)v * log10((double)whe) - (double)wh;
+        return (double)fmaf(v, log10f(wh + eps), -wh);
     }
     return 0.0;
 }

@@ -8,10 +8,15 @@
 //
 // Quality: the reference computes sum V log10(WH + eps) - WH on a side stream
 // whenever that stream is idle (nmf_base.h:70-89), so its cadence depends on
-// timing.  Here it is computed, in FP64 and fused into the iteration's pass over
-// V, every QualityInterval() iterations (default 1), and read back one quality
-// point later; the bookkeeping of QualityImprovement() is the reference's
-// (nmf_base.h:72-80: no improvement figure before the third reading).
+// timing.  Here it is computed (terms in FP32 as there, summed in FP64), fused into
+// the iteration's pass over V, every QualityInterval() iterations (default 1), and
+// read back one quality point later; the bookkeeping of QualityImprovement() is the
+// reference's (nmf_base.h:72-80: no improvement figure before the third reading).
+// DoIteration() does not wait for a quality: each one is copied into one of a few
+// pinned slots behind its iteration, and the bookkeeping readings are applied in
+// order when QualityImprovement() asks for them (or when a slot is needed again),
+// so the host keeps the stream fed while the figures stay those of the in-order
+// bookkeeping.
 #ifndef GPUMLIB_NMF_BASE_H
 #define GPUMLIB_NMF_BASE_H
 
@@ -34,15 +39,18 @@ namespace GPUMLib {
 
 class NMF {
   private:
+    static constexpr int kQualitySlots = 4;
+
     CudaStream stream;
-    int timesQualityWasObtained;
-    cudafloat lastValueQuality;
-    cudafloat deltaImprovement;
-    DeviceAccessibleVariable<double> vquality;
-    DeviceArray<double> d_quality;
+    mutable int timesQualityWasObtained;
+    mutable cudafloat lastValueQuality;
+    mutable cudafloat deltaImprovement;
+    DeviceAccessibleVariable<double> vquality[kQualitySlots];  // quality of point k in slot k % kQualitySlots
+    DeviceArray<double> d_quality;                              // the same slots on the device, + one for Objective()
     DeviceArray<unsigned char> workspace;
-    hipEvent_t qualityReady;
-    bool qualityPending;
+    hipEvent_t qualityReady[kQualitySlots];
+    long qualityPoints;            // quality points reached
+    mutable long readingsApplied;  // bookkeeping readings applied (reading k is taken at point k)
     int rule;
     int precision;
     cudafloat epsilon;
@@ -66,10 +74,13 @@ class NMF {
         timesQualityWasObtained = 0;
         deltaImprovement = CUDA_VALUE(1.0);
         lastValueQuality = CUDA_VALUE(-1.0);
-        vquality.Value() = 0.0;
-        qualityPending = false;
+        qualityPoints = 0;
+        readingsApplied = 0;
         lastError = hipSuccess;
-        if (hipEventCreateWithFlags(&qualityReady, hipEventDisableTiming) != hipSuccess) qualityReady = nullptr;
+        for (int k = 0; k < kQualitySlots; k++) {
+            vquality[k].Value() = 0.0;
+            if (hipEventCreateWithFlags(&qualityReady[k], hipEventDisableTiming) != hipSuccess) qualityReady[k] = nullptr;
+        }
         const size_t bytes =
             gpumlib_nmf_workspace_bytes(rule, precision, (int)V.Rows(), (int)V.Columns(), (int)W.Columns());
         // any rank plans (ranks above 128 run the unfused GEMM sequence); 0 means an empty or
@@ -87,19 +98,27 @@ class NMF {
         if (e != hipSuccess && lastError == hipSuccess) lastError = e;
     }
 
-    // Reads the quality computed at the previous quality point (0 before the first).
-    void ConsumeQuality() {
-        if (qualityPending) {
-            (void)hipEventSynchronize(qualityReady);
-            qualityPending = false;
+    // Applies the bookkeeping readings before reading `end`.  Reading k reads the quality
+    // computed at the previous quality point k - 1 (0 at the first point).
+    void ApplyReadings(long end) const {
+        for (; readingsApplied < end; readingsApplied++) {
+            double value = 0.0;
+            if (readingsApplied > 0) {
+                const int slot = (int)((readingsApplied - 1) % kQualitySlots);
+                if (qualityReady[slot])
+                    (void)hipEventSynchronize(qualityReady[slot]);
+                else
+                    (void)hipStreamSynchronize(stream);
+                value = vquality[slot].Value();
+            }
+            const cudafloat quality = (cudafloat)value;
+            if (timesQualityWasObtained++ > 2) {
+                deltaImprovement = quality - lastValueQuality;
+            } else {
+                timesQualityWasObtained++;
+            }
+            lastValueQuality = quality;
         }
-        const cudafloat quality = (cudafloat)vquality.Value();
-        if (timesQualityWasObtained++ > 2) {
-            deltaImprovement = quality - lastValueQuality;
-        } else {
-            timesQualityWasObtained++;
-        }
-        lastValueQuality = quality;
     }
 
   protected:
@@ -108,7 +127,7 @@ class NMF {
     DeviceMatrix<cudafloat> H;
     DeviceMatrix<cudafloat> WH;  // allocated on the first GetWH()
 
-    NMF(HostMatrix<cudafloat>& v, int r, int ruleKind) : d_quality(1), V(v) {
+    NMF(HostMatrix<cudafloat>& v, int r, int ruleKind) : d_quality(kQualitySlots + 1), V(v) {
         assert(!v.IsRowMajor());
         HostMatrix<cudafloat> aux(v.Rows(), r, ColumnMajor);
         Randomize(aux);
@@ -120,7 +139,7 @@ class NMF {
     }
 
     NMF(HostMatrix<cudafloat>& v, HostMatrix<cudafloat>& w, HostMatrix<cudafloat>& h, int ruleKind)
-        : d_quality(1), V(v), W(w), H(h) {
+        : d_quality(kQualitySlots + 1), V(v), W(w), H(h) {
         assert(!v.IsRowMajor());
         assert(!w.IsRowMajor());
         assert(!h.IsRowMajor());
@@ -136,15 +155,22 @@ class NMF {
     void Iterate(bool updateW) {
         if (workspace.Length() == 0) return;  // construction failed: LastError() says why
         const bool qualityPoint = qualityInterval > 0 && (iterationsDone % qualityInterval) == 0;
-        if (qualityPoint) ConsumeQuality();
+        const int slot = (int)(qualityPoints % kQualitySlots);
+        // the slot's previous value (point qualityPoints - kQualitySlots) is read by the reading after it
+        if (qualityPoint) ApplyReadings(qualityPoints - kQualitySlots + 2);
+        double* dq = d_quality.Pointer() + slot;
         hipError_t e = gpumlib_nmf_iterate(rule, precision, (int)V.Rows(), (int)V.Columns(), (int)W.Columns(),
                                            V.Pointer(), W.Pointer(), H.Pointer(), epsilon, 1, updateW ? 1 : 0,
                                            qualityPoint ? GPUMLIB_NMF_OBJ_QUALITY : GPUMLIB_NMF_OBJ_NONE,
-                                           qualityPoint ? d_quality.Pointer() : nullptr, workspace.Pointer(),
-                                           workspace.Length(), stream);
-        if (e == hipSuccess && qualityPoint) {
-            vquality.UpdateValueAsync(d_quality.Pointer(), stream);
-            if (qualityReady && hipEventRecord(qualityReady, stream) == hipSuccess) qualityPending = true;
+                                           qualityPoint ? dq : nullptr, workspace.Pointer(), workspace.Length(), stream);
+        if (qualityPoint) {
+            if (e == hipSuccess) {
+                vquality[slot].UpdateValueAsync(dq, stream);
+                if (qualityReady[slot]) e = hipEventRecord(qualityReady[slot], stream);
+            } else {
+                vquality[slot].Value() = 0.0;
+            }
+            qualityPoints++;
         }
         RecordError(e);
         iterationsDone++;
@@ -153,7 +179,8 @@ class NMF {
   public:
     virtual ~NMF() {
         (void)hipStreamSynchronize(stream);
-        if (qualityReady) (void)hipEventDestroy(qualityReady);
+        for (int k = 0; k < kQualitySlots; k++)
+            if (qualityReady[k]) (void)hipEventDestroy(qualityReady[k]);
     }
 
     //! Gets the W matrix (blocking read-back)
@@ -185,8 +212,12 @@ class NMF {
     //! Does one algorithm iteration, adjusting H and (if updateW) W.
     virtual void DoIteration(bool updateW = true) = 0;
 
-    //! Quality improvement measured between the last two quality readings.
-    cudafloat QualityImprovement() const { return deltaImprovement; }
+    //! Quality improvement measured between the last two quality readings (waits, if it must,
+    //! for the quality of the quality point before the last).
+    cudafloat QualityImprovement() const {
+        ApplyReadings(qualityPoints);
+        return deltaImprovement;
+    }
 
     // ---- additions (no reference counterpart) ----
     //! eps of the update ratios (reference: SMALL_VALUE_TO_ADD_DENOMINATOR = 1e-8).
@@ -200,8 +231,10 @@ class NMF {
         double value = 0.0;
         hipError_t e = gpumlib_nmf_objective(rule, precision, (int)V.Rows(), (int)V.Columns(), (int)W.Columns(),
                                              V.Pointer(), W.Pointer(), H.Pointer(), epsilon, kind,
-                                             d_quality.Pointer(), workspace.Pointer(), workspace.Length(), stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&value, d_quality.Pointer(), sizeof(double), hipMemcpyDeviceToHost, stream);
+                                             d_quality.Pointer() + kQualitySlots, workspace.Pointer(),
+                                             workspace.Length(), stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&value, d_quality.Pointer() + kQualitySlots, sizeof(double), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         RecordError(e);
         return value;

@@ -17,7 +17,7 @@ class CudaStream {
     }
     CudaStream(const CudaStream&) = delete;
     CudaStream& operator=(const CudaStream&) = delete;
-    operator hipStream_t() { return handle; }
+    operator hipStream_t() const { return handle; }
 
   private:
     hipStream_t handle;

@@ -27,6 +27,7 @@ class DeviceAccessibleVariable {
     DeviceAccessibleVariable& operator=(const DeviceAccessibleVariable&) = delete;
 
     Type& Value() { return *value; }
+    const Type& Value() const { return *value; }
     Type* Pointer() { return value; }
     void UpdateValue(Type* deviceValue) { (void)hipMemcpy(value, deviceValue, sizeof(Type), hipMemcpyDeviceToHost); }
     void UpdateValueAsync(Type* deviceValue, hipStream_t stream) {

@@ -150,7 +150,8 @@ enum {
     GPUMLIB_NMF_OBJ_NONE = 0,
     GPUMLIB_NMF_OBJ_FROBENIUS = 1, /* sum (V - WH)^2 */
     GPUMLIB_NMF_OBJ_KL = 2,        /* sum V ln(V/WH) - V + WH, 0 ln 0 = 0 */
-    GPUMLIB_NMF_OBJ_QUALITY = 3    /* reference quality, nmf_kernels_quality.cu:35 */
+    GPUMLIB_NMF_OBJ_QUALITY = 3    /* reference quality, nmf_kernels_quality.cu:35: each term in FP32 as there,
+                                      summed in FP64 */
 };
 
 /* V store: the device form of V that gpumlib_nmf_iterate reads for a precision (no reference
