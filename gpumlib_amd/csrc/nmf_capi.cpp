@@ -134,10 +134,22 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     const size_t pbytes = align256(sizeof(float) * (p1 > p3 ? p1 : p3));
     const int smax = pl->S1 > pl->S3 ? pl->S1 : pl->S3;
     const size_t sbytes = align256(sizeof(float) * (size_t)smax * pl->rpad);
-    pl->tiles_obj = (m + TILE - 1) / TILE;
-    pl->S_obj = choose_slices(pl->tiles_obj, n, KC, target, &pl->ks_obj);
-    // one objective partial per workgroup of the H pass (FP32) or of the objective pass
-    pl->nobjp = pl->tiles1 * pl->S1 > pl->tiles_obj * pl->S_obj ? pl->tiles1 * pl->S1 : pl->tiles_obj * pl->S_obj;
+    // objective pass on split factors: 128-row blocks, each over csplit ranges of 32-column
+    // blocks, aiming at kObjectiveWorkgroups work items of at least 8 column blocks (every
+    // work item loads its W pieces once: fewer, longer column ranges read them fewer times)
+    pl->ob_rp = split_rank(r);
+    pl->ob_nrows = round_up(n, 128);
+    pl->ob_ncols = round_up(m, 32);
+    pl->ob_rblocks = pl->ob_nrows / 128;
+    pl->ob_cblocks = pl->ob_ncols / 32;
+    {
+        const int want = (kObjectiveWorkgroups + pl->ob_rblocks - 1) / pl->ob_rblocks;
+        const int most = pl->ob_cblocks / 8 > 1 ? pl->ob_cblocks / 8 : 1;
+        pl->ob_csplit = want < 1 ? 1 : (want > most ? most : want);
+    }
+    // one objective partial per workgroup of the KL H pass or per work item of the objective pass
+    pl->nobjp = pl->tiles1 * pl->S1 > pl->ob_rblocks * pl->ob_csplit ? pl->tiles1 * pl->S1
+                                                                     : pl->ob_rblocks * pl->ob_csplit;
     const size_t obytes = align256(sizeof(double) * (size_t)pl->nobjp);
     size_t off = 0;
     pl->off_P = off;
@@ -150,6 +162,10 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     off += align256(sizeof(float) * 2 * (size_t)r * pl->rpad);
     pl->off_cnt = off;  // their arrival counters
     off += 256;
+    pl->off_Ws = off;  // split pieces of W^T and of H
+    off += align256((size_t)3 * pl->ob_nrows * pl->ob_rp * 2);
+    pl->off_Hs = off;
+    off += align256((size_t)3 * pl->ob_ncols * pl->ob_rp * 2);
     if (precision == GPUMLIB_NMF_BF16) {
         pl->off_Wb = off;
         off += align256((size_t)r * pl->ldv8 * 2);
@@ -227,8 +243,12 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
     for (int it = 0; it < iterations; it++) {
         const int obj = (it == 0 && d_objective) ? objective : GPUMLIB_NMF_OBJ_NONE;
         if (h_half) {
-            CK(launch_pass_h_f32(a1, kl, obj, 1, st));
-            if (obj != GPUMLIB_NMF_OBJ_NONE) CK(launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st));
+            // The KL pass forms W H for its ratio, so the objective rides on it; the Euclidean
+            // one would need W H only for the objective: that runs as its own pass.
+            const int fobj = kl ? obj : GPUMLIB_NMF_OBJ_NONE;
+            if (obj != GPUMLIB_NMF_OBJ_NONE && !kl) CK(run_objective(pl, V, W, H, eps, obj, d_objective, ws, st));
+            CK(launch_pass_h_f32(a1, kl, fobj, st));
+            if (fobj != GPUMLIB_NMF_OBJ_NONE) CK(launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st));
             if (kl)
                 CK(launch_update_h_kl(H, a1.P, a1.Psum, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, st, add));
             else
@@ -247,21 +267,40 @@ hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, fl
     return hipSuccess;
 }
 
+// Objective of (W, H) against the plan's V store: W and H split into BF16 pieces, then the
+// objective pass over V (W H on the BF16 matrix cores to FP32 accuracy), then the FP64
+// reduction of its partials.
 hipError_t run_objective(const Plan& pl, const void* V, const float* W, const float* H, float eps, int objective,
                          double* d_objective, void* ws, hipStream_t st) {
-    PassArgs a1;
-    pass_args(pl, (const float*)V, W, H, ws, eps, 1, &a1);
-    a1.tiles = pl.tiles_obj;  // V's column tiles only, slices chosen for this pass
-    a1.S = pl.S_obj;
-    a1.kslice = pl.ks_obj;
-    if (pl.precision == GPUMLIB_NMF_BF16) {
-        CK(launch_objective_bf16v(a1, objective, st));
-    } else if (pl.precision == GPUMLIB_NMF_FP8) {
-        CK(launch_objective_fp8v(a1, objective, st));
-    } else {
-        CK(launch_pass_h_f32(a1, 0, objective, 0, st));
+    char* base = (char*)ws;
+    SplitObjArgs a;
+    memset(&a, 0, sizeof(a));
+    a.V = V;
+    a.n = pl.n;
+    a.m = pl.m;
+    a.r = pl.r;
+    a.rp = pl.ob_rp;
+    a.nrows = pl.ob_nrows;
+    a.ncols = pl.ob_ncols;
+    a.rblocks = pl.ob_rblocks;
+    a.cblocks = pl.ob_cblocks;
+    a.csplit = pl.ob_csplit;
+    a.eps = eps;
+    a.Ws = (const uint16_t*)(base + pl.off_Ws);
+    a.Hs = (const uint16_t*)(base + pl.off_Hs);
+    a.objp = (double*)(base + pl.off_objp);
+    a.ldv = pl.n;
+    a.vec = (pl.n % 4 == 0) && aligned16(V);
+    a.vtiles = lp_tiles(pl.m);
+    if (pl.precision == GPUMLIB_NMF_FP8) {
+        a.vsc = (const uint8_t*)V + fp8_store_layout(pl.n, pl.m).off_sc;
+        a.nbi = pl.nbi;
     }
-    return launch_reduce_f64(a1.objp, a1.tiles * a1.S, d_objective, st);
+    const int vtype = pl.precision == GPUMLIB_NMF_FP32 ? 0 : (pl.precision == GPUMLIB_NMF_BF16 ? 1 : 2);
+    CK(launch_split_wh(W, H, pl.n, pl.m, pl.r, pl.ob_rp, pl.ob_nrows, pl.ob_ncols, (uint16_t*)(base + pl.off_Ws),
+                       (uint16_t*)(base + pl.off_Hs), st));
+    CK(launch_objective_split(a, vtype, objective, st));
+    return launch_reduce_f64(a.objp, a.rblocks * a.csplit, d_objective, st);
 }
 
 // Ranks above kMaxFusedRank: the reference's own GEMM sequence over materialised operands
@@ -684,7 +723,7 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
     }
     PassArgs a;
     pass_args(pl, (const float*)V, W, H, workspace, eps, which == 0 ? 1 : 3, &a);
-    if (which == 0) return launch_pass_h_f32(a, kl, GPUMLIB_NMF_OBJ_NONE, 1, stream);
+    if (which == 0) return launch_pass_h_f32(a, kl, GPUMLIB_NMF_OBJ_NONE, stream);
     return launch_pass_w_f32(a, kl, stream);
 }
 

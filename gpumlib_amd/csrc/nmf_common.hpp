@@ -245,6 +245,25 @@ struct PassArgs {
     int* gcount;         // arrival counter of the gram-carrying workgroups (zero between passes)
 };
 
+// Objective pass over the split factors (nmf_objective.hip): V (FP32 n x m with leading
+// dimension ldv, or a tiled BF16 / FP8 store with its column tiles and FP8 tile scales),
+// the three BF16 pieces of W^T (piece p, row i at Ws + (p*nrows + i)*rp) and of H (piece p,
+// column u at Hs + (p*ncols + u)*rp), rp = split_rank(r), zero padded; work items are
+// rblocks 128-row blocks x csplit ranges of the cblocks 32-column blocks, one FP64
+// partial each in objp.
+struct SplitObjArgs {
+    const void* V;
+    const uint8_t* vsc;
+    int nbi, vtiles, ldv, vec;
+    const uint16_t* Ws;
+    const uint16_t* Hs;
+    int nrows, ncols;
+    int n, m, r, rp;
+    float eps;
+    int rblocks, cblocks, csplit;
+    double* objp;
+};
+
 // FP8 K-contiguous pass: as LpPassArgs, with E8M0 scales.  A's scale for column a
 // and K block kb is Asc[a * nbk + kb]; B's is Bsc[(col >> bcol_shift) * bs_col + kb * bs_k].
 struct Fp8PassArgs {

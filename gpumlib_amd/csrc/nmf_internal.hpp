@@ -9,7 +9,8 @@ namespace gpumlib_nmf {
 constexpr int kMaxGramSlices = 8;
 
 // Pass kernels (nmf_pass_f32.hip).
-hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st);
+// H pass (KL: with any objective of the entry state; Euclidean: without)
+hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, hipStream_t st);
 hipError_t launch_pass_w_f32(const PassArgs& a, int kl, hipStream_t st);
 
 // BF16 storage path (nmf_pass_bf16.hip).
@@ -24,9 +25,12 @@ hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv,
                              hipStream_t st);
 hipError_t launch_shadow_fp8(const float* X, int n, int m, int r, int which, uint8_t* Xq, int ldq, uint8_t* Xsc,
                              hipStream_t st);
-// Objective of (W, H) against a BF16- or FP8-stored V (nmf_pass_f32.hip).
-hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st);
-hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st);
+// Objective pass on split factors (nmf_objective.hip): W, H -> their three BF16 pieces
+// (rp = split_rank(r), nrows a multiple of 128 >= n, ncols a multiple of 32 >= m),
+// then the per-work-item FP64 partials of the objective over V (vtype 0 FP32, 1 BF16, 2 FP8).
+hipError_t launch_split_wh(const float* W, const float* H, int n, int m, int r, int rp, int nrows, int ncols,
+                           uint16_t* Ws, uint16_t* Hs, hipStream_t st);
+hipError_t launch_objective_split(const SplitObjArgs& a, int vtype, int obj, hipStream_t st);
 
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)

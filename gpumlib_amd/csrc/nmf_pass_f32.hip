@@ -20,7 +20,6 @@
 // with one barrier per step; the next one or two steps' global loads are in
 // flight (in registers) while the current step's MFMAs run.
 #include "nmf_common.hpp"
-#include "nmf_fp8.hpp"
 #include "nmf_internal.hpp"
 
 #include <type_traits>
@@ -41,29 +40,9 @@ __device__ double block_sum_f64(double v, double* red) {
     return t;
 }
 
-// 4 consecutive elements of a BF16 column as floats (bf16 -> f32 is exact).
-__device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, int kend) {
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!ok) return v;
-    uint16_t e[4];
-    if (k + 3 < kend) {
-        const uint2 x = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned: k % 4 == 0 in a tiled column
-        e[0] = x.x & 0xffff, e[1] = x.x >> 16, e[2] = x.y & 0xffff, e[3] = x.y >> 16;
-    } else {
-        for (int j = 0; j < 4; j++) e[j] = (k + j < kend) ? p[j] : (uint16_t)0;
-    }
-    v.x = __uint_as_float((uint32_t)e[0] << 16);
-    v.y = __uint_as_float((uint32_t)e[1] << 16);
-    v.z = __uint_as_float((uint32_t)e[2] << 16);
-    v.w = __uint_as_float((uint32_t)e[3] << 16);
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
-// entry; GEMM2 false: objective only (no partials written).
-// TV: element type of V (float, or uint16_t holding BF16 for the objective of a
-// BF16-stored V, or uint8_t holding FP8; those run with GEMM2 false).
+// entry.
 // Euclidean gram W^T W: its nb x nb blocks of 32 x 32 are spread over the first
 // tiles of every slice (block t on tile t; tiles past the last column tile of V
 // carry a gram block only), and inside a tile the 4 waves each take a quarter of
@@ -74,7 +53,7 @@ __device__ __forceinline__ float4 load4_bf16(const uint16_t* p, bool ok, int k, 
 // Euclidean: H(a,u) *= WtV(a,u) / (sum_b G(a,b) H(b,u) + eps) with G = W^T W from p.G (computed by
 // the gram launch before this one, G(a,b) at p.G[b*rpad + a]); KL: H(a,u) *= num(a,u) /
 // max(colsum_a, eps), the column sums of W taken over the whole of K in this item.
-template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float, bool FUSE = false>
+template <int MB, bool KL, int OBJ, bool FUSE = false>
 __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr bool NEED_WH = KL || (OBJ != OBJ_NONE);
@@ -87,11 +66,10 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
     const int c = lane & 31, h = lane >> 5;
     const int vt = (p.m + TILE - 1) / TILE;
-    const TV* Vt = reinterpret_cast<const TV*>(p.V);
     const int col0 = t * TILE;
     const int ncols = (t < vt) ? min(TILE, p.m - col0) : 0;
     const int nblk = nb * nb;
-    const bool gcar = (!FUSE) && (!KL) && GEMM2 && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
+    const bool gcar = (!FUSE) && (!KL) && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
     const int kend = min(p.n, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -110,32 +88,13 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     // Register staging of the next chunks' global loads.  The plain Euclidean pass keeps
     // two chunks in flight (two register sets; the LDS is double-buffered); the KL and
     // objective variants, whose registers also hold H and the ratio, keep one.
-    constexpr bool DEEP = !KL && OBJ == OBJ_NONE && GEMM2 && sizeof(TV) == 4;
+    constexpr bool DEEP = !KL && OBJ == OBJ_NONE;
     float4 ra0[MB], rb0[4], ra1[DEEP ? MB : 1], rb1[DEEP ? 4 : 1];
     auto gload = [&](float4 (&rb)[4], float4 (&ra)[MB], int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
-            if constexpr (sizeof(TV) == 1) {
-                // FP8 E4M3 with the E8M0 scale of V's 32 x 32 tile (k..k+3 share one tile)
-                const int kk = k0 + kq, u = col0 + col;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (col < ncols && kk < kend) {
-                    const uint8_t* q8 = reinterpret_cast<const uint8_t*>(Vt) + fp8_tiled_index(kk, u, p.vtiles);
-                    const float sc = e8m0_value(p.vsc[(size_t)(u >> 5) * p.nbi + (kk >> 5)]);
-                    v.x = e4m3_to_f32(q8[0]) * sc;
-                    if (kk + 1 < kend) v.y = e4m3_to_f32(q8[1]) * sc;
-                    if (kk + 2 < kend) v.z = e4m3_to_f32(q8[2]) * sc;
-                    if (kk + 3 < kend) v.w = e4m3_to_f32(q8[3]) * sc;
-                }
-                rb[j] = v;
-            } else if constexpr (sizeof(TV) == 2) {
-                rb[j] = load4_bf16(reinterpret_cast<const uint16_t*>(Vt) + bf16_tiled_index(k0 + kq, col0 + col, p.vtiles),
-                                   col < ncols, k0 + kq, kend);
-            } else {
-                rb[j] = load4(reinterpret_cast<const float*>(Vt) + (size_t)(col0 + col) * p.ldv + k0 + kq,
-                              col < ncols, k0 + kq, kend, vec);
-            }
+            rb[j] = load4(p.V + (size_t)(col0 + col) * p.ldv + k0 + kq, col < ncols, k0 + kq, kend, vec);
         }
 #pragma unroll
         for (int j = 0; j < MB; j++) {
@@ -164,7 +123,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int gmb = gcar ? t / nb : 0, gnb = gcar ? t - (t / nb) * nb : 0;  // carried gram block
     double objacc = 0.0;
     float colsum = 0.f;  // KL: sum over this slice of W(i, tid) (tile 0 only)
-    const bool do_sums = KL && GEMM2 && (FUSE || t == 0) && (tid < RP);
+    const bool do_sums = KL && (FUSE || t == 0) && (tid < RP);
 
     // The gram-carrying and the plain workgroups run separate copies of the loop, so
     // that no MFMA accumulator sits under a data-dependent branch.
@@ -200,7 +159,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                     if (OBJ != OBJ_NONE) objacc += obj_term<OBJ>(vj[j], wh[g * 4 + j], p.eps);
                     if (KL) rj[j] = vj[j] / (wh[g * 4 + j] + p.eps);
                 }
-                if (KL && GEMM2) {
+                if (KL) {
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++) {
                         if (mb >= nb) continue;
@@ -213,7 +172,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 }
             }
         }
-        if (!KL && GEMM2) {
+        if (!KL) {
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const float4 bv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
@@ -330,7 +289,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 }
             }
         }
-    } else if (GEMM2) {
+    } else {
         // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
         const int ul = wave * 32 + c;
         if (ul < ncols) {
@@ -729,11 +688,11 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
 
 // ---------------------------------------------------------------------------
 // Single-problem kernels: one workgroup per work item, XCD-aware order.
-template <int MB, bool KL, int OBJ, bool GEMM2, typename TV = float>
+template <int MB, bool KL, int OBJ>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_h_f32(PassArgs p) {
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
     if (w >= p.tiles * p.S) return;
-    pass_h_body<MB, KL, OBJ, GEMM2, TV>(p, w, MB);  // MB = ceil(r/32)
+    pass_h_body<MB, KL, OBJ>(p, w, MB);  // MB = ceil(r/32)
 }
 
 template <int MB>
@@ -772,7 +731,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __r
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<2, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<2, KL, OBJ>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
@@ -940,7 +899,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<2, KL, OBJ, true, float, true>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<2, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
@@ -994,91 +953,33 @@ hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, i
 // ---------------------------------------------------------------------------
 // Host launchers.
 template <int MB>
-static hipError_t launch_h_mb(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st) {
+static hipError_t launch_h_mb(const PassArgs& a, int kl, int obj, hipStream_t st) {
     const int grid = xcd_grid(a.tiles * a.S);
-#define LH(KLV, OBJV, G2)                                                                        \
-    hipLaunchKernelGGL((k_pass_h_f32<MB, KLV, OBJV, G2>), dim3(grid), dim3(NTHREADS), 0, st, a); \
+#define LH(KLV, OBJV)                                                                          \
+    hipLaunchKernelGGL((k_pass_h_f32<MB, KLV, OBJV>), dim3(grid), dim3(NTHREADS), 0, st, a); \
     return hipGetLastError();
-#define LH_OBJ(KLV, G2)                       \
-    switch (obj) {                            \
-        case OBJ_NONE: LH(KLV, OBJ_NONE, G2)  \
-        case OBJ_FROB: LH(KLV, OBJ_FROB, G2)  \
-        case OBJ_KL: LH(KLV, OBJ_KL, G2)      \
-        case OBJ_QUALITY: LH(KLV, OBJ_QUALITY, G2) \
-        default: return hipErrorInvalidValue; \
+    if (!kl) {
+        // the Euclidean pass carries no objective (it would need W H for that alone: the
+        // objective pass on split factors computes it)
+        if (obj != OBJ_NONE) return hipErrorInvalidValue;
+        LH(false, OBJ_NONE)
     }
-    if (!gemm2) {
-        // objective only: the rule does not enter
-        switch (obj) {
-            case OBJ_FROB: LH(false, OBJ_FROB, false)
-            case OBJ_KL: LH(false, OBJ_KL, false)
-            case OBJ_QUALITY: LH(false, OBJ_QUALITY, false)
-            default: return hipErrorInvalidValue;
-        }
+    switch (obj) {
+        case OBJ_NONE: LH(true, OBJ_NONE)
+        case OBJ_FROB: LH(true, OBJ_FROB)
+        case OBJ_KL: LH(true, OBJ_KL)
+        case OBJ_QUALITY: LH(true, OBJ_QUALITY)
+        default: return hipErrorInvalidValue;
     }
-    if (kl) {
-        LH_OBJ(true, true)
-    } else {
-        LH_OBJ(false, true)
-    }
-#undef LH_OBJ
 #undef LH
 }
 
-hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, int gemm2, hipStream_t st) {
+hipError_t launch_pass_h_f32(const PassArgs& a, int kl, int obj, hipStream_t st) {
     const int rp = round_up(a.r, 32);
-    if (rp <= 32) return launch_h_mb<1>(a, kl, obj, gemm2, st);
-    if (rp <= 64) return launch_h_mb<2>(a, kl, obj, gemm2, st);
-    if (rp <= 96) return launch_h_mb<3>(a, kl, obj, gemm2, st);
-    if (rp <= 128) return launch_h_mb<4>(a, kl, obj, gemm2, st);
-    return hipErrorInvalidValue;
-}
-
-template <int MB>
-static hipError_t launch_obj_bf16v_mb(const PassArgs& a, int obj, hipStream_t st) {
-    const int grid = xcd_grid(a.tiles * a.S);
-#define LO(OBJV)                                                                                              \
-    hipLaunchKernelGGL((k_pass_h_f32<MB, false, OBJV, false, uint16_t>), dim3(grid), dim3(NTHREADS), 0, st, a); \
-    return hipGetLastError();
-    switch (obj) {
-        case OBJ_FROB: LO(OBJ_FROB)
-        case OBJ_KL: LO(OBJ_KL)
-        case OBJ_QUALITY: LO(OBJ_QUALITY)
-        default: return hipErrorInvalidValue;
-    }
-#undef LO
-}
-
-template <int MB>
-static hipError_t launch_obj_fp8v_mb(const PassArgs& a, int obj, hipStream_t st) {
-    const int grid = xcd_grid(a.tiles * a.S);
-#define LO(OBJV)                                                                                             \
-    hipLaunchKernelGGL((k_pass_h_f32<MB, false, OBJV, false, uint8_t>), dim3(grid), dim3(NTHREADS), 0, st, a); \
-    return hipGetLastError();
-    switch (obj) {
-        case OBJ_FROB: LO(OBJ_FROB)
-        case OBJ_KL: LO(OBJ_KL)
-        case OBJ_QUALITY: LO(OBJ_QUALITY)
-        default: return hipErrorInvalidValue;
-    }
-#undef LO
-}
-
-hipError_t launch_objective_fp8v(const PassArgs& a, int obj, hipStream_t st) {
-    const int rp = round_up(a.r, 32);
-    if (rp <= 32) return launch_obj_fp8v_mb<1>(a, obj, st);
-    if (rp <= 64) return launch_obj_fp8v_mb<2>(a, obj, st);
-    if (rp <= 96) return launch_obj_fp8v_mb<3>(a, obj, st);
-    if (rp <= 128) return launch_obj_fp8v_mb<4>(a, obj, st);
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_objective_bf16v(const PassArgs& a, int obj, hipStream_t st) {
-    const int rp = round_up(a.r, 32);
-    if (rp <= 32) return launch_obj_bf16v_mb<1>(a, obj, st);
-    if (rp <= 64) return launch_obj_bf16v_mb<2>(a, obj, st);
-    if (rp <= 96) return launch_obj_bf16v_mb<3>(a, obj, st);
-    if (rp <= 128) return launch_obj_bf16v_mb<4>(a, obj, st);
+    if (rp <= 32) return launch_h_mb<1>(a, kl, obj, st);
+    if (rp <= 64) return launch_h_mb<2>(a, kl, obj, st);
+    if (rp <= 96) return launch_h_mb<3>(a, kl, obj, st);
+    if (rp <= 128) return launch_h_mb<4>(a, kl, obj, st);
     return hipErrorInvalidValue;
 }
 

@@ -15,6 +15,11 @@ constexpr float kReferenceEpsilon = 10e-9f;
 constexpr int kMaxFusedRank = 128;
 // Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
 constexpr int kTargetWorkgroups = 512;
+// Work items aimed for by the objective pass on split factors.
+constexpr int kObjectiveWorkgroups = 512;
+// Padded rank of the split factors: whole MFMA steps of 16, in the kernel's four register
+// shapes (16, 32, 64, 128).
+__host__ __device__ inline int split_rank(int r) { return r <= 16 ? 16 : (r <= 32 ? 32 : (r <= 64 ? 64 : 128)); }
 
 // Update rules (GPUMLIB_NMF_* rule constants of the C-ABI): which passes a rule runs
 // (the KL ones or the Euclidean ones) and whether its update is the additive form.
@@ -31,10 +36,12 @@ struct Plan {
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;
-    int tiles_obj, S_obj, ks_obj;  // objective-only pass (FP32-style H pass over V's column tiles)
     int nobjp;
     int ld16n, ld16m, nbi, nbu;  // FP8 storage: leading dimensions (fp8_ld of n, m) and their 32-blocks
-    size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, bytes;
+    // objective pass on split factors (nmf_objective.hip): padded rank, rows and columns of the
+    // pieces, 128-row blocks, 32-column blocks and column ranges per row block
+    int ob_rp, ob_nrows, ob_ncols, ob_rblocks, ob_cblocks, ob_csplit;
+    size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, off_Ws, off_Hs, bytes;
     // unfused iteration: WH (n x m; the KL ratio in place), numerator and denominator products
     // (max(r m, n r) floats each), the r x r gram, the clamped sums (r), FP64 objective partials
     size_t off_uWH, off_uA, off_uB, off_uG, off_usum, off_uobj;
