@@ -35,10 +35,17 @@ namespace {
 // Launch tables.  Split-K engine: the passes (per rule) and the updates.  Fused engine:
 // the grams of W and H (Euclidean problems) and the fused passes (per rule).  Both: the
 // per-problem objective reduction.
+// The pass tables come in two rank classes: the problems run by the kernels built for two
+// rank blocks of 32 (r <= 64), and, when ranks are split, those with r <= 32 run by the
+// kernels built for one block (T_*1: half the accumulator registers and operand LDS).
 enum {
     T_PH_EUC = 0, T_PH_KL, T_PW_EUC, T_PW_KL, T_UH, T_UW, T_RED,
-    T_GW, T_GH, T_FH_EUC, T_FH_KL, T_FW_EUC, T_FW_KL, T_COUNT
+    T_GW, T_GH, T_FH_EUC, T_FH_KL, T_FW_EUC, T_FW_KL,
+    T_PH_EUC1, T_PH_KL1, T_PW_EUC1, T_PW_KL1, T_FH_EUC1, T_FH_KL1, T_FW_EUC1, T_FW_KL1, T_COUNT
 };
+// the two-block table of a one-block table kind (or the kind itself)
+constexpr int base_kind(int k) { return k >= T_PH_EUC1 ? (k < T_FH_EUC1 ? k - T_PH_EUC1 : k - T_FH_EUC1 + T_FH_EUC) : k; }
+constexpr bool small_kind(int k) { return k >= T_PH_EUC1; }
 enum { ENGINE_SPLIT = 0, ENGINE_FUSED = 1 };
 constexpr int kPassGrid = 1024;  // persistent workgroups of the grouped passes (before tuning)
 constexpr int kUpdGrid = 512;
@@ -64,7 +71,7 @@ struct Slot {
 };
 
 struct TuneConfig {
-    int engine, target, pass_grid, upd_grid;
+    int engine, target, pass_grid, upd_grid, split_ranks;
     double ms_per_iteration, ms_default;
     int trials;
 };
@@ -86,6 +93,7 @@ struct gpumlib_nmf_batch {
     int target = kBatchTargetWorkgroups;   // K-split target the problems are planned with
     int size_target = kBatchSizingTarget;  // target their workspaces are sized for
     int pass_grid = kPassGrid, upd_grid = kUpdGrid;
+    int split_ranks = 0;  // 1: problems with r <= 32 run on the one-rank-block pass kernels
     std::map<unsigned long long, TuneConfig> tuned;  // winners by mix signature
     std::vector<Slot> slots;
     JobArgs* d_jobs = nullptr;
@@ -122,13 +130,19 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
         int* first = t + 2;
         int* idx = t + 3 + kGroupMaxJobs;
         order.clear();
+        const int kb = base_kind(k);
         for (int s = 0; s < b->maxj; s++) {
             const Slot& sl = b->slots[s];
             if (!sl.active) continue;
             const bool euc = sl.rule == GPUMLIB_NMF_EUCLIDEAN;
+            // which rank class of the pass tables the problem goes to
+            if (kb <= T_PW_KL || (kb >= T_FH_EUC && kb <= T_FW_KL)) {
+                const bool small = b->split_ranks && sl.r <= 32;
+                if (small != small_kind(k)) continue;
+            }
             int items = 0;
             long cost = 0;  // fused tables run longest items first (the grid strides over them in order)
-            switch (k) {
+            switch (kb) {
                 case T_PH_EUC: items = !fused && euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
                 case T_PH_KL: items = !fused && !euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
                 case T_PW_EUC: items = !fused && euc ? sl.args.w.tiles * sl.args.w.S : 0; break;
@@ -151,7 +165,7 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
         for (const auto& o : order) {
             const Slot& sl = b->slots[o.second];
             int items = 0;
-            switch (k) {
+            switch (kb) {
                 case T_PH_EUC: case T_PH_KL: items = sl.args.h.tiles * sl.args.h.S; break;
                 case T_PW_EUC: case T_PW_KL: items = sl.args.w.tiles * sl.args.w.S; break;
                 case T_UH: items = (sl.m + UPD_COLS - 1) / UPD_COLS; break;
@@ -184,35 +198,42 @@ static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, 
     const int stride = table_stride();
     const int* T = b->d_tables;
     const int pg = b->pass_grid, ug = b->upd_grid;
+    const bool sr = b->split_ranks != 0;  // the one-rank-block tables are launched only when used
     if (b->engine == ENGINE_FUSED) {
         // per iteration: gram of W; fused H passes (objective of the entry state fused in);
         // gram of the new H; fused W passes
         for (int it = 0; it < iterations; it++) {
             const bool obj = it < objectives;
+            const int oe = obj ? OBJ_FROB : OBJ_NONE, ok = obj ? OBJ_KL : OBJ_NONE;
             hipError_t e;
             if ((e = launch_group_gram(b->d_jobs, T + T_GW * stride, 0, ug, st))) return e;
-            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC * stride, 0, obj ? OBJ_FROB : OBJ_NONE, pg, st)))
-                return e;
-            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_KL * stride, 1, obj ? OBJ_KL : OBJ_NONE, pg, st)))
-                return e;
+            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC * stride, 0, oe, 2, pg, st))) return e;
+            if (sr && (e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC1 * stride, 0, oe, 1, pg, st))) return e;
+            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_KL * stride, 1, ok, 2, pg, st))) return e;
+            if (sr && (e = launch_group_fused_h(b->d_jobs, T + T_FH_KL1 * stride, 1, ok, 1, pg, st))) return e;
             if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
             if ((e = launch_group_gram(b->d_jobs, T + T_GH * stride, 1, ug, st))) return e;
-            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC * stride, 0, pg, st))) return e;
-            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_KL * stride, 1, pg, st))) return e;
+            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC * stride, 0, 2, pg, st))) return e;
+            if (sr && (e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC1 * stride, 0, 1, pg, st))) return e;
+            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_KL * stride, 1, 2, pg, st))) return e;
+            if (sr && (e = launch_group_fused_w(b->d_jobs, T + T_FW_KL1 * stride, 1, 1, pg, st))) return e;
         }
         return hipSuccess;
     }
     for (int it = 0; it < iterations; it++) {
         const bool obj = it < objectives;
+        const int oe = obj ? OBJ_FROB : OBJ_NONE, ok = obj ? OBJ_KL : OBJ_NONE;
         hipError_t e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC * stride, 0, obj ? OBJ_FROB : OBJ_NONE, pg, st)))
-            return e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_KL * stride, 1, obj ? OBJ_KL : OBJ_NONE, pg, st)))
-            return e;
+        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC * stride, 0, oe, 2, pg, st))) return e;
+        if (sr && (e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC1 * stride, 0, oe, 1, pg, st))) return e;
+        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_KL * stride, 1, ok, 2, pg, st))) return e;
+        if (sr && (e = launch_group_pass_h(b->d_jobs, T + T_PH_KL1 * stride, 1, ok, 1, pg, st))) return e;
         if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
         if ((e = launch_group_update_h(b->d_jobs, T + T_UH * stride, ug, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC * stride, 0, pg, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_KL * stride, 1, pg, st))) return e;
+        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC * stride, 0, 2, pg, st))) return e;
+        if (sr && (e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC1 * stride, 0, 1, pg, st))) return e;
+        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_KL * stride, 1, 2, pg, st))) return e;
+        if (sr && (e = launch_group_pass_w(b->d_jobs, T + T_PW_KL1 * stride, 1, 1, pg, st))) return e;
         if ((e = launch_group_update_w(b->d_jobs, T + T_UW * stride, ug, st))) return e;
     }
     return hipSuccess;
@@ -325,6 +346,7 @@ hipError_t gpumlib_nmf_batch_create(int max_jobs, gpumlib_nmf_batch_handle* out)
     hipError_t e = hipMalloc((void**)&b->d_jobs, sizeof(JobArgs) * (size_t)max_jobs);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_tables, sizeof(int) * (size_t)table_stride() * T_COUNT);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_obj, sizeof(double) * 2 * (size_t)max_jobs);
+    if (e == hipSuccess) e = hipMemset(b->d_obj, 0, sizeof(double) * 2 * (size_t)max_jobs);  // unused slots read 0
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_jobs, sizeof(JobArgs) * (size_t)max_jobs, hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&b->h_tables, sizeof(int) * (size_t)table_stride() * T_COUNT, hipHostMallocDefault);
@@ -369,6 +391,18 @@ hipError_t gpumlib_nmf_batch_set_engine(gpumlib_nmf_batch_handle b, int engine) 
 }
 
 int gpumlib_nmf_batch_engine(gpumlib_nmf_batch_handle b) { return b ? b->engine : -1; }
+
+hipError_t gpumlib_nmf_batch_set_rank_split(gpumlib_nmf_batch_handle b, int on) {
+    if (!b || (on != 0 && on != 1)) return hipErrorInvalidValue;
+    if (on == b->split_ranks) return hipSuccess;
+    wait_staging(b);
+    drop_graphs(b);
+    b->split_ranks = on;
+    b->dirty = true;
+    return hipSuccess;
+}
+
+int gpumlib_nmf_batch_rank_split(gpumlib_nmf_batch_handle b) { return b ? b->split_ranks : -1; }
 
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r) {
     Plan pl;
@@ -481,18 +515,20 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         out->ms_per_iteration_before = c.ms_default;
         out->trials = cached ? 0 : c.trials;
         out->cached = cached;
+        out->split_ranks = c.split_ranks;
     };
-    const int en0 = b->engine, t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid;
+    const int en0 = b->engine, t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid, sr0 = b->split_ranks;
     const unsigned long long sig = mix_signature(b);
     auto hit = b->tuned.find(sig);
     if (hit != b->tuned.end()) {  // this mix was tuned before: apply its winner
         const TuneConfig& c = hit->second;
+        b->split_ranks = c.split_ranks;
         hipError_t e = replan_all(b, c.engine, c.target, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return e;
         b->pass_grid = c.pass_grid;
         b->upd_grid = c.upd_grid;
-        if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+        if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0 || b->split_ranks != sr0) drop_graphs(b);
         report(c, 1);
         return hipSuccess;
     }
@@ -534,8 +570,9 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         }
         return ce;
     };
-    TuneConfig best = {en0, t0, pg0, ug0, 1e30, 0.0, 0};
-    auto trial = [&](int engine, int target, int pg, int ug) -> hipError_t {
+    TuneConfig best = {en0, t0, pg0, ug0, sr0, 1e30, 0.0, 0};
+    auto trial = [&](int engine, int target, int pg, int ug, int sr) -> hipError_t {
+        b->split_ranks = sr;
         hipError_t te = replan_all(b, engine, target, stream);
         if (te != hipSuccess) return te;
         b->pass_grid = pg;
@@ -550,30 +587,34 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
             best.target = target;
             best.pass_grid = pg;
             best.upd_grid = ug;
+            best.split_ranks = sr;
             best.ms_per_iteration = ms;
         }
         return hipSuccess;
     };
     // the configuration in effect first; then the tile shape along K: the fused engine
     // (one slice, the whole contraction per tile) and the split-K targets, with the grids in
-    // effect; then the pass grids and the update (fused: gram) grids around the best so far
+    // effect; then the pass grids and the update (fused: gram) grids around the best so far;
+    // then the rank blocking: problems with r <= 32 on the one-block kernels, or not
     if (e == hipSuccess) e = copy_wh(true);
-    if (e == hipSuccess) e = trial(en0, t0, pg0, ug0);
-    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, t0, pg0, ug0);
+    if (e == hipSuccess) e = trial(en0, t0, pg0, ug0, sr0);
+    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, t0, pg0, ug0, sr0);
     for (int t : kTuneTargets)
-        if (e == hipSuccess && (en0 != ENGINE_SPLIT || t != t0) && fits(t)) e = trial(ENGINE_SPLIT, t, pg0, ug0);
+        if (e == hipSuccess && (en0 != ENGINE_SPLIT || t != t0) && fits(t)) e = trial(ENGINE_SPLIT, t, pg0, ug0, sr0);
     const int enb = best.engine, tb = best.target;
     for (int pg : kTunePassGrids)
-        if (e == hipSuccess && pg != pg0) e = trial(enb, tb, pg, ug0);
+        if (e == hipSuccess && pg != pg0) e = trial(enb, tb, pg, ug0, sr0);
     const int pgb = best.pass_grid;
     for (int ug : kTuneUpdGrids)
-        if (e == hipSuccess && ug != ug0) e = trial(enb, tb, pgb, ug);
+        if (e == hipSuccess && ug != ug0) e = trial(enb, tb, pgb, ug, sr0);
+    if (e == hipSuccess) e = trial(enb, tb, pgb, best.upd_grid, !sr0);
     // apply the winner, or the configuration in effect before if a trial failed
     const bool ok = (e == hipSuccess);
+    b->split_ranks = ok ? best.split_ranks : sr0;
     hipError_t ae = replan_all(b, ok ? best.engine : en0, ok ? best.target : t0, stream);
     b->pass_grid = ok ? best.pass_grid : pg0;
     b->upd_grid = ok ? best.upd_grid : ug0;
-    if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0) drop_graphs(b);
+    if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0 || b->split_ranks != sr0) drop_graphs(b);
     if (ae == hipSuccess) ae = hipStreamSynchronize(stream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

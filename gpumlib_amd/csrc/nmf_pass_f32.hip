@@ -722,7 +722,9 @@ __device__ __forceinline__ int group_find(const int* first, int J, int item) {
     return lo;
 }
 
-template <bool KL, int OBJ>
+// MB: rank blocks of 32 the kernel is built for (1: every problem of the table has r <= 32;
+// 2: r <= 64); a problem runs ceil(r/32) of them.
+template <bool KL, int OBJ, int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -731,11 +733,11 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __r
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<2, KL, OBJ>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<MB, KL, OBJ>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-template <bool KL>
+template <bool KL, int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_w(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -745,15 +747,17 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_w(const JobArgs* __r
         const PassArgs p = jobs[idx[j]].w;
         __syncthreads();
         if (KL)
-            pass_w_kl_body<2>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_kl_body<MB>(p, item - first[j], (p.r + 31) >> 5);
         else
-            pass_w_euc_body<2>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_euc_body<MB>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int grid, hipStream_t st) {
-#define LG(K, O)                                                                                 \
-    hipLaunchKernelGGL((k_group_pass_h<K, O>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+template <int MB>
+static hipError_t launch_group_pass_h_mb(const JobArgs* jobs, const int* table, int kl, int obj, int grid,
+                                         hipStream_t st) {
+#define LG(K, O)                                                                                     \
+    hipLaunchKernelGGL((k_group_pass_h<K, O, MB>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
     return hipGetLastError();
     if (kl) {
         if (obj == OBJ_NONE) { LG(true, OBJ_NONE) }
@@ -766,11 +770,24 @@ hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, in
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
-    if (kl)
-        hipLaunchKernelGGL((k_group_pass_w<true>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+                               hipStream_t st) {
+    if (mb == 1) return launch_group_pass_h_mb<1>(jobs, table, kl, obj, grid, st);
+    if (mb == 2) return launch_group_pass_h_mb<2>(jobs, table, kl, obj, grid, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st) {
+    if (mb == 1 && kl)
+        hipLaunchKernelGGL((k_group_pass_w<true, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 1)
+        hipLaunchKernelGGL((k_group_pass_w<false, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 2 && kl)
+        hipLaunchKernelGGL((k_group_pass_w<true, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 2)
+        hipLaunchKernelGGL((k_group_pass_w<false, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     else
-        hipLaunchKernelGGL((k_group_pass_w<false>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -890,7 +907,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_gram(const JobArgs* __restri
     }
 }
 
-template <bool KL, int OBJ>
+template <bool KL, int OBJ, int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -899,11 +916,11 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<2, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<MB, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-template <bool KL>
+template <bool KL, int MB>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_w(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -913,9 +930,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_w(const JobArgs* __
         const PassArgs p = jobs[idx[j]].w;
         __syncthreads();
         if (KL)
-            pass_w_kl_body<2, true>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_kl_body<MB, true>(p, item - first[j], (p.r + 31) >> 5);
         else
-            pass_w_euc_body<2, true>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_euc_body<MB, true>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
@@ -927,9 +944,11 @@ hipError_t launch_group_gram(const JobArgs* jobs, const int* table, int which, i
     return hipGetLastError();
 }
 
-hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int grid, hipStream_t st) {
-#define LF(K, O)                                                                                  \
-    hipLaunchKernelGGL((k_group_fused_h<K, O>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+template <int MB>
+static hipError_t launch_group_fused_h_mb(const JobArgs* jobs, const int* table, int kl, int obj, int grid,
+                                          hipStream_t st) {
+#define LF(K, O)                                                                                      \
+    hipLaunchKernelGGL((k_group_fused_h<K, O, MB>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
     return hipGetLastError();
     if (kl) {
         if (obj == OBJ_NONE) { LF(true, OBJ_NONE) }
@@ -942,11 +961,24 @@ hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, i
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
-    if (kl)
-        hipLaunchKernelGGL((k_group_fused_w<true>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+                                hipStream_t st) {
+    if (mb == 1) return launch_group_fused_h_mb<1>(jobs, table, kl, obj, grid, st);
+    if (mb == 2) return launch_group_fused_h_mb<2>(jobs, table, kl, obj, grid, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st) {
+    if (mb == 1 && kl)
+        hipLaunchKernelGGL((k_group_fused_w<true, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 1)
+        hipLaunchKernelGGL((k_group_fused_w<false, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 2 && kl)
+        hipLaunchKernelGGL((k_group_fused_w<true, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+    else if (mb == 2)
+        hipLaunchKernelGGL((k_group_fused_w<false, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     else
-        hipLaunchKernelGGL((k_group_fused_w<false>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -156,10 +156,16 @@ struct gpumlib_nmf_server {
     std::thread engine;
 
     double* d_obj = nullptr;  // one-at-a-time mode: the round's two objectives
+    // continuous batching: uploads of the next round's problems and copies of finished ones
+    // run on their own stream while a round runs on `stream`
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_staged = nullptr, ev_retired = nullptr;
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
+    void run_batched();
     hipError_t run_one(Job& j);
-    bool admit_locked(std::vector<std::shared_ptr<Job>>& admitted);
+    bool admit_locked(std::vector<std::shared_ptr<Job>>& admitted, int reserved = 0);
+    void fail_all_locked(hipError_t e, std::vector<std::shared_ptr<Job>>& extra);
 };
 
 // Device bytes of a problem: V, W, H and its workspace (the batch's plan, or in the
@@ -175,8 +181,10 @@ static size_t job_footprint(gpumlib_nmf_batch_handle b, bool sequential, Job& j)
     return j.offWs + align256(j.ws_bytes);
 }
 
-bool gpumlib_nmf_server::admit_locked(std::vector<std::shared_ptr<Job>>& admitted) {
-    int free_slots = cfg.max_active - (int)std::count_if(active.begin(), active.end(), [](auto& p) { return !!p; });
+// reserved: slots already promised to problems admitted but not yet in the batch.
+bool gpumlib_nmf_server::admit_locked(std::vector<std::shared_ptr<Job>>& admitted, int reserved) {
+    int free_slots =
+        cfg.max_active - reserved - (int)std::count_if(active.begin(), active.end(), [](auto& p) { return !!p; });
     bool any = false;
     for (auto it = pending.begin(); it != pending.end() && free_slots > 0;) {
         Job& j = **it;
@@ -285,50 +293,113 @@ void gpumlib_nmf_server::run() {
             if (e != hipSuccess) return;
         }
     }
+    run_batched();
+}
+
+// Marks every unfinished problem (queued, staged or active, plus `extra`) as failed with e.
+void gpumlib_nmf_server::fail_all_locked(hipError_t e, std::vector<std::shared_ptr<Job>>& extra) {
+    fatal = e;
+    auto fail = [&](const std::shared_ptr<Job>& jp) {
+        if (!jp || jp->done) return;
+        jp->status = e;
+        jp->done = true;
+    };
+    for (auto& jp : pending) fail(jp);
+    pending.clear();
+    for (auto& jp : active) fail(jp);
+    for (auto& jp : extra) fail(jp);
+}
+
+// Continuous batching, pipelined: while round k runs on `stream`, the engine thread
+// completes the retirement of the problems that finished after round k - 1 (their W and H
+// were copied back on the copy stream), releases their memory, admits queued problems
+// into it and uploads them on the copy stream; at the round boundary only the objectives
+// are read, the finished problems leave the batch (their copies queued) and the uploaded
+// ones join it (the round stream waits for their upload).
+void gpumlib_nmf_server::run_batched() {
+    const int rounds_iters = cfg.round_iterations;
+    std::vector<std::shared_ptr<Job>> staged;    // uploaded (or uploading) on copy_stream, not yet in the batch
+    std::vector<std::shared_ptr<Job>> retiring;  // finished, W and H being copied back on copy_stream
+    hipError_t e = hipSuccess;
+    auto upload = [&](std::vector<std::shared_ptr<Job>>& admitted) {
+        for (auto& jp : admitted) {
+            Job& j = *jp;
+            char* base = d_pool + j.off;
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(base + j.offV, j.V.data(), sizeof(float) * j.V.size(), hipMemcpyHostToDevice,
+                                   copy_stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(base + j.offW, j.W.data(), sizeof(float) * j.W.size(), hipMemcpyHostToDevice,
+                                   copy_stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(base + j.offH, j.H.data(), sizeof(float) * j.H.size(), hipMemcpyHostToDevice,
+                                   copy_stream);
+            staged.push_back(jp);
+        }
+        if (e == hipSuccess && !admitted.empty()) e = hipEventRecord(ev_staged, copy_stream);
+        admitted.clear();
+    };
+    auto finish_retiring = [&]() {
+        if (retiring.empty()) return;
+        const hipError_t ce = hipEventSynchronize(ev_retired);
+        if (e == hipSuccess) e = ce;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& jp : retiring) {
+                Job& j = *jp;
+                pool.release(j.off, j.bytes);
+                if (j.status == hipSuccess) j.status = ce;
+                j.t_finish = now();
+                j.done = true;
+                stats.completed++;
+            }
+            stats.pool_high_water = pool.high_water();
+        }
+        retiring.clear();
+        cv_done.notify_all();
+    };
     for (;;) {
+        // admission: wait for work when nothing is resident, staged or retiring
         std::vector<std::shared_ptr<Job>> admitted;
+        bool have_active = false;
         {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
                 if (abandon) return;
-                admit_locked(admitted);
-                const bool have_active = std::any_of(active.begin(), active.end(), [](auto& p) { return !!p; });
-                if (!admitted.empty() || have_active) break;
+                admit_locked(admitted, (int)staged.size());
+                have_active = std::any_of(active.begin(), active.end(), [](auto& p) { return !!p; });
+                if (!admitted.empty() || have_active || !staged.empty() || !retiring.empty()) break;
                 if (stop) return;
                 cv_work.wait(lk);
             }
         }
-        // upload and register the admitted problems
-        hipError_t e = hipSuccess;
-        for (auto& jp : admitted) {
+        upload(admitted);
+        finish_retiring();
+        if (!have_active && staged.empty()) continue;  // only retirements were pending
+        // round boundary: the staged problems join the batch
+        std::vector<std::shared_ptr<Job>> joined;
+        if (e == hipSuccess && !staged.empty()) e = hipStreamWaitEvent(stream, ev_staged, 0);
+        for (auto& jp : staged) {
             Job& j = *jp;
             char* base = d_pool + j.off;
-            float* dV = (float*)(base + j.offV);
-            float* dW = (float*)(base + j.offW);
-            float* dH = (float*)(base + j.offH);
-            if (e == hipSuccess) e = hipMemcpyAsync(dV, j.V.data(), sizeof(float) * j.V.size(), hipMemcpyHostToDevice, stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(dW, j.W.data(), sizeof(float) * j.W.size(), hipMemcpyHostToDevice, stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(dH, j.H.data(), sizeof(float) * j.H.size(), hipMemcpyHostToDevice, stream);
             int slot = -1;
             if (e == hipSuccess)
-                e = gpumlib_nmf_batch_add(batch, j.rule, j.n, j.m, j.r, dV, dW, dH, j.eps, base + j.offWs, j.ws_bytes,
-                                          &slot);
+                e = gpumlib_nmf_batch_add(batch, j.rule, j.n, j.m, j.r, (float*)(base + j.offV),
+                                          (float*)(base + j.offW), (float*)(base + j.offH), j.eps, base + j.offWs,
+                                          j.ws_bytes, &slot);
             j.slot = slot;
             j.t_admit = now();
             if (e == hipSuccess) {
                 active[slot] = jp;
+                joined.push_back(jp);
             } else {
-                // not admitted after all: return its memory and report the error to its waiter
                 std::lock_guard<std::mutex> lk(mu);
                 pool.release(j.off, j.bytes);
                 j.status = e;
                 j.done = true;
             }
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);  // host input buffers may be released after this
-        for (auto& jp : admitted) {
-            std::vector<float>().swap(jp->V);
-        }
+        staged.clear();
         if (e == hipSuccess && cfg.autotune > 0) {
             const int resident = gpumlib_nmf_batch_active(batch);
             if (resident >= 4 && (tuned_at == 0 || resident >= 2 * tuned_at || 2 * resident <= tuned_at)) {
@@ -342,6 +413,7 @@ void gpumlib_nmf_server::run() {
                     stats.tuned_pass_grid = t.pass_grid;
                     stats.tuned_update_grid = t.update_grid;
                     stats.tuned_engine = t.engine;
+                    stats.tuned_split_ranks = t.split_ranks;
                 }
             }
         }
@@ -351,6 +423,21 @@ void gpumlib_nmf_server::run() {
                 e = gpumlib_nmf_batch_graph_launch(batch, rounds_iters, rounds_iters >= 2 ? 2 : 1, stream);
             else
                 e = gpumlib_nmf_batch_iterate(batch, rounds_iters, rounds_iters >= 2 ? 2 : 1, stream);
+        }
+        // while it runs: the joined problems' host inputs are released once uploaded, and
+        // queued problems are admitted and uploaded for the next round
+        if (!joined.empty()) {
+            const hipError_t ce = hipEventSynchronize(ev_staged);
+            if (e == hipSuccess) e = ce;
+            for (auto& jp : joined) std::vector<float>().swap(jp->V);
+        }
+        if (e == hipSuccess) {
+            std::vector<std::shared_ptr<Job>> next;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                admit_locked(next, 0);
+            }
+            upload(next);
         }
         if (e == hipSuccess) e = gpumlib_nmf_batch_objectives(batch, h_obj, stream);
         std::vector<std::shared_ptr<Job>> finished;
@@ -366,42 +453,44 @@ void gpumlib_nmf_server::run() {
             if (rounds_iters >= 2 && rel < j.tol) j.converged = 1;
             if (e != hipSuccess || j.converged || j.iters >= j.max_iters) finished.push_back(jp);
         }
+        // the finished problems leave the batch; their results come back on the copy stream
+        // (the round is complete: the objectives were read after it)
         for (auto& jp : finished) {
             Job& j = *jp;
             char* base = d_pool + j.off;
-            hipError_t e2 = e;
-            if (e2 == hipSuccess)
-                e2 = hipMemcpyAsync(j.W.data(), base + j.offW, sizeof(float) * j.W.size(), hipMemcpyDeviceToHost, stream);
-            if (e2 == hipSuccess)
-                e2 = hipMemcpyAsync(j.H.data(), base + j.offH, sizeof(float) * j.H.size(), hipMemcpyDeviceToHost, stream);
-            if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
-            j.status = e2;
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(j.W.data(), base + j.offW, sizeof(float) * j.W.size(), hipMemcpyDeviceToHost,
+                                   copy_stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(j.H.data(), base + j.offH, sizeof(float) * j.H.size(), hipMemcpyDeviceToHost,
+                                   copy_stream);
+            j.status = e;
+            (void)gpumlib_nmf_batch_remove(batch, j.slot);
+            active[j.slot].reset();
+            retiring.push_back(jp);
         }
+        if (e == hipSuccess && !finished.empty()) e = hipEventRecord(ev_retired, copy_stream);
         {
             std::lock_guard<std::mutex> lk(mu);
             stats.rounds++;
             stats.iterations_batched += (long long)in_round * rounds_iters;
             stats.peak_active = std::max(stats.peak_active, in_round);
-            for (auto& jp : finished) {
-                Job& j = *jp;
-                (void)gpumlib_nmf_batch_remove(batch, j.slot);
-                active[j.slot].reset();
-                pool.release(j.off, j.bytes);
-                j.t_finish = now();
-                j.done = true;
+        }
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(copy_stream);
+            (void)hipStreamSynchronize(stream);
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& jp : retiring) {
+                pool.release(jp->off, jp->bytes);
+                jp->status = e;
+                jp->t_finish = now();
+                jp->done = true;
                 stats.completed++;
             }
-            stats.pool_high_water = pool.high_water();
-            if (e != hipSuccess) fatal = e;
-        }
-        cv_done.notify_all();
-        if (e != hipSuccess) {
-            std::lock_guard<std::mutex> lk(mu);
-            for (auto& jp : pending) {
-                jp->status = e;
-                jp->done = true;
-            }
-            pending.clear();
+            retiring.clear();
+            for (auto& jp : staged) pool.release(jp->off, jp->bytes);
+            fail_all_locked(e, staged);
+            staged.clear();
             cv_done.notify_all();
             return;
         }
@@ -418,6 +507,7 @@ LiveServers::~LiveServers() {
         s->cv_work.notify_all();
         if (s->engine.joinable()) s->engine.join();
         (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamSynchronize(s->copy_stream);
     }
     set.clear();
 }
@@ -442,12 +532,18 @@ hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpuml
     if (e == hipSuccess && cfg->target_workgroups > 0) e = gpumlib_nmf_batch_set_target(s->batch, cfg->target_workgroups);
     if (e == hipSuccess && cfg->split_k) e = gpumlib_nmf_batch_set_engine(s->batch, GPUMLIB_NMF_BATCH_SPLIT_K);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_staged, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_retired, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_obj, sizeof(double) * 2 * cfg->max_active, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_obj, 2 * sizeof(double));
     if (e != hipSuccess) {
         if (s->d_obj) (void)hipFree(s->d_obj);
         if (s->h_obj) (void)hipHostFree(s->h_obj);
         if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+        if (s->ev_staged) (void)hipEventDestroy(s->ev_staged);
+        if (s->ev_retired) (void)hipEventDestroy(s->ev_retired);
         if (s->batch) gpumlib_nmf_batch_destroy(s->batch);
         if (s->d_pool) (void)hipFree(s->d_pool);
         delete s;
@@ -547,9 +643,13 @@ void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s) {
     s->cv_work.notify_all();
     if (s->engine.joinable()) s->engine.join();  // finishes every admitted and queued problem first
     (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->copy_stream);
     (void)hipHostFree(s->h_obj);
     (void)hipFree(s->d_obj);
     (void)hipStreamDestroy(s->stream);
+    (void)hipStreamDestroy(s->copy_stream);
+    (void)hipEventDestroy(s->ev_staged);
+    (void)hipEventDestroy(s->ev_retired);
     gpumlib_nmf_batch_destroy(s->batch);
     (void)hipFree(s->d_pool);
     delete s;

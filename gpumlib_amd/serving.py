@@ -131,7 +131,9 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
             "autotune_trial_iterations": autotune, "autotunes": st.autotunes,
             "tuned": {"engine": "fused" if st.tuned_engine == capi.BATCH_FUSED else "split_k",
                       "target_workgroups": st.tuned_target, "pass_grid": st.tuned_pass_grid,
-                      "update_grid": st.tuned_update_grid} if st.autotunes else None}
+                      "update_grid": st.tuned_update_grid,
+                      "rank_blocking": "r<=32 on 1-block kernels" if st.tuned_split_ranks else "2-block kernels"}
+            if st.autotunes else None}
 
 
 def queued_latency(service_s, rate, seed=12345):

@@ -47,7 +47,7 @@ EDGE_JOBS = [  # odd shapes for the fused engine's tiles: partial tiles, unalign
 
 
 class Batch:
-    def __init__(self, hip, lib, max_jobs=16, engine=None):
+    def __init__(self, hip, lib, max_jobs=16, engine=None, rank_split=None):
         self.hip, self.lib = hip, lib
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_batch_create(max_jobs, ctypes.byref(h)))
@@ -55,6 +55,9 @@ class Batch:
         if engine is not None:
             capi.check(lib.gpumlib_nmf_batch_set_engine(h, engine))
             assert lib.gpumlib_nmf_batch_engine(h) == engine
+        if rank_split is not None:
+            capi.check(lib.gpumlib_nmf_batch_set_rank_split(h, rank_split))
+            assert lib.gpumlib_nmf_batch_rank_split(h) == rank_split
         self.keep = []
         self.stream = hip.Stream()
 
@@ -85,8 +88,9 @@ class Batch:
 
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("jobs", ["mix", "edge"])
-def test_batch_matches_oracle(native, orc, cuda, engine, jobs):
-    b = Batch(cuda, native, engine=engine)
+@pytest.mark.parametrize("rank_split", [0, 1])
+def test_batch_matches_oracle(native, orc, cuda, engine, jobs, rank_split):
+    b = Batch(cuda, native, engine=engine, rank_split=rank_split)
     outs = []
     for k, (rule, n, m, r) in enumerate(JOBS if jobs == "mix" else EDGE_JOBS):
         V, W, H = make_job(k, rule, n, m, r)
@@ -121,6 +125,27 @@ def test_alone_equals_mix(native, cuda, engine, k):
     assert np.array_equal(h1.download(np.float32), h2.download(np.float32))
     b1.close()
     b2.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_rank_split_is_bit_identical(native, cuda, engine):
+    """Problems with r <= 32 on the one-rank-block kernels give the bits of the two-block
+    kernels (the same products in the same order), eager and graph-replayed."""
+    outs = []
+    for split in (0, 1):
+        b = Batch(cuda, native, engine=engine, rank_split=split)
+        bufs = []
+        for k, job in enumerate(JOBS + EDGE_JOBS):
+            _, dW, dH = b.add(*job, *make_job(k, *job))
+            bufs.append((dW, dH))
+        b.iterate(6, objectives=2)
+        b.iterate(6, objectives=1, graph=True)
+        outs.append(([(w.download(np.float32), h.download(np.float32)) for w, h in bufs],
+                     b.objectives()[:2 * len(bufs)]))
+        b.close()
+    for (w0, h0), (w1, h1) in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(w0, w1) and np.array_equal(h0, h1)
+    assert np.array_equal(outs[0][1], outs[1][1])
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -192,6 +217,7 @@ def test_autotune_keeps_state_and_parity(native, orc, cuda):
           f"trials={t.trials}", flush=True)
     assert t.cached == 0 and t.trials >= 3
     assert t.engine in (capi.BATCH_FUSED, capi.BATCH_SPLIT_K) and native.gpumlib_nmf_batch_engine(b.h) == t.engine
+    assert t.split_ranks in (0, 1) and native.gpumlib_nmf_batch_rank_split(b.h) == t.split_ranks
     assert t.target_workgroups in TUNE_TARGETS | {32} and t.pass_grid in TUNE_PASS and t.update_grid in TUNE_UPD
     assert 0 < t.ms_per_iteration <= t.ms_per_iteration_before
     for (w0, h0), (*_, dW, dH) in zip(before, outs):
@@ -199,8 +225,8 @@ def test_autotune_keeps_state_and_parity(native, orc, cuda):
     t2 = capi.BatchTuning()
     capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t2)))
     assert t2.cached == 1 and t2.trials == 0
-    assert (t2.engine, t2.target_workgroups, t2.pass_grid, t2.update_grid) == \
-        (t.engine, t.target_workgroups, t.pass_grid, t.update_grid)
+    assert (t2.engine, t2.target_workgroups, t2.pass_grid, t2.update_grid, t2.split_ranks) == \
+        (t.engine, t.target_workgroups, t.pass_grid, t.update_grid, t.split_ranks)
     b.iterate(10)
     b.iterate(12, graph=True)  # graphs captured after tuning use the tuned grids
     for rule, n, m, r, V, W, H, dW, dH in outs:
