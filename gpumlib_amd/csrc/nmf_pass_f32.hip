@@ -554,6 +554,9 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     const int kbeg = s * p.kslice;
     const int kend = min(p.m, kbeg + p.kslice);
     const bool vec = p.vec != 0;
+    // H chunk (u in the chunk, all rank rows a): each thread loads 4 consecutive rank rows of
+    // one column u (16 bytes) and stores them down its column of sH; consecutive threads take
+    // consecutive u, so each of the 4 scalar stores of a wave lands on consecutive banks
     constexpr int BQ = RP / 4;
     constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
 
@@ -577,8 +580,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         }
 #pragma unroll
         for (int j = 0; j < BLOADS; j++) {
-            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
-            rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
+            const int q = tid + NTHREADS * j, uu = q % KC, aq = (q / KC) * 4;
+            rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (aq < RP) && (k0 + uu < kend), aq, p.r, vec);
         }
     };
     auto sstore = [&](int buf) __attribute__((always_inline)) {
@@ -589,8 +592,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         }
 #pragma unroll
         for (int j = 0; j < BLOADS; j++) {
-            const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
-            if (uu < KC) {
+            const int q = tid + NTHREADS * j, uu = q % KC, aq = (q / KC) * 4;
+            if (aq < RP) {
                 sH[buf][aq + 0][uu] = rb[j].x;
                 sH[buf][aq + 1][uu] = rb[j].y;
                 sH[buf][aq + 2][uu] = rb[j].z;
