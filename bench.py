@@ -17,7 +17,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "hbm_gbs": 8000.0}
+# fp32x: FP32 products as six BF16 piece products each, so its matrix-core ceiling in FP32 flops
+# is the BF16 peak / 6
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0,
+        "fp32x_mfma_tflops": 2500.0 / 6, "hbm_gbs": 8000.0}
 
 WORKLOADS = {
     # name: (rule, n, m, r, r0, seed)
@@ -59,9 +62,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="euclid-16384x8192-r64", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp8"],
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp8", "fp32x"],
                     help="precision of the headline value (the FP32 path is always measured beside it)")
-    ap.add_argument("--extra-legs", default="fp8",
+    ap.add_argument("--extra-legs", default="fp8,fp32x",
                     help="further precisions measured and reported beside the headline (comma list, '' = none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -136,11 +139,11 @@ def main():
         flops_iter = 8.0 * n * m * r
 
     def leg(prec_name):
-        prec = {"fp32": G.FP32, "bf16": G.BF16, "fp8": G.FP8}[prec_name]
+        prec = {"fp32": G.FP32, "bf16": G.BF16, "fp8": G.FP8, "fp32x": G.FP32X}[prec_name]
         t_setup = time.perf_counter()
         d = G.DeviceNMF(rule, V, W0, H0, n, m, r, eps=eps, precision=prec)
         t_setup = time.perf_counter() - t_setup
-        vbytes_el = {G.FP32: 4, G.BF16: 2, G.FP8: 1}[prec]
+        vbytes_el = {G.FP32: 4, G.BF16: 2, G.FP8: 1, G.FP32X: 4}[prec]
         obj0 = d.objective(obj_kind)
         d.launch(args.warmup)
         d.stream.synchronize()

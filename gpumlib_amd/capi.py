@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libgpumlib_nmf.so")
 
 EUCLIDEAN, KL = 0, 1
 ADDITIVE_EUCLIDEAN, ADDITIVE_KL, ADDITIVE_EUCLIDEAN_WITH_H = 2, 3, 4
-FP32, BF16, FP8 = 0, 1, 2
+FP32, BF16, FP8, FP32X = 0, 1, 2, 3
 BATCH_SPLIT_K, BATCH_FUSED = 0, 1
 OBJ_NONE, OBJ_FROBENIUS, OBJ_KL, OBJ_QUALITY = 0, 1, 2, 3
 REFERENCE_EPS = 1e-8  # float(10e-9f) == float(1e-8f)

@@ -36,6 +36,7 @@ static int choose_slices(int tiles, int K, int kc, int target, int* kslice) {
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 size_t vstore_bf16_offset_t(int n, int m) { return align256(bf16_tiled_elems(n, m) * 2); }
+size_t vstore_x3_offset_t(int n, int m) { return align256(x3_tiled_elems(n, m) * 4); }
 
 Fp8Store fp8_store_layout(int n, int m) {
     Fp8Store l;
@@ -50,6 +51,7 @@ size_t vstore_bytes(int precision, int n, int m) {
     if (precision == GPUMLIB_NMF_FP32) return (size_t)n * m * 4;
     if (precision == GPUMLIB_NMF_BF16) return vstore_bf16_offset_t(n, m) + align256(bf16_tiled_elems(m, n) * 2);
     if (precision == GPUMLIB_NMF_FP8) return fp8_store_layout(n, m).bytes;
+    if (precision == GPUMLIB_NMF_FP32X) return vstore_x3_offset_t(n, m) + align256(x3_tiled_elems(m, n) * 4);
     return 0;
 }
 
@@ -84,7 +86,9 @@ static bool make_unfused_plan(int rule, int n, int m, int r, Plan* pl) {
 
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
     if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
-    if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8) return false;
+    if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8 &&
+        precision != GPUMLIB_NMF_FP32X)
+        return false;
     if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
     if (n < 1 || m < 1 || r < 1) return false;
     if (r > kMaxFusedRank) {
@@ -106,29 +110,37 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     pl->ldt8 = bf16_ld(m);
     pl->ld16n = fp8_ld(n);
     pl->ld16m = fp8_ld(m);
+    pl->ldx_n = x3_ld(n);
+    pl->ldx_m = x3_ld(m);
     pl->nbi = pl->ld16n / 32;
     pl->nbu = pl->ld16m / 32;
     const int euc = !rule_is_kl(rule);
-    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
+    const int x3 = precision == GPUMLIB_NMF_FP32X;
+    const int kc = (precision == GPUMLIB_NMF_FP32) ? KC
+                   : x3                            ? 128
+                                                   : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
     // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
-    // first tiles of every slice, one per tile (tiny problems get gram-only tiles);
-    // the BF16 / FP8 passes put block t on column tile t when there are enough column
-    // tiles (each block's k steps split over the waves), else every block on tile 0.
-    const int nbk = (r + 31) / 32, gram_tiles = (precision == GPUMLIB_NMF_FP32 && euc) ? nbk * nbk : 0;
+    // first tiles of every slice, one per tile (tiny problems get gram-only tiles), and so
+    // do the FP32X passes; the BF16 / FP8 passes put block t on column tile t when there
+    // are enough column tiles (each block's k steps split over the waves), else every block
+    // on tile 0.
+    const int nbk = (r + 31) / 32;
+    const int gram_tiles = ((precision == GPUMLIB_NMF_FP32 && euc) || x3) ? nbk * nbk : 0;
     // The BF16 passes take two 64-column blocks per workgroup of 8 waves for r <= 64: each A
     // chunk (the W or H^T shadow, re-read from L2 by every workgroup) then serves twice the
     // columns of V, halving the A loads; one such workgroup fills a CU (101 KB of LDS, the
-    // same 8 waves as two 4-wave workgroups), so the pass aims at half the workgroups.
-    pl->lp_nt = (precision == GPUMLIB_NMF_BF16 && r <= 64) ? 2 : 1;
+    // same 8 waves as two 4-wave workgroups), so the pass aims at half the workgroups.  The
+    // FP32X passes do the same (one 8-wave workgroup per CU for every rank, up to 153 KB of LDS).
+    pl->lp_nt = ((precision == GPUMLIB_NMF_BF16 || x3) && r <= 64) ? 2 : 1;
     const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP * pl->lp_nt;  // columns (rows) per workgroup
-    const int tgt = target / pl->lp_nt;
+    const int tgt = x3 ? target / 2 : target / pl->lp_nt;
     const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
     pl->tiles1 = vt1 > gram_tiles ? vt1 : gram_tiles;
     pl->S1 = choose_slices(pl->tiles1, n, kc, tgt, &pl->ks1);
     pl->pstride1 = (long)(m + r) * pl->ldr;
     pl->tiles3 = vt3 > gram_tiles ? vt3 : gram_tiles;
     pl->S3 = choose_slices(pl->tiles3, m, kc, tgt, &pl->ks3);
-    // FP32 W pass writes P[s][a*ldp + i]; the BF16 pass over V^T writes P[s][i*ldr + a].
+    // FP32 W pass writes P[s][a*ldp + i]; the BF16 / FP8 / FP32X passes over V^T write P[s][i*ldr + a].
     pl->pstride3 = (precision == GPUMLIB_NMF_FP32) ? (long)r * pl->ldp : (long)(n + r) * pl->ldr;
     const size_t p1 = (size_t)pl->S1 * pl->pstride1, p3 = (size_t)pl->S3 * pl->pstride3;
     const size_t pbytes = align256(sizeof(float) * (p1 > p3 ? p1 : p3));
@@ -171,6 +183,11 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         off += align256((size_t)r * pl->ldv8 * 2);
         pl->off_Hb = off;
         off += align256((size_t)r * pl->ldt8 * 2);
+    } else if (x3) {  // three BF16 piece planes of W and of H^T
+        pl->off_Wb = off;
+        off += align256((size_t)3 * r * pl->ldx_n * 2);
+        pl->off_Hb = off;
+        off += align256((size_t)3 * r * pl->ldx_m * 2);
     } else if (precision == GPUMLIB_NMF_FP8) {
         pl->off_Wb = off;
         off += align256((size_t)r * pl->ld16n);
@@ -296,7 +313,10 @@ hipError_t run_objective(const Plan& pl, const void* V, const float* W, const fl
         a.vsc = (const uint8_t*)V + fp8_store_layout(pl.n, pl.m).off_sc;
         a.nbi = pl.nbi;
     }
-    const int vtype = pl.precision == GPUMLIB_NMF_FP32 ? 0 : (pl.precision == GPUMLIB_NMF_BF16 ? 1 : 2);
+    const int vtype = pl.precision == GPUMLIB_NMF_FP32    ? 0
+                      : pl.precision == GPUMLIB_NMF_BF16  ? 1
+                      : pl.precision == GPUMLIB_NMF_FP32X ? 3
+                                                          : 2;
     CK(launch_split_wh(W, H, pl.n, pl.m, pl.r, pl.ob_rp, pl.ob_nrows, pl.ob_ncols, (uint16_t*)(base + pl.off_Ws),
                        (uint16_t*)(base + pl.off_Hs), st));
     CK(launch_objective_split(a, vtype, objective, st));
@@ -427,6 +447,63 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
         if (!update_w) continue;
         CK(launch_pass_tn_bf16(aw, st));
         CK(launch_update_w_euc_t(W, P, G3, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st, 0, gs3));
+    }
+    return hipSuccess;
+}
+
+// The FP32X passes over the plan's V store: H pass (which 0: A = the W pieces, B = V) or W
+// pass (which 1: A = the H^T pieces, B = V^T).
+static void x3_args(const Plan& pl, const void* Vstore, void* ws, int which, LpPassArgs* a) {
+    char* base = (char*)ws;
+    memset(a, 0, sizeof(*a));
+    a->P = (float*)(base + pl.off_P);
+    a->r = pl.r;
+    a->ldr = pl.ldr;
+    a->nt = pl.lp_nt;
+    a->G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)pl.r * pl.rpad);
+    a->gcount = (int*)(base + pl.off_cnt) + which;
+    if (which == 0) {
+        a->A = (const uint16_t*)(base + pl.off_Wb);
+        a->B = (const uint16_t*)Vstore;
+        a->K = pl.n, a->N = pl.m, a->lda = pl.ldx_n;
+        a->kslice = pl.ks1, a->S = pl.S1, a->tiles = pl.tiles1, a->pstride = pl.pstride1;
+    } else {
+        a->A = (const uint16_t*)(base + pl.off_Hb);
+        a->B = (const uint16_t*)((const char*)Vstore + vstore_x3_offset_t(pl.n, pl.m));
+        a->K = pl.m, a->N = pl.n, a->lda = pl.ldx_m;
+        a->kslice = pl.ks3, a->S = pl.S3, a->tiles = pl.tiles3, a->pstride = pl.pstride3;
+    }
+    a->aplane = (long)pl.r * a->lda;
+}
+
+// FP32X storage: V store = [V | V^T] in FP32, each tiled for its pass (x3_tiled_index).  W and H
+// stay FP32; the passes take them as three BF16 piece planes, which the updates rewrite.
+hipError_t run_iterations_x3(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                             int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
+    char* base = (char*)ws;
+    uint16_t* Wp = (uint16_t*)(base + pl.off_Wb);
+    uint16_t* Hp = (uint16_t*)(base + pl.off_Hb);
+    float* P = (float*)(base + pl.off_P);
+    CK(launch_shadow_x3(W, H, pl.n, pl.m, pl.r, Wp, pl.ldx_n, Hp, pl.ldx_m, st));
+    CK(launch_zero_i32((int*)(base + pl.off_cnt), 2, st));
+    LpPassArgs ah, aw;
+    x3_args(pl, Vstore, ws, 0, &ah);
+    x3_args(pl, Vstore, ws, 1, &aw);
+    const int gs1 = update_gram_slices(pl.S1, pl.r), gs3 = update_gram_slices(pl.S3, pl.r);
+    ah.gkeep = gs1 > 0;
+    aw.gkeep = gs3 > 0;
+    const float* G1 = gs1 ? P + (size_t)pl.m * pl.ldr : ah.G;  // slice 0's gram columns, or the reduced gram
+    const float* G3 = gs3 ? P + (size_t)pl.n * pl.ldr : aw.G;
+    for (int it = 0; it < iterations; it++) {
+        if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
+            CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
+        CK(launch_pass_tn_x3(ah, st));
+        CK(launch_update_h_euc(H, P, G1, pl.m, pl.r, pl.ldr, pl.S1, pl.pstride1, eps, Hp, pl.ldx_m, st, 0, gs1,
+                               aw.aplane));
+        if (!update_w) continue;
+        CK(launch_pass_tn_x3(aw, st));
+        CK(launch_update_w_euc_t(W, P, G3, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wp, pl.ldx_n, st, 0, gs3,
+                                 ah.aplane));
     }
     return hipSuccess;
 }
@@ -645,6 +722,8 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
         return run_iterations_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     if (precision == GPUMLIB_NMF_FP8)
         return run_iterations_fp8(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
+    if (precision == GPUMLIB_NMF_FP32X)
+        return run_iterations_x3(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     return run_iterations(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective, workspace,
                           stream);
 }
@@ -674,6 +753,9 @@ hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float* V, 
         uint8_t* base = (uint8_t*)vstore;
         return launch_pack_v_fp8(V, n, m, base, fp8_ld(n), base + L.off_t, fp8_ld(m), base + L.off_sc, stream);
     }
+    if (precision == GPUMLIB_NMF_FP32X)
+        return launch_pack_v_x3(V, n, m, (float*)vstore, x3_ld(n), (float*)((char*)vstore + vstore_x3_offset_t(n, m)),
+                                x3_ld(m), stream);
     uint16_t* Vb = (uint16_t*)vstore;
     uint16_t* Vt = (uint16_t*)((char*)vstore + vstore_bf16_offset_t(n, m));
     return launch_pack_v_bf16(V, n, m, Vb, bf16_ld(n), Vt, bf16_ld(m), stream);
@@ -696,6 +778,13 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         fp8_args(pl, V, workspace, which, &a);
         a.gkeep = gslices > 0;
         return launch_pass_tn_fp8(a, stream);
+    }
+    if (precision == GPUMLIB_NMF_FP32X) {
+        // one FP32X pass over the W/H pieces the last iterate call left in this workspace
+        LpPassArgs a;
+        x3_args(pl, V, workspace, which, &a);
+        a.gkeep = gslices > 0;
+        return launch_pass_tn_x3(a, stream);
     }
     if (precision == GPUMLIB_NMF_BF16) {
         // one BF16 pass over the W/H shadows the last iterate call left in this workspace

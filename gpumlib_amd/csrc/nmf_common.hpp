@@ -45,6 +45,12 @@ __host__ __device__ inline size_t fp8_tiled_index(int k, int col, int ntiles) {
     return (((size_t)(k >> 9) * ntiles + (col >> 6)) * 64 + (col & 63)) * 512 + (k & 511);
 }
 
+// The same for the FP32 V store of the split-BF16 path: blocks of 64 columns x 128
+// contraction elements (32 KB).
+__host__ __device__ inline size_t x3_tiled_index(int k, int col, int ntiles) {
+    return (((size_t)(k >> 7) * ntiles + (col >> 6)) * 64 + (col & 63)) * 128 + (k & 127);
+}
+
 // Objective selector of the objective-carrying passes and kernels.
 enum ObjKind { OBJ_NONE = 0, OBJ_FROB = 1, OBJ_KL = 2, OBJ_QUALITY = 3 };
 
@@ -90,6 +96,24 @@ __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
         return (double)fmaf(v, log10f(wh + eps), -wh);
     }
     return 0.0;
+}
+
+// Three-way BF16 split of the pair (a, b), round to nearest even at every step:
+// x = x1 + x2 + x3 + e with x1 = bf16(x), x2 = bf16(x - x1), x3 = bf16(x - x1 - x2); both
+// differences are exact in FP32, |x2| <= 2^-9 |x|, |x3| <= 2^-18 |x|, |e| <= 2^-27 |x|
+// (normal range).  Piece p of (a, b) comes packed in wp, a in the low half.
+__device__ __forceinline__ void bf16x2_split3(float a, float b, uint32_t& w1, uint32_t& w2, uint32_t& w3) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    auto pack = [](f32x2_t x) { return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2_t)); };
+    auto lo = [](uint32_t w) { return __uint_as_float(w << 16); };
+    auto hi = [](uint32_t w) { return __uint_as_float(w & 0xffff0000u); };
+    const f32x2_t x = {a, b};
+    w1 = pack(x);
+    const f32x2_t r1 = {a - lo(w1), b - hi(w1)};
+    w2 = pack(r1);
+    const f32x2_t r2 = {r1[0] - lo(w2), r1[1] - hi(w2)};
+    w3 = pack(r2);
 }
 
 // Row of the 32x32 MFMA result held in accumulator register v by half-wave h.
@@ -300,6 +324,8 @@ struct LpPassArgs {
     float* G;
     int* gcount;
     int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
+    long aplane; // FP32X pass (B: the tiled FP32 store, x3_tiled_index): element stride between
+                 // the three BF16 piece planes of A
 };
 
 // One problem of a grouped (batched) launch: the arguments of its two passes and

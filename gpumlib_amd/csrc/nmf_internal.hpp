@@ -19,6 +19,12 @@ hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ld
                               hipStream_t st);
 hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
                               int ldh, hipStream_t st);
+// FP32X storage path (nmf_pass_x3.hip): FP32 V and V^T tiled by x3_tiled_index (contraction
+// padded to ldv / ldt, multiples of 128), the A operand as three BF16 piece planes.
+hipError_t launch_pass_tn_x3(const LpPassArgs& a, hipStream_t st);
+hipError_t launch_pack_v_x3(const float* V, int n, int m, float* Vx, int ldv, float* Vtx, int ldt, hipStream_t st);
+hipError_t launch_shadow_x3(const float* W, const float* H, int n, int m, int r, uint16_t* Wp, int ldw, uint16_t* Hp,
+                            int ldh, hipStream_t st);
 // FP8 storage path (nmf_pass_fp8.hip).
 hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st);
 hipError_t launch_pack_v_fp8(const float* V, int n, int m, uint8_t* Vq, int ldv, uint8_t* Vtq, int ldt, uint8_t* Vsc,
@@ -27,21 +33,24 @@ hipError_t launch_shadow_fp8(const float* X, int n, int m, int r, int which, uin
                              hipStream_t st);
 // Objective pass on split factors (nmf_objective.hip): W, H -> their three BF16 pieces
 // (rp = split_rank(r), nrows a multiple of 128 >= n, ncols a multiple of 32 >= m),
-// then the per-work-item FP64 partials of the objective over V (vtype 0 FP32, 1 BF16, 2 FP8).
+// then the per-work-item FP64 partials of the objective over V (vtype 0 FP32, 1 BF16, 2 FP8,
+// 3 the tiled FP32 store of the FP32X path).
 hipError_t launch_split_wh(const float* W, const float* H, int n, int m, int r, int rp, int nrows, int ncols,
                            uint16_t* Ws, uint16_t* Hs, hipStream_t st);
 hipError_t launch_objective_split(const SplitObjArgs& a, int vtype, int obj, hipStream_t st);
 
 // Update kernels (nmf_update.hip).
 // Euclidean H:  H(a,u) *= WtV(a,u) / (sum_b WtW(a,b) H(b,u) + eps)
-// Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs).
+// Hs (nullable): BF16 copy of the new H written as H^T column-major (ld ldhs); with shplane > 0
+// (FP32X), its three BF16 pieces in planes shplane elements apart (ldhs a multiple of 128).
 // G: the reduced gram W^T W written by the H pass, WtW(a,b) = G[b*ldr + a];  with
 // gsl in [1, kMaxGramSlices] (r <= 64), the first of the gsl slice grams of the pass (stride pstride),
 // which the update sums in slice order.
 // add != 0 (every Euclidean / KL update below): the additive-update rule of
 // NMF_AdditiveEuclidian / NMF_AdditiveDivergence in place of the multiplicative one.
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add = 0, int gsl = 0);
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add = 0, int gsl = 0,
+                               long shplane = 0);
 // Euclidean W:  W(i,a) *= VHt(i,a) / (sum_b W(i,b) HHt(b,a) + eps)
 // G: the reduced gram H H^T written by the W pass, HHt(a,b) = G[b*rpad + a].
 hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, int r, int ldp, int S, long pstride,
@@ -49,7 +58,8 @@ hipError_t launch_update_w_euc(float* W, const float* P, const float* G, int n, 
 // Euclidean W from partials laid out P[s][i*ldr + a] (gram rows at i >= n), as the
 // BF16 pass over V^T writes them; Wb (nullable): BF16 copy of the new W (ld ldwb).
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add = 0, int gsl = 0);
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add = 0, int gsl = 0,
+                                 long shplane = 0);
 // FP8 storage path: the same updates, also writing the E4M3 shadow of the new factor
 // (one E8M0 scale per 32 rows along the contraction).
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,

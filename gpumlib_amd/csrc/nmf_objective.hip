@@ -1,7 +1,7 @@
 // Objective pass: sum over V's elements of a per-element term of (V(i,u), (W H)(i,u)),
 // accumulated in FP64 (Frobenius (v - wh)^2, KL v ln(v / wh) - v + wh, and the
 // reference quality v log10(wh + eps) - wh of nmf_kernels_quality.cu:35), for V
-// stored in FP32, BF16 or FP8.
+// stored in FP32 (plain, or tiled as the split-BF16 passes read it), BF16 or FP8.
 //
 // W H is formed on BF16 MFMAs from a three-way BF16 split of the FP32 factors:
 // x = x1 + x2 + x3 with x1 = bf16(x), x2 = bf16(x - x1), x3 = bf16(x - x1 - x2)
@@ -152,6 +152,10 @@ struct VBlock<1> {  // BF16 (tiled store)
     uint2 x[4];
 };
 template <>
+struct VBlock<4> {  // FP32 (tiled store of the split-BF16 path)
+    float4 x[4];
+};
+template <>
 struct VBlock<2> {  // FP8 E4M3 (tiled store) and the E8M0 scale of its 32 x 32 tile
     uint32_t x[4];
     float sc;
@@ -180,6 +184,12 @@ __device__ __forceinline__ void load_vblock(const SplitObjArgs& p, int i4, int u
         const float* V = reinterpret_cast<const float*>(p.V);
 #pragma unroll
         for (int g = 0; g < 4; g++) vb.x[g] = load4(V + (size_t)u * p.ldv + i4 + 8 * g, u < p.m, i4 + 8 * g, p.n, false);
+    } else if constexpr (TV == 4) {
+        // zero padded to whole 128-row chunks and 64-column tiles: no bounds; four rows of a
+        // group are adjacent in a chunk column
+        const float* V = reinterpret_cast<const float*>(p.V);
+#pragma unroll
+        for (int g = 0; g < 4; g++) vb.x[g] = *reinterpret_cast<const float4*>(V + x3_tiled_index(i4 + 8 * g, u, p.vtiles));
     } else if constexpr (TV == 1) {
         // the store is zero padded to whole 256-row chunks and 64-column tiles: no bounds
         const uint16_t* V = reinterpret_cast<const uint16_t*>(p.V);
@@ -196,7 +206,7 @@ __device__ __forceinline__ void load_vblock(const SplitObjArgs& p, int i4, int u
 
 template <int TV>
 __device__ __forceinline__ float vblock_elem(const VBlock<TV>& vb, int g, int j) {
-    if constexpr (TV == 0 || TV == 3) {
+    if constexpr (TV == 0 || TV == 3 || TV == 4) {
         const float4 q = vb.x[g];
         return j == 0 ? q.x : (j == 1 ? q.y : (j == 2 ? q.z : q.w));
     } else if constexpr (TV == 1) {
@@ -219,7 +229,7 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // NK: K steps of 16 of the pieces (rp = 16 NK); TV: V element type (0 FP32 with n % 4 == 0
-// and 16-byte alignment, 3 any FP32, 1 BF16 store, 2 FP8 store); OBJ: the term.
+// and 16-byte alignment, 3 any FP32, 1 BF16 store, 2 FP8 store, 4 tiled FP32 store); OBJ: the term.
 template <int NK, int TV, int OBJ>
 __global__ __launch_bounds__(NTHREADS) void k_objective_split(SplitObjArgs p) {
     constexpr int RP = NK * 16;                    // the pieces' padded rank (p.rp)
@@ -454,6 +464,7 @@ hipError_t launch_objective_split(const SplitObjArgs& a, int vtype, int obj, hip
         case 0: return a.vec ? launch_tv<0>(a, obj, st) : launch_tv<3>(a, obj, st);
         case 1: return launch_tv<1>(a, obj, st);
         case 2: return launch_tv<2>(a, obj, st);
+        case 3: return launch_tv<4>(a, obj, st);
         default: return hipErrorInvalidValue;
     }
 }

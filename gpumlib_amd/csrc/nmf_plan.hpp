@@ -32,7 +32,8 @@ struct Plan {
     int npad, rpad, ldr, ldp;
     int ldv8, ldt8;       // BF16 storage: padded contraction lengths (bf16_ld of n and of m): the K extent
                           // of the tiled V and V^T and the leading dimensions of the W / H^T shadows
-    int lp_nt;            // BF16 passes: 64-column blocks per workgroup (1 or 2)
+    int ldx_n, ldx_m;     // FP32X storage: padded contraction lengths (x3_ld of n and of m), as ldv8 / ldt8
+    int lp_nt;            // BF16 / FP32X passes: 64-column blocks per workgroup (1 or 2)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;
@@ -52,6 +53,7 @@ struct Plan {
 // padding zero-filled, so the streaming passes load every chunk without bounds checks.
 inline int bf16_ld(int k) { return (k + 255) / 256 * 256; }
 inline int fp8_ld(int k) { return (k + 511) / 512 * 512; }
+inline int x3_ld(int k) { return (k + 127) / 128 * 128; }
 
 // Byte layout of the FP8 V store: Vq (contraction i, m columns) | Vtq (contraction u, n columns),
 // each tiled in blocks of 64 columns x 512 contraction bytes (fp8_tiled_index; contraction padded
@@ -72,6 +74,10 @@ Fp8Store fp8_store_layout(int n, int m);
 inline int lp_tiles(int cols) { return (cols + 63) / 64; }
 inline size_t bf16_tiled_elems(int K, int cols) { return (size_t)bf16_ld(K) * 64 * lp_tiles(cols); }
 size_t vstore_bf16_offset_t(int n, int m);
+// FP32X V store: V then V^T (from a 256-byte boundary) in FP32, each tiled the same way in blocks
+// of 64 columns x 128 contraction elements (x3_tiled_index; contraction padded to x3_ld).
+inline size_t x3_tiled_elems(int K, int cols) { return (size_t)x3_ld(K) * 64 * lp_tiles(cols); }
+size_t vstore_x3_offset_t(int n, int m);
 size_t vstore_bytes(int precision, int n, int m);
 
 // target: workgroups aimed for per pass (K slices are chosen to reach it).
@@ -90,5 +96,7 @@ hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, floa
                               int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
                                int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
+hipError_t run_iterations_x3(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                             int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 
 }  // namespace gpumlib_nmf

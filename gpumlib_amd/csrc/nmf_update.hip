@@ -191,7 +191,9 @@ struct RegStage4 {
 //   X(u,a) = H(a,u) at H[u*r + a], C(b,a) = WtW(a,b) = G[b*ldg + a]  (gt = 0)
 // W update from a pass over V^T (nmf_multiplicative_euclidian.cpp:53-66): j = i,
 //   X(i,a) = W(i,a) at W[a*n + i], C(b,a) = HHt(b,a) = G[a*ldg + b]  (gt = 1)
-// Sh (optional) receives the BF16 shadow of the new X: X(j,a) at Sh[a*ldsh + j].
+// Sh (optional) receives the BF16 shadow of the new X: X(j,a) at Sh[a*ldsh + j]; with
+// shplane > 0, its three BF16 pieces (bf16x2_split3) in planes shplane elements apart, the
+// workgroup's rows past count written as zeros (ldsh, a multiple of 128, covers them).
 // Each thread owns (row jj, 4 consecutive a); JT rows per workgroup.  RP is r rounded
 // up to 32: the staged C and X are zero beyond r, so the inner product runs over RP
 // terms (the added terms are exact zeros), on the matrix cores.
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
                                                               long pstride, float eps, uint16_t* __restrict__ Sh,
                                                               int ldsh, uint8_t* __restrict__ Q, int ldq,
                                                               uint8_t* __restrict__ Qsc, int nblk, int add,
-                                                              long gps, int gsl) {
+                                                              long gps, int gsl, long shplane) {
     extern __shared__ __attribute__((aligned(16))) float smr[];
     constexpr int RS = RP + 4, NQ = RP / 4, ITEMS = JT * NQ, MAXI = (ITEMS + NTHREADS - 1) / NTHREADS;
     float* sG = smr;            // sG[b*RS + a] = C(b,a), zero for a or b >= r
@@ -351,7 +353,19 @@ __global__ __launch_bounds__(NTHREADS) void k_update_rows_euc(float* __restrict_
             if (jj < nj) X[(size_t)(j0 + jj) * xs_j + (size_t)a * xs_a] = sN[jj * RS + a];
         }
     }
-    if (Sh) {
+    if (Sh && shplane) {
+        // pairs of rows: 4-byte aligned (j0 and ldsh are even)
+        for (int idx = tid; idx < JT / 2 * r; idx += NTHREADS) {
+            const int jj = 2 * (idx % (JT / 2)), a = idx / (JT / 2);
+            const float x0 = jj < nj ? sN[jj * RS + a] : 0.f, x1 = jj + 1 < nj ? sN[(jj + 1) * RS + a] : 0.f;
+            uint32_t w1, w2, w3;
+            bf16x2_split3(x0, x1, w1, w2, w3);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(Sh + (size_t)a * ldsh + j0 + jj);
+            dst[0] = w1;
+            dst[shplane / 2] = w2;
+            dst[shplane] = w3;
+        }
+    } else if (Sh) {
         for (int idx = tid; idx < JT * r; idx += NTHREADS) {
             const int jj = idx % JT, a = idx / JT;
             if (jj < nj) Sh[(size_t)a * ldsh + j0 + jj] = __builtin_bit_cast(uint16_t, (__bf16)sN[jj * RS + a]);
@@ -854,12 +868,13 @@ template <int JT, int RP>
 static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                                 int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
                                 uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, long gps, int gsl,
-                                hipStream_t st) {
+                                long shplane, hipStream_t st) {
     const size_t sm = (size_t)(RP + 2 * JT) * (RP + 4) * sizeof(float);
     hipError_t e = set_smem(k_update_rows_euc<JT, RP>, sm);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_update_rows_euc<JT, RP>), dim3((count + JT - 1) / JT), dim3(NTHREADS), sm, st, X, xs_j, xs_a,
-                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk, add, gps, gsl);
+                       P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, Qsc, nblk, add, gps, gsl,
+                       shplane);
     return hipGetLastError();
 }
 
@@ -868,13 +883,15 @@ static hipError_t launch_rows_t(float* X, long xs_j, long xs_a, const float* P, 
 static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, const float* G, int ldg, int gt,
                               int count, int r, int ldr, int S, long pstride, float eps, uint16_t* Sh, int ldsh,
                               uint8_t* Q, int ldq, uint8_t* Qsc, int nblk, int add, long gps, int gsl,
-                              hipStream_t st) {
+                              hipStream_t st, long shplane = 0) {
     if (r < 1 || r > 128 || (Q && (r > 64 || ldq % 32 || ldq < count))) return hipErrorInvalidValue;
+    if (shplane && (!Sh || shplane % 2 || ldsh % 128 || ldsh < count || shplane < (long)r * ldsh))
+        return hipErrorInvalidValue;
     if (gsl < 0 || gsl > kMaxGramSlices || (gsl > 0 && r > 64)) return hipErrorInvalidValue;
     const bool wide = count >= 16384 || Q != nullptr;
 #define LR(JT, RP)                                                                                                \
     return launch_rows_t<JT, RP>(X, xs_j, xs_a, P, G, ldg, gt, count, r, ldr, S, pstride, eps, Sh, ldsh, Q, ldq, \
-                                 Qsc, nblk, add, gps, gsl, st)
+                                 Qsc, nblk, add, gps, gsl, shplane, st)
     if (r <= 32) { if (wide) LR(32, 32); LR(16, 32); }
     if (r <= 64) { if (wide) LR(32, 64); LR(16, 64); }
     LR(16, 128);
@@ -882,15 +899,15 @@ static hipError_t launch_rows(float* X, long xs_j, long xs_a, const float* P, co
 }
 
 hipError_t launch_update_h_euc(float* H, const float* P, const float* G, int m, int r, int ldr, int S, long pstride,
-                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add, int gsl) {
+                               float eps, uint16_t* Hs, int ldhs, hipStream_t st, int add, int gsl, long shplane) {
     return launch_rows(H, r, 1, P, G, ldr, 0, m, r, ldr, S, pstride, eps, Hs, ldhs, nullptr, 0, nullptr, 0, add,
-                       pstride, gsl, st);
+                       pstride, gsl, st, shplane);
 }
 
 hipError_t launch_update_w_euc_t(float* W, const float* P, const float* G, int n, int r, int ldr, int S, long pstride,
-                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add, int gsl) {
+                                 float eps, uint16_t* Wb, int ldwb, hipStream_t st, int add, int gsl, long shplane) {
     return launch_rows(W, 1, n, P, G, ldr, 1, n, r, ldr, S, pstride, eps, Wb, ldwb, nullptr, 0, nullptr, 0, add,
-                       pstride, gsl, st);
+                       pstride, gsl, st, shplane);
 }
 
 hipError_t launch_update_h_euc_q8(float* H, const float* P, const float* G, int m, int r, int ldr, int S,

@@ -11,12 +11,12 @@ import ctypes
 import numpy as np
 
 from . import capi
-from .capi import (ADDITIVE_EUCLIDEAN, ADDITIVE_EUCLIDEAN_WITH_H, ADDITIVE_KL, BF16, EUCLIDEAN, FP8, FP32, KL,
-                   OBJ_FROBENIUS, OBJ_KL, OBJ_NONE, OBJ_QUALITY, REFERENCE_EPS)
+from .capi import (ADDITIVE_EUCLIDEAN, ADDITIVE_EUCLIDEAN_WITH_H, ADDITIVE_KL, BF16, EUCLIDEAN, FP8, FP32, FP32X,
+                   KL, OBJ_FROBENIUS, OBJ_KL, OBJ_NONE, OBJ_QUALITY, REFERENCE_EPS)
 
 __all__ = ["colmajor", "as_matrix", "gen_euclidean", "gen_poisson", "gen_init", "DeviceNMF",
            "EUCLIDEAN", "KL", "ADDITIVE_EUCLIDEAN", "ADDITIVE_KL", "ADDITIVE_EUCLIDEAN_WITH_H",
-           "FP32", "BF16", "FP8", "OBJ_NONE", "OBJ_FROBENIUS", "OBJ_KL", "OBJ_QUALITY", "REFERENCE_EPS"]
+           "FP32", "BF16", "FP8", "FP32X", "OBJ_NONE", "OBJ_FROBENIUS", "OBJ_KL", "OBJ_QUALITY", "REFERENCE_EPS"]
 
 
 def colmajor(x):
@@ -61,7 +61,7 @@ def gen_init(n, m, r, seed):
 class DeviceNMF:
     """One NMF problem resident on the GPU, driven through gpumlib_nmf_iterate.
 
-    precision BF16/FP8: V is packed into the V store of that precision on the device.
+    precision BF16/FP8/FP32X: V is packed into the V store of that precision on the device.
     setup_s: wall-clock seconds of the set-up phases, "upload" (device buffers allocated and
     V, W, H copied from pageable host memory) and "pack" (the V store built on the device)."""
 

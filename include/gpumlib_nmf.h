@@ -145,7 +145,10 @@ enum {
     GPUMLIB_NMF_ADDITIVE_KL = 3,
     GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H = 4
 };
-enum { GPUMLIB_NMF_FP32 = 0, GPUMLIB_NMF_BF16 = 1, GPUMLIB_NMF_FP8 = 2 };
+/* Storage precision of V (W, H and every update stay FP32).  GPUMLIB_NMF_FP32X: V in FP32, the pass
+ * products formed on the BF16 matrix cores from three-way BF16 splits of both operands (six piece
+ * products, FP32 accumulation): FP32 results, at FP32 storage traffic. */
+enum { GPUMLIB_NMF_FP32 = 0, GPUMLIB_NMF_BF16 = 1, GPUMLIB_NMF_FP8 = 2, GPUMLIB_NMF_FP32X = 3 };
 enum {
     GPUMLIB_NMF_OBJ_NONE = 0,
     GPUMLIB_NMF_OBJ_FROBENIUS = 1, /* sum (V - WH)^2 */
@@ -165,8 +168,11 @@ enum {
  * col/64) * 64 + col%64) * 512 + k%512, the contraction padded to a multiple of 512 and the columns to 64 T with
  * zeros), 256-byte aligned parts, followed by one E8M0 scale per 32 x 32 tile of the padded V (ldv = 512 *
  * ceil(n/512), ldt = 512 * ceil(m/512); scale of tile (i/32, u/32) at byte (u/32) * (ldv/32) + i/32; padding
- * tiles hold 127 = 2^0), shared by both orientations.  _pack builds the store from an FP32 device V on `stream`.  BF16 and FP8 storage run
- * the Euclidean rule; FP8 supports r <= 64. */
+ * tiles hold 127 = 2^0), shared by both orientations.  GPUMLIB_NMF_FP32X: V then V^T (from a 256-byte
+ * boundary) in FP32, tiled the same way in blocks of 64 columns x 128 contraction elements (element (k, col) at
+ * ((k/128 * T + col/64) * 64 + col%64) * 128 + k%128, the contraction padded to a multiple of 128 and the
+ * columns to 64 T with zeros).  _pack builds the store from an FP32 device V on `stream`.  BF16, FP8 and
+ * FP32X storage run the Euclidean rule; FP8 supports r <= 64, BF16 and FP32X r <= 128. */
 size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
                                    hipStream_t stream);

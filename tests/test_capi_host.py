@@ -54,6 +54,27 @@ def test_workspace_bytes(native):
         assert native.gpumlib_nmf_workspace_bytes(*ok) > 0
 
 
+def test_fp32x_plan_and_store(native):
+    """FP32X storage: Euclidean only, ranks up to 128; the V store holds V and V^T in FP32,
+    each padded to 128-deep chunks and 64-column tiles; the workspace holds three BF16
+    piece planes of W and of H^T beside the partials."""
+    n, m = 1000, 700
+    ws32 = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32, n, m, 64)
+    wsx = native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32X, n, m, 64)
+    assert wsx > 0 and wsx >= 3 * 2 * 64 * (1024 + 768)
+    assert native.gpumlib_nmf_workspace_bytes(capi.EUCLIDEAN, capi.FP32X, n, m, 128) > 0
+    for bad in [(capi.EUCLIDEAN, capi.FP32X, n, m, 129), (capi.KL, capi.FP32X, n, m, 8),
+                (capi.ADDITIVE_EUCLIDEAN, capi.FP32X, n, m, 8), (capi.EUCLIDEAN, capi.FP32X, 0, m, 8)]:
+        assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
+    assert ws32 > 0
+    pad = lambda k: (k + 127) // 128 * 128  # noqa: E731
+    tiles = lambda c: (c + 63) // 64  # noqa: E731
+    a256 = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    for nn, mm in [(1000, 700), (1, 1), (128, 64), (4097, 130)]:
+        want = a256(pad(nn) * 64 * tiles(mm) * 4) + a256(pad(mm) * 64 * tiles(nn) * 4)
+        assert native.gpumlib_nmf_vstore_bytes(capi.FP32X, nn, mm) == want
+
+
 def test_additive_rules_plan_fp32_only(native):
     """The additive rules run the FP32 passes (same workspace as their multiplicative
     counterparts); no narrow-precision store; the object API takes FP32 only."""
