@@ -157,7 +157,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (OBJ != OBJ_NONE) objacc += obj_term<OBJ>(vj[j], wh[g * 4 + j], p.eps);
-                    if (KL) rj[j] = vj[j] / (wh[g * 4 + j] + p.eps);
+                    if (KL) rj[j] = kl_ratio(vj[j], wh[g * 4 + j], p.eps);
                 }
                 if (KL) {
 #pragma unroll
@@ -635,7 +635,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float vv = sV[buf][g * 8 + h * 4 + j][wave * 32 + c];
-                rj[j] = vv / (wt[g * 4 + j] + p.eps);
+                rj[j] = kl_ratio(vv, wt[g * 4 + j], p.eps);
             }
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
