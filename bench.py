@@ -120,8 +120,8 @@ def main():
         return
     rule_name, n, m, r, r0, seed = WORKLOADS[args.workload]
     rule = G.EUCLIDEAN if rule_name == "euclidean" else G.KL
-    if rule == G.KL:
-        args.precision = "fp32"  # the narrow storage paths run the Euclidean rule
+    if rule == G.KL and args.precision not in ("fp32", "bf16"):
+        args.precision = "fp32"  # the KL rule runs with FP32 or BF16 storage
     eps = G.REFERENCE_EPS if rule == G.EUCLIDEAN else 1e-9
 
     t_gen = time.time()
@@ -227,15 +227,16 @@ def main():
     legs = {"fp32": leg("fp32")}
     if args.precision != "fp32":
         legs[args.precision] = leg(args.precision)
-    if rule == G.EUCLIDEAN:  # the narrow storage paths run the Euclidean rule
-        for extra in (x.strip() for x in args.extra_legs.split(",")):
-            if extra and extra not in legs:
-                legs[extra] = leg(extra)
-                legs[extra]["speedup_vs_fp32_path"] = legs[extra]["iters_per_s"] / legs["fp32"]["iters_per_s"]
-                if gap_iters:
-                    legs[extra]["objective_rel_gap_vs_fp32_path"] = {
-                        str(t): abs(legs[extra]["objective_at"][t] - legs["fp32"]["objective_at"][t])
-                        / legs["fp32"]["objective_at"][t] for t in gap_iters}
+    # storage precisions each rule runs: the KL rule has FP32 and BF16 storage
+    runs = ("fp32", "bf16", "fp8", "fp32x") if rule == G.EUCLIDEAN else ("fp32", "bf16")
+    for extra in (x.strip() for x in args.extra_legs.split(",")):
+        if extra and extra not in legs and extra in runs:
+            legs[extra] = leg(extra)
+            legs[extra]["speedup_vs_fp32_path"] = legs[extra]["iters_per_s"] / legs["fp32"]["iters_per_s"]
+            if gap_iters:
+                legs[extra]["objective_rel_gap_vs_fp32_path"] = {
+                    str(t): abs(legs[extra]["objective_at"][t] - legs["fp32"]["objective_at"][t])
+                    / legs["fp32"]["objective_at"][t] for t in gap_iters}
     # Additive-update variants on the same workload (FP32, the same pass kernels with the
     # additive update): iterations per second and objectives, for the record.
     def additive_leg(arule):

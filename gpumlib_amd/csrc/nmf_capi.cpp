@@ -89,8 +89,12 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8 &&
         precision != GPUMLIB_NMF_FP32X)
         return false;
-    if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN) return false;
+    // the narrow storage paths run the Euclidean rule; BF16 storage also runs the
+    // multiplicative KL rule (up to r = 64)
+    const bool kl_bf16 = precision == GPUMLIB_NMF_BF16 && rule == GPUMLIB_NMF_KL;
+    if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN && !kl_bf16) return false;
     if (n < 1 || m < 1 || r < 1) return false;
+    if (kl_bf16 && r > 64) return false;
     if (r > kMaxFusedRank) {
         if (precision != GPUMLIB_NMF_FP32) return false;
         return make_unfused_plan(rule, n, m, r, pl);
@@ -117,7 +121,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     const int euc = !rule_is_kl(rule);
     const int x3 = precision == GPUMLIB_NMF_FP32X;
     const int kc = (precision == GPUMLIB_NMF_FP32) ? KC
-                   : x3                            ? 128
+                   : (x3 || kl_bf16)               ? 128
                                                    : (precision == GPUMLIB_NMF_BF16 ? 256 : 512);
     // Euclidean grams: the FP32 passes spread the ceil(r/32)^2 gram blocks over the
     // first tiles of every slice, one per tile (tiny problems get gram-only tiles), and so
@@ -183,6 +187,13 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         off += align256((size_t)r * pl->ldv8 * 2);
         pl->off_Hb = off;
         off += align256((size_t)r * pl->ldt8 * 2);
+        if (kl_bf16) {
+            pl->rps = round_up(r, 32);
+            pl->off_Wr = off;
+            off += align256((size_t)pl->ldv8 * pl->rps * 2);
+            pl->off_Hr = off;
+            off += align256((size_t)pl->ldt8 * pl->rps * 2);
+        }
     } else if (x3) {  // three BF16 piece planes of W and of H^T
         pl->off_Wb = off;
         off += align256((size_t)3 * r * pl->ldx_n * 2);
@@ -447,6 +458,68 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
         if (!update_w) continue;
         CK(launch_pass_tn_bf16(aw, st));
         CK(launch_update_w_euc_t(W, P, G3, pl.n, pl.r, pl.ldr, pl.S3, pl.pstride3, eps, Wb, pl.ldv8, st, 0, gs3));
+    }
+    return hipSuccess;
+}
+
+// The KL passes over the plan's BF16 V store: H pass (which 0: A = W, C = H, B = V) or W pass
+// (which 1: A = H^T, C = W^T, B = V^T), over the shadows in the workspace.
+void kl_bf16_args(const Plan& pl, const void* Vstore, void* ws, float eps, int which, KlLpArgs* a) {
+    char* base = (char*)ws;
+    memset(a, 0, sizeof(*a));
+    const uint16_t* Wk = (const uint16_t*)(base + pl.off_Wb);
+    const uint16_t* Hk = (const uint16_t*)(base + pl.off_Hb);
+    const uint16_t* Wr = (const uint16_t*)(base + pl.off_Wr);
+    const uint16_t* Hr = (const uint16_t*)(base + pl.off_Hr);
+    a->P = (float*)(base + pl.off_P);
+    a->Psum = (float*)(base + pl.off_Psum);
+    a->r = pl.r;
+    a->rps = pl.rps;
+    a->ldr = pl.ldr;
+    a->rpad = pl.rpad;
+    a->nt = pl.lp_nt;
+    a->eps = eps;
+    if (which == 0) {
+        a->Ak = Wk, a->Ar = Wr, a->Cr = Hr, a->B = (const uint16_t*)Vstore;
+        a->K = pl.n, a->N = pl.m, a->lda = pl.ldv8;
+        a->kslice = pl.ks1, a->S = pl.S1, a->tiles = pl.tiles1, a->pstride = pl.pstride1;
+    } else {
+        a->Ak = Hk, a->Ar = Hr, a->Cr = Wr;
+        a->B = (const uint16_t*)((const char*)Vstore + vstore_bf16_offset_t(pl.n, pl.m));
+        a->K = pl.m, a->N = pl.n, a->lda = pl.ldt8;
+        a->kslice = pl.ks3, a->S = pl.S3, a->tiles = pl.tiles3, a->pstride = pl.pstride3;
+    }
+}
+
+// KL rule, BF16 storage (nmf_multiplicative_divergence.cpp:29-50): the shadows of W and H in
+// both layouts are built once, then per iteration the H pass (W H of the old factors, formed
+// in the pass) and the H update (which rewrites the H shadows), the W pass (W H with the new
+// H) and the W update.
+hipError_t run_iterations_kl_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                                  int update_w, int objective, double* d_objective, void* ws, hipStream_t st) {
+    if (!(eps > 0.f)) return hipErrorInvalidValue;  // padded rows rely on V / (0 + eps) = 0
+    char* base = (char*)ws;
+    uint16_t* Wk = (uint16_t*)(base + pl.off_Wb);
+    uint16_t* Hk = (uint16_t*)(base + pl.off_Hb);
+    uint16_t* Wr = (uint16_t*)(base + pl.off_Wr);
+    uint16_t* Hr = (uint16_t*)(base + pl.off_Hr);
+    const int n = pl.n, m = pl.m, r = pl.r;
+    CK(launch_shadow_bf16(W, H, n, m, r, Wk, pl.ldv8, Hk, pl.ldt8, st));
+    CK(launch_shadow_rank_bf16(W, 1, n, n, r, pl.ldv8, Wr, pl.rps, st));
+    CK(launch_shadow_rank_bf16(H, r, 1, m, r, pl.ldt8, Hr, pl.rps, st));
+    KlLpArgs ah, aw;
+    kl_bf16_args(pl, Vstore, ws, eps, 0, &ah);
+    kl_bf16_args(pl, Vstore, ws, eps, 1, &aw);
+    for (int it = 0; it < iterations; it++) {
+        if (it == 0 && d_objective && objective != GPUMLIB_NMF_OBJ_NONE)
+            CK(run_objective(pl, Vstore, W, H, eps, objective, d_objective, ws, st));
+        CK(launch_pass_kl_bf16(ah, st));
+        CK(launch_update_kl_rows_bf16(H, r, 1, ah.P, ah.Psum, m, r, pl.ldr, pl.rpad, pl.S1, pl.pstride1, eps, Hk,
+                                      pl.ldt8, Hr, pl.rps, st));
+        if (!update_w) continue;
+        CK(launch_pass_kl_bf16(aw, st));
+        CK(launch_update_kl_rows_bf16(W, 1, n, aw.P, aw.Psum, n, r, pl.ldr, pl.rpad, pl.S3, pl.pstride3, eps, Wk,
+                                      pl.ldv8, Wr, pl.rps, st));
     }
     return hipSuccess;
 }
@@ -718,6 +791,9 @@ hipError_t gpumlib_nmf_iterate(int rule, int precision, int n, int m, int r, con
     if (pl.unfused)
         return run_iterations_unfused(pl, (const float*)V, W, H, eps, iterations, update_w, objective, d_objective,
                                       workspace, stream);
+    if (precision == GPUMLIB_NMF_BF16 && rule_is_kl(rule))
+        return run_iterations_kl_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace,
+                                      stream);
     if (precision == GPUMLIB_NMF_BF16)
         return run_iterations_bf16(pl, V, W, H, eps, iterations, update_w, objective, d_objective, workspace, stream);
     if (precision == GPUMLIB_NMF_FP8)
@@ -785,6 +861,12 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         x3_args(pl, V, workspace, which, &a);
         a.gkeep = gslices > 0;
         return launch_pass_tn_x3(a, stream);
+    }
+    if (precision == GPUMLIB_NMF_BF16 && kl) {
+        // one KL pass over the W/H shadows the last iterate call left in this workspace
+        KlLpArgs a;
+        kl_bf16_args(pl, V, workspace, eps, which, &a);
+        return launch_pass_kl_bf16(a, stream);
     }
     if (precision == GPUMLIB_NMF_BF16) {
         // one BF16 pass over the W/H shadows the last iterate call left in this workspace

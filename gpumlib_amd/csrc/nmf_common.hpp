@@ -338,6 +338,27 @@ struct LpPassArgs {
                  // the three BF16 piece planes of A
 };
 
+// KL pass over the BF16 V store: P[s](a, col) = sum_{k in slice s} A(k, a) R(k, col) with the ratio
+// R = bf16(V(k, col) / ((A C)(k, col) + eps)), A C formed per 32 x 32 block on the matrix cores and
+// never stored.  A comes twice, K-contiguous (Ak[a*lda + k], the second product) and
+// rank-contiguous (Ar[k*rps + a], the A C product); C rank-contiguous (Cr[col*rps + a]).  Both
+// rank-contiguous shadows are zero for a in [r, rps) and past the last row.  Tile 0 of every
+// slice also writes the slice's column sums of A, Psum[s*rpad + a] = sum_k A(k, a).
+struct KlLpArgs {
+    const uint16_t* Ak;
+    const uint16_t* Ar;
+    const uint16_t* Cr;
+    const uint16_t* B;   // the tiled store of one V orientation (bf16_tiled_index, vtiles column tiles)
+    float* P;
+    float* Psum;
+    int K, N, r;
+    int lda, rps, ldr, rpad;
+    int kslice, S, tiles;
+    int nt;              // 64-column store blocks per workgroup tile (2 for r <= 64)
+    long pstride;
+    float eps;
+};
+
 // One problem of a grouped (batched) launch: the arguments of its two passes and
 // of its updates.  P, Psum and objp point into the problem's own workspace, so
 // no value depends on which other problems share the launch.

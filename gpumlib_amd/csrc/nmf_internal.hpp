@@ -19,6 +19,17 @@ hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ld
                               hipStream_t st);
 hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
                               int ldh, hipStream_t st);
+// KL rule with BF16 storage (nmf_kl_bf16.hip; r <= 64, eps > 0, rps = r rounded up to 32, kslice
+// a multiple of 128, nt = 2).
+hipError_t launch_pass_kl_bf16(const KlLpArgs& a, hipStream_t st);
+// Xr[j*rps + a] = bf16(X[j*xs_j + a*xs_a]) for j < count, a < r; zeros up to rows x rps.
+hipError_t launch_shadow_rank_bf16(const float* X, long xs_j, long xs_a, int count, int r, int rows, uint16_t* Xr,
+                                   int rps, hipStream_t st);
+// X(j,a) *= sum_s P[s][j*ldr + a] / max(sum_s Psum[s*rpad + a], eps), X(j,a) at X[j*xs_j + a*xs_a];
+// then Xk[a*ldk + j] and Xr[j*rps + a] = bf16 of the new X.
+hipError_t launch_update_kl_rows_bf16(float* X, long xs_j, long xs_a, const float* P, const float* Psum, int count,
+                                      int r, int ldr, int rpad, int S, long pstride, float eps, uint16_t* Xk, int ldk,
+                                      uint16_t* Xr, int rps, hipStream_t st);
 // FP32X storage path (nmf_pass_x3.hip): FP32 V and V^T tiled by x3_tiled_index (contraction
 // padded to ldv / ldt, multiples of 128), the A operand as three BF16 piece planes.
 hipError_t launch_pass_tn_x3(const LpPassArgs& a, hipStream_t st);

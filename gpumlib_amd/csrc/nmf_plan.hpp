@@ -43,6 +43,10 @@ struct Plan {
     // pieces, 128-row blocks, 32-column blocks and column ranges per row block
     int ob_rp, ob_nrows, ob_ncols, ob_rblocks, ob_cblocks, ob_csplit;
     size_t off_P, off_Psum, off_objp, off_G, off_cnt, off_Wb, off_Hb, off_Wsc, off_Hsc, off_Ws, off_Hs, bytes;
+    // KL rule with BF16 storage: rank-contiguous BF16 shadows of W (ldv8 rows) and of H (ldt8 rows),
+    // rows of rps = r rounded up to 32 elements (off_Wb / off_Hb hold the K-contiguous ones)
+    int rps;
+    size_t off_Wr, off_Hr;
     // unfused iteration: WH (n x m; the KL ratio in place), numerator and denominator products
     // (max(r m, n r) floats each), the r x r gram, the clamped sums (r), FP64 objective partials
     size_t off_uWH, off_uA, off_uB, off_uG, off_usum, off_uobj;
@@ -96,6 +100,9 @@ hipError_t run_iterations_fp8(const Plan& pl, const void* Vstore, float* W, floa
                               int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
                                int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
+hipError_t run_iterations_kl_bf16(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
+                                  int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
+void kl_bf16_args(const Plan& pl, const void* Vstore, void* ws, float eps, int which, KlLpArgs* a);
 hipError_t run_iterations_x3(const Plan& pl, const void* Vstore, float* W, float* H, float eps, int iterations,
                              int update_w, int objective, double* d_objective, void* ws, hipStream_t st);
 

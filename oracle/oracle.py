@@ -300,6 +300,24 @@ def np_euc_iter_shadow(Vq, W, H, eps, shadow_w, shadow_h, update_w=True):
     return W, H
 
 
+def np_kl_iter_shadow(Vq, W, H, eps, shadow, ratio_round, update_w=True):
+    """KL MU (the rules of np_kl_iter, nmf_kernels.cu:63-341) with the passes fed shadow
+    operands, as the BF16 storage path runs it: W H from shadow(W) and shadow(H); the ratio
+    V / (W H + eps) rounded by ratio_round (bf16_round: to BF16 through FP32) before the second product;
+    numerator and the clamped column (row) sums from the same shadows; the FP32 masters
+    updated.  The W half takes shadow(new H) and the shadow of the old W."""
+    Ws, Hs = shadow(W), shadow(H)
+    R = ratio_round(Vq / (Ws @ Hs + eps))
+    sumW = np.maximum(Ws.sum(axis=0), eps)
+    H = H * ((Ws.T @ R) / sumW[:, None])
+    if update_w:
+        Hs = shadow(H)
+        R = ratio_round(Vq / (Ws @ Hs + eps))
+        sumH = np.maximum(Hs.sum(axis=1), eps)
+        W = W * ((R @ Hs.T) / sumH[None, :])
+    return W, H
+
+
 def np_frob(V, W, H):
     D = V - W @ H
     return float(np.sum(D * D))

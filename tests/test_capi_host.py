@@ -54,6 +54,27 @@ def test_workspace_bytes(native):
         assert native.gpumlib_nmf_workspace_bytes(*ok) > 0
 
 
+def test_kl_bf16_plan(native):
+    """KL with BF16 storage plans ranks up to 64 (the multiplicative rule only); its workspace
+    adds the rank-contiguous BF16 shadows of W and H (rows of r rounded up to 32) to what
+    the Euclidean BF16 path holds; eps must be positive (refused before any device work)."""
+    import ctypes
+
+    n, m = 1000, 700
+    for r in (1, 17, 32, 33, 64):
+        kl = native.gpumlib_nmf_workspace_bytes(capi.KL, capi.BF16, n, m, r)
+        rps = (r + 31) // 32 * 32
+        assert kl >= 2 * rps * (1024 + 768), r
+    for bad in [(capi.KL, capi.BF16, n, m, 65), (capi.ADDITIVE_KL, capi.BF16, n, m, 8), (capi.KL, capi.FP8, n, m, 8),
+                (capi.KL, capi.FP32X, n, m, 8)]:
+        assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: eps is checked first
+    ws = native.gpumlib_nmf_workspace_bytes(capi.KL, capi.BF16, n, m, 16)
+    for eps in (0.0, -1e-9):
+        assert native.gpumlib_nmf_iterate(capi.KL, capi.BF16, n, m, 16, fake, fake, fake, eps, 1, 1, capi.OBJ_NONE,
+                                          None, fake, ws, None) == HIP_INVALID_VALUE
+
+
 def test_fp32x_plan_and_store(native):
     """FP32X storage: Euclidean only, ranks up to 128; the V store holds V and V^T in FP32,
     each padded to 128-deep chunks and 64-column tiles; the workspace holds three BF16

@@ -340,3 +340,53 @@ def test_shadow_iteration_order(orc):
     assert np.array_equal(w1, w2) and np.array_equal(h1, h2)
     assert [k for k, _ in seen] == ["w", "h"]
     assert np.array_equal(seen[0][1], W) and np.array_equal(seen[1][1], h2)
+
+
+def test_kl_shadow_identity_is_np_kl_iter(orc):
+    """Identity shadows and ratio rounding give np_kl_iter bit for bit; the shadow is taken of
+    the old W, the old H and then the new H, and the W half keeps the old W's shadow."""
+    V, W, H = poisson_problem(30, 22, 4, 31)
+    seen = []
+    ident = lambda X: (seen.append(X.copy()), X)[1]
+    w1, h1 = orc.np_kl_iter_shadow(V, W, H, 1e-9, ident, lambda R: R)
+    w2, h2 = orc.np_kl_iter(V, W, H, 1e-9)
+    assert np.array_equal(w1, w2) and np.array_equal(h1, h2)
+    assert len(seen) == 3
+    assert np.array_equal(seen[0], W) and np.array_equal(seen[1], H) and np.array_equal(seen[2], h2)
+
+
+def test_kl_shadow_transpose_duality(orc):
+    """With BF16 shadows too, the W half on (V, W, H) is the H half on (V^T, H^T, W^T)."""
+    V, W, H = poisson_problem(26, 33, 5, 32)
+    w2, h2 = orc.np_kl_iter_shadow(V, W, H, 1e-9, orc.bf16_round, orc.bf16_round)
+    _, wt = orc.np_kl_iter_shadow(V.T, h2.T, W.T, 1e-9, orc.bf16_round, orc.bf16_round, update_w=False)
+    assert rel_fro(wt.T, w2) < 1e-13
+
+
+def test_kl_shadow_fixed_point_uses_shadow_sums(orc):
+    """V = bf16(W0) bf16(H0) exactly, masters W0, H0 not representable in BF16: the ratio rounds to
+    exactly 1, so the factors stay put only if the numerator and the clamped sums both come from
+    the shadows (sums of the FP32 masters would move every entry by ~2^-9)."""
+    rng = np.random.default_rng(33)
+    W0 = rng.random((24, 3)) + 0.5
+    H0 = rng.random((3, 17)) + 0.5
+    Ws, Hs = orc.bf16_round(W0), orc.bf16_round(H0)
+    assert not np.array_equal(Ws, W0)
+    V = Ws @ Hs
+    w, h = orc.np_kl_iter_shadow(V, W0, H0, 1e-9, orc.bf16_round, orc.bf16_round)
+    assert rel_fro(h, H0) < 1e-14 and rel_fro(w, W0) < 1e-14
+    sums_from_masters = H0 * ((Ws.T @ np.ones_like(V)) / W0.sum(axis=0)[:, None])
+    assert rel_fro(sums_from_masters, H0) > 1e-4
+
+
+def test_kl_shadow_tracks_exact_iteration(orc):
+    """BF16 operands move a Poisson problem's objective by far less than the iterations do."""
+    V, W, H = poisson_problem(60, 50, 4, 34)
+    n, m = V.shape
+    w, h, we, he = W, H, W, H
+    for _ in range(30):
+        w, h = orc.np_kl_iter(V, w, h, 1e-9)
+        we, he = orc.np_kl_iter_shadow(V, we, he, 1e-9, orc.bf16_round, orc.bf16_round)
+    d0 = orc.kl_div(n, m, 4, cm(V), cm(W), cm(H))
+    d, de = orc.kl_div(n, m, 4, cm(V), cm(w), cm(h)), orc.kl_div(n, m, 4, cm(V), cm(we), cm(he))
+    assert abs(de - d) / d < 1e-3 and d < 0.5 * d0

@@ -24,7 +24,8 @@ def test_noalloc_rejects_oversize_and_bad_problems(native):
     args = (G.EUCLIDEAN, G.FP32, 16384, 8192, 128, None, None, None, 1e-8, 1, 1, G.OBJ_NONE, None, None)
     assert native.gpumlib_nmf_iterate_noalloc(*args) == HIP_ERROR_OUT_OF_MEMORY
     assert native.gpumlib_nmf_workspace_bytes(G.EUCLIDEAN, G.FP32, 16384, 8192, 64) <= SCRATCH_BYTES
-    for bad in [(G.EUCLIDEAN, G.FP32, 10, 10, 0), (7, G.FP32, 10, 10, 4), (G.KL, G.BF16, 10, 10, 4)]:
+    for bad in [(G.EUCLIDEAN, G.FP32, 10, 10, 0), (7, G.FP32, 10, 10, 4), (G.KL, G.FP8, 10, 10, 4),
+                (G.KL, G.BF16, 10, 10, 65)]:
         a = (*bad, None, None, None, 1e-8, 1, 1, G.OBJ_NONE, None, None)
         assert native.gpumlib_nmf_iterate_noalloc(*a) == 1  # hipErrorInvalidValue
 
@@ -72,6 +73,7 @@ CASES = [
     (G.KL, G.FP32, (1200, 1000, 40), (257, 130, 17)),
     (G.EUCLIDEAN, G.BF16, (2048, 1536, 64), (600, 500, 32)),
     (G.EUCLIDEAN, G.FP8, (2048, 1024, 64), (700, 300, 16)),
+    (G.KL, G.BF16, (1800, 1300, 64), (333, 250, 20)),
     (G.ADDITIVE_KL, G.FP32, (800, 600, 24), (100, 90, 8)),
     (G.ADDITIVE_EUCLIDEAN_WITH_H, G.FP32, (640, 480, 32), (64, 64, 4)),
 ]
