@@ -757,15 +757,14 @@ hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, floa
 
 hipError_t gpumlib_nmf_gemm_bf16(char transa, char transb, int M, int N, int K, float alpha, const uint16_t* A,
                                  int lda, const uint16_t* B, int ldb, float beta, float* C, int ldc, hipStream_t stream) {
-    return launch_gemm(1, transa, transb, M, N, K, alpha, A, lda, 1.f, B, ldb, 1.f, beta, C, ldc, stream);
+    return launch_gemm(1, transa, transb, M, N, K, alpha, A, lda, 127, B, ldb, 127, beta, C, ldc, stream);
 }
 
 hipError_t gpumlib_nmf_gemm_fp8(char transa, char transb, int M, int N, int K, float alpha, const uint8_t* A, int lda,
                                 int scale_a, const uint8_t* B, int ldb, int scale_b, float beta, float* C, int ldc,
                                 hipStream_t stream) {
     if (scale_a < 0 || scale_a > 254 || scale_b < 0 || scale_b > 254) return hipErrorInvalidValue;
-    return launch_gemm(2, transa, transb, M, N, K, alpha, A, lda, std::ldexp(1.f, scale_a - 127), B, ldb,
-                       std::ldexp(1.f, scale_b - 127), beta, C, ldc, stream);
+    return launch_gemm(2, transa, transb, M, N, K, alpha, A, lda, scale_a, B, ldb, scale_b, beta, C, ldc, stream);
 }
 
 size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r) {

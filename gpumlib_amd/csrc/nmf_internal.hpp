@@ -139,8 +139,8 @@ hipError_t launch_objective_wh(const float* V, const float* WH, long count, floa
 hipError_t launch_sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,
                         const float* B, int ldb, float beta, float* C, int ldc, hipStream_t st);
 // The same with operands in format fmt (0 FP32, 1 BF16 bit patterns, 2 OCP E4M3 times the
-// operand's scale sa / sb; sa, sb are ignored for 0 and 1).
-hipError_t launch_gemm(int fmt, char ta, char tb, int M, int N, int K, float alpha, const void* A, int lda, float sa,
-                       const void* B, int ldb, float sb, float beta, float* C, int ldc, hipStream_t st);
+// operand's E8M0 scale 2^(sa-127) / 2^(sb-127), sa, sb in [0, 254]; ignored for 0 and 1).
+hipError_t launch_gemm(int fmt, char ta, char tb, int M, int N, int K, float alpha, const void* A, int lda, int sa,
+                       const void* B, int ldb, int sb, float beta, float* C, int ldc, hipStream_t st);
 
 }  // namespace gpumlib_nmf

@@ -111,8 +111,10 @@ hipError_t gpumlib_nmf_sgemm(char transa, char transb, int M, int N, int K, floa
 /* Narrow-operand variants of gpumlib_nmf_sgemm (no reference counterpart; for callers that keep the
  * DeviceMatrix::Multiply operands in BF16 or FP8 to halve or quarter their bytes): the same contract with
  * A and B in BF16 (uint16 bit patterns) or in OCP E4M3 bytes with one E8M0 scale per operand (element value
- * e4m3(q) * 2^(scale - 127), scale in [0, 254]); C and the accumulation stay FP32.  Every product of two
- * such values is exact in FP32, so C is the FP32-accumulated GEMM of the stored values. */
+ * e4m3(q) * 2^(scale - 127), scale in [0, 254]); C and the accumulation stay FP32.  Both run on the
+ * matrix cores of their operand format.  Every product of two such values is exact in FP32: the BF16 form
+ * is the FP32-accumulated GEMM of the stored values; the FP8 matrix cores sum each 64-term block of K at
+ * less than FP32 accuracy (measured: up to ~6e-5 relative on full-range random operands). */
 hipError_t gpumlib_nmf_gemm_bf16(char transa, char transb, int M, int N, int K, float alpha, const uint16_t *A,
                                  int lda, const uint16_t *B, int ldb, float beta, float *C, int ldc, hipStream_t stream);
 hipError_t gpumlib_nmf_gemm_fp8(char transa, char transb, int M, int N, int K, float alpha, const uint8_t *A, int lda,

@@ -33,7 +33,10 @@ def get(buf, dtype=np.float32):
 
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("tb", ["N", "T"])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (64, 64, 64), (130, 257, 95), (1000, 3, 700)])
+# (256, 384, 320): whole 128 x 128 tiles and whole stages of K, 16-byte loads; (300, 260, 1000):
+# interior tiles on the vector path beside edge tiles, and a partial last stage of K
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (64, 64, 64), (130, 257, 95), (1000, 3, 700),
+                                 (256, 384, 320), (300, 260, 1000)])
 def test_sgemm(native, orc, cuda, ta, tb, mnk):
     M, N, K = mnk
     rng = np.random.default_rng(M * 7 + N * 3 + K)
@@ -84,7 +87,8 @@ def e4m3_value(q):
 
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("tb", ["N", "T"])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (130, 257, 95), (1000, 3, 700)])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 17, 5), (130, 257, 95), (1000, 3, 700), (256, 384, 320),
+                                 (264, 136, 1000)])
 def test_gemm_bf16(native, orc, cuda, ta, tb, mnk):
     """BF16 operands, FP32 accumulation: the FP64 GEMM of the stored values within FP32
     accumulation error (the products are exact in FP32)."""
@@ -105,9 +109,13 @@ def test_gemm_bf16(native, orc, cuda, ta, tb, mnk):
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
-@pytest.mark.parametrize("mnk", [(33, 17, 5), (130, 257, 95), (64, 64, 512)])
+@pytest.mark.parametrize("mnk", [(33, 17, 5), (130, 257, 95), (64, 64, 512), (256, 384, 320), (272, 144, 1000)])
 def test_gemm_fp8(native, orc, cuda, ta, tb, mnk):
-    """E4M3 operands with one E8M0 scale each: the FP64 GEMM of the scaled stored values."""
+    """E4M3 operands with one E8M0 scale each, on the FP8 matrix cores: the FP64 GEMM of the
+    scaled stored values within 2e-4.  The products are exact, but the matrix cores sum each
+    64-term block of K at less than FP32 accuracy (measured on these operands: 3.8e-5 relative
+    at K = 5, 5.3e-5 at K = 64, e.g. 143808 for an exact 143819.4), so the bound is that of
+    the hardware's block sums, not of FP32 accumulation."""
     M, N, K = mnk
     rng = np.random.default_rng(M * 13 + N + K * 3)
     Ar, Ac = (M, K) if ta == "N" else (K, M)
@@ -123,7 +131,7 @@ def test_gemm_fp8(native, orc, cuda, ta, tb, mnk):
     ref = C.astype(np.float64)
     orc.gemm(ta, tb, M, N, K, 1.25, e4m3_value(A) * 2.0 ** (sa - 127), Ar, e4m3_value(B) * 2.0 ** (sb - 127), Br,
              0.5, ref, M)
-    assert rel_fro(get(dC), ref) < 1e-5
+    assert rel_fro(get(dC), ref) < 2e-4
     # the decoding is the library's own host decoder's
     assert all(native.gpumlib_nmf_fp8_decode(int(c), 127) == e4m3_value(np.array([c]))[0] for c in codes)
 
