@@ -88,6 +88,8 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run) and exit")
     ap.add_argument("--no-reduction", action="store_true",
                     help="skip the reduction-framework leg (sum / min / max over the FP32 V)")
+    ap.add_argument("--no-gemm", action="store_true",
+                    help="skip the container-GEMM leg (W H of the workload, square 4096^3 GEMMs per operand format)")
     ap.add_argument("--no-class", action="store_true",
                     help="skip the drop-in class legs (DoIteration through the object handle, FP32)")
     ap.add_argument("--no-additive", action="store_true",
@@ -338,6 +340,58 @@ def main():
 
     reduction = None if args.no_reduction else reduction_leg()
 
+    # Container GEMM (DeviceMatrix::Multiply / GetWH): W H of the workload (n x m from n x r times
+    # r x m, FP32: bound by writing the n x m product), and a square 4096^3 GEMM in each operand
+    # format against that format's matrix-core peak.  HIP events on the launching stream.
+    def gemm_leg():
+        st = hip.Stream()
+
+        def timed(fn, reps):
+            for _ in range(2):
+                capi.check(fn(), "gemm")
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(st)
+            for _ in range(reps):
+                capi.check(fn(), "gemm")
+            e1.record(st)
+            e1.synchronize()
+            return e0.elapsed_ms(e1) * 1e3 / reps
+
+        Wd = hip.Buffer.from_array(np.asarray(W0, dtype=np.float32))
+        Hd = hip.Buffer.from_array(np.asarray(H0, dtype=np.float32))
+        WH = hip.Buffer(4 * n * m)
+        us = timed(lambda: lib.gpumlib_nmf_sgemm(b"N", b"N", n, m, r, 1.0, Wd.ptr, n, Hd.ptr, r, 0.0, WH.ptr, n,
+                                                 st.handle), max(5, args.kernel_reps))
+        out = {"get_wh": {"shape": f"{n}x{m}x{r}", "us_per_call": us,
+                          "tflops": 2.0 * n * m * r / (us * 1e-6) / 1e12,
+                          "write_gbs": 4.0 * n * m / (us * 1e-6) / 1e9}}
+        for x in (Wd, Hd, WH):
+            x.close()
+        S = 4096
+        A, B, Cm = hip.Buffer(4 * S * S), hip.Buffer(4 * S * S), hip.Buffer(4 * S * S)
+        A.fill_zero()
+        B.fill_zero()
+        calls = {
+            "fp32": (lambda: lib.gpumlib_nmf_sgemm(b"N", b"N", S, S, S, 1.0, A.ptr, S, B.ptr, S, 0.0, Cm.ptr, S,
+                                                   st.handle), PEAK["fp32_mfma_tflops"]),
+            "bf16": (lambda: lib.gpumlib_nmf_gemm_bf16(b"N", b"N", S, S, S, 1.0, A.ptr, S, B.ptr, S, 0.0, Cm.ptr, S,
+                                                       st.handle), PEAK["bf16_mfma_tflops"]),
+            "fp8": (lambda: lib.gpumlib_nmf_gemm_fp8(b"N", b"N", S, S, S, 1.0, A.ptr, S, 127, B.ptr, S, 127, 0.0,
+                                                     Cm.ptr, S, st.handle), PEAK["fp8_mfma_tflops"]),
+        }
+        for name, (fn, peak) in calls.items():
+            us = timed(fn, 5)
+            tf = 2.0 * S ** 3 / (us * 1e-6) / 1e12
+            out[f"square_{name}"] = {"shape": f"{S}x{S}x{S}", "us_per_call": us, "tflops": tf,
+                                     "frac_of_mfma_peak": tf / peak}
+        for x in (A, B, Cm):
+            x.close()
+        out["note"] = ("gpumlib_nmf_sgemm / _gemm_bf16 / _gemm_fp8, column-major 'N','N'; 128 x 128 tiles on the "
+                       "operand format's matrix cores")
+        return out
+
+    gemm = None if args.no_gemm else gemm_leg()
+
     head = legs[args.precision]
     line = {
         "metric": "NMF iter/s (16384x8192, r=64)" if args.workload.startswith("euclid-16384") else f"NMF iter/s ({args.workload})",
@@ -373,6 +427,8 @@ def main():
         line["additive_legs"] = additive
     if reduction:
         line["reduction"] = reduction
+    if gemm:
+        line["gemm"] = gemm
 
     if args.serve_jobs > 0:
         from gpumlib_amd import serving
