@@ -29,6 +29,15 @@ typedef unsigned int ku32x4 __attribute__((ext_vector_type(4)));
 constexpr int KKL = 128;        // K per chunk: half of a 256-deep block of the tiled store
 constexpr int LDK2 = KKL + 8;   // LDS pitch (bf16) of the K-contiguous tiles: 272 bytes
 
+// The K-contiguous tiles are read 8 bytes per lane (4 k of rows c = 0..31): with a 68-dword
+// pitch, rows c and c + 8 fall on the same LDS banks.  Rows with bit 3 set therefore hold
+// each 8-byte unit at k ^ 4 (the halves of every 16-byte unit swapped), so those two rows
+// differ by 2 banks; a 16-byte unit keeps its place.
+__device__ __forceinline__ int kswz(int row, int k) { return k ^ (((row >> 3) & 1) << 2); }
+__device__ __forceinline__ ku32x4 kswz16(int row, ku32x4 v) {
+    return ((row >> 3) & 1) ? ku32x4{v[2], v[3], v[0], v[1]} : v;
+}
+
 __device__ __forceinline__ f32x16 mfma_kl(ku32x4 a, ku32x4 b, f32x16 acc) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kbf16x8, a), __builtin_bit_cast(kbf16x8, b),
                                                    acc, 0, 0, 0);
@@ -134,12 +143,12 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
         for (int j = 0; j < BL; j++) {
             const int q = tid + THR * j, tb = j / BLB, ql = q - tb * BLB * THR;
             const int col = tb * TILE_LP + ql / (KKL / 8), kq = (ql % (KKL / 8)) * 8;
-            *reinterpret_cast<ku32x4*>(&sB[col][kq]) = rb[j];
+            *reinterpret_cast<ku32x4*>(&sB[col][kq]) = kswz16(col, rb[j]);
         }
 #pragma unroll
         for (int j = 0; j < AKL; j++) {
             const int q = tid + THR * j, a = q / (KKL / 8), kq = (q % (KKL / 8)) * 8;
-            *reinterpret_cast<ku32x4*>(&sAk[a][kq]) = rk[j];
+            *reinterpret_cast<ku32x4*>(&sAk[a][kq]) = kswz16(a, rk[j]);
         }
 #pragma unroll
         for (int j = 0; j < ARL; j++) {
@@ -175,7 +184,8 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
             ku32x4 rv[2];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint2 vv = *reinterpret_cast<const uint2*>(&sB[ub * 32 + c][kb * 32 + 8 * q + 4 * h]);
+                const uint2 vv =
+                    *reinterpret_cast<const uint2*>(&sB[ub * 32 + c][kswz(c, kb * 32 + 8 * q + 4 * h)]);
                 const float v0 = __uint_as_float(vv.x << 16), v1 = __uint_as_float(vv.x & 0xffff0000u);
                 const float v2 = __uint_as_float(vv.y << 16), v3 = __uint_as_float(vv.y & 0xffff0000u);
                 const uint32_t p01 = pack_bf16x2(kl_ratio(v0, wh[4 * q], eps), kl_ratio(v1, wh[4 * q + 1], eps));
@@ -187,9 +197,10 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
             for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
                 for (int st = 0; st < 2; st++) {
-                    const uint2 a0 = *reinterpret_cast<const uint2*>(&sAk[mb * 32 + c][kb * 32 + 16 * st + 4 * h]);
+                    const uint2 a0 =
+                        *reinterpret_cast<const uint2*>(&sAk[mb * 32 + c][kswz(c, kb * 32 + 16 * st + 4 * h)]);
                     const uint2 a1 =
-                        *reinterpret_cast<const uint2*>(&sAk[mb * 32 + c][kb * 32 + 16 * st + 8 + 4 * h]);
+                        *reinterpret_cast<const uint2*>(&sAk[mb * 32 + c][kswz(c, kb * 32 + 16 * st + 8 + 4 * h)]);
                     const ku32x4 av = {a0.x, a0.y, a1.x, a1.y};
                     acc[mb] = mfma_kl(av, rv[st], acc[mb]);
                 }
