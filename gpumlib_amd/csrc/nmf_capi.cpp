@@ -90,11 +90,10 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         precision != GPUMLIB_NMF_FP32X)
         return false;
     // the narrow storage paths run the Euclidean rule; BF16 storage also runs the
-    // multiplicative KL rule (up to r = 64)
+    // multiplicative KL rule (up to r = 128, as the Euclidean rule)
     const bool kl_bf16 = precision == GPUMLIB_NMF_BF16 && rule == GPUMLIB_NMF_KL;
     if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN && !kl_bf16) return false;
     if (n < 1 || m < 1 || r < 1) return false;
-    if (kl_bf16 && r > 64) return false;
     if (r > kMaxFusedRank) {
         if (precision != GPUMLIB_NMF_FP32) return false;
         return make_unfused_plan(rule, n, m, r, pl);
@@ -135,7 +134,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     // columns of V, halving the A loads; one such workgroup fills a CU (101 KB of LDS, the
     // same 8 waves as two 4-wave workgroups), so the pass aims at half the workgroups.  The
     // FP32X passes do the same (one 8-wave workgroup per CU for every rank, up to 153 KB of LDS).
-    pl->lp_nt = ((precision == GPUMLIB_NMF_BF16 || x3) && r <= 64) ? 2 : 1;
+    pl->lp_nt = (((precision == GPUMLIB_NMF_BF16 || x3) && r <= 64) || kl_bf16) ? 2 : 1;
     const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP * pl->lp_nt;  // columns (rows) per workgroup
     const int tgt = x3 ? target / 2 : target / pl->lp_nt;
     const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
@@ -188,7 +187,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         pl->off_Hb = off;
         off += align256((size_t)r * pl->ldt8 * 2);
         if (kl_bf16) {
-            pl->rps = round_up(r, 32);
+            pl->rps = kl_rank_pitch(r);
             pl->off_Wr = off;
             off += align256((size_t)pl->ldv8 * pl->rps * 2);
             pl->off_Hr = off;

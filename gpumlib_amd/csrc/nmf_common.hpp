@@ -344,6 +344,10 @@ struct LpPassArgs {
 // rank-contiguous (Ar[k*rps + a], the A C product); C rank-contiguous (Cr[col*rps + a]).  Both
 // rank-contiguous shadows are zero for a in [r, rps) and past the last row.  Tile 0 of every
 // slice also writes the slice's column sums of A, Psum[s*rpad + a] = sum_k A(k, a).
+// Row pitch of the rank-contiguous shadows of the KL rule with BF16 storage: the rank rounded up
+// to the pass's rank blocks (32, 64 or 128).
+__host__ __device__ inline int kl_rank_pitch(int r) { return r <= 32 ? 32 : (r <= 64 ? 64 : 128); }
+
 struct KlLpArgs {
     const uint16_t* Ak;
     const uint16_t* Ar;

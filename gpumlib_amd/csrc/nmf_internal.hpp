@@ -19,7 +19,7 @@ hipError_t launch_pack_v_bf16(const float* V, int n, int m, uint16_t* Vb, int ld
                               hipStream_t st);
 hipError_t launch_shadow_bf16(const float* W, const float* H, int n, int m, int r, uint16_t* Wb, int ldw, uint16_t* Ht,
                               int ldh, hipStream_t st);
-// KL rule with BF16 storage (nmf_kl_bf16.hip; r <= 64, eps > 0, rps = r rounded up to 32, kslice
+// KL rule with BF16 storage (nmf_kl_bf16.hip; r <= 128, eps > 0, rps = kl_rank_pitch(r), kslice
 // a multiple of 128, nt = 2).
 hipError_t launch_pass_kl_bf16(const KlLpArgs& a, hipStream_t st);
 // Xr[j*rps + a] = bf16(X[j*xs_j + a*xs_a]) for j < count, a < r; zeros up to rows x rps.

@@ -80,10 +80,23 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     constexpr int RQ = RP / 8;                    // 16-byte groups per rank-contiguous row
     static_assert(WPC * NCB == NW && KPW * WPC == NKB, "wave split");
     static_assert(BLB >= 1 && AKL >= 1 && ARL >= 1 && CLD >= 1, "load split");
-    __shared__ __attribute__((aligned(16))) uint16_t sB[TW][LDK2];    // V chunk, [col][k]
-    __shared__ __attribute__((aligned(16))) uint16_t sAk[RP][LDK2];   // A chunk, [a][k]
-    __shared__ __attribute__((aligned(16))) uint16_t sAr[KKL][LDR2];  // A chunk, [k][a]
-    __shared__ __attribute__((aligned(16))) uint16_t sC[TW][LDR2];    // C of the tile, [col][a]
+    // the V chunk [col][k], the A chunk [a][k] and [k][a], the tile's C [col][a]; after the
+    // last chunk the same LDS holds the waves' partial products
+    struct Stage {
+        uint16_t sB[TW][LDK2];
+        uint16_t sAk[RP][LDK2];
+        uint16_t sAr[KKL][LDR2];
+        uint16_t sC[TW][LDR2];
+    };
+    union Lds {
+        Stage st;
+        float red[NCB * MB * 16 * 64];
+    };
+    __shared__ __attribute__((aligned(16))) Lds lds;
+    auto& sB = lds.st.sB;
+    auto& sAk = lds.st.sAk;
+    auto& sAr = lds.st.sAr;
+    auto& sC = lds.st.sC;
 
     const int total = p.tiles * p.S;
     const int w = xcd_work_index(blockIdx.x, gridDim.x);
@@ -166,14 +179,13 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     float csum = 0.f;
 
     const float eps = p.eps;
-    auto compute = [&](auto sums_tag) __attribute__((always_inline)) {
-#pragma unroll
-        for (int j = 0; j < KPW; j++) {
-            const int kb = kb0 + WPC * j;
+    // one 32 x 32 block (k block kb, column block ub) of the chunk
+    auto block = [&](int kb) __attribute__((always_inline)) {
+        {
             f32x16 wh;
 #pragma unroll
             for (int v = 0; v < 16; v++) wh[v] = 0.f;
-#pragma unroll
+#pragma unroll(MB <= 2 ? RP / 16 : 2)
             for (int g = 0; g < RP / 16; g++) {
                 const ku32x4 av = *reinterpret_cast<const ku32x4*>(&sAr[kb * 32 + c][g * 16 + h * 8]);
                 const ku32x4 bv = *reinterpret_cast<const ku32x4*>(&sC[ub * 32 + c][g * 16 + h * 8]);
@@ -197,6 +209,7 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
             for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
                 for (int st = 0; st < 2; st++) {
+                    if constexpr (MB > 2) __builtin_amdgcn_sched_barrier(0);  // no hoisting across rank blocks
                     const uint2 a0 =
                         *reinterpret_cast<const uint2*>(&sAk[mb * 32 + c][kswz(c, kb * 32 + 16 * st + 4 * h)]);
                     const uint2 a1 =
@@ -205,6 +218,16 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
                     acc[mb] = mfma_kl(av, rv[st], acc[mb]);
                 }
             }
+        }
+    };
+    auto compute = [&](auto sums_tag) __attribute__((always_inline)) {
+        if constexpr (MB <= 2) {
+#pragma unroll
+            for (int j = 0; j < KPW; j++) block(kb0 + WPC * j);
+        } else {
+            // four rank blocks: one k block at a time keeps the registers within a wave's share
+#pragma unroll 1
+            for (int j = 0; j < KPW; j++) block(kb0 + WPC * j);
         }
         if constexpr (decltype(sums_tag)::value) {
             if (tid < 4 * RP) {
@@ -253,8 +276,7 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
 
     // the WPC waves of a column block add their products in wave order through the free LDS
     static_assert(WPC == 2, "two waves per column block");
-    static_assert(NCB * MB * 16 * 64 * sizeof(float) <= sizeof(sB), "partials fit in sB");
-    float* red = reinterpret_cast<float*>(&sB[0][0]);
+    float* red = lds.red;
     if (kb0 == 1) {
 #pragma unroll
         for (int mb = 0; mb < MB; mb++)
@@ -301,18 +323,18 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_rank_bf16(const float* __re
 // operand of the other pass) and Xr[j*rps + a] (rank-contiguous).  JT rows per workgroup; X
 // staged through LDS in whichever order its layout reads coalesced; each thread owns (row jj,
 // 4 consecutive a) items and issues all its slice loads (eight slices at a time) before adding.
-template <int JT>
+template <int JT, int RMAX>
 __global__ __launch_bounds__(NTHREADS) void k_update_kl_rows_bf16(float* __restrict__ X, long xs_j, long xs_a,
                                                                   const float* __restrict__ P,
                                                                   const float* __restrict__ Psum, int count, int r,
                                                                   int ldr, int rpad, int S, long pstride, float eps,
                                                                   uint16_t* __restrict__ Xk, int ldk,
                                                                   uint16_t* __restrict__ Xr, int rps) {
-    constexpr int XS = 64 + 1;                     // LDS row pitch (r <= 64)
-    constexpr int MAXI = (JT * 16 + NTHREADS - 1) / NTHREADS;
-    constexpr int XU = (JT * 64 + NTHREADS - 1) / NTHREADS;
+    constexpr int XS = RMAX + 1;                   // LDS row pitch (r <= RMAX)
+    constexpr int MAXI = (JT * RMAX / 4 + NTHREADS - 1) / NTHREADS;
+    constexpr int XU = (JT * RMAX + NTHREADS - 1) / NTHREADS;
     __shared__ float sX[JT * XS];
-    __shared__ float ssum[64];
+    __shared__ float ssum[RMAX];
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * JT, nj = min(JT, count - j0);
     const int rq = ldr >> 2;
@@ -411,12 +433,14 @@ static void launch_kl(const KlLpArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_pass_kl_bf16(const KlLpArgs& a, hipStream_t st) {
-    if (a.r < 1 || a.r > 64 || a.nt != 2 || a.kslice % KKL || a.rps != round_up(a.r, 32) || !(a.eps > 0.f))
+    if (a.r < 1 || a.r > 128 || a.nt != 2 || a.kslice % KKL || a.rps != kl_rank_pitch(a.r) || !(a.eps > 0.f))
         return hipErrorInvalidValue;
     if (a.rps == 32)
         launch_kl<1, 2>(a, st);
-    else
+    else if (a.rps == 64)
         launch_kl<2, 2>(a, st);
+    else
+        launch_kl<4, 2>(a, st);
     return hipGetLastError();
 }
 
@@ -434,15 +458,18 @@ hipError_t launch_shadow_rank_bf16(const float* X, long xs_j, long xs_a, int cou
 hipError_t launch_update_kl_rows_bf16(float* X, long xs_j, long xs_a, const float* P, const float* Psum, int count,
                                       int r, int ldr, int rpad, int S, long pstride, float eps, uint16_t* Xk, int ldk,
                                       uint16_t* Xr, int rps, hipStream_t st) {
-    if (r < 1 || r > 64 || ldr % 4 || ldr < r || rpad < r || pstride % 4 || S < 1) return hipErrorInvalidValue;
-    // 32 rows per workgroup for long factors (at least 256 workgroups), else 16
-    if (count >= 8192) {
-        hipLaunchKernelGGL(k_update_kl_rows_bf16<32>, dim3((count + 31) / 32), dim3(NTHREADS), 0, st, X, xs_j, xs_a, P,
-                           Psum, count, r, ldr, rpad, S, pstride, eps, Xk, ldk, Xr, rps);
-    } else {
-        hipLaunchKernelGGL(k_update_kl_rows_bf16<16>, dim3((count + 15) / 16), dim3(NTHREADS), 0, st, X, xs_j, xs_a, P,
-                           Psum, count, r, ldr, rpad, S, pstride, eps, Xk, ldk, Xr, rps);
-    }
+    if (r < 1 || r > 128 || ldr % 4 || ldr < r || rpad < r || pstride % 4 || S < 1) return hipErrorInvalidValue;
+    // r <= 64: 32 rows per workgroup for long factors (at least 256 workgroups), else 16; above, 16
+#define KLU(JT, RMAX)                                                                                             \
+    hipLaunchKernelGGL((k_update_kl_rows_bf16<JT, RMAX>), dim3((count + JT - 1) / JT), dim3(NTHREADS), 0, st, X, \
+                       xs_j, xs_a, P, Psum, count, r, ldr, rpad, S, pstride, eps, Xk, ldk, Xr, rps)
+    if (r > 64)
+        KLU(16, 128);
+    else if (count >= 8192)
+        KLU(32, 64);
+    else
+        KLU(16, 64);
+#undef KLU
     return hipGetLastError();
 }
 

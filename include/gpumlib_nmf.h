@@ -174,7 +174,9 @@ enum {
  * boundary) in FP32, tiled the same way in blocks of 64 columns x 128 contraction elements (element (k, col) at
  * ((k/128 * T + col/64) * 64 + col%64) * 128 + k%128, the contraction padded to a multiple of 128 and the
  * columns to 64 T with zeros).  _pack builds the store from an FP32 device V on `stream`.  BF16, FP8 and
- * FP32X storage run the Euclidean rule; FP8 supports r <= 64, BF16 and FP32X r <= 128. */
+ * FP32X storage run the Euclidean rule; FP8 supports r <= 64, BF16 and FP32X r <= 128.  BF16 storage also
+ * runs the multiplicative KL rule (GPUMLIB_NMF_KL, r <= 128, eps > 0: W H formed per block from BF16 copies
+ * of W and H, the FP32 ratio rounded to BF16 for the second product, FP32 sums and updates). */
 size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
                                    hipStream_t stream);

@@ -55,17 +55,17 @@ def test_workspace_bytes(native):
 
 
 def test_kl_bf16_plan(native):
-    """KL with BF16 storage plans ranks up to 64 (the multiplicative rule only); its workspace
-    adds the rank-contiguous BF16 shadows of W and H (rows of r rounded up to 32) to what
-    the Euclidean BF16 path holds; eps must be positive (refused before any device work)."""
+    """KL with BF16 storage plans ranks up to 128 (the multiplicative rule only); its workspace
+    adds the rank-contiguous BF16 shadows of W and H (rows of r rounded up to 32, 64 or 128) to
+    what the Euclidean BF16 path holds; eps must be positive (refused before any device work)."""
     import ctypes
 
     n, m = 1000, 700
-    for r in (1, 17, 32, 33, 64):
+    for r in (1, 17, 32, 33, 64, 65, 100, 128):
         kl = native.gpumlib_nmf_workspace_bytes(capi.KL, capi.BF16, n, m, r)
-        rps = (r + 31) // 32 * 32
+        rps = 32 if r <= 32 else (64 if r <= 64 else 128)
         assert kl >= 2 * rps * (1024 + 768), r
-    for bad in [(capi.KL, capi.BF16, n, m, 65), (capi.ADDITIVE_KL, capi.BF16, n, m, 8), (capi.KL, capi.FP8, n, m, 8),
+    for bad in [(capi.KL, capi.BF16, n, m, 129), (capi.ADDITIVE_KL, capi.BF16, n, m, 8), (capi.KL, capi.FP8, n, m, 8),
                 (capi.KL, capi.FP32X, n, m, 8)]:
         assert native.gpumlib_nmf_workspace_bytes(*bad) == 0
     fake = ctypes.c_void_p(0x1000)  # never dereferenced: eps is checked first

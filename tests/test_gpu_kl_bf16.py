@@ -51,8 +51,10 @@ def _gaps(orc, n, m, r, iters, seed, r0=None, update_w=True):
     return out, p, W
 
 
+# ranks on one, two and four rank blocks of 32 (r = 65 and 128 bound the four-block kernel)
 @pytest.mark.parametrize("shape", [(97, 61, 5), (130, 257, 33), (300, 200, 64), (1, 9, 1), (4100, 90, 32),
-                                   (70, 4100, 20), (129, 300, 48), (600, 700, 64), (2000, 130, 17)])
+                                   (70, 4100, 20), (129, 300, 48), (600, 700, 64), (2000, 130, 17),
+                                   (129, 300, 65), (300, 200, 100), (600, 500, 128)])
 def test_kl_bf16_shapes(native, orc, cuda, shape):
     n, m, r = shape
     g, _, _ = _gaps(orc, n, m, r, 20, seed=n + 3 * m + r)

@@ -25,7 +25,7 @@ def test_noalloc_rejects_oversize_and_bad_problems(native):
     assert native.gpumlib_nmf_iterate_noalloc(*args) == HIP_ERROR_OUT_OF_MEMORY
     assert native.gpumlib_nmf_workspace_bytes(G.EUCLIDEAN, G.FP32, 16384, 8192, 64) <= SCRATCH_BYTES
     for bad in [(G.EUCLIDEAN, G.FP32, 10, 10, 0), (7, G.FP32, 10, 10, 4), (G.KL, G.FP8, 10, 10, 4),
-                (G.KL, G.BF16, 10, 10, 65)]:
+                (G.KL, G.BF16, 10, 10, 129)]:
         a = (*bad, None, None, None, 1e-8, 1, 1, G.OBJ_NONE, None, None)
         assert native.gpumlib_nmf_iterate_noalloc(*a) == 1  # hipErrorInvalidValue
 
