@@ -41,13 +41,15 @@ class ServerStats(ctypes.Structure):
                 ("iterations_batched", _c.c_longlong), ("peak_active", _c.c_int), ("pool_bytes", _c.c_size_t),
                 ("pool_high_water", _c.c_size_t), ("autotunes", _c.c_longlong), ("tuned_target", _c.c_int),
                 ("tuned_pass_grid", _c.c_int), ("tuned_update_grid", _c.c_int), ("tuned_engine", _c.c_int),
-                ("tuned_split_ranks", _c.c_int)]
+                ("tuned_split_ranks", _c.c_int), ("tuned_bucket_targets", _c.c_int * 3),
+                ("tuned_bucket_tile_cols", _c.c_int * 3), ("tuned_bucket_split_ranks", _c.c_int * 3)]
 
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("engine", _c.c_int), ("target_workgroups", _c.c_int), ("pass_grid", _c.c_int), ("update_grid", _c.c_int),
                 ("ms_per_iteration", _c.c_double), ("ms_per_iteration_before", _c.c_double), ("trials", _c.c_int),
-                ("cached", _c.c_int), ("split_ranks", _c.c_int)]
+                ("cached", _c.c_int), ("split_ranks", _c.c_int), ("bucket_targets", _c.c_int * 3),
+                ("bucket_tile_cols", _c.c_int * 3), ("bucket_split_ranks", _c.c_int * 3)]
 
 
 # name -> (restype, argtypes)
@@ -97,6 +99,7 @@ SIGNATURES = {
     "gpumlib_nmf_batch_engine": (_i, [_vp]),
     "gpumlib_nmf_batch_set_rank_split": (_i, [_vp, _i]),
     "gpumlib_nmf_batch_rank_split": (_i, [_vp]),
+    "gpumlib_nmf_batch_set_tile_cols": (_i, [_vp, _i]),
     "gpumlib_nmf_batch_workspace_bytes": (_sz, [_vp, _i, _i, _i, _i]),
     "gpumlib_nmf_batch_add": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _f, _vp, _sz, _c.POINTER(_i)]),
     "gpumlib_nmf_batch_remove": (_i, [_vp, _i]),

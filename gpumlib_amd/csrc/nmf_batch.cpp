@@ -8,10 +8,12 @@
 // the set of problems changes.  Because the launches read everything from that
 // table, a hipGraph captured once stays valid across changes of the set.
 //
-// Autotuner (gpumlib_nmf_batch_autotune): the tile shape along K (how many K slices,
-// hence how deep each pass work item is, set through the per-problem workgroup
-// target) and the persistent grid sizes of the grouped launches are chosen by timing
-// trial iterations of the current mix on the device.  W and H are saved before and
+// Autotuner (gpumlib_nmf_batch_autotune): the tile shape of the pass work items -- along K
+// (the engine: whole contraction per item, or K slices whose depth follows the per-problem
+// workgroup target), across the columns (H pass) / rows (W pass) of V (128 or 64 per item)
+// and across the ranks (r <= 32 on the one-rank-block kernels, or not), each chosen per
+// shape bucket of problems -- and the persistent grid sizes of the grouped launches are
+// chosen by timing trial iterations of the current mix on the device.  W and H are saved before and
 // restored after every trial, so tuning changes no problem's state; the winner is
 // cached per mix signature.
 #include <hip/hip_runtime.h>
@@ -35,17 +37,17 @@ namespace {
 // Launch tables.  Split-K engine: the passes (per rule) and the updates.  Fused engine:
 // the grams of W and H (Euclidean problems) and the fused passes (per rule).  Both: the
 // per-problem objective reduction.
-// The pass tables come in two rank classes: the problems run by the kernels built for two
-// rank blocks of 32 (r <= 64), and, when ranks are split, those with r <= 32 run by the
-// kernels built for one block (T_*1: half the accumulator registers and operand LDS).
-enum {
-    T_PH_EUC = 0, T_PH_KL, T_PW_EUC, T_PW_KL, T_UH, T_UW, T_RED,
-    T_GW, T_GH, T_FH_EUC, T_FH_KL, T_FW_EUC, T_FW_KL,
-    T_PH_EUC1, T_PH_KL1, T_PW_EUC1, T_PW_KL1, T_FH_EUC1, T_FH_KL1, T_FW_EUC1, T_FW_KL1, T_COUNT
-};
-// the two-block table of a one-block table kind (or the kind itself)
-constexpr int base_kind(int k) { return k >= T_PH_EUC1 ? (k < T_FH_EUC1 ? k - T_PH_EUC1 : k - T_FH_EUC1 + T_FH_EUC) : k; }
-constexpr bool small_kind(int k) { return k >= T_PH_EUC1; }
+// Every pass table comes in four variants, one per pass kernel shape: bit 0 set, the kernels
+// built for one rank block of 32 (problems with r <= 32 when their bucket splits ranks: half
+// the accumulator registers and operand LDS), else two (r <= 64); bit 1 set, 64-column (row)
+// tiles, else 128.  A problem sits in the variant of its shape.
+enum { T_UH = 0, T_UW, T_RED, T_GW, T_GH, T_PASS0 };
+enum { P_PH_EUC = 0, P_PH_KL, P_PW_EUC, P_PW_KL, P_FH_EUC, P_FH_KL, P_FW_EUC, P_FW_KL, P_COUNT };
+constexpr int kVariants = 4;
+constexpr int T_COUNT = T_PASS0 + P_COUNT * kVariants;
+constexpr int pass_table(int pk, int v) { return T_PASS0 + pk * kVariants + v; }
+constexpr int variant_mb(int v) { return (v & 1) ? 1 : 2; }
+constexpr int variant_tile(int v) { return (v & 2) ? 64 : TILE; }
 enum { ENGINE_SPLIT = 0, ENGINE_FUSED = 1 };
 constexpr int kPassGrid = 1024;  // persistent workgroups of the grouped passes (before tuning)
 constexpr int kUpdGrid = 512;
@@ -56,12 +58,30 @@ constexpr int kBatchSizingTarget = 128;     // workspaces are sized for targets 
 // Candidates of the autotuner.  Targets above the batch's sizing target are skipped
 // (a problem's workspace would be too small for that many K slices).
 constexpr int kTuneTargets[] = {4, 8, 16, 32, 64, 128};
+// Shape buckets of the K-split target: problems of fewer than 2^18 elements of V, fewer than
+// 2^21, and the rest (the serving mix spans 4 K to 8 M elements).  Each bucket's problems are
+// planned with their own target, so small and large problems can take different K-tile depths.
+constexpr int kBuckets = 3;
+int shape_bucket(int n, int m) {
+    const long e = (long)n * m;
+    return e < (1L << 18) ? 0 : (e < (1L << 21) ? 1 : 2);
+}
 constexpr int kTunePassGrids[] = {512, 1024, 2048};
 constexpr int kTuneUpdGrids[] = {256, 512, 1024};
 
+// Tile shape of one shape bucket's problems.
+struct BucketCfg {
+    int target = kBatchTargetWorkgroups;  // K-split target (split-K engine)
+    int tile = TILE;                      // columns (H pass) / rows (W pass) per work item
+    int split = 0;                        // 1: problems with r <= 32 run on the one-rank-block kernels
+    bool operator==(const BucketCfg& o) const { return target == o.target && tile == o.tile && split == o.split; }
+};
+typedef BucketCfg BucketSet[kBuckets];
+
 struct Slot {
     bool active = false;
-    int rule = 0, n = 0, m = 0, r = 0;
+    int rule = 0, n = 0, m = 0, r = 0, bucket = 0;
+    int tile = TILE;  // tile width the problem is planned with
     const float* V = nullptr;
     float *W = nullptr, *H = nullptr;
     void* ws = nullptr;
@@ -71,7 +91,8 @@ struct Slot {
 };
 
 struct TuneConfig {
-    int engine, target, pass_grid, upd_grid, split_ranks;
+    int engine, pass_grid, upd_grid;
+    BucketSet bk;
     double ms_per_iteration, ms_default;
     int trials;
 };
@@ -81,8 +102,8 @@ struct TuneConfig {
 // partial per column tile of the H pass (inside the split-K plan's workspace: the fused
 // engine never reads the split-K partials and the split-K passes rewrite them).
 size_t fused_objp_offset(int r) { return (2 * (size_t)r * round_up(r, 4) * sizeof(float) + 255) / 256 * 256; }
-size_t fused_bytes(int m, int r) {
-    return fused_objp_offset(r) + ((size_t)((m + TILE - 1) / TILE) * sizeof(double) + 255) / 256 * 256;
+size_t fused_bytes(int m, int r, int tile) {
+    return fused_objp_offset(r) + ((size_t)((m + tile - 1) / tile) * sizeof(double) + 255) / 256 * 256;
 }
 
 }  // namespace
@@ -90,10 +111,9 @@ size_t fused_bytes(int m, int r) {
 struct gpumlib_nmf_batch {
     int maxj = 0;
     int engine = ENGINE_FUSED;             // how the problems are planned and launched
-    int target = kBatchTargetWorkgroups;   // K-split target the problems are planned with
+    BucketSet bk;                          // tile shape per shape bucket (new problems take their bucket's)
     int size_target = kBatchSizingTarget;  // target their workspaces are sized for
     int pass_grid = kPassGrid, upd_grid = kUpdGrid;
-    int split_ranks = 0;  // 1: problems with r <= 32 run on the one-rank-block pass kernels
     std::map<unsigned long long, TuneConfig> tuned;  // winners by mix signature
     std::vector<Slot> slots;
     JobArgs* d_jobs = nullptr;
@@ -116,67 +136,63 @@ static void wait_staging(gpumlib_nmf_batch* b) {
     }
 }
 
+// Work items of slot sl in a table of base kind kb (0: not in it), and the key the fused tables
+// order their problems by (longest first: the grid strides over them in order).
+static int table_items(const gpumlib_nmf_batch* b, const Slot& sl, int kind, long* cost) {
+    const bool fused = b->engine == ENGINE_FUSED;
+    const bool euc = sl.rule == GPUMLIB_NMF_EUCLIDEAN;
+    *cost = 0;
+    switch (kind) {
+        case T_UH: return !fused ? (sl.m + UPD_COLS - 1) / UPD_COLS : 0;
+        case T_UW: return !fused ? (sl.n + UPD_ROWS - 1) / UPD_ROWS : 0;
+        case T_RED: return 1;
+        case T_GW: *cost = (long)sl.n * sl.r; return fused && euc ? 1 : 0;
+        case T_GH: *cost = (long)sl.m * sl.r; return fused && euc ? 1 : 0;
+        default: break;
+    }
+    const int pk = (kind - T_PASS0) / kVariants, v = (kind - T_PASS0) % kVariants;
+    const bool small = b->bk[sl.bucket].split && sl.r <= 32;
+    if (small != ((v & 1) != 0) || sl.tile != variant_tile(v)) return 0;
+    switch (pk) {
+        case P_PH_EUC: return !fused && euc ? sl.args.h.tiles * sl.args.h.S : 0;
+        case P_PH_KL: return !fused && !euc ? sl.args.h.tiles * sl.args.h.S : 0;
+        case P_PW_EUC: return !fused && euc ? sl.args.w.tiles * sl.args.w.S : 0;
+        case P_PW_KL: return !fused && !euc ? sl.args.w.tiles * sl.args.w.S : 0;
+        case P_FH_EUC: *cost = sl.n; return fused && euc ? sl.args.h.tiles : 0;
+        case P_FH_KL: *cost = sl.n; return fused && !euc ? sl.args.h.tiles : 0;
+        case P_FW_EUC: *cost = sl.m; return fused && euc ? sl.args.w.tiles : 0;
+        case P_FW_KL: *cost = sl.m; return fused && !euc ? sl.args.w.tiles : 0;
+    }
+    return 0;
+}
+
 // Rebuilds the launch tables from the active slots and stages them to the device.
 static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
     if (!b->dirty) return hipSuccess;
     wait_staging(b);
     const int stride = table_stride();
     memset(b->h_tables, 0, sizeof(int) * (size_t)stride * T_COUNT);
-    const bool fused = b->engine == ENGINE_FUSED;
-    std::vector<std::pair<long, int>> order;  // (cost, slot)
+    std::vector<std::pair<long, int>> order;  // (-cost, slot)
+    std::vector<int> count(b->maxj);
     for (int k = 0; k < T_COUNT; k++) {
         int* t = b->h_tables + (size_t)k * stride;
         int J = 0, T = 0;
         int* first = t + 2;
         int* idx = t + 3 + kGroupMaxJobs;
         order.clear();
-        const int kb = base_kind(k);
         for (int s = 0; s < b->maxj; s++) {
             const Slot& sl = b->slots[s];
             if (!sl.active) continue;
-            const bool euc = sl.rule == GPUMLIB_NMF_EUCLIDEAN;
-            // which rank class of the pass tables the problem goes to
-            if (kb <= T_PW_KL || (kb >= T_FH_EUC && kb <= T_FW_KL)) {
-                const bool small = b->split_ranks && sl.r <= 32;
-                if (small != small_kind(k)) continue;
-            }
-            int items = 0;
-            long cost = 0;  // fused tables run longest items first (the grid strides over them in order)
-            switch (kb) {
-                case T_PH_EUC: items = !fused && euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
-                case T_PH_KL: items = !fused && !euc ? sl.args.h.tiles * sl.args.h.S : 0; break;
-                case T_PW_EUC: items = !fused && euc ? sl.args.w.tiles * sl.args.w.S : 0; break;
-                case T_PW_KL: items = !fused && !euc ? sl.args.w.tiles * sl.args.w.S : 0; break;
-                case T_UH: items = !fused ? (sl.m + UPD_COLS - 1) / UPD_COLS : 0; break;
-                case T_UW: items = !fused ? (sl.n + UPD_ROWS - 1) / UPD_ROWS : 0; break;
-                case T_RED: items = 1; break;
-                case T_GW: items = fused && euc ? 1 : 0, cost = (long)sl.n * sl.r; break;
-                case T_GH: items = fused && euc ? 1 : 0, cost = (long)sl.m * sl.r; break;
-                case T_FH_EUC: items = fused && euc ? sl.args.h.tiles : 0, cost = sl.n; break;
-                case T_FH_KL: items = fused && !euc ? sl.args.h.tiles : 0, cost = sl.n; break;
-                case T_FW_EUC: items = fused && euc ? sl.args.w.tiles : 0, cost = sl.m; break;
-                case T_FW_KL: items = fused && !euc ? sl.args.w.tiles : 0, cost = sl.m; break;
-            }
-            if (items == 0) continue;
-            order.emplace_back(-cost, s);
+            long cost = 0;
+            count[s] = table_items(b, sl, k, &cost);
+            if (count[s] > 0) order.emplace_back(-cost, s);
         }
         std::stable_sort(order.begin(), order.end(),
                          [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first < y.first; });
         for (const auto& o : order) {
-            const Slot& sl = b->slots[o.second];
-            int items = 0;
-            switch (kb) {
-                case T_PH_EUC: case T_PH_KL: items = sl.args.h.tiles * sl.args.h.S; break;
-                case T_PW_EUC: case T_PW_KL: items = sl.args.w.tiles * sl.args.w.S; break;
-                case T_UH: items = (sl.m + UPD_COLS - 1) / UPD_COLS; break;
-                case T_UW: items = (sl.n + UPD_ROWS - 1) / UPD_ROWS; break;
-                case T_RED: case T_GW: case T_GH: items = 1; break;
-                case T_FH_EUC: case T_FH_KL: items = sl.args.h.tiles; break;
-                case T_FW_EUC: case T_FW_KL: items = sl.args.w.tiles; break;
-            }
             first[J] = T;
             idx[J] = o.second;
-            T += items;
+            T += count[o.second];
             J++;
         }
         first[J] = T;
@@ -194,11 +210,31 @@ static hipError_t upload(gpumlib_nmf_batch* b, hipStream_t st) {
     return hipSuccess;
 }
 
+// Pass-table variants launched: those some bucket's configuration can put a problem in (whether
+// or not one is there now, so that a captured graph stays valid while problems come and go).
+static unsigned variants_launched(const gpumlib_nmf_batch* b) {
+    unsigned set = 0;
+    for (int k = 0; k < kBuckets; k++) {
+        const int v = b->bk[k].tile == 64 ? 2 : 0;
+        set |= 1u << v;
+        if (b->bk[k].split) set |= 1u << (v | 1);
+    }
+    return set;
+}
+
 static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, hipStream_t st) {
     const int stride = table_stride();
     const int* T = b->d_tables;
     const int pg = b->pass_grid, ug = b->upd_grid;
-    const bool sr = b->split_ranks != 0;  // the one-rank-block tables are launched only when used
+    const unsigned vs = variants_launched(b);
+    auto tab = [&](int pk, int v) { return T + (size_t)pass_table(pk, v) * stride; };
+    // every launched variant of one pass kind
+#define EACH_VARIANT(call)                          \
+    for (int v = 0; v < kVariants; v++) {           \
+        if (!((vs >> v) & 1)) continue;             \
+        const int mb = variant_mb(v), tw = variant_tile(v); \
+        if ((e = (call)) != hipSuccess) return e;   \
+    }
     if (b->engine == ENGINE_FUSED) {
         // per iteration: gram of W; fused H passes (objective of the entry state fused in);
         // gram of the new H; fused W passes
@@ -207,16 +243,12 @@ static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, 
             const int oe = obj ? OBJ_FROB : OBJ_NONE, ok = obj ? OBJ_KL : OBJ_NONE;
             hipError_t e;
             if ((e = launch_group_gram(b->d_jobs, T + T_GW * stride, 0, ug, st))) return e;
-            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC * stride, 0, oe, 2, pg, st))) return e;
-            if (sr && (e = launch_group_fused_h(b->d_jobs, T + T_FH_EUC1 * stride, 0, oe, 1, pg, st))) return e;
-            if ((e = launch_group_fused_h(b->d_jobs, T + T_FH_KL * stride, 1, ok, 2, pg, st))) return e;
-            if (sr && (e = launch_group_fused_h(b->d_jobs, T + T_FH_KL1 * stride, 1, ok, 1, pg, st))) return e;
+            EACH_VARIANT(launch_group_fused_h(b->d_jobs, tab(P_FH_EUC, v), 0, oe, mb, tw, pg, st))
+            EACH_VARIANT(launch_group_fused_h(b->d_jobs, tab(P_FH_KL, v), 1, ok, mb, tw, pg, st))
             if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
             if ((e = launch_group_gram(b->d_jobs, T + T_GH * stride, 1, ug, st))) return e;
-            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC * stride, 0, 2, pg, st))) return e;
-            if (sr && (e = launch_group_fused_w(b->d_jobs, T + T_FW_EUC1 * stride, 0, 1, pg, st))) return e;
-            if ((e = launch_group_fused_w(b->d_jobs, T + T_FW_KL * stride, 1, 2, pg, st))) return e;
-            if (sr && (e = launch_group_fused_w(b->d_jobs, T + T_FW_KL1 * stride, 1, 1, pg, st))) return e;
+            EACH_VARIANT(launch_group_fused_w(b->d_jobs, tab(P_FW_EUC, v), 0, mb, tw, pg, st))
+            EACH_VARIANT(launch_group_fused_w(b->d_jobs, tab(P_FW_KL, v), 1, mb, tw, pg, st))
         }
         return hipSuccess;
     }
@@ -224,26 +256,23 @@ static hipError_t enqueue(gpumlib_nmf_batch* b, int iterations, int objectives, 
         const bool obj = it < objectives;
         const int oe = obj ? OBJ_FROB : OBJ_NONE, ok = obj ? OBJ_KL : OBJ_NONE;
         hipError_t e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC * stride, 0, oe, 2, pg, st))) return e;
-        if (sr && (e = launch_group_pass_h(b->d_jobs, T + T_PH_EUC1 * stride, 0, oe, 1, pg, st))) return e;
-        if ((e = launch_group_pass_h(b->d_jobs, T + T_PH_KL * stride, 1, ok, 2, pg, st))) return e;
-        if (sr && (e = launch_group_pass_h(b->d_jobs, T + T_PH_KL1 * stride, 1, ok, 1, pg, st))) return e;
+        EACH_VARIANT(launch_group_pass_h(b->d_jobs, tab(P_PH_EUC, v), 0, oe, mb, tw, pg, st))
+        EACH_VARIANT(launch_group_pass_h(b->d_jobs, tab(P_PH_KL, v), 1, ok, mb, tw, pg, st))
         if (obj && (e = launch_group_reduce_obj(b->d_jobs, T + T_RED * stride, it, kRedGrid, st))) return e;
         if ((e = launch_group_update_h(b->d_jobs, T + T_UH * stride, ug, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC * stride, 0, 2, pg, st))) return e;
-        if (sr && (e = launch_group_pass_w(b->d_jobs, T + T_PW_EUC1 * stride, 0, 1, pg, st))) return e;
-        if ((e = launch_group_pass_w(b->d_jobs, T + T_PW_KL * stride, 1, 2, pg, st))) return e;
-        if (sr && (e = launch_group_pass_w(b->d_jobs, T + T_PW_KL1 * stride, 1, 1, pg, st))) return e;
+        EACH_VARIANT(launch_group_pass_w(b->d_jobs, tab(P_PW_EUC, v), 0, mb, tw, pg, st))
+        EACH_VARIANT(launch_group_pass_w(b->d_jobs, tab(P_PW_KL, v), 1, mb, tw, pg, st))
         if ((e = launch_group_update_w(b->d_jobs, T + T_UW * stride, ug, st))) return e;
     }
+#undef EACH_VARIANT
     return hipSuccess;
 }
 
-// Plans slot sl for K-split target `target` in its own workspace and zeroes the
-// arrival counters at the plan's offset (stream-ordered, outside graph capture).
-static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStream_t st) {
+// Plans slot sl with its bucket's tile shape c in its own workspace and zeroes the arrival
+// counters at the plan's offset (stream-ordered, outside graph capture).
+static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, const BucketCfg& c, hipStream_t st) {
     Plan pl;
-    if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target) || pl.bytes > sl.ws_bytes)
+    if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, c.target, c.tile) || pl.bytes > sl.ws_bytes)
         return hipErrorInvalidValue;
     JobArgs a;
     memset(&a, 0, sizeof(a));
@@ -254,8 +283,9 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStrea
     a.obj = sl.args.obj;
     a.rule = sl.rule;
     a.nobj = pl.tiles1 * pl.S1;
+    sl.tile = c.tile;
     if (b->engine == ENGINE_FUSED) {
-        if (fused_bytes(sl.m, sl.r) > sl.ws_bytes) return hipErrorInvalidValue;
+        if (fused_bytes(sl.m, sl.r, c.tile) > sl.ws_bytes) return hipErrorInvalidValue;
         char* base = (char*)sl.ws;
         const size_t gsz = (size_t)sl.r * round_up(sl.r, 4);
         for (PassArgs* p : {&a.h, &a.w}) {
@@ -264,11 +294,11 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStrea
             p->Psum = nullptr;
             p->gcount = nullptr;
         }
-        a.h.tiles = (sl.m + TILE - 1) / TILE;
+        a.h.tiles = (sl.m + c.tile - 1) / c.tile;
         a.h.kslice = round_up(sl.n, KC);
         a.h.G = (float*)base;  // W^T W
         a.h.objp = (double*)(base + fused_objp_offset(sl.r));
-        a.w.tiles = (sl.n + TILE - 1) / TILE;
+        a.w.tiles = (sl.n + c.tile - 1) / c.tile;
         a.w.kslice = round_up(sl.m, KC);
         a.w.G = (float*)base + gsz;  // H H^T
         a.w.objp = nullptr;
@@ -282,15 +312,15 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, int target, hipStrea
     return hipMemsetAsync(a.h.gcount, 0, 2 * sizeof(int), st);
 }
 
-static hipError_t replan_all(gpumlib_nmf_batch* b, int engine, int target, hipStream_t st) {
+static hipError_t replan_all(gpumlib_nmf_batch* b, int engine, const BucketSet& bk, hipStream_t st) {
     b->engine = engine;
+    for (int k = 0; k < kBuckets; k++) b->bk[k] = bk[k];
+    b->dirty = true;
     for (Slot& sl : b->slots) {
         if (!sl.active) continue;
-        const hipError_t e = plan_slot(b, sl, target, st);
+        const hipError_t e = plan_slot(b, sl, bk[sl.bucket], st);
         if (e != hipSuccess) return e;
     }
-    b->target = target;
-    b->dirty = true;
     return hipSuccess;
 }
 
@@ -375,8 +405,8 @@ void gpumlib_nmf_batch_destroy(gpumlib_nmf_batch_handle b) {
 
 hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgroups_per_pass) {
     if (!b || workgroups_per_pass < 1) return hipErrorInvalidValue;
-    b->target = workgroups_per_pass;
     b->size_target = workgroups_per_pass;
+    for (int k = 0; k < kBuckets; k++) b->bk[k].target = workgroups_per_pass;
     return hipSuccess;
 }
 
@@ -385,7 +415,7 @@ hipError_t gpumlib_nmf_batch_set_engine(gpumlib_nmf_batch_handle b, int engine) 
     if (engine == b->engine) return hipSuccess;
     wait_staging(b);
     drop_graphs(b);
-    hipError_t e = replan_all(b, engine, b->target, nullptr);
+    hipError_t e = replan_all(b, engine, b->bk, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     return e;
 }
@@ -394,22 +424,51 @@ int gpumlib_nmf_batch_engine(gpumlib_nmf_batch_handle b) { return b ? b->engine 
 
 hipError_t gpumlib_nmf_batch_set_rank_split(gpumlib_nmf_batch_handle b, int on) {
     if (!b || (on != 0 && on != 1)) return hipErrorInvalidValue;
-    if (on == b->split_ranks) return hipSuccess;
+    bool same = true;
+    for (int k = 0; k < kBuckets; k++) same = same && b->bk[k].split == on;
+    if (same) return hipSuccess;
     wait_staging(b);
     drop_graphs(b);
-    b->split_ranks = on;
+    for (int k = 0; k < kBuckets; k++) b->bk[k].split = on;
     b->dirty = true;
     return hipSuccess;
 }
 
-int gpumlib_nmf_batch_rank_split(gpumlib_nmf_batch_handle b) { return b ? b->split_ranks : -1; }
+int gpumlib_nmf_batch_rank_split(gpumlib_nmf_batch_handle b) {
+    if (!b) return -1;
+    int any = 0;
+    for (int k = 0; k < kBuckets; k++) any |= b->bk[k].split;
+    return any;
+}
+
+hipError_t gpumlib_nmf_batch_set_tile_cols(gpumlib_nmf_batch_handle b, int cols) {
+    if (!b || (cols != 64 && cols != TILE)) return hipErrorInvalidValue;
+    bool same = true;
+    for (int k = 0; k < kBuckets; k++) same = same && b->bk[k].tile == cols;
+    if (same) return hipSuccess;
+    wait_staging(b);
+    drop_graphs(b);
+    BucketSet bk;
+    for (int k = 0; k < kBuckets; k++) {
+        bk[k] = b->bk[k];
+        bk[k].tile = cols;
+    }
+    hipError_t e = replan_all(b, b->engine, bk, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
+}
 
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r) {
-    Plan pl;
-    if (!b || r > 64 || (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) ||
-        !make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, std::max(b->target, b->size_target)))
-        return 0;
-    return pl.bytes;
+    if (!b || r > 64 || (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL)) return 0;
+    // the largest plan of the targets up to the sizing target, at either tile width
+    size_t most = 0;
+    for (int tile : {TILE, 64}) {
+        Plan pl;
+        if (!make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, std::max(kBatchTargetWorkgroups, b->size_target), tile))
+            return 0;
+        most = std::max(most, std::max(pl.bytes, fused_bytes(m, r, tile)));
+    }
+    return most;
 }
 
 hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r, const float* V, float* W,
@@ -417,7 +476,9 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     Plan pl;
     if (!b || !slot || !V || !W || !H || !workspace || r > 64) return hipErrorInvalidValue;
     if (rule != GPUMLIB_NMF_EUCLIDEAN && rule != GPUMLIB_NMF_KL) return hipErrorInvalidValue;  // multiplicative only
-    if (!make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->target) || workspace_bytes < pl.bytes)
+    const int bucket = shape_bucket(n, m);
+    if (!make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, b->bk[bucket].target, b->bk[bucket].tile) ||
+        workspace_bytes < pl.bytes)
         return hipErrorInvalidValue;
     int s = 0;
     while (s < b->maxj && b->slots[s].active) s++;
@@ -428,6 +489,7 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     sl.n = n;
     sl.m = m;
     sl.r = r;
+    sl.bucket = bucket;
     sl.V = V;
     sl.W = W;
     sl.H = H;
@@ -436,7 +498,7 @@ hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, in
     sl.eps = eps;
     sl.args.obj = b->d_obj + 2 * (size_t)s;
     // split-K: the gram arrival counters start at zero (null stream, outside any captured graph)
-    hipError_t e = plan_slot(b, sl, b->target, nullptr);
+    hipError_t e = plan_slot(b, sl, b->bk[bucket], nullptr);
     if (e == hipSuccess && b->engine == ENGINE_SPLIT) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return e;
     sl.active = true;
@@ -505,39 +567,62 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
                                                  gpumlib_nmf_batch_tuning* out) {
     if (!b || !stream || trial_iterations < 1) return hipErrorInvalidValue;
     if (gpumlib_nmf_batch_active(b) == 0) return hipErrorInvalidValue;
+    // buckets with active problems (and with problems of r <= 32, the ones rank blocking moves);
+    // the single target reported is that of the largest bucket present
+    bool present[kBuckets] = {false, false, false}, narrow[kBuckets] = {false, false, false};
+    for (const Slot& sl : b->slots)
+        if (sl.active) {
+            present[sl.bucket] = true;
+            narrow[sl.bucket] = narrow[sl.bucket] || sl.r <= 32;
+        }
+    int top = 0;
+    for (int k = 0; k < kBuckets; k++)
+        if (present[k]) top = k;
     auto report = [&](const TuneConfig& c, int cached) {
         if (!out) return;
         out->engine = c.engine;
-        out->target_workgroups = c.target;
+        out->target_workgroups = c.bk[top].target;
         out->pass_grid = c.pass_grid;
         out->update_grid = c.upd_grid;
         out->ms_per_iteration = c.ms_per_iteration;
         out->ms_per_iteration_before = c.ms_default;
         out->trials = cached ? 0 : c.trials;
         out->cached = cached;
-        out->split_ranks = c.split_ranks;
+        out->split_ranks = 0;
+        for (int k = 0; k < kBuckets; k++) {
+            out->bucket_targets[k] = present[k] ? c.bk[k].target : 0;
+            out->bucket_tile_cols[k] = present[k] ? c.bk[k].tile : 0;
+            out->bucket_split_ranks[k] = present[k] ? c.bk[k].split : 0;
+            out->split_ranks |= out->bucket_split_ranks[k];
+        }
     };
-    const int en0 = b->engine, t0 = b->target, pg0 = b->pass_grid, ug0 = b->upd_grid, sr0 = b->split_ranks;
+    BucketSet bk0;
+    for (int k = 0; k < kBuckets; k++) bk0[k] = b->bk[k];
+    const int en0 = b->engine, pg0 = b->pass_grid, ug0 = b->upd_grid;
+    const unsigned vs0 = variants_launched(b);
+    auto changed = [&]() {
+        return b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0 || variants_launched(b) != vs0;
+    };
     const unsigned long long sig = mix_signature(b);
     auto hit = b->tuned.find(sig);
     if (hit != b->tuned.end()) {  // this mix was tuned before: apply its winner
         const TuneConfig& c = hit->second;
-        b->split_ranks = c.split_ranks;
-        hipError_t e = replan_all(b, c.engine, c.target, stream);
+        hipError_t e = replan_all(b, c.engine, c.bk, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return e;
         b->pass_grid = c.pass_grid;
         b->upd_grid = c.upd_grid;
-        if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0 || b->split_ranks != sr0) drop_graphs(b);
+        if (changed()) drop_graphs(b);
         report(c, 1);
         return hipSuccess;
     }
-    // every problem's workspace must hold the plan of a candidate target
-    auto fits = [&](int target) {
+    // every problem of a bucket (all buckets: bucket < 0) must hold the plan of a candidate shape
+    auto fits = [&](int target, int tile, int bucket) {
         for (const Slot& sl : b->slots) {
-            if (!sl.active) continue;
+            if (!sl.active || (bucket >= 0 && sl.bucket != bucket)) continue;
             Plan pl;
-            if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target) || pl.bytes > sl.ws_bytes)
+            if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target, tile) || pl.bytes > sl.ws_bytes ||
+                fused_bytes(sl.m, sl.r, tile) > sl.ws_bytes)
                 return false;
         }
         return true;
@@ -570,10 +655,16 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         }
         return ce;
     };
-    TuneConfig best = {en0, t0, pg0, ug0, sr0, 1e30, 0.0, 0};
-    auto trial = [&](int engine, int target, int pg, int ug, int sr) -> hipError_t {
-        b->split_ranks = sr;
-        hipError_t te = replan_all(b, engine, target, stream);
+    TuneConfig best;
+    best.engine = en0;
+    best.pass_grid = pg0;
+    best.upd_grid = ug0;
+    for (int k = 0; k < kBuckets; k++) best.bk[k] = bk0[k];
+    best.ms_per_iteration = 1e30;
+    best.ms_default = 0.0;
+    best.trials = 0;
+    auto trial = [&](int engine, const BucketSet& bk, int pg, int ug) -> hipError_t {
+        hipError_t te = replan_all(b, engine, bk, stream);
         if (te != hipSuccess) return te;
         b->pass_grid = pg;
         b->upd_grid = ug;
@@ -584,37 +675,92 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         if (++best.trials == 1) best.ms_default = ms;
         if (ms < best.ms_per_iteration) {
             best.engine = engine;
-            best.target = target;
+            for (int k = 0; k < kBuckets; k++) best.bk[k] = bk[k];
             best.pass_grid = pg;
             best.upd_grid = ug;
-            best.split_ranks = sr;
             best.ms_per_iteration = ms;
         }
         return hipSuccess;
     };
-    // the configuration in effect first; then the tile shape along K: the fused engine
-    // (one slice, the whole contraction per tile) and the split-K targets, with the grids in
-    // effect; then the pass grids and the update (fused: gram) grids around the best so far;
-    // then the rank blocking: problems with r <= 32 on the one-block kernels, or not
+    // a copy of the best bucket shapes with bucket k's replaced
+    auto with = [&](int k, const BucketCfg& c, BucketSet& bk) {
+        for (int j = 0; j < kBuckets; j++) bk[j] = best.bk[j];
+        bk[k] = c;
+    };
+    // The configuration in effect first.  Then the tile depth along K: the fused engine (one
+    // slice, the whole contraction per tile) and the split-K targets, one for every bucket; then,
+    // split-K, each present bucket's target one step deeper and shallower than the best (the
+    // other buckets kept).  Then, per present bucket, the other tile width; then the pass grids
+    // and the update (fused: gram) grids around the best so far; then, per bucket with problems
+    // of r <= 32, the other rank blocking.
     if (e == hipSuccess) e = copy_wh(true);
-    if (e == hipSuccess) e = trial(en0, t0, pg0, ug0, sr0);
-    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, t0, pg0, ug0, sr0);
-    for (int t : kTuneTargets)
-        if (e == hipSuccess && (en0 != ENGINE_SPLIT || t != t0) && fits(t)) e = trial(ENGINE_SPLIT, t, pg0, ug0, sr0);
-    const int enb = best.engine, tb = best.target;
-    for (int pg : kTunePassGrids)
-        if (e == hipSuccess && pg != pg0) e = trial(enb, tb, pg, ug0, sr0);
-    const int pgb = best.pass_grid;
-    for (int ug : kTuneUpdGrids)
-        if (e == hipSuccess && ug != ug0) e = trial(enb, tb, pgb, ug, sr0);
-    if (e == hipSuccess) e = trial(enb, tb, pgb, best.upd_grid, !sr0);
+    if (e == hipSuccess) e = trial(en0, bk0, pg0, ug0);
+    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, bk0, pg0, ug0);
+    for (int t : kTuneTargets) {
+        BucketSet bk;
+        bool same = en0 == ENGINE_SPLIT;
+        for (int k = 0; k < kBuckets; k++) {
+            bk[k] = bk0[k];
+            bk[k].target = t;
+            same = same && bk0[k].target == t;
+        }
+        if (e == hipSuccess && !same && fits(t, TILE, -1)) {
+            // (the tile widths in effect stay; a bucket on 64-wide tiles is checked for them too)
+            bool ok = true;
+            for (int k = 0; k < kBuckets; k++) ok = ok && (!present[k] || fits(t, bk[k].tile, k));
+            if (ok) e = trial(ENGINE_SPLIT, bk, pg0, ug0);
+        }
+    }
+    const int nt = (int)(sizeof(kTuneTargets) / sizeof(kTuneTargets[0]));
+    if (best.engine == ENGINE_SPLIT) {
+        for (int k = kBuckets - 1; k >= 0; k--) {
+            if (!present[k]) continue;
+            int cur = 0;
+            while (cur < nt && kTuneTargets[cur] != best.bk[k].target) cur++;
+            for (int d : {-1, 1}) {
+                const int c = cur + d;
+                if (e != hipSuccess || cur == nt || c < 0 || c >= nt || !fits(kTuneTargets[c], best.bk[k].tile, k))
+                    continue;
+                BucketCfg nc = best.bk[k];
+                nc.target = kTuneTargets[c];
+                BucketSet bk;
+                with(k, nc, bk);
+                e = trial(ENGINE_SPLIT, bk, pg0, ug0);
+            }
+        }
+    }
+    for (int k = kBuckets - 1; k >= 0; k--) {
+        BucketCfg nc = best.bk[k];
+        nc.tile = nc.tile == TILE ? 64 : TILE;
+        if (e != hipSuccess || !present[k] || !fits(nc.target, nc.tile, k)) continue;
+        BucketSet bk;
+        with(k, nc, bk);
+        e = trial(best.engine, bk, pg0, ug0);
+    }
+    {
+        const int enb = best.engine;
+        BucketSet bkb;
+        for (int k = 0; k < kBuckets; k++) bkb[k] = best.bk[k];
+        for (int pg : kTunePassGrids)
+            if (e == hipSuccess && pg != pg0) e = trial(enb, bkb, pg, ug0);
+        const int pgb = best.pass_grid;
+        for (int ug : kTuneUpdGrids)
+            if (e == hipSuccess && ug != ug0) e = trial(enb, bkb, pgb, ug);
+    }
+    for (int k = kBuckets - 1; k >= 0; k--) {
+        BucketCfg nc = best.bk[k];
+        nc.split = !nc.split;
+        if (e != hipSuccess || !narrow[k]) continue;
+        BucketSet bk;
+        with(k, nc, bk);
+        e = trial(best.engine, bk, best.pass_grid, best.upd_grid);
+    }
     // apply the winner, or the configuration in effect before if a trial failed
     const bool ok = (e == hipSuccess);
-    b->split_ranks = ok ? best.split_ranks : sr0;
-    hipError_t ae = replan_all(b, ok ? best.engine : en0, ok ? best.target : t0, stream);
+    hipError_t ae = ok ? replan_all(b, best.engine, best.bk, stream) : replan_all(b, en0, bk0, stream);
     b->pass_grid = ok ? best.pass_grid : pg0;
     b->upd_grid = ok ? best.upd_grid : ug0;
-    if (b->engine != en0 || b->pass_grid != pg0 || b->upd_grid != ug0 || b->split_ranks != sr0) drop_graphs(b);
+    if (changed()) drop_graphs(b);
     if (ae == hipSuccess) ae = hipStreamSynchronize(stream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

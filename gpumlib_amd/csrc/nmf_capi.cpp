@@ -84,7 +84,7 @@ static bool make_unfused_plan(int rule, int n, int m, int r, Plan* pl) {
     return true;
 }
 
-bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target) {
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target, int tile) {
     if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
     if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8 &&
         precision != GPUMLIB_NMF_FP32X)
@@ -94,6 +94,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     const bool kl_bf16 = precision == GPUMLIB_NMF_BF16 && rule == GPUMLIB_NMF_KL;
     if (precision != GPUMLIB_NMF_FP32 && rule != GPUMLIB_NMF_EUCLIDEAN && !kl_bf16) return false;
     if (n < 1 || m < 1 || r < 1) return false;
+    if (tile != TILE && (tile != 64 || precision != GPUMLIB_NMF_FP32 || r > 64)) return false;
     if (r > kMaxFusedRank) {
         if (precision != GPUMLIB_NMF_FP32) return false;
         return make_unfused_plan(rule, n, m, r, pl);
@@ -135,7 +136,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
     // same 8 waves as two 4-wave workgroups), so the pass aims at half the workgroups.  The
     // FP32X passes do the same (one 8-wave workgroup per CU for every rank, up to 153 KB of LDS).
     pl->lp_nt = (((precision == GPUMLIB_NMF_BF16 || x3) && r <= 64) || kl_bf16) ? 2 : 1;
-    const int tw = (precision == GPUMLIB_NMF_FP32) ? TILE : TILE_LP * pl->lp_nt;  // columns (rows) per workgroup
+    const int tw = (precision == GPUMLIB_NMF_FP32) ? tile : TILE_LP * pl->lp_nt;  // columns (rows) per workgroup
     const int tgt = x3 ? target / 2 : target / pl->lp_nt;
     const int vt1 = (m + tw - 1) / tw, vt3 = (n + tw - 1) / tw;
     pl->tiles1 = vt1 > gram_tiles ? vt1 : gram_tiles;

@@ -94,18 +94,22 @@ hipError_t launch_reduce_f64(const double* in, int count, double* out, hipStream
 // is the persistent workgroup count; each workgroup strides over the table's items.
 // mb: rank blocks of 32 the grouped pass kernels are built for (1: the table's problems have
 // r <= 32; 2: r <= 64)
-hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+// Grouped passes: mb rank blocks of 32 (1 or 2), tw columns (H pass) / rows (W pass) per work
+// item (128 or 64), as the table's problems were planned.
+hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int tw, int grid,
                                hipStream_t st);
-hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st);
+hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int mb, int tw, int grid,
+                               hipStream_t st);
 hipError_t launch_group_update_h(const JobArgs* jobs, const int* table, int grid, hipStream_t st);
 hipError_t launch_group_update_w(const JobArgs* jobs, const int* table, int grid, hipStream_t st);
 // Fused full-K grouped launches (one slice per problem; each item updates its own tile):
 // the gram of W (which 0, into h.G) or of H (which 1, into w.G) per Euclidean problem, and
 // the fused H / W passes.
 hipError_t launch_group_gram(const JobArgs* jobs, const int* table, int which, int grid, hipStream_t st);
-hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int tw, int grid,
                                 hipStream_t st);
-hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st);
+hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int mb, int tw, int grid,
+                                hipStream_t st);
 hipError_t launch_group_reduce_obj(const JobArgs* jobs, const int* table, int slot, int grid, hipStream_t st);
 
 // Reference-shaped kernels over materialised operands.

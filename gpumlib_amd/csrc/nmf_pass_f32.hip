@@ -16,7 +16,11 @@
 //
 // Every workgroup has 4 waves; a wave owns 32 columns (H pass) or 32 rows
 // (W pass) of the tile and all r (padded to MB*32) rank rows, i.e. MB 32x32
-// MFMA accumulators.  K is staged through double-buffered LDS in steps of KC
+// MFMA accumulators.  The grouped launches also come with 64-column (64-row)
+// tiles (TW = 64): there two waves share each 32-column block, each taking
+// half of the k steps of every chunk, and the pair's partials are added in
+// LDS after the loop (more, narrower work items for problems with few columns
+// or rows).  K is staged through double-buffered LDS in steps of KC
 // with one barrier per step; the next one or two steps' global loads are in
 // flight (in registers) while the current step's MFMAs run.
 #include "nmf_common.hpp"
@@ -40,6 +44,27 @@ __device__ double block_sum_f64(double v, double* red) {
     return t;
 }
 
+// 64-wide tiles: the wave of k half 1 hands its partial blocks to the wave of k half 0 that owns
+// the same 32-column block (cw), which adds them (half 0 + half 1, a fixed order).  Every thread
+// of the workgroup calls it; red holds 2 * MB * 16 * 64 floats and is free again on return.
+template <int MB>
+__device__ __forceinline__ void pair_reduce(f32x16 (&acc)[MB], float* red, int cw, int kh, int lane) {
+    if (kh == 1) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) red[((cw * MB + mb) * 16 + v) * 64 + lane] = acc[mb][v];
+    }
+    __syncthreads();
+    if (kh == 0) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[mb][v] += red[((cw * MB + mb) * 16 + v) * 64 + lane];
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
 // entry.
@@ -53,21 +78,28 @@ __device__ double block_sum_f64(double v, double* red) {
 // Euclidean: H(a,u) *= WtV(a,u) / (sum_b G(a,b) H(b,u) + eps) with G = W^T W from p.G (computed by
 // the gram launch before this one, G(a,b) at p.G[b*rpad + a]); KL: H(a,u) *= num(a,u) /
 // max(colsum_a, eps), the column sums of W taken over the whole of K in this item.
-template <int MB, bool KL, int OBJ, bool FUSE = false>
+template <int MB, bool KL, int OBJ, bool FUSE = false, int TW = TILE>
 __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
     constexpr bool NEED_WH = KL || (OBJ != OBJ_NONE);
-    __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];    // W chunk, [a][i]
-    __shared__ __attribute__((aligned(16))) float sB[2][TILE][LDK];  // V (or W) chunk, [u][i]
+    static_assert(TW == 64 || TW == 128, "tile of 64 or 128 columns");
+    constexpr int NCB = TW / 32;                        // 32-column blocks of the tile
+    constexpr int KH = 4 / NCB;                         // waves per block, each on its k groups
+    constexpr int VL = TW * KC / 4 / NTHREADS;          // float4 of the V chunk per thread
+    __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];  // W chunk, [a][i]
+    __shared__ __attribute__((aligned(16))) float sB[2][TW][LDK];  // V (or W) chunk, [u][i]
     __shared__ double red[NTHREADS / 64];
+    static_assert(KH == 1 || 2 * TW * LDK >= 2 * MB * 16 * 64, "pair reduction fits the V chunk buffers");
 
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
+    const int cw = wave % NCB, kh = wave / NCB;  // the wave's column block and k half (TW = 64)
+    const bool owner = KH == 1 || kh == 0;       // holds the block's total after the pair reduction
     const int c = lane & 31, h = lane >> 5;
-    const int vt = (p.m + TILE - 1) / TILE;
-    const int col0 = t * TILE;
-    const int ncols = (t < vt) ? min(TILE, p.m - col0) : 0;
+    const int vt = (p.m + TW - 1) / TW;
+    const int col0 = t * TW;
+    const int ncols = (t < vt) ? min(TW, p.m - col0) : 0;
     const int nblk = nb * nb;
     const bool gcar = (!FUSE) && (!KL) && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
@@ -77,7 +109,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     // Per-wave B operand of the W H product: H(b, u) for b = 2*kk + h, u = this lane's column.
     float hreg[NEED_WH ? MB * 16 : 1];
     if (NEED_WH) {
-        const int u = col0 + wave * 32 + c;
+        const int u = col0 + cw * 32 + c;
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
             const int b = 2 * kk + h;
@@ -89,10 +121,10 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     // two chunks in flight (two register sets; the LDS is double-buffered); the KL and
     // objective variants, whose registers also hold H and the ratio, keep one.
     constexpr bool DEEP = !KL && OBJ == OBJ_NONE;
-    float4 ra0[MB], rb0[4], ra1[DEEP ? MB : 1], rb1[DEEP ? 4 : 1];
-    auto gload = [&](float4 (&rb)[4], float4 (&ra)[MB], int k0) __attribute__((always_inline)) {
+    float4 ra0[MB], rb0[VL], ra1[DEEP ? MB : 1], rb1[DEEP ? VL : 1];
+    auto gload = [&](float4 (&rb)[VL], float4 (&ra)[MB], int k0) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < VL; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
             rb[j] = load4(p.V + (size_t)(col0 + col) * p.ldv + k0 + kq, col < ncols, k0 + kq, kend, vec);
         }
@@ -102,9 +134,9 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             ra[j] = load4(p.W + (size_t)a * p.n + k0 + kq, a < p.r, k0 + kq, kend, vec);
         }
     };
-    auto sstore = [&](const float4 (&rb)[4], const float4 (&ra)[MB], int buf) __attribute__((always_inline)) {
+    auto sstore = [&](const float4 (&rb)[VL], const float4 (&ra)[MB], int buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < VL; j++) {
             const int q = tid + NTHREADS * j, col = q >> 3, kq = (q & 7) * 4;
             *reinterpret_cast<float4*>(&sB[buf][col][kq]) = rb[j];
         }
@@ -148,10 +180,12 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
             for (int kk = 0; kk < MB * 16; kk++)
                 if (kk < nb * 16) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
-            // wh[v] is row i = acc_row(v, h) of the chunk, column c of the wave.
+            // wh[v] is row i = acc_row(v, h) of the chunk, column c of the wave's block; with
+            // 64-column tiles the wave takes the k groups g of its half only.
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                const float4 vv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
+                if (KH == 2 && (g >> 1) != kh) continue;
+                const float4 vv = *reinterpret_cast<const float4*>(&sB[buf][cw * 32 + c][g * 8 + h * 4]);
                 const float vj[4] = {vv.x, vv.y, vv.z, vv.w};
                 float rj[4];
 #pragma unroll
@@ -174,8 +208,9 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         }
         if (!KL) {
 #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const float4 bv = *reinterpret_cast<const float4*>(&sB[buf][wave * 32 + c][g * 8 + h * 4]);
+            for (int gg = 0; gg < 4 / KH; gg++) {
+                const int g = kh * (4 / KH) + gg;
+                const float4 bv = *reinterpret_cast<const float4*>(&sB[buf][cw * 32 + c][g * 8 + h * 4]);
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
                     if (mb >= nb) continue;
@@ -201,7 +236,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             // chunk ch is in LDS, the register set passed holds chunk ch + 1 (in flight), the
             // other set chunk ch + 2; after the multiply chunk ch + 1 goes to the other LDS
             // buffer and its register set takes the loads of chunk ch + 3
-            auto step = [&](int ch, float4 (&rbn)[4], float4 (&ran)[MB]) __attribute__((always_inline)) {
+            auto step = [&](int ch, float4 (&rbn)[VL], float4 (&ran)[MB]) __attribute__((always_inline)) {
                 chunk(gram_tag, ch);
                 if (ch + 1 < nchunks) {
                     sstore(rbn, ran, (ch + 1) & 1);
@@ -234,12 +269,13 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         mainloop(std::true_type{});
     else
         mainloop(std::false_type{});
+    if constexpr (KH == 2) pair_reduce<MB>(acc, &sB[0][0][0], cw, kh, lane);
 
     if constexpr (FUSE) {
-        // acc[mb][v] = WtV or the KL numerator at (a = mb*32 + acc_row(v,h), u = col0 + wave*32 + c)
+        // acc[mb][v] = WtV or the KL numerator at (a = mb*32 + acc_row(v,h), u = col0 + cw*32 + c)
         constexpr int GS = RP + 1;
         float* sG = &sA[0][0][0];  // LDS is free after the loop's last barrier
-        const int ul = wave * 32 + c, u = col0 + ul;
+        const int ul = cw * 32 + c, u = col0 + ul;
         const bool ucol = ul < ncols;
         if (KL) {
             if (do_sums) sG[tid] = (colsum < p.eps) ? p.eps : colsum;
@@ -264,7 +300,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         }
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) {
-            if (mb >= nb) continue;
+            if (mb >= nb || !owner) continue;
             f32x16 d;
             for (int v = 0; v < 16; v++) d[v] = 0.f;
             if constexpr (!KL) {
@@ -290,9 +326,9 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             }
         }
     } else {
-        // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = wave*32 + c)
-        const int ul = wave * 32 + c;
-        if (ul < ncols) {
+        // acc[mb][v] = P(a = mb*32 + acc_row(v,h), u = cw*32 + c)
+        const int ul = cw * 32 + c;
+        if (owner && ul < ncols) {
             float* dst = p.P + (size_t)s * p.pstride + (size_t)(col0 + ul) * p.ldr;
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
@@ -336,23 +372,31 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 // over the first tiles of every slice as in the H pass (k quarters per wave).
 // FUSE: as in the H pass, the workgroup applies W(i,a) *= VHt(i,a) / (sum_b W(i,b) G(b,a) + eps)
 // to its rows, G = H H^T from p.G (the gram launch before this one).
-template <int MB, bool FUSE = false>
+template <int MB, bool FUSE = false, int TW = TILE>
 __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
-    constexpr int PA = TILE + 32;  // rows k and k+1 land 32 banks apart
+    static_assert(TW == 64 || TW == 128, "tile of 64 or 128 rows");
+    constexpr int NCB = TW / 32;    // 32-row blocks of the tile
+    constexpr int KH = 4 / NCB;     // waves per block, each on its k steps
+    constexpr int VQ = TW / 4;      // float4 per chunk row u
+    constexpr int VL = KC * VQ / NTHREADS;
+    constexpr int PA = TW + 32;  // rows k and k+1 land 32 banks apart
     constexpr int PB = RP + 32;
     __shared__ __attribute__((aligned(16))) float sA[2][KC][PA];  // [u][i]
     __shared__ __attribute__((aligned(16))) float sB[2][KC][PB];  // [u][a]
+    static_assert(KH == 1 || 2 * KC * PB >= 2 * MB * 16 * 64, "pair reduction fits the H chunk buffers");
 
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
+    const int cw = wave % NCB, kh = wave / NCB;  // the wave's row block and k half (TW = 64)
+    const bool owner = KH == 1 || kh == 0;
     const int c = lane & 31, h = lane >> 5;
-    const int vt = (p.n + TILE - 1) / TILE;
+    const int vt = (p.n + TW - 1) / TW;
     const float* A = p.V;
     const int lda = p.n;
-    const int row0 = t * TILE;
-    const int nrows = (t < vt) ? min(TILE, p.n - row0) : 0;
+    const int row0 = t * TW;
+    const int nrows = (t < vt) ? min(TW, p.n - row0) : 0;
     const int nblk = nb * nb;
     const bool gcar = !FUSE && p.G != nullptr && t < nblk;  // the plan gives tiles >= nblk
     const int kbeg = s * p.kslice;
@@ -364,11 +408,11 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     // two chunks of global loads in flight (two register sets, double-buffered LDS) up to
     // r = 96; one at r = 128, where a second set would not fit two workgroups per CU
     constexpr bool DEEP = MB <= 3;
-    float4 ra0[4], rb0[BLOADS], ra1[DEEP ? 4 : 1], rb1[DEEP ? BLOADS : 1];
-    auto gload = [&](float4 (&ra)[4], float4 (&rb)[BLOADS], int k0) __attribute__((always_inline)) {
+    float4 ra0[VL], rb0[BLOADS], ra1[DEEP ? VL : 1], rb1[DEEP ? BLOADS : 1];
+    auto gload = [&](float4 (&ra)[VL], float4 (&rb)[BLOADS], int k0) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+        for (int j = 0; j < VL; j++) {
+            const int q = tid + NTHREADS * j, uu = q / VQ, iq = (q % VQ) * 4;
             ra[j] = load4(A + (size_t)(k0 + uu) * lda + row0 + iq, k0 + uu < kend, iq, nrows, vec);
         }
 #pragma unroll
@@ -377,10 +421,10 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             rb[j] = load4(p.H + (size_t)(k0 + uu) * p.r + aq, (uu < KC) && (k0 + uu < kend), aq, p.r, vec);
         }
     };
-    auto sstore = [&](const float4 (&ra)[4], const float4 (&rb)[BLOADS], int buf) __attribute__((always_inline)) {
+    auto sstore = [&](const float4 (&ra)[VL], const float4 (&rb)[BLOADS], int buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+        for (int j = 0; j < VL; j++) {
+            const int q = tid + NTHREADS * j, uu = q / VQ, iq = (q % VQ) * 4;
             *reinterpret_cast<float4*>(&sA[buf][uu][iq]) = ra[j];
         }
 #pragma unroll
@@ -404,8 +448,9 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
         const int buf = ch & 1;
         {
 #pragma unroll
-            for (int kk = 0; kk < KC / 2; kk++) {
-                const float av = sA[buf][2 * kk + h][wave * 32 + c];
+            for (int k2 = 0; k2 < KC / 2 / KH; k2++) {
+                const int kk = kh * (KC / 2 / KH) + k2;  // the wave's k steps of the chunk
+                const float av = sA[buf][2 * kk + h][cw * 32 + c];
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++)
                     if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
@@ -421,7 +466,7 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     };
     auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
         if constexpr (DEEP) {  // the schedule of the H pass
-            auto step = [&](int ch, float4 (&ran)[4], float4 (&rbn)[BLOADS]) __attribute__((always_inline)) {
+            auto step = [&](int ch, float4 (&ran)[VL], float4 (&rbn)[BLOADS]) __attribute__((always_inline)) {
                 chunk(gram_tag, ch);
                 if (ch + 1 < nchunks) {
                     sstore(ran, rbn, (ch + 1) & 1);
@@ -454,9 +499,10 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
         mainloop(std::true_type{});
     else
         mainloop(std::false_type{});
+    if constexpr (KH == 2) pair_reduce<MB>(acc, &sB[0][0][0], cw, kh, lane);
 
     if constexpr (FUSE) {
-        // acc[mb][v] = VHt(i = row0 + wave*32 + acc_row(v,h), a = mb*32 + c)
+        // acc[mb][v] = VHt(i = row0 + cw*32 + acc_row(v,h), a = mb*32 + c)
         constexpr int GS = RP + 1;
         float* sG = &sA[0][0][0];  // free after the loop's last barrier; sG[b*GS + a] = G(b, a)
         const int rpad = round_up(p.r, 4);
@@ -465,7 +511,8 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             sG[b * GS + a] = (a < p.r && b < p.r) ? p.G[(size_t)a * rpad + b] : 0.f;
         }
         __syncthreads();
-        const int il = wave * 32 + c;  // A operand row of W G: W(row0 + il, b = 2kk + h)
+        if (!owner) return;
+        const int il = cw * 32 + c;  // A operand row of W G: W(row0 + il, b = 2kk + h)
         float wa[MB * 16];
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
@@ -487,24 +534,24 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
             for (int g = 0; g < 4; g++) {
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int i = wave * 32 + g * 8 + h * 4 + j;
+                    const int i = cw * 32 + g * 8 + h * 4 + j;
                     if (i < nrows) wcol[i] *= (acc[mb][g * 4 + j] / (d[g * 4 + j] + p.eps));
                 }
             }
         }
         return;
     }
-    // acc[mb][v] = P(i = wave*32 + acc_row(v,h), a = mb*32 + c)
+    // acc[mb][v] = P(i = cw*32 + acc_row(v,h), a = mb*32 + c)
     const int orow0 = row0;
     const int rlim = p.npad - row0;
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) {
         const int a = mb * 32 + c;
-        if (a >= p.r) continue;
+        if (a >= p.r || !owner) continue;
         float* dst = p.P + (size_t)s * p.pstride + (size_t)a * p.ldp + orow0;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-            const int il = wave * 32 + g * 8 + h * 4;
+            const int il = cw * 32 + g * 8 + h * 4;
             if (il < rlim) {
                 float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
                 *reinterpret_cast<float4*>(dst + il) = o;
@@ -538,19 +585,27 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
 // layout of the second product without a shuffle.
 // FUSE: the workgroup applies W(i,a) *= num(i,a) / max(rowsum_a, eps) to its rows, the row sums
 // of H taken over the whole of K in this item.
-template <int MB, bool FUSE = false>
+template <int MB, bool FUSE = false, int TW = TILE>
 __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, const int nb) {
     constexpr int RP = MB * 32;
-    constexpr int PV = TILE + 8;  // rows u and u+4 land 32 banks apart
+    static_assert(TW == 64 || TW == 128, "tile of 64 or 128 rows");
+    constexpr int NCB = TW / 32;    // 32-row blocks of the tile
+    constexpr int KH = 4 / NCB;     // waves per block, each on its k groups
+    constexpr int VQ = TW / 4;      // float4 per chunk row u
+    constexpr int VL = KC * VQ / NTHREADS;
+    constexpr int PV = TW + 8;  // rows u and u+4 land 32 banks apart
     __shared__ __attribute__((aligned(16))) float sV[2][KC][PV];   // [u][i]
     __shared__ __attribute__((aligned(16))) float sH[2][RP][LDK];  // [a][u]
+    static_assert(KH == 1 || 2 * KC * PV >= 2 * MB * 16 * 64, "pair reduction fits the V chunk buffers");
 
     const int s = w / p.tiles, t = w - s * p.tiles;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: branches on it stay scalar
+    const int cw = wave % NCB, kh = wave / NCB;  // the wave's row block and k half (TW = 64)
+    const bool owner = KH == 1 || kh == 0;
     const int c = lane & 31, h = lane >> 5;
-    const int row0 = t * TILE;
-    const int nrows = min(TILE, p.n - row0);
+    const int row0 = t * TW;
+    const int nrows = min(TW, p.n - row0);
     const int kbeg = s * p.kslice;
     const int kend = min(p.m, kbeg + p.kslice);
     const bool vec = p.vec != 0;
@@ -563,7 +618,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     // B operand of H^T W^T: W(i, b) for i = this lane's row, b = 2*kk + h.
     float wreg[MB * 16];
     {
-        const int i = row0 + wave * 32 + c;
+        const int i = row0 + cw * 32 + c;
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
             const int b = 2 * kk + h;
@@ -571,11 +626,11 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         }
     }
 
-    float4 ra[4], rb[BLOADS];
+    float4 ra[VL], rb[BLOADS];
     auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+        for (int j = 0; j < VL; j++) {
+            const int q = tid + NTHREADS * j, uu = q / VQ, iq = (q % VQ) * 4;
             ra[j] = load4(p.V + (size_t)(k0 + uu) * p.n + row0 + iq, k0 + uu < kend, iq, nrows, vec);
         }
 #pragma unroll
@@ -586,8 +641,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     };
     auto sstore = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int q = tid + NTHREADS * j, uu = q >> 5, iq = (q & 31) * 4;
+        for (int j = 0; j < VL; j++) {
+            const int q = tid + NTHREADS * j, uu = q / VQ, iq = (q % VQ) * 4;
             *reinterpret_cast<float4*>(&sV[buf][uu][iq]) = ra[j];
         }
 #pragma unroll
@@ -623,7 +678,8 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
                 rowsum += (x.x + x.y) + (x.z + x.w);
             }
         }
-        // whT(u, i): rows u = acc_row(v,h) of this step, column i = wave*32 + c.
+        // whT(u, i): rows u = acc_row(v,h) of this step, column i = cw*32 + c; with 64-row
+        // tiles the wave takes the k groups g of its half only.
         f32x16 wt;
         for (int v = 0; v < 16; v++) wt[v] = 0.f;
 #pragma unroll
@@ -631,10 +687,11 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
             if (kk < nb * 16) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
 #pragma unroll
         for (int g = 0; g < 4; g++) {
+            if (KH == 2 && (g >> 1) != kh) continue;
             float rj[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float vv = sV[buf][g * 8 + h * 4 + j][wave * 32 + c];
+                const float vv = sV[buf][g * 8 + h * 4 + j][cw * 32 + c];
                 rj[j] = kl_ratio(vv, wt[g * 4 + j], p.eps);
             }
 #pragma unroll
@@ -650,6 +707,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         if (ch + 1 < nchunks) sstore(buf ^ 1);
         __syncthreads();
     }
+    if constexpr (KH == 2) pair_reduce<MB>(acc, &sV[0][0][0], cw, kh, lane);
 
     if constexpr (FUSE) {
         float* sS = &sH[0][0][0];  // free after the loop's last barrier
@@ -658,13 +716,13 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) {
             const int a = mb * 32 + c;
-            if (mb >= nb || a >= p.r) continue;
+            if (mb >= nb || a >= p.r || !owner) continue;
             float* wcol = const_cast<float*>(p.W) + (size_t)a * p.n + row0;  // the problem's own W
 #pragma unroll
             for (int g = 0; g < 4; g++) {
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int i = wave * 32 + g * 8 + h * 4 + j;
+                    const int i = cw * 32 + g * 8 + h * 4 + j;
                     if (i < nrows) wcol[i] *= (acc[mb][g * 4 + j] / sS[a]);
                 }
             }
@@ -675,11 +733,11 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) {
         const int a = mb * 32 + c;
-        if (a >= p.r) continue;
+        if (a >= p.r || !owner) continue;
         float* dst = p.P + (size_t)s * p.pstride + (size_t)a * p.ldp + row0;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-            const int il = wave * 32 + g * 8 + h * 4;
+            const int il = cw * 32 + g * 8 + h * 4;
             if (il < rlim) {
                 float4 o = make_float4(acc[mb][g * 4], acc[mb][g * 4 + 1], acc[mb][g * 4 + 2], acc[mb][g * 4 + 3]);
                 *reinterpret_cast<float4*>(dst + il) = o;
@@ -726,8 +784,9 @@ __device__ __forceinline__ int group_find(const int* first, int J, int item) {
 }
 
 // MB: rank blocks of 32 the kernel is built for (1: every problem of the table has r <= 32;
-// 2: r <= 64); a problem runs ceil(r/32) of them.
-template <bool KL, int OBJ, int MB>
+// 2: r <= 64); a problem runs ceil(r/32) of them.  TW: columns (H pass) or rows (W pass) per
+// work item, 128 or 64; the table's problems are planned with tiles of that width.
+template <bool KL, int OBJ, int MB, int TW>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -736,11 +795,11 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_h(const JobArgs* __r
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<MB, KL, OBJ>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<MB, KL, OBJ, false, TW>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-template <bool KL, int MB>
+template <bool KL, int MB, int TW>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_w(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -750,17 +809,17 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_pass_w(const JobArgs* __r
         const PassArgs p = jobs[idx[j]].w;
         __syncthreads();
         if (KL)
-            pass_w_kl_body<MB>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_kl_body<MB, false, TW>(p, item - first[j], (p.r + 31) >> 5);
         else
-            pass_w_euc_body<MB>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_euc_body<MB, false, TW>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-template <int MB>
+template <int MB, int TW>
 static hipError_t launch_group_pass_h_mb(const JobArgs* jobs, const int* table, int kl, int obj, int grid,
                                          hipStream_t st) {
-#define LG(K, O)                                                                                     \
-    hipLaunchKernelGGL((k_group_pass_h<K, O, MB>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+#define LG(K, O)                                                                                         \
+    hipLaunchKernelGGL((k_group_pass_h<K, O, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
     return hipGetLastError();
     if (kl) {
         if (obj == OBJ_NONE) { LG(true, OBJ_NONE) }
@@ -773,25 +832,31 @@ static hipError_t launch_group_pass_h_mb(const JobArgs* jobs, const int* table, 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+hipError_t launch_group_pass_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int tw, int grid,
                                hipStream_t st) {
-    if (mb == 1) return launch_group_pass_h_mb<1>(jobs, table, kl, obj, grid, st);
-    if (mb == 2) return launch_group_pass_h_mb<2>(jobs, table, kl, obj, grid, st);
+    if (mb == 1 && tw == 128) return launch_group_pass_h_mb<1, 128>(jobs, table, kl, obj, grid, st);
+    if (mb == 2 && tw == 128) return launch_group_pass_h_mb<2, 128>(jobs, table, kl, obj, grid, st);
+    if (mb == 1 && tw == 64) return launch_group_pass_h_mb<1, 64>(jobs, table, kl, obj, grid, st);
+    if (mb == 2 && tw == 64) return launch_group_pass_h_mb<2, 64>(jobs, table, kl, obj, grid, st);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st) {
-    if (mb == 1 && kl)
-        hipLaunchKernelGGL((k_group_pass_w<true, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 1)
-        hipLaunchKernelGGL((k_group_pass_w<false, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 2 && kl)
-        hipLaunchKernelGGL((k_group_pass_w<true, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 2)
-        hipLaunchKernelGGL((k_group_pass_w<false, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+template <int MB, int TW>
+static hipError_t launch_group_pass_w_mb(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
+    if (kl)
+        hipLaunchKernelGGL((k_group_pass_w<true, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_group_pass_w<false, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     return hipGetLastError();
+}
+
+hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, int mb, int tw, int grid,
+                               hipStream_t st) {
+    if (mb == 1 && tw == 128) return launch_group_pass_w_mb<1, 128>(jobs, table, kl, grid, st);
+    if (mb == 2 && tw == 128) return launch_group_pass_w_mb<2, 128>(jobs, table, kl, grid, st);
+    if (mb == 1 && tw == 64) return launch_group_pass_w_mb<1, 64>(jobs, table, kl, grid, st);
+    if (mb == 2 && tw == 64) return launch_group_pass_w_mb<2, 64>(jobs, table, kl, grid, st);
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -910,7 +975,7 @@ __global__ __launch_bounds__(NTHREADS) void k_group_gram(const JobArgs* __restri
     }
 }
 
-template <bool KL, int OBJ, int MB>
+template <bool KL, int OBJ, int MB, int TW>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -919,11 +984,11 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_h(const JobArgs* __
         const int j = group_find(first, J, item);
         const PassArgs p = jobs[idx[j]].h;
         __syncthreads();
-        pass_h_body<MB, KL, OBJ, true>(p, item - first[j], (p.r + 31) >> 5);
+        pass_h_body<MB, KL, OBJ, true, TW>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
-template <bool KL, int MB>
+template <bool KL, int MB, int TW>
 __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_w(const JobArgs* __restrict__ jobs, const int* table) {
     const int J = table[0], T = table[1];
     const int* first = table + 2;
@@ -933,9 +998,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_group_fused_w(const JobArgs* __
         const PassArgs p = jobs[idx[j]].w;
         __syncthreads();
         if (KL)
-            pass_w_kl_body<MB, true>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_kl_body<MB, true, TW>(p, item - first[j], (p.r + 31) >> 5);
         else
-            pass_w_euc_body<MB, true>(p, item - first[j], (p.r + 31) >> 5);
+            pass_w_euc_body<MB, true, TW>(p, item - first[j], (p.r + 31) >> 5);
     }
 }
 
@@ -947,11 +1012,11 @@ hipError_t launch_group_gram(const JobArgs* jobs, const int* table, int which, i
     return hipGetLastError();
 }
 
-template <int MB>
+template <int MB, int TW>
 static hipError_t launch_group_fused_h_mb(const JobArgs* jobs, const int* table, int kl, int obj, int grid,
                                           hipStream_t st) {
-#define LF(K, O)                                                                                      \
-    hipLaunchKernelGGL((k_group_fused_h<K, O, MB>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
+#define LF(K, O)                                                                                          \
+    hipLaunchKernelGGL((k_group_fused_h<K, O, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table); \
     return hipGetLastError();
     if (kl) {
         if (obj == OBJ_NONE) { LF(true, OBJ_NONE) }
@@ -964,25 +1029,31 @@ static hipError_t launch_group_fused_h_mb(const JobArgs* jobs, const int* table,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int grid,
+hipError_t launch_group_fused_h(const JobArgs* jobs, const int* table, int kl, int obj, int mb, int tw, int grid,
                                 hipStream_t st) {
-    if (mb == 1) return launch_group_fused_h_mb<1>(jobs, table, kl, obj, grid, st);
-    if (mb == 2) return launch_group_fused_h_mb<2>(jobs, table, kl, obj, grid, st);
+    if (mb == 1 && tw == 128) return launch_group_fused_h_mb<1, 128>(jobs, table, kl, obj, grid, st);
+    if (mb == 2 && tw == 128) return launch_group_fused_h_mb<2, 128>(jobs, table, kl, obj, grid, st);
+    if (mb == 1 && tw == 64) return launch_group_fused_h_mb<1, 64>(jobs, table, kl, obj, grid, st);
+    if (mb == 2 && tw == 64) return launch_group_fused_h_mb<2, 64>(jobs, table, kl, obj, grid, st);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int mb, int grid, hipStream_t st) {
-    if (mb == 1 && kl)
-        hipLaunchKernelGGL((k_group_fused_w<true, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 1)
-        hipLaunchKernelGGL((k_group_fused_w<false, 1>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 2 && kl)
-        hipLaunchKernelGGL((k_group_fused_w<true, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
-    else if (mb == 2)
-        hipLaunchKernelGGL((k_group_fused_w<false, 2>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
+template <int MB, int TW>
+static hipError_t launch_group_fused_w_mb(const JobArgs* jobs, const int* table, int kl, int grid, hipStream_t st) {
+    if (kl)
+        hipLaunchKernelGGL((k_group_fused_w<true, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_group_fused_w<false, MB, TW>), dim3(grid), dim3(NTHREADS), 0, st, jobs, table);
     return hipGetLastError();
+}
+
+hipError_t launch_group_fused_w(const JobArgs* jobs, const int* table, int kl, int mb, int tw, int grid,
+                                hipStream_t st) {
+    if (mb == 1 && tw == 128) return launch_group_fused_w_mb<1, 128>(jobs, table, kl, grid, st);
+    if (mb == 2 && tw == 128) return launch_group_fused_w_mb<2, 128>(jobs, table, kl, grid, st);
+    if (mb == 1 && tw == 64) return launch_group_fused_w_mb<1, 64>(jobs, table, kl, grid, st);
+    if (mb == 2 && tw == 64) return launch_group_fused_w_mb<2, 64>(jobs, table, kl, grid, st);
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------

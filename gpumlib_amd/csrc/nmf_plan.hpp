@@ -84,8 +84,11 @@ inline size_t x3_tiled_elems(int K, int cols) { return (size_t)x3_ld(K) * 64 * l
 size_t vstore_x3_offset_t(int n, int m);
 size_t vstore_bytes(int precision, int n, int m);
 
-// target: workgroups aimed for per pass (K slices are chosen to reach it).
-bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target = kTargetWorkgroups);
+// target: workgroups aimed for per pass (K slices are chosen to reach it).  tile: columns (H
+// pass) and rows (W pass) per work item of the FP32 passes, 128 or 64 (the grouped launches
+// have both widths; the single-problem launches take 128).
+bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target = kTargetWorkgroups,
+               int tile = TILE);
 void pass_args(const Plan& pl, const float* V, const float* W, const float* H, void* ws, float eps, int which,
                PassArgs* a);
 hipError_t run_iterations(const Plan& pl, const float* V, float* W, float* H, float eps, int iterations, int update_w,

@@ -414,6 +414,11 @@ void gpumlib_nmf_server::run_batched() {
                     stats.tuned_update_grid = t.update_grid;
                     stats.tuned_engine = t.engine;
                     stats.tuned_split_ranks = t.split_ranks;
+                    for (int k = 0; k < 3; k++) {
+                        stats.tuned_bucket_targets[k] = t.bucket_targets[k];
+                        stats.tuned_bucket_tile_cols[k] = t.bucket_tile_cols[k];
+                        stats.tuned_bucket_split_ranks[k] = t.bucket_split_ranks[k];
+                    }
                 }
             }
         }

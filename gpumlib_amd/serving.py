@@ -129,11 +129,27 @@ def run_server(lib, jobs, data, rate, tol=1e-4, max_iters=500, budget=32 << 30, 
             "max_active": 1 if sequential else max_active,
             "engine": "one_at_a_time" if sequential else ("split_k" if split_k else "fused"),
             "autotune_trial_iterations": autotune, "autotunes": st.autotunes,
-            "tuned": {"engine": "fused" if st.tuned_engine == capi.BATCH_FUSED else "split_k",
-                      "target_workgroups": st.tuned_target, "pass_grid": st.tuned_pass_grid,
-                      "update_grid": st.tuned_update_grid,
-                      "rank_blocking": "r<=32 on 1-block kernels" if st.tuned_split_ranks else "2-block kernels"}
+            "tuned": _tuned_report(st)
             if st.autotunes else None}
+
+
+_BUCKETS = ("V<2^18", "V<2^21", "V>=2^21")
+
+
+def _tuned_report(st):
+    """The batch configuration the autotuner chose last: the engine, the grids, and per shape bucket
+    with problems the pass tile shape -- K-split target (split-K engine), tile width, rank blocking."""
+    fused = st.tuned_engine == capi.BATCH_FUSED
+    buckets = {}
+    for name, t, w, sp in zip(_BUCKETS, st.tuned_bucket_targets, st.tuned_bucket_tile_cols,
+                              st.tuned_bucket_split_ranks):
+        if not w:
+            continue
+        buckets[name] = {"k_split": "whole contraction per item" if fused else f"target {t} workgroups",
+                         "tile_cols": w,
+                         "rank_blocking": "r<=32 on 1-block kernels" if sp else "2-block kernels"}
+    return {"engine": "fused" if fused else "split_k", "pass_grid": st.tuned_pass_grid,
+            "update_grid": st.tuned_update_grid, "tile_shapes": buckets}
 
 
 def queued_latency(service_s, rate, seed=12345):

@@ -275,12 +275,20 @@ hipError_t gpumlib_nmf_batch_set_target(gpumlib_nmf_batch_handle b, int workgrou
 enum { GPUMLIB_NMF_BATCH_SPLIT_K = 0, GPUMLIB_NMF_BATCH_FUSED = 1 };
 hipError_t gpumlib_nmf_batch_set_engine(gpumlib_nmf_batch_handle b, int engine);
 int gpumlib_nmf_batch_engine(gpumlib_nmf_batch_handle b);
-/* Rank blocking of the grouped pass kernels: 0 (default): every problem on the kernels built for two
- * rank blocks of 32 (r <= 64; a problem with r <= 32 skips the second block); 1: problems with r <= 32
- * on kernels built for one block (half the accumulator registers and operand LDS), launched after the
- * two-block kernels of each pass.  Results are the same either way (same arithmetic, same order). */
+/* Rank blocking of the grouped pass kernels, for every shape bucket: 0 (default): every problem on the
+ * kernels built for two rank blocks of 32 (r <= 64; a problem with r <= 32 skips the second block); 1:
+ * problems with r <= 32 on kernels built for one block (half the accumulator registers and operand LDS),
+ * launched after the two-block kernels of each pass.  Results are the same either way (same arithmetic,
+ * same order).  gpumlib_nmf_batch_rank_split returns 1 when any bucket splits (the autotuner chooses
+ * per bucket). */
 hipError_t gpumlib_nmf_batch_set_rank_split(gpumlib_nmf_batch_handle b, int on);
 int gpumlib_nmf_batch_rank_split(gpumlib_nmf_batch_handle b);
+/* Tile width of the grouped pass kernels, for every shape bucket: 128 (default) or 64 columns of V per
+ * H-pass work item and rows per W-pass work item.  A 64-wide work item has two waves on each 32-column
+ * block, each over half of every K chunk, their partials added after the loop: twice the work items
+ * for problems with few columns or rows (results within FP32 rounding of the 128-wide ones, not
+ * bit-identical).  Re-plans every problem; not under graph capture. */
+hipError_t gpumlib_nmf_batch_set_tile_cols(gpumlib_nmf_batch_handle b, int cols);
 size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r);
 hipError_t gpumlib_nmf_batch_add(gpumlib_nmf_batch_handle b, int rule, int n, int m, int r, const float *V, float *W,
                                  float *H, float eps, void *workspace, size_t workspace_bytes, int *slot);
@@ -299,9 +307,13 @@ const double *gpumlib_nmf_batch_objective_device_ptr(gpumlib_nmf_batch_handle b)
 /* Autotuner: times `trial_iterations` iterations of the active mix (after one untimed iteration) for
  * each candidate configuration -- the tile shape along K: the fused engine (a tile's whole contraction
  * in one work item) or the split-K engine with a K-split target per problem (4, 8, 16, 32, 64 or 128
- * workgroups per problem and pass, those that fit every problem's workspace), then the persistent grid
- * sizes of the grouped passes (512, 1024, 2048) and of the updates or, fused, the gram launches
- * (256, 512, 1024), then the rank blocking (gpumlib_nmf_batch_set_rank_split) -- and keeps the fastest.  W and H of every problem are saved before and restored
+ * workgroups per problem and pass, those that fit every problem's workspace), first one target for
+ * all problems, then, split-K, one step up and down from the best for each shape bucket of problems
+ * on its own (buckets of fewer than 2^18 elements of V, fewer than 2^21, and the rest); then the tile
+ * width (gpumlib_nmf_batch_set_tile_cols) per bucket; then the persistent grid sizes of the grouped
+ * passes (512, 1024, 2048) and of the updates or, fused, the gram launches (256, 512, 1024); then the
+ * rank blocking (gpumlib_nmf_batch_set_rank_split) per bucket with problems of r <= 32 -- and keeps
+ * the fastest.  W and H of every problem are saved before and restored
  * after each trial (a temporary device buffer of their size is allocated for that), so no problem's
  * state changes; a re-planned problem sums its K slices in the new grouping afterwards (results within
  * the FP32 tolerance of the untuned ones, not bit-identical to them).  The winner is cached per mix
@@ -314,7 +326,12 @@ typedef struct {
     double ms_per_iteration_before; /* of the configuration in effect before the call */
     int trials;                     /* configurations timed (0 when a cached choice was applied) */
     int cached;
-    int split_ranks;                /* rank blocking chosen (gpumlib_nmf_batch_set_rank_split) */
+    int split_ranks;                /* 1: some bucket present splits ranks (gpumlib_nmf_batch_set_rank_split) */
+    /* per shape bucket (0 in all three for a bucket with no problem); target_workgroups is that of
+       the largest bucket present */
+    int bucket_targets[3];          /* K-split target */
+    int bucket_tile_cols[3];        /* tile width, 128 or 64 */
+    int bucket_split_ranks[3];      /* rank blocking */
 } gpumlib_nmf_batch_tuning;
 hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int trial_iterations, hipStream_t stream,
                                       gpumlib_nmf_batch_tuning *out);
@@ -362,6 +379,9 @@ typedef struct {
     int tuned_target, tuned_pass_grid, tuned_update_grid;  /* configuration chosen last */
     int tuned_engine;                                       /* GPUMLIB_NMF_BATCH_FUSED or _SPLIT_K */
     int tuned_split_ranks;                                  /* rank blocking chosen last */
+    int tuned_bucket_targets[3];                            /* per shape bucket, chosen last */
+    int tuned_bucket_tile_cols[3];
+    int tuned_bucket_split_ranks[3];
 } gpumlib_nmf_server_stats;
 
 hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config *cfg, gpumlib_nmf_server_handle *out);

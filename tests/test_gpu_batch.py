@@ -47,7 +47,7 @@ EDGE_JOBS = [  # odd shapes for the fused engine's tiles: partial tiles, unalign
 
 
 class Batch:
-    def __init__(self, hip, lib, max_jobs=16, engine=None, rank_split=None):
+    def __init__(self, hip, lib, max_jobs=16, engine=None, rank_split=None, tile_cols=None):
         self.hip, self.lib = hip, lib
         h = ctypes.c_void_p()
         capi.check(lib.gpumlib_nmf_batch_create(max_jobs, ctypes.byref(h)))
@@ -58,6 +58,8 @@ class Batch:
         if rank_split is not None:
             capi.check(lib.gpumlib_nmf_batch_set_rank_split(h, rank_split))
             assert lib.gpumlib_nmf_batch_rank_split(h) == rank_split
+        if tile_cols is not None:
+            capi.check(lib.gpumlib_nmf_batch_set_tile_cols(h, tile_cols))
         self.keep = []
         self.stream = hip.Stream()
 
@@ -89,8 +91,9 @@ class Batch:
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("jobs", ["mix", "edge"])
 @pytest.mark.parametrize("rank_split", [0, 1])
-def test_batch_matches_oracle(native, orc, cuda, engine, jobs, rank_split):
-    b = Batch(cuda, native, engine=engine, rank_split=rank_split)
+@pytest.mark.parametrize("tile_cols", [128, 64])
+def test_batch_matches_oracle(native, orc, cuda, engine, jobs, rank_split, tile_cols):
+    b = Batch(cuda, native, engine=engine, rank_split=rank_split, tile_cols=tile_cols)
     outs = []
     for k, (rule, n, m, r) in enumerate(JOBS if jobs == "mix" else EDGE_JOBS):
         V, W, H = make_job(k, rule, n, m, r)
@@ -128,12 +131,14 @@ def test_alone_equals_mix(native, cuda, engine, k):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-def test_rank_split_is_bit_identical(native, cuda, engine):
+@pytest.mark.parametrize("tile_cols", [128, 64])
+def test_rank_split_is_bit_identical(native, cuda, engine, tile_cols):
     """Problems with r <= 32 on the one-rank-block kernels give the bits of the two-block
-    kernels (the same products in the same order), eager and graph-replayed."""
+    kernels (the same products in the same order), eager and graph-replayed, at either tile
+    width."""
     outs = []
     for split in (0, 1):
-        b = Batch(cuda, native, engine=engine, rank_split=split)
+        b = Batch(cuda, native, engine=engine, rank_split=split, tile_cols=tile_cols)
         bufs = []
         for k, job in enumerate(JOBS + EDGE_JOBS):
             _, dW, dH = b.add(*job, *make_job(k, *job))
@@ -149,8 +154,10 @@ def test_rank_split_is_bit_identical(native, cuda, engine):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-def test_graph_replays_equal_eager(native, cuda, engine):
-    eager, graphed = Batch(cuda, native, engine=engine), Batch(cuda, native, engine=engine)
+@pytest.mark.parametrize("tile_cols", [128, 64])
+def test_graph_replays_equal_eager(native, cuda, engine, tile_cols):
+    eager = Batch(cuda, native, engine=engine, tile_cols=tile_cols)
+    graphed = Batch(cuda, native, engine=engine, tile_cols=tile_cols)
     pairs = []
     for k, job in enumerate(JOBS[:4]):
         V, W, H = make_job(k, *job)
@@ -175,8 +182,9 @@ def test_graph_replays_equal_eager(native, cuda, engine):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-def test_batch_objectives_match_single(native, orc, cuda, engine):
-    b = Batch(cuda, native, engine=engine)
+@pytest.mark.parametrize("tile_cols", [128, 64])
+def test_batch_objectives_match_single(native, orc, cuda, engine, tile_cols):
+    b = Batch(cuda, native, engine=engine, tile_cols=tile_cols)
     slots = []
     for k, job in enumerate(JOBS[:3]):
         V, W, H = make_job(k, *job)
@@ -214,11 +222,18 @@ def test_autotune_keeps_state_and_parity(native, orc, cuda):
     capi.check(native.gpumlib_nmf_batch_autotune(b.h, 2, b.stream.handle, ctypes.byref(t)))
     print(f"tuned engine={t.engine} target={t.target_workgroups} pass_grid={t.pass_grid} "
           f"update_grid={t.update_grid} ms {t.ms_per_iteration_before:.3f} -> {t.ms_per_iteration:.3f} "
-          f"trials={t.trials}", flush=True)
+          f"trials={t.trials} buckets={list(t.bucket_targets)} tiles={list(t.bucket_tile_cols)} "
+          f"split={list(t.bucket_split_ranks)}", flush=True)
     assert t.cached == 0 and t.trials >= 3
     assert t.engine in (capi.BATCH_FUSED, capi.BATCH_SPLIT_K) and native.gpumlib_nmf_batch_engine(b.h) == t.engine
-    assert t.split_ranks in (0, 1) and native.gpumlib_nmf_batch_rank_split(b.h) == t.split_ranks
+    assert t.split_ranks == int(any(t.bucket_split_ranks)) and native.gpumlib_nmf_batch_rank_split(b.h) == t.split_ranks
     assert t.target_workgroups in TUNE_TARGETS | {32} and t.pass_grid in TUNE_PASS and t.update_grid in TUNE_UPD
+    # the mix fills the buckets of fewer than 2^18 and fewer than 2^21 elements of V, not the third;
+    # the single target reported is that of the largest bucket present
+    assert t.bucket_targets[0] in TUNE_TARGETS and t.bucket_targets[1] in TUNE_TARGETS
+    assert t.bucket_targets[2] == 0 and t.target_workgroups == t.bucket_targets[1]
+    assert t.bucket_tile_cols[0] in (64, 128) and t.bucket_tile_cols[1] in (64, 128) and t.bucket_tile_cols[2] == 0
+    assert all(x in (0, 1) for x in t.bucket_split_ranks) and t.bucket_split_ranks[2] == 0
     assert 0 < t.ms_per_iteration <= t.ms_per_iteration_before
     for (w0, h0), (*_, dW, dH) in zip(before, outs):
         assert np.array_equal(dW.download(np.float32), w0) and np.array_equal(dH.download(np.float32), h0)
@@ -227,6 +242,9 @@ def test_autotune_keeps_state_and_parity(native, orc, cuda):
     assert t2.cached == 1 and t2.trials == 0
     assert (t2.engine, t2.target_workgroups, t2.pass_grid, t2.update_grid, t2.split_ranks) == \
         (t.engine, t.target_workgroups, t.pass_grid, t.update_grid, t.split_ranks)
+    assert list(t2.bucket_targets) == list(t.bucket_targets)
+    assert list(t2.bucket_tile_cols) == list(t.bucket_tile_cols)
+    assert list(t2.bucket_split_ranks) == list(t.bucket_split_ranks)
     b.iterate(10)
     b.iterate(12, graph=True)  # graphs captured after tuning use the tuned grids
     for rule, n, m, r, V, W, H, dW, dH in outs:
@@ -315,6 +333,9 @@ def test_server_with_autotuning_matches_oracle(native, orc, cuda):
     st = s.stats()
     print(f"autotunes={st.autotunes} target={st.tuned_target} grids={st.tuned_pass_grid}/{st.tuned_update_grid}")
     assert st.autotunes >= 1 and st.tuned_target in TUNE_TARGETS
+    assert all(t in TUNE_TARGETS | {0} for t in st.tuned_bucket_targets)
+    assert st.tuned_target in list(st.tuned_bucket_targets)
+    assert all(t in (0, 64, 128) for t in st.tuned_bucket_tile_cols)
     s.close()
 
 
