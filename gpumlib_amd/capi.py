@@ -15,6 +15,7 @@ ADDITIVE_EUCLIDEAN, ADDITIVE_KL, ADDITIVE_EUCLIDEAN_WITH_H = 2, 3, 4
 FP32, BF16, FP8, FP32X = 0, 1, 2, 3
 BATCH_SPLIT_K, BATCH_FUSED = 0, 1
 OBJ_NONE, OBJ_FROBENIUS, OBJ_KL, OBJ_QUALITY = 0, 1, 2, 3
+STREAM_AUTO, STREAM_NONTEMPORAL, STREAM_CACHED = 0, 1, 2
 REFERENCE_EPS = 1e-8  # float(10e-9f) == float(1e-8f)
 
 _c = ctypes
@@ -74,6 +75,7 @@ SIGNATURES = {
     "gpumlib_nmf_sgemm": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_gemm_bf16": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i, _vp]),
     "gpumlib_nmf_gemm_fp8": (_i, [_c.c_char, _c.c_char, _i, _i, _i, _f, _vp, _i, _i, _vp, _i, _i, _f, _vp, _i, _vp]),
+    "gpumlib_nmf_set_stream_policy": (_i, [_i]),
     "gpumlib_nmf_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "gpumlib_nmf_vstore_bytes": (_sz, [_i, _i, _i]),
     "gpumlib_nmf_vstore_pack": (_i, [_i, _i, _i, _vp, _vp, _sz, _vp]),

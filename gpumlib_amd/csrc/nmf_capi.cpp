@@ -1,5 +1,6 @@
 // C-ABI of the NMF library: argument checking, workspace planning and the
 // launch sequence of one multiplicative-update iteration.
+#include <atomic>
 #include <cmath>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -83,6 +84,9 @@ static bool make_unfused_plan(int rule, int n, int m, int r, Plan* pl) {
     pl->bytes = off;
     return true;
 }
+
+// Cache policy of the BF16 passes' V reads (gpumlib_nmf_set_stream_policy).
+static std::atomic<int> g_stream_policy{GPUMLIB_NMF_STREAM_AUTO};
 
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target, int tile) {
     if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
@@ -210,6 +214,10 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         off += align256((size_t)r * pl->nbu);
     }
     pl->bytes = off;
+    const int policy = g_stream_policy.load(std::memory_order_relaxed);
+    pl->stream_nt = precision == GPUMLIB_NMF_BF16 &&
+                    (policy == GPUMLIB_NMF_STREAM_NONTEMPORAL ||
+                     (policy == GPUMLIB_NMF_STREAM_AUTO && vstore_bytes(precision, n, m) > kStreamNtBytes));
     return true;
 }
 
@@ -432,6 +440,7 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     ah.S = pl.S1;
     ah.tiles = pl.tiles1;
     ah.nt = pl.lp_nt;
+    ah.snt = pl.stream_nt;
     ah.pstride = pl.pstride1;
     aw = ah;
     aw.G = G + (size_t)pl.r * pl.rpad;
@@ -478,6 +487,7 @@ void kl_bf16_args(const Plan& pl, const void* Vstore, void* ws, float eps, int w
     a->ldr = pl.ldr;
     a->rpad = pl.rpad;
     a->nt = pl.lp_nt;
+    a->snt = pl.stream_nt;
     a->eps = eps;
     if (which == 0) {
         a->Ak = Wk, a->Ar = Wr, a->Cr = Hr, a->B = (const uint16_t*)Vstore;
@@ -767,6 +777,13 @@ hipError_t gpumlib_nmf_gemm_fp8(char transa, char transb, int M, int N, int K, f
     return launch_gemm(2, transa, transb, M, N, K, alpha, A, lda, scale_a, B, ldb, scale_b, beta, C, ldc, stream);
 }
 
+hipError_t gpumlib_nmf_set_stream_policy(int policy) {
+    if (policy != GPUMLIB_NMF_STREAM_AUTO && policy != GPUMLIB_NMF_STREAM_NONTEMPORAL && policy != GPUMLIB_NMF_STREAM_CACHED)
+        return hipErrorInvalidValue;
+    g_stream_policy.store(policy, std::memory_order_relaxed);
+    return hipSuccess;
+}
+
 size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r) {
     Plan pl;
     if (!make_plan(rule, precision, n, m, r, &pl)) return 0;
@@ -880,6 +897,7 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         a.G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)r * pl.rpad);
         a.gcount = (int*)(base + pl.off_cnt) + which;
         a.nt = pl.lp_nt;
+        a.snt = pl.stream_nt;
         a.gkeep = gslices > 0;
         if (which == 0) {
             a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = pl.ldv8;

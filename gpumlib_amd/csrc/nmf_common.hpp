@@ -64,6 +64,18 @@ __device__ __forceinline__ f32x16 mfma32x2(float a, float b, f32x16 acc) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
 }
 
+// Load of the streamed operand of a pass (V) with the non-temporal cache policy (nt), for V
+// too large to stay cached from one iteration to the next: the stream then does not displace
+// the L2-resident factor chunks every workgroup re-reads.  A read probe of the BF16 pass's
+// pattern (one contiguous 64 KB chunk per workgroup and step, 256 workgroups of 512 threads,
+// staged through LDS) over 537 MB on the MI355X: 6.7 TB/s with the nt policy, 6.1 TB/s
+// without.  V that fits the 256 MB last-level cache is read with the default policy (with nt
+// its reuse across iterations is lost: the KL pass at 4096 x 4096 ran 7% slower).
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // KL ratio V / (WH + eps) of the fused passes (nmf_kernels.cu:271,274,319,322), as V times
 // the hardware reciprocal of WH + eps (v_rcp_f32, within 1 ulp): the ratio only feeds a
 // sum over the contraction, where the <= 1.5 ulp difference per term from a correctly
@@ -336,6 +348,7 @@ struct LpPassArgs {
     int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
     long aplane; // FP32X pass (B: the tiled FP32 store, x3_tiled_index): element stride between
                  // the three BF16 piece planes of A
+    int snt;     // BF16 pass: 1 reads B with the non-temporal policy (stream_load)
 };
 
 // KL pass over the BF16 V store: P[s](a, col) = sum_{k in slice s} A(k, a) R(k, col) with the ratio
@@ -361,6 +374,7 @@ struct KlLpArgs {
     int nt;              // 64-column store blocks per workgroup tile (2 for r <= 64)
     long pstride;
     float eps;
+    int snt;             // 1: B read with the non-temporal policy (stream_load)
 };
 
 // One problem of a grouped (batched) launch: the arguments of its two passes and

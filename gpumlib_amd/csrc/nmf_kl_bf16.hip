@@ -61,7 +61,8 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
 // order at the end.  V, the shadows and C are padded with zeros to whole chunks and tiles:
 // padded rows contribute V = 0 (ratio 0, eps > 0); padded columns are not stored; rank rows
 // >= r of P (A rows clamped to r - 1) are not read by the update.
-template <int MB, int NT>
+// SNT: the V loads take the non-temporal policy (V too large to stay cached between passes).
+template <int MB, int NT, bool SNT>
 __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     constexpr int THR = NTHREADS * NT;
     constexpr int NW = THR / 64;
@@ -136,7 +137,8 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
             const int q = tid + THR * j, tb = j / BLB, ql = q - tb * BLB * THR;
             const int b = min(t * NT + tb, nt64 - 1);
             const int col = ql / (KKL / 8), kq = (ql % (KKL / 8)) * 8;
-            rb[j] = *reinterpret_cast<const ku32x4*>(crow + (size_t)b * (TILE_LP * 256) + col * 256 + kq);
+            const ku32x4* src = reinterpret_cast<const ku32x4*>(crow + (size_t)b * (TILE_LP * 256) + col * 256 + kq);
+            rb[j] = SNT ? stream_load(src) : *src;
         }
 #pragma unroll
         for (int j = 0; j < AKL; j++) {
@@ -429,7 +431,11 @@ __global__ __launch_bounds__(NTHREADS) void k_update_kl_rows_bf16(float* __restr
 // ---------------------------------------------------------------------------
 template <int MB, int NT>
 static void launch_kl(const KlLpArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_pass_kl_bf16<MB, NT>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
+    if (a.snt)
+        hipLaunchKernelGGL((k_pass_kl_bf16<MB, NT, true>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_pass_kl_bf16<MB, NT, false>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st,
+                           a);
 }
 
 hipError_t launch_pass_kl_bf16(const KlLpArgs& a, hipStream_t st) {

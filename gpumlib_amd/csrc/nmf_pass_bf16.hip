@@ -54,7 +54,8 @@ __device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
 // bounds check; lanes past the last column or rank row re-read the last valid one:
 // those products land only in output columns that are not stored and in rank rows
 // >= r of P and of the gram, which no update reads.
-template <int MB, int NT>
+// SNT: the V loads take the non-temporal policy (V too large to stay cached between passes).
+template <int MB, int NT, bool SNT>
 __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassArgs p) {
     constexpr int THR = NTHREADS * NT;                // threads: 4 waves per 64 columns
     constexpr int NW = THR / 64;                      // waves
@@ -98,8 +99,9 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
         for (int j = 0; j < BL; j++) {
             const int q = tid + THR * j, tb = j / BLB;  // store block of this load
             const int b = min(t * NT + tb, nt64 - 1);
-            rb[j] = *reinterpret_cast<const u32x4*>(crow + (size_t)b * (TILE_LP * KB) +
-                                                    (size_t)(q - tb * BLB * THR) * 8);
+            const u32x4* src = reinterpret_cast<const u32x4*>(crow + (size_t)b * (TILE_LP * KB) +
+                                                              (size_t)(q - tb * BLB * THR) * 8);
+            rb[j] = SNT ? stream_load(src) : *src;
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
@@ -300,7 +302,11 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_bf16(const float* __restric
 // ---------------------------------------------------------------------------
 template <int MB, int NT>
 static void launch_tn(const LpPassArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
+    if (a.snt)
+        hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT, true>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT, false>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st,
+                           a);
 }
 
 hipError_t launch_pass_tn_bf16(const LpPassArgs& a, hipStream_t st) {

@@ -15,6 +15,11 @@ constexpr float kReferenceEpsilon = 10e-9f;
 constexpr int kMaxFusedRank = 128;
 // Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
 constexpr int kTargetWorkgroups = 512;
+// V stores (both orientations) larger than this are streamed by the BF16 passes with the
+// non-temporal cache policy: 1.5 x the MI355X's 256 MB last-level cache, past which no part of
+// V survives from one pass over it to the next (the 268 MB FP8 store at 16384 x 8192 still
+// gains from what does; the 537 MB BF16 store does not).
+constexpr size_t kStreamNtBytes = 384ull << 20;
 // Work items aimed for by the objective pass on split factors.
 constexpr int kObjectiveWorkgroups = 512;
 // Padded rank of the split factors: whole MFMA steps of 16, in the kernel's four register
@@ -34,6 +39,7 @@ struct Plan {
                           // of the tiled V and V^T and the leading dimensions of the W / H^T shadows
     int ldx_n, ldx_m;     // FP32X storage: padded contraction lengths (x3_ld of n and of m), as ldv8 / ldt8
     int lp_nt;            // BF16 / FP32X passes: 64-column blocks per workgroup (1 or 2)
+    int stream_nt;        // BF16 passes: V read with the non-temporal policy (its store > kStreamNtBytes)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;

@@ -181,6 +181,16 @@ size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
                                    hipStream_t stream);
 
+/* Cache policy of the BF16 passes' reads of the V store (process-wide, read when an iteration or pass
+ * starts; no reference counterpart).  GPUMLIB_NMF_STREAM_AUTO (default): non-temporal when the store
+ * exceeds 384 MiB (no part of it survives in the 256 MB last-level cache from one pass over it to the
+ * next; 16384 x 8192: 537 MB), the default policy below that (the store is then re-read from that cache);
+ * _NONTEMPORAL / _CACHED force one.  Values are unaffected: only the cache policy of the loads differs. */
+#define GPUMLIB_NMF_STREAM_AUTO 0
+#define GPUMLIB_NMF_STREAM_NONTEMPORAL 1
+#define GPUMLIB_NMF_STREAM_CACHED 2
+hipError_t gpumlib_nmf_set_stream_policy(int policy);
+
 /* Device workspace needed by gpumlib_nmf_iterate / gpumlib_nmf_objective for one problem. */
 size_t gpumlib_nmf_workspace_bytes(int rule, int precision, int n, int m, int r);
 

@@ -124,3 +124,25 @@ def test_bf16_deterministic(native, cuda):
         p.iterate(10)
         outs.append((p.get_w(), p.get_h()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("rule", [G.EUCLIDEAN, G.KL])
+def test_bf16_stream_policy_bit_identical(native, cuda, rule):
+    """The BF16 passes read V with the non-temporal cache policy when the store is too large to stay
+    cached (16384 x 8192 and up) and with the default policy below: both policies forced on one
+    problem give the same bits (only the loads' cache policy differs)."""
+    from gpumlib_amd import capi
+    n, m, r = 900, 700, 32
+    V = G.gen_euclidean(n, m, r, 7) if rule == G.EUCLIDEAN else G.gen_poisson(n, m, r, 7)
+    W, H = G.gen_init(n, m, r, 7)
+    outs = []
+    try:
+        for policy in (capi.STREAM_NONTEMPORAL, capi.STREAM_CACHED):
+            capi.check(native.gpumlib_nmf_set_stream_policy(policy))
+            p = G.DeviceNMF(rule, V, W, H, n, m, r, eps=G.REFERENCE_EPS, precision=G.BF16)
+            p.iterate(10)
+            outs.append((p.get_w(), p.get_h()))
+    finally:
+        capi.check(native.gpumlib_nmf_set_stream_policy(capi.STREAM_AUTO))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert native.gpumlib_nmf_set_stream_policy(3) != 0
