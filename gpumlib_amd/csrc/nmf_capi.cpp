@@ -88,6 +88,25 @@ static bool make_unfused_plan(int rule, int n, int m, int r, Plan* pl) {
 // Cache policy of the BF16 passes' V reads (gpumlib_nmf_set_stream_policy).
 static std::atomic<int> g_stream_policy{GPUMLIB_NMF_STREAM_AUTO};
 
+// From which contraction index the narrow passes read V with the non-temporal policy: k0[0] for
+// the H pass (over the n rows of V), k0[1] for the W pass (over the m columns); n or m: none.
+static void stream_policy_k0(int precision, int n, int m, int (&k0)[2]) {
+    k0[0] = n;
+    k0[1] = m;
+    if (precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8) return;
+    const int policy = g_stream_policy.load(std::memory_order_relaxed);
+    if (policy == GPUMLIB_NMF_STREAM_NONTEMPORAL) {
+        k0[0] = k0[1] = 0;
+    } else if (policy == GPUMLIB_NMF_STREAM_AUTO && precision == GPUMLIB_NMF_BF16 &&
+               vstore_bytes(precision, n, m) > kStreamNtBytes) {
+        // the first kStreamCacheBytes of V (whole 256-row chunks of the tiled store) stay cached
+        const size_t row_bytes = (size_t)2 * 64 * lp_tiles(m);
+        const size_t rows = kStreamCacheBytes / row_bytes / 256 * 256;
+        k0[0] = rows < (size_t)n ? (int)rows : n;
+        k0[1] = 0;
+    }
+}
+
 bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int target, int tile) {
     if (rule < GPUMLIB_NMF_EUCLIDEAN || rule > GPUMLIB_NMF_ADDITIVE_EUCLIDEAN_WITH_H) return false;
     if (precision != GPUMLIB_NMF_FP32 && precision != GPUMLIB_NMF_BF16 && precision != GPUMLIB_NMF_FP8 &&
@@ -214,10 +233,7 @@ bool make_plan(int rule, int precision, int n, int m, int r, Plan* pl, int targe
         off += align256((size_t)r * pl->nbu);
     }
     pl->bytes = off;
-    const int policy = g_stream_policy.load(std::memory_order_relaxed);
-    pl->stream_nt = precision == GPUMLIB_NMF_BF16 &&
-                    (policy == GPUMLIB_NMF_STREAM_NONTEMPORAL ||
-                     (policy == GPUMLIB_NMF_STREAM_AUTO && vstore_bytes(precision, n, m) > kStreamNtBytes));
+    stream_policy_k0(precision, n, m, pl->nt_k0);
     return true;
 }
 
@@ -440,9 +456,10 @@ hipError_t run_iterations_bf16(const Plan& pl, const void* Vstore, float* W, flo
     ah.S = pl.S1;
     ah.tiles = pl.tiles1;
     ah.nt = pl.lp_nt;
-    ah.snt = pl.stream_nt;
+    ah.snt_k0 = pl.nt_k0[0];
     ah.pstride = pl.pstride1;
     aw = ah;
+    aw.snt_k0 = pl.nt_k0[1];
     aw.G = G + (size_t)pl.r * pl.rpad;
     aw.gcount = cnt + 1;
     aw.A = Hb;
@@ -487,7 +504,7 @@ void kl_bf16_args(const Plan& pl, const void* Vstore, void* ws, float eps, int w
     a->ldr = pl.ldr;
     a->rpad = pl.rpad;
     a->nt = pl.lp_nt;
-    a->snt = pl.stream_nt;
+    a->snt_k0 = pl.nt_k0[which];
     a->eps = eps;
     if (which == 0) {
         a->Ak = Wk, a->Ar = Wr, a->Cr = Hr, a->B = (const uint16_t*)Vstore;
@@ -595,6 +612,7 @@ static void fp8_args(const Plan& pl, const void* Vstore, void* ws, int which, Fp
     char* base = (char*)ws;
     const Fp8Store L = fp8_store_layout(pl.n, pl.m);
     memset(a, 0, sizeof(*a));
+    a->snt_k0 = pl.nt_k0[which];
     a->P = (float*)(base + pl.off_P);
     a->r = pl.r;
     a->ldr = pl.ldr;
@@ -897,7 +915,7 @@ hipError_t gpumlib_nmf_pass(int rule, int precision, int which, int n, int m, in
         a.G = (float*)(base + pl.off_G) + (which == 0 ? 0 : (size_t)r * pl.rpad);
         a.gcount = (int*)(base + pl.off_cnt) + which;
         a.nt = pl.lp_nt;
-        a.snt = pl.stream_nt;
+        a.snt_k0 = pl.nt_k0[which];
         a.gkeep = gslices > 0;
         if (which == 0) {
             a.A = Wb, a.B = (const uint16_t*)V, a.K = n, a.N = m, a.lda = pl.ldv8;

@@ -327,6 +327,7 @@ struct Fp8PassArgs {
     float* G;
     int* gcount;
     int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
+    int snt_k0;  // B chunks from contraction index snt_k0 on: non-temporal policy (K or more: none)
 };
 
 // Low-precision K-contiguous pass: P[s](a, u) = sum_k A(k, a) B(k, u) over slice s.
@@ -348,7 +349,8 @@ struct LpPassArgs {
     int gkeep;   // 1: leave the S gram partials (columns N.. of every slice) for the update to sum
     long aplane; // FP32X pass (B: the tiled FP32 store, x3_tiled_index): element stride between
                  // the three BF16 piece planes of A
-    int snt;     // BF16 pass: 1 reads B with the non-temporal policy (stream_load)
+    int snt_k0;  // BF16 pass: B chunks from contraction index snt_k0 on are read with the non-temporal
+                 // policy (stream_load); K or more: none
 };
 
 // KL pass over the BF16 V store: P[s](a, col) = sum_{k in slice s} A(k, a) R(k, col) with the ratio
@@ -374,7 +376,7 @@ struct KlLpArgs {
     int nt;              // 64-column store blocks per workgroup tile (2 for r <= 64)
     long pstride;
     float eps;
-    int snt;             // 1: B read with the non-temporal policy (stream_load)
+    int snt_k0;          // B chunks from contraction index snt_k0 on: non-temporal policy (K or more: none)
 };
 
 // One problem of a grouped (batched) launch: the arguments of its two passes and

@@ -61,7 +61,8 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
 // order at the end.  V, the shadows and C are padded with zeros to whole chunks and tiles:
 // padded rows contribute V = 0 (ratio 0, eps > 0); padded columns are not stored; rank rows
 // >= r of P (A rows clamped to r - 1) are not read by the update.
-// SNT: the V loads take the non-temporal policy (V too large to stay cached between passes).
+// SNT: the K slices that start at or after contraction index p.snt_k0 load their V chunks with
+// the non-temporal policy (each workgroup runs the loop copy of its slice's policy).
 template <int MB, int NT, bool SNT>
 __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     constexpr int THR = NTHREADS * NT;
@@ -129,17 +130,17 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     }
 
     ku32x4 rb[BL], rk[AKL], rr[ARL];
-    auto gload = [&](int k0) __attribute__((always_inline)) {
+    auto gload = [&](auto nt_tag, int k0) __attribute__((always_inline)) {
         // the chunk is half of a 256-deep block: 256 contiguous bytes of each of its 64 columns
         const uint16_t* crow = p.B + (size_t)(k0 >> 8) * nt64 * (TILE_LP * 256) + (k0 & 255);
-#pragma unroll
-        for (int j = 0; j < BL; j++) {
+        auto src = [&](int j) __attribute__((always_inline)) {
             const int q = tid + THR * j, tb = j / BLB, ql = q - tb * BLB * THR;
             const int b = min(t * NT + tb, nt64 - 1);
             const int col = ql / (KKL / 8), kq = (ql % (KKL / 8)) * 8;
-            const ku32x4* src = reinterpret_cast<const ku32x4*>(crow + (size_t)b * (TILE_LP * 256) + col * 256 + kq);
-            rb[j] = SNT ? stream_load(src) : *src;
-        }
+            return reinterpret_cast<const ku32x4*>(crow + (size_t)b * (TILE_LP * 256) + col * 256 + kq);
+        };
+#pragma unroll
+        for (int j = 0; j < BL; j++) rb[j] = decltype(nt_tag)::value ? stream_load(src(j)) : *src(j);
 #pragma unroll
         for (int j = 0; j < AKL; j++) {
             const int q = tid + THR * j, a = q / (KKL / 8), kq = (q % (KKL / 8)) * 8;
@@ -248,13 +249,13 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
     };
 
     const int nchunks = (kend - kbeg) / KKL;
-    auto mainloop = [&](auto sums_tag) __attribute__((always_inline)) {
-        gload(kbeg);
+    auto mainloop = [&](auto sums_tag, auto nt_tag) __attribute__((always_inline)) {
+        gload(nt_tag, kbeg);
         sstore();
         __syncthreads();
         for (int ch = 0; ch < nchunks; ch++) {
             const bool more = ch + 1 < nchunks;
-            if (more) gload(kbeg + (ch + 1) * KKL);
+            if (more) gload(nt_tag, kbeg + (ch + 1) * KKL);
             compute(sums_tag);
             __syncthreads();
             if (more) {
@@ -263,10 +264,16 @@ __global__ __launch_bounds__(NTHREADS * NT) void k_pass_kl_bf16(KlLpArgs p) {
             }
         }
     };
-    if (t == 0)
-        mainloop(std::integral_constant<bool, true>{});
+    auto run = [&](auto nt_tag) __attribute__((always_inline)) {
+        if (t == 0)
+            mainloop(std::integral_constant<bool, true>{}, nt_tag);
+        else
+            mainloop(std::integral_constant<bool, false>{}, nt_tag);
+    };
+    if (SNT && kbeg >= p.snt_k0)
+        run(std::bool_constant<SNT>{});
     else
-        mainloop(std::integral_constant<bool, false>{});
+        run(std::false_type{});
 
     if (t == 0 && tid < 4 * RP) {
         // quarters of rank a sit in lanes 4a .. 4a + 3 of one wave: (q0 + q1) + (q2 + q3)
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update_kl_rows_bf16(float* __restr
 // ---------------------------------------------------------------------------
 template <int MB, int NT>
 static void launch_kl(const KlLpArgs& a, hipStream_t st) {
-    if (a.snt)
+    if (a.snt_k0 < a.K)
         hipLaunchKernelGGL((k_pass_kl_bf16<MB, NT, true>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
     else
         hipLaunchKernelGGL((k_pass_kl_bf16<MB, NT, false>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st,

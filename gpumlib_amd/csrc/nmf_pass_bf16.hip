@@ -54,7 +54,9 @@ __device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 acc) {
 // bounds check; lanes past the last column or rank row re-read the last valid one:
 // those products land only in output columns that are not stored and in rank rows
 // >= r of P and of the gram, which no update reads.
-// SNT: the V loads take the non-temporal policy (V too large to stay cached between passes).
+// SNT: the K slices that start at or after contraction index p.snt_k0 load their V chunks with
+// the non-temporal policy (the part of a large V that the last-level cache cannot keep from one
+// pass to the next); each workgroup runs the loop copy of its slice's policy.
 template <int MB, int NT, bool SNT>
 __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassArgs p) {
     constexpr int THR = NTHREADS * NT;                // threads: 4 waves per 64 columns
@@ -88,21 +90,20 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
     const int kend = min(round_up(p.K, KB), kbeg + p.kslice);
 
     u32x4 rb[BL], ra[AL];
-    auto gload = [&](int k0) __attribute__((always_inline)) {
+    auto gload = [&](auto nt_tag, int k0) __attribute__((always_inline)) {
         // the chunk's NT store blocks of 64 columns (32 KB each, adjacent in the store: one
         // contiguous read): element (kq, col) of block b at b*TILE_LP*KB + col*KB + kq, so load q
         // (16 bytes) lands at sB[q / LPC][(q % LPC) * 8].  A block past the store's last (the
         // second half of a last, odd tile) re-reads the last one: those columns are not stored.
         const int nt64 = (p.N + TILE_LP - 1) / TILE_LP;
         const uint16_t* crow = p.B + (size_t)(k0 / KB) * nt64 * (TILE_LP * KB);
-#pragma unroll
-        for (int j = 0; j < BL; j++) {
+        auto src = [&](int j) __attribute__((always_inline)) {
             const int q = tid + THR * j, tb = j / BLB;  // store block of this load
             const int b = min(t * NT + tb, nt64 - 1);
-            const u32x4* src = reinterpret_cast<const u32x4*>(crow + (size_t)b * (TILE_LP * KB) +
-                                                              (size_t)(q - tb * BLB * THR) * 8);
-            rb[j] = SNT ? stream_load(src) : *src;
-        }
+            return reinterpret_cast<const u32x4*>(crow + (size_t)b * (TILE_LP * KB) + (size_t)(q - tb * BLB * THR) * 8);
+        };
+#pragma unroll
+        for (int j = 0; j < BL; j++) rb[j] = decltype(nt_tag)::value ? stream_load(src(j)) : *src(j);
 #pragma unroll
         for (int j = 0; j < AL; j++) {
             const int q = tid + THR * j, a = q / LPC, kq = (q % LPC) * 8;
@@ -177,13 +178,13 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
     };
 
     const int nchunks = (kend - kbeg) / KB;  // whole chunks (padded operands)
-    auto mainloop = [&](auto gram_tag) __attribute__((always_inline)) {
-        gload(kbeg);
+    auto mainloop = [&](auto gram_tag, auto nt_tag) __attribute__((always_inline)) {
+        gload(nt_tag, kbeg);
         sstore();
         __syncthreads();
         for (int ch = 0; ch < nchunks; ch++) {
             const bool more = ch + 1 < nchunks;
-            if (more) gload(kbeg + (ch + 1) * KB);
+            if (more) gload(nt_tag, kbeg + (ch + 1) * KB);
             compute(gram_tag);
             __syncthreads();
             if (more) {
@@ -192,12 +193,18 @@ __global__ __launch_bounds__(NTHREADS * NT, 2 / NT) void k_pass_tn_bf16(LpPassAr
             }
         }
     };
-    if (gram && spread)
-        mainloop(std::integral_constant<int, 2>{});
-    else if (gram)
-        mainloop(std::integral_constant<int, 1>{});
+    auto run = [&](auto nt_tag) __attribute__((always_inline)) {
+        if (gram && spread)
+            mainloop(std::integral_constant<int, 2>{}, nt_tag);
+        else if (gram)
+            mainloop(std::integral_constant<int, 1>{}, nt_tag);
+        else
+            mainloop(std::integral_constant<int, 0>{}, nt_tag);
+    };
+    if (SNT && kbeg >= p.snt_k0)
+        run(std::bool_constant<SNT>{});  // (true here; SNT keeps the copy out of the plain kernel)
     else
-        mainloop(std::integral_constant<int, 0>{});
+        run(std::false_type{});
 
     // acc[q][v] = P(a = mb*32 + acc_row(v,h), u = nb*32 + c) for block wave*BPW + q
     float* slice = p.P + (size_t)s * p.pstride;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_bf16(const float* __restric
 // ---------------------------------------------------------------------------
 template <int MB, int NT>
 static void launch_tn(const LpPassArgs& a, hipStream_t st) {
-    if (a.snt)
+    if (a.snt_k0 < a.K)
         hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT, true>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st, a);
     else
         hipLaunchKernelGGL((k_pass_tn_bf16<MB, NT, false>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS * NT), 0, st,

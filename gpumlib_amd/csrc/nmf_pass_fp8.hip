@@ -52,7 +52,9 @@ __device__ __forceinline__ f32x16 mfma_fp8(u32x4 a0, u32x4 a1, u32x4 b0, u32x4 b
 // needs a bounds check; lanes past the last column or rank row re-read the last valid one
 // (their products land only in output columns that are not stored and in rank rows >= r,
 // which no update reads).
-template <int MB>
+// SNT: the V loads take the non-temporal policy (launched when p.snt_k0 < K: every slice; this
+// pass takes the policy for all of V or none of it).
+template <int MB, bool SNT>
 __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
     constexpr int RP = MB * 32;
     constexpr int NBLK = 2 * MB;
@@ -92,8 +94,8 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pass_tn_fp8(Fp8PassArgs p) {
         const uint8_t* bblk = p.B + ((size_t)(k0 / K8) * p.tiles + t) * (TILE_LP * K8);
 #pragma unroll
         for (int j = 0; j < BL; j++) {
-            const int q = tid + NTHREADS * j;
-            rb[j] = *reinterpret_cast<const u32x4*>(bblk + (size_t)q * 16);
+            const u32x4* src = reinterpret_cast<const u32x4*>(bblk + (size_t)(tid + NTHREADS * j) * 16);
+            rb[j] = SNT ? stream_load(src) : *src;
         }
 #pragma unroll
         for (int j = 0; j < AL; j++) {
@@ -338,7 +340,10 @@ __global__ __launch_bounds__(NTHREADS) void k_shadow_fp8(const float* __restrict
 // ---------------------------------------------------------------------------
 template <int MB>
 static void launch_tn8(const Fp8PassArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_pass_tn_fp8<MB>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS), 0, st, a);
+    if (a.snt_k0 < a.K)
+        hipLaunchKernelGGL((k_pass_tn_fp8<MB, true>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_pass_tn_fp8<MB, false>), dim3(xcd_grid(a.tiles * a.S)), dim3(NTHREADS), 0, st, a);
 }
 
 hipError_t launch_pass_tn_fp8(const Fp8PassArgs& a, hipStream_t st) {

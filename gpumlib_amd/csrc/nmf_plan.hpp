@@ -15,11 +15,16 @@ constexpr float kReferenceEpsilon = 10e-9f;
 constexpr int kMaxFusedRank = 128;
 // Workgroups per pass aimed for: two resident workgroups on each of 256 CUs.
 constexpr int kTargetWorkgroups = 512;
-// V stores (both orientations) larger than this are streamed by the BF16 passes with the
-// non-temporal cache policy: 1.5 x the MI355X's 256 MB last-level cache, past which no part of
-// V survives from one pass over it to the next (the 268 MB FP8 store at 16384 x 8192 still
-// gains from what does; the 537 MB BF16 store does not).
+// V stores (both orientations) larger than this are read partly with the non-temporal cache
+// policy by the BF16 passes: 1.5 x the MI355X's 256 MB last-level cache.  The H pass's K slices
+// that start within the first kStreamCacheBytes of V keep the default policy (that part stays in
+// the cache from one iteration to the next); the other slices and all of V^T are streamed
+// non-temporally, so that they do not evict it.  Smaller stores are read with the default policy
+// throughout (they are re-read from the cache).  At 16384 x 8192 (V 268 MB per orientation, H
+// pass in 4 slices) the iteration measured 121.6 us with everything non-temporal, 113.0 with the
+// first 2 slices cached, 112.2 with 3 (this budget), 116.1 with all of V.
 constexpr size_t kStreamNtBytes = 384ull << 20;
+constexpr size_t kStreamCacheBytes = 192ull << 20;
 // Work items aimed for by the objective pass on split factors.
 constexpr int kObjectiveWorkgroups = 512;
 // Padded rank of the split factors: whole MFMA steps of 16, in the kernel's four register
@@ -39,7 +44,8 @@ struct Plan {
                           // of the tiled V and V^T and the leading dimensions of the W / H^T shadows
     int ldx_n, ldx_m;     // FP32X storage: padded contraction lengths (x3_ld of n and of m), as ldv8 / ldt8
     int lp_nt;            // BF16 / FP32X passes: 64-column blocks per workgroup (1 or 2)
-    int stream_nt;        // BF16 passes: V read with the non-temporal policy (its store > kStreamNtBytes)
+    int nt_k0[2];         // BF16 / FP8 passes (H pass over n, W pass over m): V chunks from this
+                          // contraction index on are read with the non-temporal policy (n or m: none)
     int tiles1, S1, ks1;  // H pass: column tiles (+ gram tile), K slices over n
     int tiles3, S3, ks3;  // W pass: row tiles (+ gram tile), K slices over m
     long pstride1, pstride3;

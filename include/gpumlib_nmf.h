@@ -181,11 +181,13 @@ size_t gpumlib_nmf_vstore_bytes(int precision, int n, int m);
 hipError_t gpumlib_nmf_vstore_pack(int precision, int n, int m, const float *V, void *vstore, size_t vstore_bytes,
                                    hipStream_t stream);
 
-/* Cache policy of the BF16 passes' reads of the V store (process-wide, read when an iteration or pass
- * starts; no reference counterpart).  GPUMLIB_NMF_STREAM_AUTO (default): non-temporal when the store
- * exceeds 384 MiB (no part of it survives in the 256 MB last-level cache from one pass over it to the
- * next; 16384 x 8192: 537 MB), the default policy below that (the store is then re-read from that cache);
- * _NONTEMPORAL / _CACHED force one.  Values are unaffected: only the cache policy of the loads differs. */
+/* Cache policy of the BF16 / FP8 passes' reads of the V store (process-wide, read when an iteration or
+ * pass starts; no reference counterpart).  GPUMLIB_NMF_STREAM_AUTO (default): for a BF16 store above
+ * 384 MiB (16384 x 8192: 537 MB), the H pass's K slices within the first 192 MiB of V keep the default
+ * policy and stay in the 256 MB last-level cache between iterations, and the rest of V and all of V^T
+ * are read non-temporally so as not to evict them; smaller stores, and FP8 stores, are read with the
+ * default policy (they are re-read from that cache).  _NONTEMPORAL / _CACHED force one policy for all of
+ * V.  Values are unaffected: only the cache policy of the loads differs. */
 #define GPUMLIB_NMF_STREAM_AUTO 0
 #define GPUMLIB_NMF_STREAM_NONTEMPORAL 1
 #define GPUMLIB_NMF_STREAM_CACHED 2

@@ -146,3 +146,24 @@ def test_bf16_stream_policy_bit_identical(native, cuda, rule):
         capi.check(native.gpumlib_nmf_set_stream_policy(capi.STREAM_AUTO))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     assert native.gpumlib_nmf_set_stream_policy(3) != 0
+
+
+def test_bf16_partial_stream_policy_bit_identical(native, cuda):
+    """At 16384 x 8192 (a 537 MB BF16 store) the default policy mixes the two cache policies by
+    K slice (the H pass's first slices cached, the rest of V and all of V^T non-temporal): the
+    same bits as the default policy everywhere."""
+    from gpumlib_amd import capi
+    n, m, r = 16384, 8192, 64
+    V = G.gen_euclidean(n, m, r, 11)
+    W, H = G.gen_init(n, m, r, 11)
+    outs = []
+    try:
+        for policy in (capi.STREAM_AUTO, capi.STREAM_CACHED):
+            capi.check(native.gpumlib_nmf_set_stream_policy(policy))
+            p = BF16Problem(V, W, H, n, m, r)
+            p.iterate(3)
+            outs.append((p.get_w(), p.get_h()))
+            del p
+    finally:
+        capi.check(native.gpumlib_nmf_set_stream_policy(capi.STREAM_AUTO))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
