@@ -65,6 +65,13 @@ __device__ __forceinline__ void pair_reduce(f32x16 (&acc)[MB], float* red, int c
     __syncthreads();
 }
 
+// Rank b that lane half h feeds at step kk of a W H product on the 32x32x2 MFMAs, in the passes
+// that hold a rank-contiguous copy of the A operand (r <= 64): blocks of 32 ranks, each lane half
+// taking 16 consecutive ones, so that a lane's operands of one block are 16 contiguous floats (four
+// 16-byte LDS reads instead of sixteen 4-byte ones).  Block by block, so a problem with r <= 32
+// sums in the same order on the kernels built for one or for two rank blocks.
+__device__ __forceinline__ int wh_rank(int kk, int h) { return (kk >> 4) * 32 + h * 16 + (kk & 15); }
+
 // ---------------------------------------------------------------------------
 // H pass.  KL: ratio B operand; OBJ: fused objective of the state (W, H) on
 // entry.
@@ -86,8 +93,11 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     constexpr int NCB = TW / 32;                        // 32-column blocks of the tile
     constexpr int KH = 4 / NCB;                         // waves per block, each on its k groups
     constexpr int VL = TW * KC / 4 / NTHREADS;          // float4 of the V chunk per thread
+    // TRK: the W H product reads W from a rank-contiguous copy of the chunk (wh_rank order)
+    constexpr bool TRK = NEED_WH && MB <= 2;
     __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];  // W chunk, [a][i]
     __shared__ __attribute__((aligned(16))) float sB[2][TW][LDK];  // V (or W) chunk, [u][i]
+    __shared__ __attribute__((aligned(16))) float sAt[TRK ? 2 : 1][TRK ? KC : 1][RP + 4];  // W chunk, [i][a]
     __shared__ double red[NTHREADS / 64];
     static_assert(KH == 1 || 2 * TW * LDK >= 2 * MB * 16 * 64, "pair reduction fits the V chunk buffers");
 
@@ -106,13 +116,14 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int kend = min(p.n, kbeg + p.kslice);
     const bool vec = p.vec != 0;
 
-    // Per-wave B operand of the W H product: H(b, u) for b = 2*kk + h, u = this lane's column.
+    // Per-wave B operand of the W H product: H(b, u) for b = wh_rank(kk, h) (TRK) or 2*kk + h,
+    // u = this lane's column.
     float hreg[NEED_WH ? MB * 16 : 1];
     if (NEED_WH) {
         const int u = col0 + cw * 32 + c;
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
-            const int b = 2 * kk + h;
+            const int b = TRK ? wh_rank(kk, h) : 2 * kk + h;
             hreg[kk] = (u < p.m && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
         }
     }
@@ -144,6 +155,12 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
         for (int j = 0; j < MB; j++) {
             const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 4;
             *reinterpret_cast<float4*>(&sA[buf][a][kq]) = ra[j];
+            if constexpr (TRK) {
+                sAt[buf][kq][a] = ra[j].x;
+                sAt[buf][kq + 1][a] = ra[j].y;
+                sAt[buf][kq + 2][a] = ra[j].z;
+                sAt[buf][kq + 3][a] = ra[j].w;
+            }
         }
     };
 
@@ -177,9 +194,25 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             // wh(i, u) for the 32 rows of this step and the wave's 32 columns.
             f32x16 wh;
             for (int v = 0; v < 16; v++) wh[v] = 0.f;
+            if constexpr (TRK) {
 #pragma unroll
-            for (int kk = 0; kk < MB * 16; kk++)
-                if (kk < nb * 16) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
+                for (int blk = 0; blk < MB; blk++) {
+                    if (blk >= nb) continue;
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; q4++) {
+                        const float4 w4 = *reinterpret_cast<const float4*>(&sAt[buf][c][blk * 32 + h * 16 + q4 * 4]);
+                        const int kk = blk * 16 + q4 * 4;
+                        wh = mfma32x2(w4.x, hreg[kk], wh);
+                        wh = mfma32x2(w4.y, hreg[kk + 1], wh);
+                        wh = mfma32x2(w4.z, hreg[kk + 2], wh);
+                        wh = mfma32x2(w4.w, hreg[kk + 3], wh);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < MB * 16; kk++)
+                    if (kk < nb * 16) wh = mfma32x2(sA[buf][2 * kk + h][c], hreg[kk], wh);
+            }
             // wh[v] is row i = acc_row(v, h) of the chunk, column c of the wave's block; with
             // 64-column tiles the wave takes the k groups g of its half only.
 #pragma unroll
@@ -292,7 +325,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
             for (int kk = 0; kk < MB * 16; kk++) {
                 const int b = 2 * kk + h;
-                if constexpr (NEED_WH)
+                if constexpr (NEED_WH && !TRK)
                     hb[kk] = hreg[kk];
                 else
                     hb[kk] = (ucol && b < p.r) ? p.H[(size_t)u * p.r + b] : 0.f;
@@ -594,8 +627,10 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     constexpr int VQ = TW / 4;      // float4 per chunk row u
     constexpr int VL = KC * VQ / NTHREADS;
     constexpr int PV = TW + 8;  // rows u and u+4 land 32 banks apart
+    constexpr bool TRK = MB <= 2;  // the H^T W^T product reads H from a rank-contiguous copy (wh_rank order)
     __shared__ __attribute__((aligned(16))) float sV[2][KC][PV];   // [u][i]
     __shared__ __attribute__((aligned(16))) float sH[2][RP][LDK];  // [a][u]
+    __shared__ __attribute__((aligned(16))) float sHu[TRK ? 2 : 1][TRK ? KC : 1][RP + 4];  // [u][a]
     static_assert(KH == 1 || 2 * KC * PV >= 2 * MB * 16 * 64, "pair reduction fits the V chunk buffers");
 
     const int s = w / p.tiles, t = w - s * p.tiles;
@@ -615,13 +650,13 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
     constexpr int BQ = RP / 4;
     constexpr int BLOADS = (KC * BQ + NTHREADS - 1) / NTHREADS;
 
-    // B operand of H^T W^T: W(i, b) for i = this lane's row, b = 2*kk + h.
+    // B operand of H^T W^T: W(i, b) for i = this lane's row, b = wh_rank(kk, h) (TRK) or 2*kk + h.
     float wreg[MB * 16];
     {
         const int i = row0 + cw * 32 + c;
 #pragma unroll
         for (int kk = 0; kk < MB * 16; kk++) {
-            const int b = 2 * kk + h;
+            const int b = TRK ? wh_rank(kk, h) : 2 * kk + h;
             wreg[kk] = (i < p.n && b < p.r) ? p.W[(size_t)b * p.n + i] : 0.f;
         }
     }
@@ -653,6 +688,7 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
                 sH[buf][aq + 1][uu] = rb[j].y;
                 sH[buf][aq + 2][uu] = rb[j].z;
                 sH[buf][aq + 3][uu] = rb[j].w;
+                if constexpr (TRK) *reinterpret_cast<float4*>(&sHu[buf][uu][aq]) = rb[j];
             }
         }
     };
@@ -682,9 +718,25 @@ __device__ __forceinline__ void pass_w_kl_body(const PassArgs& p, const int w, c
         // tiles the wave takes the k groups g of its half only.
         f32x16 wt;
         for (int v = 0; v < 16; v++) wt[v] = 0.f;
+        if constexpr (TRK) {
 #pragma unroll
-        for (int kk = 0; kk < MB * 16; kk++)
-            if (kk < nb * 16) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
+            for (int blk = 0; blk < MB; blk++) {
+                if (blk >= nb) continue;
+#pragma unroll
+                for (int q4 = 0; q4 < 4; q4++) {
+                    const float4 h4 = *reinterpret_cast<const float4*>(&sHu[buf][c][blk * 32 + h * 16 + q4 * 4]);
+                    const int kk = blk * 16 + q4 * 4;
+                    wt = mfma32x2(h4.x, wreg[kk], wt);
+                    wt = mfma32x2(h4.y, wreg[kk + 1], wt);
+                    wt = mfma32x2(h4.z, wreg[kk + 2], wt);
+                    wt = mfma32x2(h4.w, wreg[kk + 3], wt);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < MB * 16; kk++)
+                if (kk < nb * 16) wt = mfma32x2(sH[buf][2 * kk + h][c], wreg[kk], wt);
+        }
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             if (KH == 2 && (g >> 1) != kh) continue;
