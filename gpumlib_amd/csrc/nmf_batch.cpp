@@ -688,28 +688,34 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
         bk[k] = c;
     };
     // The configuration in effect first.  Then the tile depth along K: the fused engine (one
-    // slice, the whole contraction per tile) and the split-K targets, one for every bucket; then,
-    // split-K, each present bucket's target one step deeper and shallower than the best (the
-    // other buckets kept).  Then, per present bucket, the other tile width; then the pass grids
-    // and the update (fused: gram) grids around the best so far; then, per bucket with problems
-    // of r <= 32, the other rank blocking.
+    // slice, the whole contraction per tile) with every bucket on 128- and on 64-wide tiles, and
+    // the split-K targets, one for every bucket, on 128-wide tiles; then, split-K, each present
+    // bucket's target one step deeper and shallower than the best (the other buckets kept).  Then,
+    // per present bucket, the other tile width; then the pass grids and the update (fused: gram)
+    // grids around the best so far; then, per bucket with problems of r <= 32, the other rank
+    // blocking.
     if (e == hipSuccess) e = copy_wh(true);
-    if (e == hipSuccess) e = trial(en0, bk0, pg0, ug0);
-    if (e == hipSuccess && en0 != ENGINE_FUSED) e = trial(ENGINE_FUSED, bk0, pg0, ug0);
-    for (int t : kTuneTargets) {
-        BucketSet bk;
-        bool same = en0 == ENGINE_SPLIT;
+    // the shapes in effect with every bucket on one tile width (and the split-K target of each)
+    auto uniform = [&](int tile, int target, BucketSet& bk) {
+        bool same = true;
         for (int k = 0; k < kBuckets; k++) {
             bk[k] = bk0[k];
-            bk[k].target = t;
-            same = same && bk0[k].target == t;
+            bk[k].tile = tile;
+            if (target > 0) bk[k].target = target;
+            same = same && bk[k] == bk0[k];
         }
-        if (e == hipSuccess && !same && fits(t, TILE, -1)) {
-            // (the tile widths in effect stay; a bucket on 64-wide tiles is checked for them too)
-            bool ok = true;
-            for (int k = 0; k < kBuckets; k++) ok = ok && (!present[k] || fits(t, bk[k].tile, k));
-            if (ok) e = trial(ENGINE_SPLIT, bk, pg0, ug0);
-        }
+        return same;
+    };
+    if (e == hipSuccess) e = trial(en0, bk0, pg0, ug0);
+    for (int tile : {TILE, 64}) {  // the fused engine at either uniform width
+        BucketSet bk;
+        const bool same = uniform(tile, 0, bk) && en0 == ENGINE_FUSED;
+        if (e == hipSuccess && !same && fits(kBatchTargetWorkgroups, tile, -1)) e = trial(ENGINE_FUSED, bk, pg0, ug0);
+    }
+    for (int t : kTuneTargets) {  // split-K targets on 128-wide tiles
+        BucketSet bk;
+        const bool same = uniform(TILE, t, bk) && en0 == ENGINE_SPLIT;
+        if (e == hipSuccess && !same && fits(t, TILE, -1)) e = trial(ENGINE_SPLIT, bk, pg0, ug0);
     }
     const int nt = (int)(sizeof(kTuneTargets) / sizeof(kTuneTargets[0]));
     if (best.engine == ENGINE_SPLIT) {
