@@ -97,14 +97,32 @@ struct TuneConfig {
     int trials;
 };
 
-// Fused engine: one slice per pass, the whole contraction per work item.  Its workspace
-// use: the grams of W and H (r x rpad floats each) from offset 0, then one objective
-// partial per column tile of the H pass (inside the split-K plan's workspace: the fused
-// engine never reads the split-K partials and the split-K passes rewrite them).
-size_t fused_objp_offset(int r) { return (2 * (size_t)r * round_up(r, 4) * sizeof(float) + 255) / 256 * 256; }
-size_t fused_bytes(int m, int r, int tile) {
-    return fused_objp_offset(r) + ((size_t)((m + tile - 1) / tile) * sizeof(double) + 255) / 256 * 256;
+// Fused engine: one slice per pass, the whole contraction per work item.  The grams of W
+// (over the n rows) and of H (over the m columns) come from the gram launches in K slices of
+// at most kGramSlice elements, one work item each (so a long contraction no longer holds the
+// launch back on one workgroup); the fused passes sum a gram's slices in slice order as they
+// stage it.  Workspace use: the W gram's slices (r x rpad floats each) from offset 0, then the
+// H gram's, then one objective partial per column tile of the H pass (inside the split-K plan's
+// workspace when that is larger: the fused engine never reads the split-K partials and the
+// split-K passes rewrite them).
+constexpr int kGramSlice = 1024;
+struct FusedLayout {
+    int gsl_w, gks_w, gsl_h, gks_h;  // gram slices and their length (W gram over n, H gram over m)
+    size_t off_gh, off_objp, bytes;
+};
+FusedLayout fused_layout(int n, int m, int r, int tile) {
+    FusedLayout L;
+    L.gsl_w = (n + kGramSlice - 1) / kGramSlice;
+    L.gks_w = round_up((n + L.gsl_w - 1) / L.gsl_w, 128);
+    L.gsl_h = (m + kGramSlice - 1) / kGramSlice;
+    L.gks_h = round_up((m + L.gsl_h - 1) / L.gsl_h, 128);
+    const size_t g = (size_t)r * round_up(r, 4) * sizeof(float);
+    L.off_gh = L.gsl_w * g;
+    L.off_objp = (L.off_gh + L.gsl_h * g + 255) / 256 * 256;
+    L.bytes = L.off_objp + ((size_t)((m + tile - 1) / tile) * sizeof(double) + 255) / 256 * 256;
+    return L;
 }
+size_t fused_bytes(int n, int m, int r, int tile) { return fused_layout(n, m, r, tile).bytes; }
 
 }  // namespace
 
@@ -146,8 +164,8 @@ static int table_items(const gpumlib_nmf_batch* b, const Slot& sl, int kind, lon
         case T_UH: return !fused ? (sl.m + UPD_COLS - 1) / UPD_COLS : 0;
         case T_UW: return !fused ? (sl.n + UPD_ROWS - 1) / UPD_ROWS : 0;
         case T_RED: return 1;
-        case T_GW: *cost = (long)sl.n * sl.r; return fused && euc ? 1 : 0;
-        case T_GH: *cost = (long)sl.m * sl.r; return fused && euc ? 1 : 0;
+        case T_GW: *cost = (long)sl.args.h.gks * sl.r; return fused && euc ? sl.args.h.gsl : 0;
+        case T_GH: *cost = (long)sl.args.w.gks * sl.r; return fused && euc ? sl.args.w.gsl : 0;
         default: break;
     }
     const int pk = (kind - T_PASS0) / kVariants, v = (kind - T_PASS0) % kVariants;
@@ -285,9 +303,9 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, const BucketCfg& c, 
     a.nobj = pl.tiles1 * pl.S1;
     sl.tile = c.tile;
     if (b->engine == ENGINE_FUSED) {
-        if (fused_bytes(sl.m, sl.r, c.tile) > sl.ws_bytes) return hipErrorInvalidValue;
+        const FusedLayout L = fused_layout(sl.n, sl.m, sl.r, c.tile);
+        if (L.bytes > sl.ws_bytes) return hipErrorInvalidValue;
         char* base = (char*)sl.ws;
-        const size_t gsz = (size_t)sl.r * round_up(sl.r, 4);
         for (PassArgs* p : {&a.h, &a.w}) {
             p->S = 1;
             p->P = nullptr;
@@ -296,11 +314,15 @@ static hipError_t plan_slot(gpumlib_nmf_batch* b, Slot& sl, const BucketCfg& c, 
         }
         a.h.tiles = (sl.m + c.tile - 1) / c.tile;
         a.h.kslice = round_up(sl.n, KC);
-        a.h.G = (float*)base;  // W^T W
-        a.h.objp = (double*)(base + fused_objp_offset(sl.r));
+        a.h.G = (float*)base;  // W^T W, in gsl slices over n
+        a.h.gsl = L.gsl_w;
+        a.h.gks = L.gks_w;
+        a.h.objp = (double*)(base + L.off_objp);
         a.w.tiles = (sl.n + c.tile - 1) / c.tile;
         a.w.kslice = round_up(sl.m, KC);
-        a.w.G = (float*)base + gsz;  // H H^T
+        a.w.G = (float*)(base + L.off_gh);  // H H^T, in gsl slices over m
+        a.w.gsl = L.gsl_h;
+        a.w.gks = L.gks_h;
         a.w.objp = nullptr;
         a.nobj = a.h.tiles;
         sl.args = a;
@@ -466,7 +488,7 @@ size_t gpumlib_nmf_batch_workspace_bytes(gpumlib_nmf_batch_handle b, int rule, i
         Plan pl;
         if (!make_plan(rule, GPUMLIB_NMF_FP32, n, m, r, &pl, std::max(kBatchTargetWorkgroups, b->size_target), tile))
             return 0;
-        most = std::max(most, std::max(pl.bytes, fused_bytes(m, r, tile)));
+        most = std::max(most, std::max(pl.bytes, fused_bytes(n, m, r, tile)));
     }
     return most;
 }
@@ -622,7 +644,7 @@ extern "C" hipError_t gpumlib_nmf_batch_autotune(gpumlib_nmf_batch_handle b, int
             if (!sl.active || (bucket >= 0 && sl.bucket != bucket)) continue;
             Plan pl;
             if (!make_plan(sl.rule, GPUMLIB_NMF_FP32, sl.n, sl.m, sl.r, &pl, target, tile) || pl.bytes > sl.ws_bytes ||
-                fused_bytes(sl.m, sl.r, tile) > sl.ws_bytes)
+                fused_bytes(sl.n, sl.m, sl.r, tile) > sl.ws_bytes)
                 return false;
         }
         return true;

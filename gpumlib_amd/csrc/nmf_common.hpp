@@ -289,6 +289,8 @@ struct PassArgs {
     int nbi;             // FP8 storage: 32-row blocks of V
     float* G;            // reduced gram of this pass's factor: gram(a,b) at G[b*rpad + a]
     int* gcount;         // arrival counter of the gram-carrying workgroups (zero between passes)
+    int gsl, gks;        // fused batch engine: the gram comes in gsl K slices of gks elements (slice s
+                         // at G + s*r*rpad), summed in slice order where it is read
 };
 
 // Objective pass over the split factors (nmf_objective.hip): V (FP32 n x m with leading

@@ -316,7 +316,10 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             const int rpad = round_up(p.r, 4);
             for (int e = tid; e < RP * RP; e += NTHREADS) {
                 const int a = e / RP, b = e - a * RP;
-                sG[a * GS + b] = (a < p.r && b < p.r) ? p.G[(size_t)b * rpad + a] : 0.f;
+                float g = 0.f;
+                if (a < p.r && b < p.r)
+                    for (int gs = 0; gs < p.gsl; gs++) g += p.G[(size_t)gs * p.r * rpad + (size_t)b * rpad + a];
+                sG[a * GS + b] = g;
             }
         }
         __syncthreads();
@@ -541,7 +544,10 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
         const int rpad = round_up(p.r, 4);
         for (int e = tid; e < RP * RP; e += NTHREADS) {
             const int b = e / RP, a = e - b * RP;
-            sG[b * GS + a] = (a < p.r && b < p.r) ? p.G[(size_t)a * rpad + b] : 0.f;
+            float g = 0.f;
+            if (a < p.r && b < p.r)
+                for (int gs = 0; gs < p.gsl; gs++) g += p.G[(size_t)gs * p.r * rpad + (size_t)a * rpad + b];
+            sG[b * GS + a] = g;
         }
         __syncthreads();
         if (!owner) return;
@@ -917,14 +923,14 @@ hipError_t launch_group_pass_w(const JobArgs* jobs, const int* table, int kl, in
 // contraction, and applies that tile's update itself (pass bodies with FUSE).  The
 // Euclidean updates take their gram from a gram launch that precedes the pass.
 
-// G = X^T X over the whole contraction of one factor, written G(a,b) at G[b*rpad + a]:
-// WHICH 0: X = W (n x r, column-major, k = i); WHICH 1: X = H^T (H r x m, k = u).
-// Stages of KG contraction elements through LDS (loads of the next stage in flight);
-// the 4 waves take interleaved k steps of every stage, and their partial blocks are
-// added in wave order.
+// G = X^T X over the contraction range [k0, k1) of one factor (k0 a multiple of 128), written
+// G(a,b) at G[b*rpad + a]: WHICH 0: X = W (n x r, column-major, k = i); WHICH 1: X = H^T (H r x m,
+// k = u).  Stages of KG contraction elements through LDS (loads of the next stage in flight);
+// the 4 waves take interleaved k steps of every stage, and their partial blocks are added in
+// wave order.
 template <int WHICH>
-__device__ __forceinline__ void gram_body(const float* __restrict__ X, int K, int r, int ld, float* __restrict__ G,
-                                          bool vec) {
+__device__ __forceinline__ void gram_body(const float* __restrict__ X, int k0, int k1, int r, int ld,
+                                          float* __restrict__ G, bool vec) {
     constexpr int RP = 64, KG = 128, XP = RP + 4;
     constexpr int LOADS = RP * KG / 4 / NTHREADS;  // float4 per thread per stage
     __shared__ __attribute__((aligned(16))) float sX[KG][XP];  // sX[k][a] = X(k, a)
@@ -933,16 +939,16 @@ __device__ __forceinline__ void gram_body(const float* __restrict__ X, int K, in
     const int c = lane & 31, h = lane >> 5;
     const int nb = (r + 31) >> 5;
     float4 rx[LOADS];
-    auto gload = [&](int k0) __attribute__((always_inline)) {
+    auto gload = [&](int ks) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < LOADS; j++) {
             const int q = tid + NTHREADS * j;
             if (WHICH == 0) {  // W(k, a) = W[a*ld + k]: 4 consecutive k of row a
                 const int a = q / (KG / 4), kq = (q % (KG / 4)) * 4;
-                rx[j] = load4(X + (size_t)a * ld + k0 + kq, a < r, k0 + kq, K, vec);
+                rx[j] = load4(X + (size_t)a * ld + ks + kq, a < r, ks + kq, k1, vec);
             } else {  // H(a, k) = H[k*ld + a]: 4 consecutive a of column k
                 const int k = q / (RP / 4), aq = (q % (RP / 4)) * 4;
-                rx[j] = load4(X + (size_t)(k0 + k) * ld + aq, k0 + k < K, aq, r, vec);
+                rx[j] = load4(X + (size_t)(ks + k) * ld + aq, ks + k < k1, aq, r, vec);
             }
         }
     };
@@ -968,12 +974,12 @@ __device__ __forceinline__ void gram_body(const float* __restrict__ X, int K, in
 #pragma unroll
         for (int y = 0; y < 2; y++)
             for (int v = 0; v < 16; v++) gacc[x][y][v] = 0.f;
-    const int stages = (K + KG - 1) / KG;
-    if (stages > 0) gload(0);
+    const int stages = (k1 - k0 + KG - 1) / KG;
+    if (stages > 0) gload(k0);
     for (int st = 0; st < stages; st++) {
         sstore();
         __syncthreads();
-        if (st + 1 < stages) gload((st + 1) * KG);
+        if (st + 1 < stages) gload(k0 + (st + 1) * KG);
 #pragma unroll
         for (int j = 0; j < KG / 8; j++) {
             const int k = 2 * (wave + 4 * j) + h;
@@ -1019,11 +1025,15 @@ __global__ __launch_bounds__(NTHREADS) void k_group_gram(const JobArgs* __restri
         const int j = group_find(first, J, item);
         const JobArgs& ja = jobs[idx[j]];
         const PassArgs& p = WHICH == 0 ? ja.h : ja.w;
+        // K slice s of the gram: [s*gks, min(K, (s+1)*gks)), its partial at G + s*r*rpad
+        const int s = item - first[j], K = WHICH == 0 ? p.n : p.m;
+        const int k0 = s * p.gks, k1 = min(K, k0 + p.gks);
+        float* G = p.G + (size_t)s * p.r * round_up(p.r, 4);
         __syncthreads();
         if (WHICH == 0)
-            gram_body<0>(p.W, p.n, p.r, p.n, p.G, p.vec != 0);
+            gram_body<0>(p.W, k0, k1, p.r, p.n, G, p.vec != 0);
         else
-            gram_body<1>(p.H, p.m, p.r, p.r, p.G, p.vec != 0);
+            gram_body<1>(p.H, k0, k1, p.r, p.r, G, p.vec != 0);
     }
 }
 

@@ -43,6 +43,7 @@ EDGE_JOBS = [  # odd shapes for the fused engine's tiles: partial tiles, unalign
     (G.KL, 50, 4096, 17),
     (G.EUCLIDEAN, 333, 129, 63),
     (G.KL, 129, 1000, 64),
+    (G.EUCLIDEAN, 90, 3000, 40),  # fused engine: the H gram in 3 K slices (the W gram of 2048 rows in 2)
 ]
 
 
