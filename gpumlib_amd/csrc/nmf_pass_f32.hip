@@ -223,7 +223,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
                 float rj[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    if (OBJ != OBJ_NONE) objacc += obj_term<OBJ>(vj[j], wh[g * 4 + j], p.eps);
+                    if (OBJ != OBJ_NONE) objacc += (double)obj_term_f32<OBJ>(vj[j], wh[g * 4 + j], p.eps);
                     if (KL) rj[j] = kl_ratio(vj[j], wh[g * 4 + j], p.eps);
                 }
                 if (KL) {
