@@ -159,7 +159,7 @@ struct gpumlib_nmf_server {
     // continuous batching: uploads of the next round's problems and copies of finished ones
     // run on their own stream while a round runs on `stream`
     hipStream_t copy_stream = nullptr;
-    hipEvent_t ev_staged = nullptr, ev_retired = nullptr;
+    hipEvent_t ev_staged = nullptr, ev_retired = nullptr, ev_ready = nullptr;
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
     void run_batched();
@@ -315,7 +315,9 @@ void gpumlib_nmf_server::fail_all_locked(hipError_t e, std::vector<std::shared_p
 // were copied back on the copy stream), releases their memory, admits queued problems
 // into it and uploads them on the copy stream; at the round boundary only the objectives
 // are read, the finished problems leave the batch (their copies queued) and the uploaded
-// ones join it (the round stream waits for their upload).
+// ones join it (the round stream waits for their upload, long done).  Problems admitted at
+// the boundary itself are uploaded then and join at the next boundary, unless the batch
+// would otherwise have nothing to run.
 void gpumlib_nmf_server::run_batched() {
     const int rounds_iters = cfg.round_iterations;
     std::vector<std::shared_ptr<Job>> staged;    // uploaded (or uploading) on copy_stream, not yet in the batch
@@ -373,13 +375,23 @@ void gpumlib_nmf_server::run_batched() {
                 cv_work.wait(lk);
             }
         }
+        // The problems uploaded while the last round ran join at this boundary; those admitted
+        // just now are uploaded now and join at the next one, so that the round does not wait
+        // for their copies (from pageable host memory) -- unless nothing would run otherwise.
+        std::vector<std::shared_ptr<Job>> ready;
+        ready.swap(staged);
+        if (e == hipSuccess && !ready.empty()) e = hipEventRecord(ev_ready, copy_stream);
         upload(admitted);
         finish_retiring();
-        if (!have_active && staged.empty()) continue;  // only retirements were pending
-        // round boundary: the staged problems join the batch
+        if (!have_active && ready.empty()) {
+            if (staged.empty()) continue;  // only retirements were pending
+            ready.swap(staged);
+            if (e == hipSuccess) e = hipEventRecord(ev_ready, copy_stream);
+        }
+        // round boundary: the ready problems join the batch
         std::vector<std::shared_ptr<Job>> joined;
-        if (e == hipSuccess && !staged.empty()) e = hipStreamWaitEvent(stream, ev_staged, 0);
-        for (auto& jp : staged) {
+        if (e == hipSuccess && !ready.empty()) e = hipStreamWaitEvent(stream, ev_ready, 0);
+        for (auto& jp : ready) {
             Job& j = *jp;
             char* base = d_pool + j.off;
             int slot = -1;
@@ -399,7 +411,7 @@ void gpumlib_nmf_server::run_batched() {
                 j.done = true;
             }
         }
-        staged.clear();
+        ready.clear();
         if (e == hipSuccess && cfg.autotune > 0) {
             const int resident = gpumlib_nmf_batch_active(batch);
             if (resident >= 4 && (tuned_at == 0 || resident >= 2 * tuned_at || 2 * resident <= tuned_at)) {
@@ -432,7 +444,7 @@ void gpumlib_nmf_server::run_batched() {
         // while it runs: the joined problems' host inputs are released once uploaded, and
         // queued problems are admitted and uploaded for the next round
         if (!joined.empty()) {
-            const hipError_t ce = hipEventSynchronize(ev_staged);
+            const hipError_t ce = hipEventSynchronize(ev_ready);
             if (e == hipSuccess) e = ce;
             for (auto& jp : joined) std::vector<float>().swap(jp->V);
         }
@@ -440,7 +452,7 @@ void gpumlib_nmf_server::run_batched() {
             std::vector<std::shared_ptr<Job>> next;
             {
                 std::lock_guard<std::mutex> lk(mu);
-                admit_locked(next, 0);
+                admit_locked(next, (int)staged.size());  // (staged: admitted at the boundary, not yet in the batch)
             }
             upload(next);
         }
@@ -540,6 +552,7 @@ hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpuml
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_staged, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_retired, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_ready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_obj, sizeof(double) * 2 * cfg->max_active, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_obj, 2 * sizeof(double));
     if (e != hipSuccess) {
@@ -549,6 +562,7 @@ hipError_t gpumlib_nmf_server_create(const gpumlib_nmf_server_config* cfg, gpuml
         if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
         if (s->ev_staged) (void)hipEventDestroy(s->ev_staged);
         if (s->ev_retired) (void)hipEventDestroy(s->ev_retired);
+        if (s->ev_ready) (void)hipEventDestroy(s->ev_ready);
         if (s->batch) gpumlib_nmf_batch_destroy(s->batch);
         if (s->d_pool) (void)hipFree(s->d_pool);
         delete s;
@@ -655,6 +669,7 @@ void gpumlib_nmf_server_destroy(gpumlib_nmf_server_handle s) {
     (void)hipStreamDestroy(s->copy_stream);
     (void)hipEventDestroy(s->ev_staged);
     (void)hipEventDestroy(s->ev_retired);
+    (void)hipEventDestroy(s->ev_ready);
     gpumlib_nmf_batch_destroy(s->batch);
     (void)hipFree(s->d_pool);
     delete s;
