@@ -15,7 +15,7 @@
  *   H is r x m, H(a,u) = H[u*r + a]
  *
  * Two arithmetic flavours exist:
- *   *_f64  double precision: the judge for every parity test;
+ *   *_f64  double precision: the expected values of every parity test;
  *   *_f32  single precision, reference-faithful operation order (FP32 is the
  *          reference's cudafloat, src/cuda/definitions.h:68);
  *   *_f32_omp  the same FP32 rules, cache-blocked and OpenMP-parallel: the

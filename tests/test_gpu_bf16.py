@@ -1,6 +1,5 @@
 """BF16 storage path: V packed to BF16 (V and V^T), BF16 MFMA passes with FP32
-accumulation, FP32 updates.  Two objective gaps are checked (the north-star
-protocol): (i) against the FP64 oracle on the same BF16-quantised V (pure
+accumulation, FP32 updates.  Two objective gaps are checked: (i) against the FP64 oracle on the same BF16-quantised V (pure
 arithmetic error of the BF16 operands) and (ii) against the FP64 oracle on the
 FP32 V (storage error included)."""
 import numpy as np

@@ -1,6 +1,6 @@
 """Parity of the fused gfx950 iteration (gpumlib_nmf_iterate) with the FP64 oracle.
 
-Gate (north star): after a fixed number of iterations from the same seeded
+Gate: after a fixed number of iterations from the same seeded
 inputs, W, H and the objective match the FP64 restatement of the reference
 update rules within 1e-4 relative Frobenius error.
 """
