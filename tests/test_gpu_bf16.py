@@ -125,11 +125,12 @@ def test_bf16_deterministic(native, cuda):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("rule", [G.EUCLIDEAN, G.KL])
-def test_bf16_stream_policy_bit_identical(native, cuda, rule):
+@pytest.mark.parametrize("rule,precision", [(G.EUCLIDEAN, G.BF16), (G.KL, G.BF16), (G.EUCLIDEAN, G.FP8)])
+def test_bf16_stream_policy_bit_identical(native, cuda, rule, precision):
     """The BF16 passes read V with the non-temporal cache policy when the store is too large to stay
     cached (16384 x 8192 and up) and with the default policy below: both policies forced on one
-    problem give the same bits (only the loads' cache policy differs)."""
+    problem give the same bits (only the loads' cache policy differs); the FP8 passes take the
+    forced policy too."""
     from gpumlib_amd import capi
     n, m, r = 900, 700, 32
     V = G.gen_euclidean(n, m, r, 7) if rule == G.EUCLIDEAN else G.gen_poisson(n, m, r, 7)
@@ -138,7 +139,7 @@ def test_bf16_stream_policy_bit_identical(native, cuda, rule):
     try:
         for policy in (capi.STREAM_NONTEMPORAL, capi.STREAM_CACHED):
             capi.check(native.gpumlib_nmf_set_stream_policy(policy))
-            p = G.DeviceNMF(rule, V, W, H, n, m, r, eps=G.REFERENCE_EPS, precision=G.BF16)
+            p = G.DeviceNMF(rule, V, W, H, n, m, r, eps=G.REFERENCE_EPS, precision=precision)
             p.iterate(10)
             outs.append((p.get_w(), p.get_h()))
     finally:
