@@ -95,7 +95,11 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     constexpr int VL = TW * KC / 4 / NTHREADS;          // float4 of the V chunk per thread
     // TRK: the W H product reads W from a rank-contiguous copy of the chunk (wh_rank order)
     constexpr bool TRK = NEED_WH && MB <= 2;
-    __shared__ __attribute__((aligned(16))) float sA[2][RP][LDK];  // W chunk, [a][i]
+    // SWA: the plain Euclidean pass keeps the W chunk unpadded, its 16-byte groups XOR-swizzled by
+    // row (as free of bank conflicts as the padded rows, 3 KB less LDS per workgroup)
+    constexpr bool SWA = !NEED_WH && !FUSE;
+    constexpr int LDA = SWA ? KC : LDK;
+    __shared__ __attribute__((aligned(16))) float sA[2][RP][LDA];  // W chunk, [a][i]
     __shared__ __attribute__((aligned(16))) float sB[2][TW][LDK];  // V (or W) chunk, [u][i]
     __shared__ __attribute__((aligned(16))) float sAt[TRK ? 2 : 1][TRK ? KC : 1][RP + 4];  // W chunk, [i][a]
     __shared__ double red[NTHREADS / 64];
@@ -115,6 +119,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
     const int kbeg = s * p.kslice;
     const int kend = min(p.n, kbeg + p.kslice);
     const bool vec = p.vec != 0;
+    auto ac = [](int row, int col) __attribute__((always_inline)) { return SWA ? (col ^ ((row & 7) << 2)) : col; };
 
     // Per-wave B operand of the W H product: H(b, u) for b = wh_rank(kk, h) (TRK) or 2*kk + h,
     // u = this lane's column.
@@ -154,7 +159,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
         for (int j = 0; j < MB; j++) {
             const int q = tid + NTHREADS * j, a = q >> 3, kq = (q & 7) * 4;
-            *reinterpret_cast<float4*>(&sA[buf][a][kq]) = ra[j];
+            *reinterpret_cast<float4*>(&sA[buf][a][ac(a, kq)]) = ra[j];
             if constexpr (TRK) {
                 sAt[buf][kq][a] = ra[j].x;
                 sAt[buf][kq + 1][a] = ra[j].y;
@@ -247,7 +252,7 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
                     if (mb >= nb) continue;
-                    const float4 av = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][g * 8 + h * 4]);
+                    const float4 av = *reinterpret_cast<const float4*>(&sA[buf][mb * 32 + c][ac(c, g * 8 + h * 4)]);
                     acc[mb] = mfma32x2(av.x, bv.x, acc[mb]);
                     acc[mb] = mfma32x2(av.y, bv.y, acc[mb]);
                     acc[mb] = mfma32x2(av.z, bv.z, acc[mb]);
@@ -256,8 +261,8 @@ __device__ __forceinline__ void pass_h_body(const PassArgs& p, const int w, cons
             }
         }
         if constexpr (GR) {  // this wave's k group (wave*8 .. wave*8+7) of the chunk for the carried block
-            const float4 av = *reinterpret_cast<const float4*>(&sA[buf][gmb * 32 + c][wave * 8 + h * 4]);
-            const float4 bv = *reinterpret_cast<const float4*>(&sA[buf][gnb * 32 + c][wave * 8 + h * 4]);
+            const float4 av = *reinterpret_cast<const float4*>(&sA[buf][gmb * 32 + c][ac(c, wave * 8 + h * 4)]);
+            const float4 bv = *reinterpret_cast<const float4*>(&sA[buf][gnb * 32 + c][ac(c, wave * 8 + h * 4)]);
             gacc = mfma32x2(av.x, bv.x, gacc);
             gacc = mfma32x2(av.y, bv.y, gacc);
             gacc = mfma32x2(av.z, bv.z, gacc);
@@ -416,8 +421,14 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
     constexpr int KH = 4 / NCB;     // waves per block, each on its k steps
     constexpr int VQ = TW / 4;      // float4 per chunk row u
     constexpr int VL = KC * VQ / NTHREADS;
-    constexpr int PA = TW + 32;  // rows k and k+1 land 32 banks apart
-    constexpr int PB = RP + 32;
+    // rows k and k+1 land 32 banks apart: by padding (fused pass, whose gram staging reuses the
+    // buffers) or, unpadded, by swapping the two 32-float halves of every odd row (16 KB less LDS
+    // per workgroup at r = 64)
+    constexpr bool SW = !FUSE;
+    constexpr int PA = SW ? TW : TW + 32;
+    constexpr int PB = SW ? RP : RP + 32;
+    constexpr int XA = SW ? 32 : 0;
+    constexpr int XB = (SW && RP % 64 == 0) ? 32 : 0;
     __shared__ __attribute__((aligned(16))) float sA[2][KC][PA];  // [u][i]
     __shared__ __attribute__((aligned(16))) float sB[2][KC][PB];  // [u][a]
     static_assert(KH == 1 || 2 * KC * PB >= 2 * MB * 16 * 64, "pair reduction fits the H chunk buffers");
@@ -461,12 +472,12 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
 #pragma unroll
         for (int j = 0; j < VL; j++) {
             const int q = tid + NTHREADS * j, uu = q / VQ, iq = (q % VQ) * 4;
-            *reinterpret_cast<float4*>(&sA[buf][uu][iq]) = ra[j];
+            *reinterpret_cast<float4*>(&sA[buf][uu][iq ^ ((uu & 1) ? XA : 0)]) = ra[j];
         }
 #pragma unroll
         for (int j = 0; j < BLOADS; j++) {
             const int q = tid + NTHREADS * j, uu = q / BQ, aq = (q % BQ) * 4;
-            if (uu < KC) *reinterpret_cast<float4*>(&sB[buf][uu][aq]) = rb[j];
+            if (uu < KC) *reinterpret_cast<float4*>(&sB[buf][uu][aq ^ ((uu & 1) ? XB : 0)]) = rb[j];
         }
     };
 
@@ -486,16 +497,16 @@ __device__ __forceinline__ void pass_w_euc_body(const PassArgs& p, const int w, 
 #pragma unroll
             for (int k2 = 0; k2 < KC / 2 / KH; k2++) {
                 const int kk = kh * (KC / 2 / KH) + k2;  // the wave's k steps of the chunk
-                const float av = sA[buf][2 * kk + h][cw * 32 + c];
+                const float av = sA[buf][2 * kk + h][(cw * 32 + c) ^ (h ? XA : 0)];
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++)
-                    if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][mb * 32 + c], acc[mb]);
+                    if (mb < nb) acc[mb] = mfma32x2(av, sB[buf][2 * kk + h][(mb * 32 + c) ^ (h ? XB : 0)], acc[mb]);
             }
             if constexpr (GR) {  // this wave's k quarter (steps wave*4 .. wave*4+3) for the carried block
 #pragma unroll
                 for (int j = 0; j < KC / 8; j++) {
                     const int kk = wave * (KC / 8) + j;
-                    gacc = mfma32x2(sB[buf][2 * kk + h][gmb * 32 + c], sB[buf][2 * kk + h][gnb * 32 + c], gacc);
+                    gacc = mfma32x2(sB[buf][2 * kk + h][(gmb * 32 + c) ^ (h ? XB : 0)], sB[buf][2 * kk + h][(gnb * 32 + c) ^ (h ? XB : 0)], gacc);
                 }
             }
         }
